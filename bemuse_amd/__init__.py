@@ -1,0 +1,299 @@
+"""bemuse_amd: MI355X (gfx950) implementation of the BEMUse per-frequency
+hydrodynamic hot path, driven through the C-ABI of ``libbemuse_amd.so``
+(declared in ``include/bemuse_amd.h``).
+
+The Python layer is a thin ctypes binding: every arithmetic step (Rankine and
+Havelock influence assembly, complex LU, solves, reductions) runs in the HIP
+library on the GPU. There is no CPU fallback: if the library is missing the
+import of the binding fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from ctypes import POINTER, c_double, c_int, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbemuse_amd.so")
+
+MODE_FP64, MODE_MIXED, MODE_MIXED_SURROGATE = 0, 1, 2
+COMPOSE_WAVE, COMPOSE_ZERO, COMPOSE_INF = 0, 1, 2
+GRAVITY = 9.81          # BEMUse_Math_Types.h:115
+TWO_PI_INV = 0.5 / math.pi
+
+_lib = None
+
+
+class BemError(RuntimeError):
+    pass
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [
+        ("refine_iters", c_int),
+        ("fell_back_fp64", c_int),
+        ("pivot_fallback", c_int),
+        ("residual", c_double),
+        ("ms_assemble", c_double),
+        ("ms_factor", c_double),
+        ("ms_solve", c_double),
+        ("surrogate_points", c_int),
+        ("exact_points", c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_PD = POINTER(c_double)
+_PI = POINTER(c_int)
+_PP = POINTER(c_void_p)
+
+_SIGS = {
+    "bem_last_error": (ctypes.c_char_p, []),
+    "bem_mesh_from_arrays": (c_int, [c_int, _PD, c_int, _PI, _PP]),
+    "bem_mesh_half_cylinder": (c_int, [c_double, c_double, c_int, c_int, c_int, c_int, _PP]),
+    "bem_mesh_box": (c_int, [c_double, c_double, c_double, c_double, c_int, c_int, c_int, c_int, _PP]),
+    "bem_mesh_semisub3": (c_int, [c_double, c_double, c_double, c_double, c_double, c_double,
+                                  c_int, c_int, c_int, c_int, c_int, c_int, _PP]),
+    "bem_mesh_cone_spar": (c_int, [c_double, c_double, c_double, c_double, c_double,
+                                   c_int, c_int, c_int, c_int, c_int, _PP]),
+    "bem_mesh_random_body": (c_int, [c_uint64, c_int, _PD, _PP]),
+    "bem_mesh_size": (c_int, [c_void_p, _PI, _PI]),
+    "bem_mesh_export": (c_int, [c_void_p, _PD, _PI]),
+    "bem_mesh_free": (None, [c_void_p]),
+    "bem_geometry": (c_int, [c_void_p, _PD, _PD, _PD, _PD, _PD]),
+    "bem_create": (c_int, [c_int, _PP]),
+    "bem_destroy": (None, [c_void_p]),
+    "bem_body_create": (c_int, [c_void_p, c_void_p, _PP]),
+    "bem_body_destroy": (None, [c_void_p]),
+    "bem_body_n": (c_int, [c_void_p]),
+    "bem_body_assemble_rankine": (c_int, [c_void_p, c_void_p]),
+    "bem_solve_frequency": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, c_int, c_int, c_int, _PD,
+                                    _PD, _PD, _PD, _PD, _PD, _PD, _PD, POINTER(SolveInfo)]),
+    "bem_get_rankine": (c_int, [c_void_p, c_void_p, _PD, _PD, _PD, _PD]),
+    "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
+    "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
+    "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
+}
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points this binding expects the library to export."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Loads libbemuse_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BemError(f"{LIB_PATH} not built: run __graft_entry__.build() (make -C bemuse_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().bem_last_error().decode(errors="replace")
+        raise BemError(f"bemuse_amd error {rc}: {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_PD) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_PI) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# Meshes
+# ---------------------------------------------------------------------------
+class Mesh:
+    """Surface panel mesh (nodes n x 3, elements e x 4 with -1 marking triangles)."""
+
+    def __init__(self, handle):
+        self._h = c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.bem_mesh_free(self._h)
+            self._h = None
+
+    @staticmethod
+    def _make(fn, *args):
+        h = c_void_p()
+        _check(fn(*args, ctypes.byref(h)))
+        return Mesh(h.value)
+
+    @classmethod
+    def from_arrays(cls, nodes, elems):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
+        elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 4)
+        return cls._make(lib().bem_mesh_from_arrays, len(nodes), _dp(nodes), len(elems), _ip(elems))
+
+    @classmethod
+    def half_cylinder(cls, radius, draft, n_radial, n_vertical, n_azimuthal, cosine=True):
+        return cls._make(lib().bem_mesh_half_cylinder, radius, draft, n_radial, n_vertical, n_azimuthal, int(cosine))
+
+    @classmethod
+    def box(cls, L, W, z0, z1, nx, ny, nz, open_top=True):
+        return cls._make(lib().bem_mesh_box, L, W, z0, z1, nx, ny, nz, int(open_top))
+
+    @classmethod
+    def semisub3(cls, col_radius=6.0, draft=20.0, spacing=50.0, pontoon_width=6.0, pontoon_height=4.0, gap=0.5,
+                 n_azimuthal=44, n_vertical=24, n_radial=8, nx=31, ny=5, nz=4):
+        return cls._make(lib().bem_mesh_semisub3, col_radius, draft, spacing, pontoon_width, pontoon_height, gap,
+                         n_azimuthal, n_vertical, n_radial, nx, ny, nz)
+
+    @classmethod
+    def cone_spar(cls, r_top, r_bot, draft, r_plate, t_plate, n_azimuthal, n_disc, n_rim, n_annulus, n_cone):
+        return cls._make(lib().bem_mesh_cone_spar, r_top, r_bot, draft, r_plate, t_plate, n_azimuthal, n_disc,
+                         n_rim, n_annulus, n_cone)
+
+    @classmethod
+    def random_body(cls, seed, index):
+        params = np.zeros(6)
+        h = c_void_p()
+        _check(lib().bem_mesh_random_body(c_uint64(seed), index, _dp(params), ctypes.byref(h)))
+        m = Mesh(h.value)
+        m.params = dict(type="cylinder" if params[0] == 0 else "barge", a=params[1], b=params[2], draft=params[3],
+                        target=int(params[4]), n=int(params[5]))
+        return m
+
+    def size(self):
+        nn, ne = c_int(), c_int()
+        _check(lib().bem_mesh_size(self._h, ctypes.byref(nn), ctypes.byref(ne)))
+        return nn.value, ne.value
+
+    @property
+    def n_panels(self):
+        return self.size()[1]
+
+    def arrays(self):
+        nn, ne = self.size()
+        nodes = np.zeros((nn, 3))
+        elems = np.zeros((ne, 4), dtype=np.int32)
+        _check(lib().bem_mesh_export(self._h, _dp(nodes), _ip(elems)))
+        return nodes, elems
+
+    def geometry(self):
+        """Per-panel host records (see bem_internal.h PanelField / QuadField)."""
+        ne = self.n_panels
+        body = np.zeros((ne, 32)); refl = np.zeros((ne, 32)); quad = np.zeros((ne, 32))
+        recv = np.zeros((ne, 4)); gen = np.zeros((ne, 8))
+        _check(lib().bem_geometry(self._h, _dp(body), _dp(refl), _dp(quad), _dp(recv), _dp(gen)))
+        return dict(body=body, refl=refl, quad=quad, recv=recv, gen=gen)
+
+
+# ---------------------------------------------------------------------------
+# Context / body / solve
+# ---------------------------------------------------------------------------
+class Context:
+    def __init__(self, device=0):
+        h = c_void_p()
+        _check(lib().bem_create(device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().bem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def eval_havelock(self, X, Y, V):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        out = np.zeros((len(X), 8))
+        _check(lib().bem_eval_havelock(self._h, 64, len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
+        return out
+
+    def dense_solve(self, A, B, precision=64):
+        A = np.asfortranarray(A, dtype=np.complex128)
+        B = np.asfortranarray(B, dtype=np.complex128)
+        if B.ndim == 1:
+            B = B[:, None]
+        n, nrhs = B.shape
+        X = np.zeros((n, nrhs), dtype=np.complex128, order="F")
+        info = SolveInfo()
+        _check(lib().bem_dense_solve(self._h, precision, n, A.ctypes.data_as(_PD), nrhs, B.ctypes.data_as(_PD),
+                                     X.ctypes.data_as(_PD), ctypes.byref(info)))
+        return X, info.as_dict()
+
+
+class Body:
+    def __init__(self, ctx: Context, mesh: Mesh):
+        h = c_void_p()
+        _check(lib().bem_body_create(ctx._h, mesh._h, ctypes.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        self.N = lib().bem_body_n(h)
+
+    def close(self):
+        if self._h:
+            lib().bem_body_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def assemble_rankine(self):
+        _check(lib().bem_body_assemble_rankine(self.ctx._h, self._h))
+
+    def get_rankine(self):
+        N = self.N
+        out = [np.zeros((N, N), order="F") for _ in range(4)]
+        _check(lib().bem_get_rankine(self.ctx._h, self._h, *[_dp(o) for o in out]))
+        return dict(Ssrc=out[0], Srefl=out[1], Dsrc=out[2], Drefl=out[3])
+
+    def get_system(self, kappa, mode=MODE_FP64, compose=COMPOSE_WAVE):
+        N = self.N
+        S = np.zeros((N, N), dtype=np.complex128, order="F")
+        D = np.zeros((N, N), dtype=np.complex128, order="F")
+        _check(lib().bem_get_system(self.ctx._h, self._h, mode, kappa, compose, S.ctypes.data_as(_PD),
+                                    D.ctypes.data_as(_PD)))
+        return S, D
+
+    def solve(self, omega, kappa=None, betas=(), mode=MODE_FP64, compose=COMPOSE_WAVE, rad_only=False,
+              want_phi=False):
+        """One frequency. omega is the solver's internal Omega (rad/s); kappa
+        defaults to Omega^2 / 9.81 (infinite depth)."""
+        if kappa is None:
+            kappa = omega * omega / GRAVITY
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        nb = len(betas)
+        A = np.zeros(36); B = np.zeros(36)
+        F = np.zeros((6, nb), dtype=np.complex128)
+        FK = np.zeros((6, nb), dtype=np.complex128)
+        SC = np.zeros((6, nb), dtype=np.complex128)
+        phiJ = np.zeros((self.N, 6), dtype=np.complex128, order="F") if want_phi else None
+        phiS = np.zeros((self.N, max(nb, 1)), dtype=np.complex128, order="F") if want_phi else None
+        info = SolveInfo()
+        _check(lib().bem_solve_frequency(
+            self.ctx._h, self._h, mode, omega, kappa, compose, int(rad_only), nb, _dp(betas) if nb else None,
+            _dp(A), _dp(B), F.ctypes.data_as(_PD), FK.ctypes.data_as(_PD), SC.ctypes.data_as(_PD),
+            phiJ.ctypes.data_as(_PD) if want_phi else None, phiS.ctypes.data_as(_PD) if want_phi else None,
+            ctypes.byref(info)))
+        res = dict(A=A.reshape(6, 6), B=B.reshape(6, 6), F=F, FK=FK, SC=SC, info=info.as_dict())
+        if want_phi:
+            res["phiJ"] = phiJ
+            res["phiS"] = phiS[:, :nb]
+        return res

@@ -1,0 +1,166 @@
+// Device-side special functions and Green's-function kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bemuse {
+namespace dev {
+
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kFourPIinv = 0.25 / 3.14159265358979323846;
+constexpr double kEUL = 0.5772156649;   // truncated Euler constant of the reference (BEMUse_Math_Types.h:99)
+
+// Struve H0 / H1 approximations (Kernels.cpp:35-116); for x > 3 they add the
+// Bessel Y0 / Y1 (ROCm device libm) to a rational function of (3/x)^2.
+__device__ inline double struve_h0(double x, double y0) {
+    if (x <= 3.0) {
+        double yy = (x / 3.0) * (x / 3.0);
+        double s = 1.909859164 + (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 - 0.000876918 * yy) * yy) * yy) * yy) * yy;
+        return s * (x / 3.0);
+    }
+    double yy = (3.0 / x) * (3.0 / x);
+    double c1 = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
+    double c2 = kPI * x * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
+    return c1 / c2 + y0;
+}
+
+__device__ inline double struve_h1(double x, double y1) {
+    if (x <= 3.0) {
+        double yy = (x / 3.0) * (x / 3.0);
+        return (1.909859286 + (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 - 0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
+    }
+    double yy = (3.0 / x) * (3.0 / x);
+    double c1 = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
+    double c2 = kPI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
+    return c1 / c2 + y1;
+}
+
+// Polynomials A..D (local flow) and dA..dC (its horizontal derivative) of the
+// Wu/Zhang/Zhu/Li/Wan/Noblesse global approximation (Eur. J. Mech. B 65 (2017)
+// eq. 14; Liang/Wu/Noblesse, Appl. Ocean Res. 74 (2018) eq. 16), coefficients
+// as tabulated at Kernels.cpp:120-256, evaluated in Horner form.
+__device__ inline double horner(const double* c, int n, double t) {
+    double s = c[n - 1];
+    for (int i = n - 2; i >= 0; i--) s = s * t + c[i];
+    return s;
+}
+
+struct GFCoeffs {
+    double A[10], B[10], C[8], D[10], dA[10], dB[10], dC[6];
+};
+
+__device__ __constant__ const GFCoeffs kGF = {
+    {1.21, -13.328, 215.896, -1763.96, 8418.94, -24314.21, 42002.57, -41592.9, 21859.0, -4838.6},
+    {0.938, 5.737, -67.92, 796.534, -4780.77, 17137.74, -36618.81, 44894.06, -29030.24, 7671.22},
+    {1.268, -9.747, 209.653, -1397.89, 5155.67, -9844.35, 9136.4, -3272.62},
+    {0.632, -40.97, 667.16, -6072.07, 31127.39, -96293.05, 181856.75, -205690.43, 128170.2, -33744.6},
+    {2.948, -24.53, 249.69, -754.85, -1187.71, 16370.75, -48811.41, 68220.87, -46688.0, 12622.25},
+    {1.11, 2.894, -76.765, 1565.35, -11336.19, 44270.15, -97014.11, 118879.26, -76209.82, 19923.28},
+    {14.19, -148.24, 847.8, -2318.58, 3168.35, -1590.27}};
+
+// Non-dimensional free-surface Green function (HavelockGF, Kernels.cpp:342-412):
+// in (X, Y, V) = K (x_R - y_q); out GF = L0 + W and GFh = Ls + Wh (complex).
+// DGFX = GFh X/h, DGFY = GFh Y/h (0 when h <= 1e-6), DGFZ = GF.
+struct GFVal { double gr, gi, hr, hi, h; };
+
+__device__ inline GFVal havelock_gf(double X, double Y, double vv) {
+    double hh = sqrt(X * X + Y * Y);
+    double dd = sqrt(hh * hh + vv * vv);
+    double al = -vv / dd;
+    double be = hh / dd;
+    double rho = dd / (1.0 + dd);
+    double omr = 1.0 - rho;
+    double rw = rho * omr * omr * omr;
+    double dd3 = dd * dd * dd;
+    double ev = exp(vv);
+    double inv1 = 1.0 / (1.0 + dd3);
+    // local flow L0 (GF_Func_L0, :258-282)
+    double PP = (log(0.5 * (dd - vv)) + kEUL - 2.0 * dd * dd) * ev + dd * dd - vv;
+    double A = horner(kGF.A, 10, rho), B = horner(kGF.B, 10, rho);
+    double C = horner(kGF.C, 8, rho), D = horner(kGF.D, 10, rho);
+    double RR = (1.0 - be) * A - be * B - al * C / (1.0 + 6.0 * al * rho * omr) + be * (1.0 - be) * D;
+    double L0 = 2.0 * PP * inv1 + 2.0 * rw * RR;
+    // derivative local flow Ls (GF_Func_Ls, :297-325)
+    double PS = (be + hh) / (dd - vv) - 2.0 * be + 2.0 * dd * ev - hh;
+    double QS = exp(-dd) * (1.0 - be) * (1.0 + dd * inv1);
+    double dA = horner(kGF.dA, 10, rho), dB = horner(kGF.dB, 10, rho), dC = horner(kGF.dC, 6, rho);
+    double R2 = be * dA - (1.0 - al) * dB + be * (1.0 - be) * rho * (1.0 - 2.0 * rho) * dC;
+    double Ls = 2.0 * PS * inv1 - 4.0 * QS + 2.0 * rw * R2;
+    // wave parts W, Wh (GF_Func_W / GF_Func_Wh, :284-295, :327-338)
+    double j0v = j0(hh), j1v = j1(hh);
+    double y0v = 0.0, y1v = 0.0;
+    if (hh > 3.0) { y0v = y0(hh); y1v = y1(hh); }
+    double E = 2.0 * kPI * ev;
+    GFVal o;
+    o.gr = L0 + E * struve_h0(hh, y0v);
+    o.gi = -E * j0v;
+    o.hr = Ls + E * (2.0 / kPI - struve_h1(hh, y1v));
+    o.hi = E * j1v;
+    o.h = hh;
+    return o;
+}
+
+// Wave S/D influence of one mirrored panel at receiver (xr, yr, zr): sum over
+// its quadrature nodes of w G and w (grad_R G . n_q) (Panel.cpp:601-636,776-794).
+// q points to the panel's 32-double quadrature record.
+__device__ inline void wave_panel(const double* q, double xr, double yr, double zr, double K,
+                                  double& sr, double& si, double& dr, double& di) {
+    sr = si = dr = di = 0.0;
+    if (K == 0.0) return;
+    int nq = (int)q[28];
+    double s1 = K * kFourPIinv, s2 = K * K * kFourPIinv;
+    for (int k = 0; k < nq; k++) {
+        const double* p = q + 7 * k;
+        double X = (xr - p[0]) * K, Y = (yr - p[1]) * K, V = (zr - p[2]) * K;
+        GFVal g = havelock_gf(X, Y, V);
+        double w = p[6];
+        sr += w * (g.gr * s1);
+        si += w * (g.gi * s1);
+        double cx = 0.0, cy = 0.0;
+        if (g.h > 1.0e-6) { cx = X / g.h; cy = Y / g.h; }
+        double tr = (g.hr * cx) * p[3] + (g.hr * cy) * p[4] + g.gr * p[5];
+        double ti = (g.hi * cx) * p[3] + (g.hi * cy) * p[4] + g.gi * p[5];
+        dr += w * (tr * s2);
+        di += w * (ti * s2);
+    }
+}
+
+// Analytic constant-strength source/dipole influence of a flat polygon
+// (FlatTriPanel/FlatQuadPanel::Inf_SourceKernel, Panel.cpp:17-156) at point P;
+// rec is the panel's 32-double record (centroid, frame, local vertices,
+// slopes, edge lengths).
+__device__ inline void rankine_panel(const double* rec, double px, double py, double pz, double& S, double& Dv) {
+    double dx = px - rec[0], dy = py - rec[1], dz = pz - rec[2];
+    double x = rec[3] * dx + rec[4] * dy + rec[5] * dz;
+    double y = rec[6] * dx + rec[7] * dy + rec[8] * dz;
+    double z = rec[9] * dx + rec[10] * dy + rec[11] * dz;
+    int n = (int)rec[28];
+    double Z2 = z * z;
+    double DX[4], DY[4], e[4], h[4], r[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        DX[k] = x - rec[12 + k];
+        DY[k] = y - rec[16 + k];
+        e[k] = DX[k] * DX[k] + Z2;
+        h[k] = DX[k] * DY[k];
+        r[k] = sqrt(DX[k] * DX[k] + DY[k] * DY[k] + Z2);
+    }
+    double phi = 0.0, bl = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k < n) {
+            int k1 = (k + 1 == n) ? 0 : k + 1;
+            double m = rec[20 + k], d = rec[24 + k];
+            phi += atan((m * e[k] - h[k]) / (z * r[k])) - atan((m * e[k1] - h[k1]) / (z * r[k1]));
+            double ddx = rec[12 + k1] - rec[12 + k], ddy = rec[16 + k1] - rec[16 + k];
+            double B = (DX[k] * ddy - DY[k] * ddx) / d;
+            double L = log((r[k] + r[k1] + d) / (r[k] + r[k1] - d));
+            bl += B * L;
+        }
+    }
+    Dv = phi * kFourPIinv;
+    S = (z * phi - bl) * kFourPIinv;
+}
+
+}  // namespace dev
+}  // namespace bemuse

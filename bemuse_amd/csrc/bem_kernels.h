@@ -1,0 +1,41 @@
+// Launch wrappers of the device kernels (bem_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace bemuse {
+
+constexpr int kPad = 128;       // Np = roundup(N, kPad)
+constexpr int kLuNb = 64;       // LU panel width
+constexpr int kMaxRhs = 16;     // columns of the right-hand-side block (6 + nb <= 16)
+
+void launch_rankine(hipStream_t s, int N, int Np, const double* body, const double* refl, const double* recv,
+                    double* Ssrc, double* Srefl, double* Dsrc, double* Drefl);
+void launch_wave_compose(hipStream_t s, int N, int Np, double K, int compose, const double* quad, const double* recv,
+                         const double* Ssrc, const double* Srefl, const double* Dsrc, const double* Drefl,
+                         double* Sr, double* Si, double* Dr, double* Di, float* D32r, float* D32i);
+void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out);
+void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,
+                     double omega, double kappa, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
+                     double* DAr, double* DAi);
+int matvec_kslabs(int Np);
+template <typename TM>
+void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
+                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi);
+template <typename T>
+void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv);
+template <typename T, typename TX>
+void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const int* ipiv, const int* flag_any,
+                     TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch);
+void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb);
+void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb);
+void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb);
+void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
+                   const double* xi, double* out);
+void launch_rowabs(hipStream_t s, int N, int Np, const double* Ar, const double* Ai, double* rowsum);
+void launch_reduce_coeffs(hipStream_t s, int N, int Np, int nb, const double* gen, const double* Xr, const double* Xi,
+                          const double* PhiIr, const double* PhiIi, const double* DAr, const double* DAi,
+                          double* out);
+
+}  // namespace bemuse

@@ -1,0 +1,153 @@
+/*
+ * bemuse_amd: C-ABI of the MI355X (gfx950) implementation of the BEMUse
+ * per-frequency hydrodynamic hot path (Rankine + Havelock influence assembly,
+ * dense complex LU, radiation/diffraction solves, added mass / damping /
+ * excitation reductions).
+ *
+ * Conventions
+ *   - Every entry point returns 0 (BEM_OK) on success or a BEM_ERR_* code;
+ *     bem_last_error() then returns a message (thread-local).
+ *   - Buffers are caller-owned host memory. Matrices are column-major.
+ *     Complex values are interleaved (re, im) doubles, layout-compatible with
+ *     std::complex<double>.
+ *   - The reference exposes this path only through C++ virtuals
+ *     (src/Solver/Solver_Base.h:119-151, src/Solver/Hydrodynamic_Solver.h:135-160);
+ *     the C++ drop-in class in include/bemuse_hydro.h sits on top of these
+ *     entry points. Each entry point names the reference seam it replaces.
+ *   - One context per host thread. Calls on one context are serialised on the
+ *     context's HIP stream.
+ */
+#ifndef BEMUSE_AMD_H
+#define BEMUSE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BEM_OK = 0,
+    BEM_ERR_ARG = 1,      /* invalid argument */
+    BEM_ERR_HIP = 2,      /* HIP runtime error (no device, launch failure, ...) */
+    BEM_ERR_NOMEM = 3,    /* device allocation failed */
+    BEM_ERR_NUMERIC = 4,  /* zero pivot or non-finite result */
+    BEM_ERR_STATE = 5     /* call out of order (e.g. solve before Rankine assembly) */
+};
+
+/* Arithmetic mode of bem_solve_frequency. */
+enum {
+    BEM_MODE_FP64 = 0,            /* FP64 assembly, FP64-MFMA LU, FP64 solves */
+    BEM_MODE_MIXED = 1,           /* FP64 assembly, FP32-MFMA LU, FP64 iterative refinement */
+    BEM_MODE_MIXED_SURROGATE = 2  /* as MIXED, wave term from the trained surrogate inside its domain */
+};
+
+/* Influence-matrix composition case (Hydrodynamic_Solver.cpp:415-429). */
+enum {
+    BEM_COMPOSE_WAVE = 0,  /* S = Ssrc + Srefl + Swave, D = Dsrc + Drefl + Dwave */
+    BEM_COMPOSE_ZERO = 1,  /* Omega == 0: S = Ssrc + Srefl, D = Dsrc + Drefl      */
+    BEM_COMPOSE_INF = 2    /* Omega == FreqInf: S = Ssrc - Srefl, D = Dsrc - Drefl */
+};
+
+typedef struct bem_mesh bem_mesh;
+typedef struct bem_ctx bem_ctx;
+typedef struct bem_body bem_body;
+
+/* Diagnostics of one frequency solve. */
+typedef struct bem_solve_info {
+    int refine_iters;        /* iterative-refinement steps taken (MIXED modes) */
+    int fell_back_fp64;      /* 1 if refinement stagnated and the FP64 LU was used */
+    int pivot_fallback;      /* panels whose speculative (diagonal) pivots were rejected */
+    double residual;         /* final max-norm relative residual |b - A x| / (|A| |x|) */
+    double ms_assemble;      /* device time of wave assembly + compose [ms] */
+    double ms_factor;        /* device time of the LU [ms] */
+    double ms_solve;         /* device time of RHS, solves, refinement, reductions [ms] */
+    int surrogate_points;    /* wave-kernel evaluations served by the surrogate */
+    int exact_points;        /* wave-kernel evaluations served by the exact kernel */
+} bem_solve_info;
+
+const char* bem_last_error(void);
+
+/* ---- meshes (host only; no GPU needed) -------------------------------------
+ * Node/element arrays as consumed by Boundary::Get_Elements
+ * (src/Boundary/Boundary_Base.h:168-172): nodes n x 3, elements e x 4 with
+ * elems[4e+3] = -1 for a triangle. */
+int bem_mesh_from_arrays(int n_nodes, const double* nodes, int n_elems, const int* elems, bem_mesh** out);
+/* Half_Cylinder (src/Boundary/Volume_of_Revolution.cpp:167-199). */
+int bem_mesh_half_cylinder(double radius, double draft, int n_radial, int n_vertical,
+                           int n_azimuthal, int cosine, bem_mesh** out);
+/* Rectangular box x in [-L/2,L/2], y in [-W/2,W/2], z in [z0,z1]
+ * (cf. src/Boundary/Barge.cpp:34-191); open_top = 1 for a surface-piercing barge. */
+int bem_mesh_box(double L, double W, double z0, double z1, int nx, int ny, int nz, int open_top, bem_mesh** out);
+/* Three columns + three closed pontoons (the 6k-panel semi-submersible workload). */
+int bem_mesh_semisub3(double col_radius, double draft, double spacing, double pontoon_width,
+                      double pontoon_height, double gap, int n_azimuthal, int n_vertical,
+                      int n_radial, int nx, int ny, int nz, bem_mesh** out);
+/* Truncated-cone spar with heave plate (single volume of revolution). */
+int bem_mesh_cone_spar(double r_top, double r_bot, double draft, double r_plate, double t_plate,
+                       int n_azimuthal, int n_disc, int n_rim, int n_annulus, int n_cone, bem_mesh** out);
+/* Body `index` of the seeded uneven batch; params (6, optional): type (0 cyl,
+ * 1 barge), dim a, dim b, draft, target panels, actual panels. */
+int bem_mesh_random_body(uint64_t seed, int index, double* params, bem_mesh** out);
+int bem_mesh_size(const bem_mesh* m, int* n_nodes, int* n_elems);
+int bem_mesh_export(const bem_mesh* m, double* nodes, int* elems);
+void bem_mesh_free(bem_mesh* m);
+
+/* Per-panel records (host only), for inspection and tests: body panels and
+ * mirrored panels (N x 32 doubles each, see bem_internal.h PanelField),
+ * mirrored quadrature records (N x 32), BC points (N x 4), generalised
+ * normals (N x 8: n, x x n, area, 0). Any pointer may be NULL.
+ * Replaces Create_Panels / Specify_BC_Positions / Set_RHS_Mat precompute
+ * (Hydrodynamic_Solver.cpp:30-81,117-175; Solver_Base.cpp:71-83). */
+int bem_geometry(const bem_mesh* m, double* body, double* refl, double* quad, double* recv, double* gen);
+
+/* ---- context and bodies ----------------------------------------------------- */
+int bem_create(int device, bem_ctx** out);
+void bem_destroy(bem_ctx* ctx);
+
+/* Uploads the geometry of a mesh (Hydrodynamic_Radiation_Solver::Setup,
+ * Hydrodynamic_Solver.cpp:269-279, geometry part). */
+int bem_body_create(bem_ctx* ctx, const bem_mesh* m, bem_body** out);
+void bem_body_destroy(bem_body* b);
+int bem_body_n(const bem_body* b);
+
+/* Rankine source/dipole influence of the body and its image
+ * (Prepare_Linear_System, Hydrodynamic_Solver.cpp:83-115), once per body. */
+int bem_body_assemble_rankine(bem_ctx* ctx, bem_body* b);
+
+/* One frequency (Hydrodynamic_Radiation_Solver::Solve,
+ * Hydrodynamic_Solver.cpp:400-477): wave assembly, compose, LU, radiation and
+ * diffraction solves, A/B/F reductions.
+ *   omega   the solver's internal Omega (= Frequency / 2 pi, :405)
+ *   kappa   wavenumber (Omega^2 / 9.81 in infinite depth, :409)
+ *   compose BEM_COMPOSE_*; rad_only = 1 stops after A/B (:469)
+ *   betas   nb wave headings [rad]
+ * Outputs: A36, B36 row-major 6x6; F, FK, SC row-major 6 x nb complex (may be
+ * NULL); phiJ N x 6 and phiS N x nb complex column-major (may be NULL). */
+int bem_solve_frequency(bem_ctx* ctx, bem_body* b, int mode, double omega, double kappa,
+                        int compose, int rad_only, int nb, const double* betas,
+                        double* A36, double* B36, double* F, double* FK, double* SC,
+                        double* phiJ, double* phiS, bem_solve_info* info);
+
+/* ---- kernel-level access (parity checks of individual seams) ---------------- */
+/* Rankine matrices as assembled on the device: SSrc = -s, SSrcRefl = -sr,
+ * DSrc = d, DSrcRefl = dr (N x N real column-major; NULL skips). */
+int bem_get_rankine(bem_ctx* ctx, bem_body* b, double* Ssrc, double* Srefl, double* Dsrc, double* Drefl);
+/* Composed SMat and DMat for one frequency (complex N x N column-major),
+ * as produced by the device assembly in the given mode. */
+int bem_get_system(bem_ctx* ctx, bem_body* b, int mode, double kappa, int compose, double* SMat, double* DMat);
+/* Device evaluation of HavelockGF (Kernels.cpp:342-412) at n points;
+ * out: 8 doubles per point (GF, DGFX, DGFY, DGFZ as re, im). precision 64 or 32. */
+int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const double* Y,
+                      const double* V, double* out);
+/* Device LU + solve of a caller matrix (complex n x n, column-major):
+ * precision 64 (FP64 LU) or 32 (FP32-MFMA LU + FP64 refinement); X receives
+ * the solution of A X = B for nrhs columns. */
+int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
+                    double* X, bem_solve_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEMUSE_AMD_H */
