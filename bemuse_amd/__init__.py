@@ -9,6 +9,7 @@ import of the binding fails loudly.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import math
 import os
@@ -25,6 +26,17 @@ GRAVITY = 9.81          # BEMUse_Math_Types.h:115
 TWO_PI_INV = 0.5 / math.pi
 
 _lib = None
+_exiting = False
+
+
+def _mark_exit():
+    # Objects collected after interpreter shutdown starts must not call into
+    # the HIP runtime, which may already be torn down.
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_mark_exit)
 
 
 class BemError(RuntimeError):
@@ -78,6 +90,7 @@ _SIGS = {
     "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
+    "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
 }
 
 
@@ -125,7 +138,7 @@ class Mesh:
         self._h = c_void_p(handle)
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_h", None) and _lib is not None and not _exiting:
             _lib.bem_mesh_free(self._h)
             self._h = None
 
@@ -210,6 +223,8 @@ class Context:
             self._h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:
@@ -250,6 +265,8 @@ class Body:
             self._h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:
@@ -271,6 +288,13 @@ class Body:
         _check(lib().bem_get_system(self.ctx._h, self._h, mode, kappa, compose, S.ctypes.data_as(_PD),
                                     D.ctypes.data_as(_PD)))
         return S, D
+
+    def profile_lu(self, kappa, mode=MODE_FP64):
+        """Instrumented factorisation: GEMM launch timings of one LU (see bem_profile_lu)."""
+        out = np.zeros(6)
+        _check(lib().bem_profile_lu(self.ctx._h, self._h, mode, kappa, _dp(out)))
+        return dict(ms_lu=out[0], ms_gemm=out[1], gemm_flops=out[2], gemm_launches=int(out[3]),
+                    ms_gemm_per_launch=out[4], flops_per_launch=out[5])
 
     def solve(self, omega, kappa=None, betas=(), mode=MODE_FP64, compose=COMPOSE_WAVE, rad_only=False,
               want_phi=False):

@@ -573,6 +573,45 @@ int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose,
     return BEM_OK;
 }
 
+int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out) {
+    if (!c || !b || !b->have_rankine || !out) return fail(BEM_ERR_STATE, "bem_profile_lu: Rankine matrices not assembled");
+    HIP_TRY(hipSetDevice(c->device));
+    int r = ensure_freq_buffers(b);
+    if (r) return r;
+    if (mode != BEM_MODE_FP64 && (r = b->sys.init32())) return r;
+    assemble(c, b, mode, kappa, BEM_COMPOSE_WAVE);
+    LinSys& s = b->sys;
+    LuProfile* prof = new LuProfile;
+    prof->N = b->N;
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (mode == BEM_MODE_FP64)
+        launch_lu<double>(c->stream, s.Np, s.Dr, s.Di, s.sd_r, s.sd_i, s.s21_r, s.s21_i, s.flag, s.ipiv, prof);
+    else
+        launch_lu<float>(c->stream, s.Np, s.D32r, s.D32i, s.sd32_r, s.sd32_i, s.s2132_r, s.s2132_i, s.flag32,
+                         s.ipiv32, prof);
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    hipError_t e = hipStreamSynchronize(c->stream);
+    double sum = 0.0;
+    for (int i = 0; i < prof->n; i++) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, prof->ev[2 * i], prof->ev[2 * i + 1]);
+        sum += t;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[1]);
+    out[0] = tot;
+    out[1] = sum;
+    out[2] = prof->useful_flops;
+    out[3] = prof->n;
+    out[4] = prof->n ? sum / prof->n : 0.0;
+    out[5] = prof->n ? prof->useful_flops / prof->n : 0.0;
+    for (auto& ev : prof->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    delete prof;
+    if (e != hipSuccess) return fail(BEM_ERR_HIP, std::string("bem_profile_lu: ") + hipGetErrorString(e));
+    return BEM_OK;
+}
+
 int bem_eval_havelock(bem_ctx* c, int precision, int n, const double* X, const double* Y, const double* V,
                       double* out) {
     if (!c || n <= 0 || !X || !Y || !V || !out) return fail(BEM_ERR_ARG, "bem_eval_havelock: bad arguments");

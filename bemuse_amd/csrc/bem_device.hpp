@@ -12,7 +12,7 @@ constexpr double kEUL = 0.5772156649;   // truncated Euler constant of the refer
 
 // Struve H0 / H1 approximations (Kernels.cpp:35-116); for x > 3 they add the
 // Bessel Y0 / Y1 (ROCm device libm) to a rational function of (3/x)^2.
-__device__ inline double struve_h0(double x, double y0) {
+__host__ __device__ inline double struve_h0(double x, double y0) {
     if (x <= 3.0) {
         double yy = (x / 3.0) * (x / 3.0);
         double s = 1.909859164 + (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 - 0.000876918 * yy) * yy) * yy) * yy) * yy;
@@ -24,7 +24,7 @@ __device__ inline double struve_h0(double x, double y0) {
     return c1 / c2 + y0;
 }
 
-__device__ inline double struve_h1(double x, double y1) {
+__host__ __device__ inline double struve_h1(double x, double y1) {
     if (x <= 3.0) {
         double yy = (x / 3.0) * (x / 3.0);
         return (1.909859286 + (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 - 0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
@@ -39,7 +39,7 @@ __device__ inline double struve_h1(double x, double y1) {
 // Wu/Zhang/Zhu/Li/Wan/Noblesse global approximation (Eur. J. Mech. B 65 (2017)
 // eq. 14; Liang/Wu/Noblesse, Appl. Ocean Res. 74 (2018) eq. 16), coefficients
 // as tabulated at Kernels.cpp:120-256, evaluated in Horner form.
-__device__ inline double horner(const double* c, int n, double t) {
+__host__ __device__ inline double horner(const double* c, int n, double t) {
     double s = c[n - 1];
     for (int i = n - 2; i >= 0; i--) s = s * t + c[i];
     return s;
@@ -49,7 +49,7 @@ struct GFCoeffs {
     double A[10], B[10], C[8], D[10], dA[10], dB[10], dC[6];
 };
 
-__device__ __constant__ const GFCoeffs kGF = {
+constexpr GFCoeffs kGF = {
     {1.21, -13.328, 215.896, -1763.96, 8418.94, -24314.21, 42002.57, -41592.9, 21859.0, -4838.6},
     {0.938, 5.737, -67.92, 796.534, -4780.77, 17137.74, -36618.81, 44894.06, -29030.24, 7671.22},
     {1.268, -9.747, 209.653, -1397.89, 5155.67, -9844.35, 9136.4, -3272.62},
@@ -63,7 +63,7 @@ __device__ __constant__ const GFCoeffs kGF = {
 // DGFX = GFh X/h, DGFY = GFh Y/h (0 when h <= 1e-6), DGFZ = GF.
 struct GFVal { double gr, gi, hr, hi, h; };
 
-__device__ inline GFVal havelock_gf(double X, double Y, double vv) {
+__host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     double hh = sqrt(X * X + Y * Y);
     double dd = sqrt(hh * hh + vv * vv);
     double al = -vv / dd;
@@ -103,7 +103,7 @@ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
 // Wave S/D influence of one mirrored panel at receiver (xr, yr, zr): sum over
 // its quadrature nodes of w G and w (grad_R G . n_q) (Panel.cpp:601-636,776-794).
 // q points to the panel's 32-double quadrature record.
-__device__ inline void wave_panel(const double* q, double xr, double yr, double zr, double K,
+__host__ __device__ inline void wave_panel(const double* q, double xr, double yr, double zr, double K,
                                   double& sr, double& si, double& dr, double& di) {
     sr = si = dr = di = 0.0;
     if (K == 0.0) return;
@@ -129,7 +129,7 @@ __device__ inline void wave_panel(const double* q, double xr, double yr, double 
 // (FlatTriPanel/FlatQuadPanel::Inf_SourceKernel, Panel.cpp:17-156) at point P;
 // rec is the panel's 32-double record (centroid, frame, local vertices,
 // slopes, edge lengths).
-__device__ inline void rankine_panel(const double* rec, double px, double py, double pz, double& S, double& Dv) {
+__host__ __device__ inline void rankine_panel(const double* rec, double px, double py, double pz, double& S, double& Dv) {
     double dx = px - rec[0], dy = py - rec[1], dz = pz - rec[2];
     double x = rec[3] * dx + rec[4] * dy + rec[5] * dz;
     double y = rec[6] * dx + rec[7] * dy + rec[8] * dz;

@@ -23,8 +23,20 @@ int matvec_kslabs(int Np);
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
                    double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi);
+// Optional timing of the trailing-update GEMM launches of one factorisation:
+// events bracket every GEMM launch; useful_flops counts 8 M N K over the
+// unpadded part (order N) of each launch.
+struct LuProfile {
+    static constexpr int kMax = 1024;
+    hipEvent_t ev[2 * kMax] = {};
+    int n = 0;
+    int N = 0;
+    double useful_flops = 0.0;
+};
+
 template <typename T>
-void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv);
+void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv,
+               LuProfile* prof = nullptr);
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const int* ipiv, const int* flag_any,
                      TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch);

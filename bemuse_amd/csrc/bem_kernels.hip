@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 namespace bemuse {
 
 using namespace dev;
@@ -466,19 +468,23 @@ __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai) {
 // multiples of 64, K multiple of 16.
 //
 // MFMA 16x16x4 operand layout (lane l = i + 16 g): A supplies A[i][k(g)],
-// B supplies B[k(g)][j], the accumulator holds D[4g + r][j], r = 0..3. Within
-// a chunk of 16 k, lane group g covers k = 4g + t at step t, so each lane
-// reads 4 consecutive k of a B column as one vector.
+// B supplies B[k(g)][j]. The accumulator register r of lane l holds D[row][j]
+// with row = 4g + r for the f32 instruction and row = g + 4r for the f64
+// instruction on gfx950 (measured). Within a chunk of 16 k, lane group g
+// covers k = 4g + t at step t, so each lane reads 4 consecutive k of a B
+// column as one vector.
 // ---------------------------------------------------------------------------
 template <typename T> struct MfmaT;
 template <> struct MfmaT<double> {
     typedef d4 acc;
     static __device__ inline acc mma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int r) { return g + 4 * r; }
     static constexpr int WM = 2, WN = 2;   // wave tile 32 x 32
 };
 template <> struct MfmaT<float> {
     typedef f4 acc;
     static __device__ inline acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int r) { return 4 * g + r; }
     static constexpr int WM = 4, WN = 4;   // wave tile 64 x 64
 };
 
@@ -540,11 +546,12 @@ __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, T* __res
     for (int a = 0; a < WM; a++)
 #pragma unroll
         for (int b = 0; b < WN; b++) {
-            size_t off = (size_t)(cc + col0 + 16 * b + i) * ldl + rc + row0 + 16 * a + 4 * g;
+            size_t off = (size_t)(cc + col0 + 16 * b + i) * ldl + rc + row0 + 16 * a;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                Pr[off + r] -= cr[a][b][r];
-                Pi[off + r] -= ci[a][b][r];
+                int rr = MT::row(g, r);
+                Pr[off + rr] -= cr[a][b][r];
+                Pi[off + rr] -= ci[a][b][r];
             }
         }
 }
@@ -824,7 +831,8 @@ template void launch_matvec<double>(hipStream_t, int, int, const double*, const 
                                     double*, double*, const double*, const double*, int, double*, double*);
 
 template <typename T>
-void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv) {
+void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv,
+               LuProfile* prof) {
     int npanel = Np / LU_NB;
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv);
     for (int p = 0; p < npanel; p++) {
@@ -841,14 +849,30 @@ void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, 
             constexpr int TM = 32 * MfmaT<T>::WM, TN = 32 * MfmaT<T>::WN;
             int tiles = (int)(cdiv(rest, TM) * cdiv(rest, TN));
             int per = (tiles + 7) / 8;
+            bool timed = prof && prof->n < LuProfile::kMax;
+            if (timed) {
+                if (!prof->ev[2 * prof->n]) {
+                    (void)hipEventCreate(&prof->ev[2 * prof->n]);
+                    (void)hipEventCreate(&prof->ev[2 * prof->n + 1]);
+                }
+                (void)hipEventRecord(prof->ev[2 * prof->n], s);
+            }
             hipLaunchKernelGGL(k_cgemm_sub<T>, dim3(per * 8), dim3(256), 0, s, rest, rest, LU_NB, Ar, Ai, Np,
                                k0 + LU_NB, k0, k0, k0 + LU_NB, k0 + LU_NB, k0 + LU_NB);
+            if (timed) {
+                (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
+                double u = (double)std::max(0, prof->N - (k0 + LU_NB));
+                prof->useful_flops += 8.0 * u * u * LU_NB;
+                prof->n++;
+            }
         }
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
 }
-template void launch_lu<double>(hipStream_t, int, double*, double*, double*, double*, double*, double*, int*, int*);
-template void launch_lu<float>(hipStream_t, int, float*, float*, float*, float*, float*, float*, int*, int*);
+template void launch_lu<double>(hipStream_t, int, double*, double*, double*, double*, double*, double*, int*, int*,
+                                 LuProfile*);
+template void launch_lu<float>(hipStream_t, int, float*, float*, float*, float*, float*, float*, int*, int*,
+                                LuProfile*);
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const int* ipiv, const int* flag_any,

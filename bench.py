@@ -1,0 +1,173 @@
+"""Benchmark: frequency-solves/s of the per-frequency BEM path at ~6k panels.
+
+Workload (the 6k-panel headline configuration): a three-column semi-submersible
+(columns r = 6 m, draft 20 m, at the vertices of a 50 m triangle, joined by
+closed 6 m x 4 m pontoons), 44 x (24 + 8) panels per column and 598 per
+pontoon (N = 6018 panels), 40 wave frequencies uniform in 0.05-2.5 rad/s,
+6 radiation DOF + 5 headings (0, 45, 90, 135, 180 deg). Synthetic geometry,
+no input data.
+
+A step is one frequency solve (wave assembly + compose, LU, radiation and
+diffraction solves, refinement, A/B/F reductions); steps cycle through the 40
+frequencies. Rankine assembly (once per body) is outside the timed region and
+reported separately. The headline mode is MIXED (FP32-MFMA LU + FP64
+refinement); the FP64 mode is timed in the same run for the ratio, and every
+MIXED result is compared with the FP64 GPU result at the same frequency and
+with the FP64 CPU oracle at the sampled frequency.
+
+Prints one JSON line (rank 0, N=1 only).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64 = 78.6e12      # MI355X dense FP64 (vector and matrix), flop/s
+PEAK_FP32 = 157.3e12     # MI355X dense FP32 matrix, flop/s
+METRIC = "frequency-solves/sec at 6k panels (mixed+surrogate vs FP64) at <=1e-6 rel err"
+
+
+def rel(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+
+
+def abf_err(g, r):
+    return max(rel(g["A"], r["A"]), rel(g["B"], r["B"]), rel(g["F"], r["F"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1, help="frequencies of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--small", action="store_true", help="1k-panel cylinder instead of the 6k semi-sub (quick check)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(frequencies are independent: a multi-GPU run would shard the frequency list).", file=sys.stderr)
+        sys.exit(2)
+
+    import bemuse_amd as bm
+
+    if args.small:
+        mesh = bm.Mesh.half_cylinder(10.0, 20.0, 8, 24, 32)
+        workload = "floating cylinder r=10 m draft 20 m, 1024 panels"
+    else:
+        mesh = bm.Mesh.semisub3()
+        workload = "three-column semi-sub (r=6 m, draft 20 m, 50 m triangle, 6x4 m pontoons)"
+    omegas = np.linspace(0.05, 2.5, 40)
+    betas = np.deg2rad([0.0, 45.0, 90.0, 135.0, 180.0])
+
+    ctx = bm.Context(0)
+    body = bm.Body(ctx, mesh)
+    N = body.N
+    t0 = time.perf_counter()
+    body.assemble_rankine()
+    t_rankine = time.perf_counter() - t0
+    print(f"[bench] N={N} Rankine assembly {t_rankine * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    def run(mode, steps, warmup):
+        for i in range(warmup):
+            body.solve(omegas[i % len(omegas)], betas=betas, mode=mode)
+        res = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            res.append(body.solve(omegas[i % len(omegas)], betas=betas, mode=mode))
+        dt = time.perf_counter() - t0
+        return dt, res
+
+    dt64, r64 = run(bm.MODE_FP64, args.steps, args.warmup)
+    print(f"[bench] FP64 {args.steps} solves in {dt64:.3f} s ({args.steps / dt64:.2f}/s)", file=sys.stderr, flush=True)
+    dtmx, rmx = run(bm.MODE_MIXED, args.steps, args.warmup)
+    print(f"[bench] MIXED {args.steps} solves in {dtmx:.3f} s ({args.steps / dtmx:.2f}/s)", file=sys.stderr, flush=True)
+    err_mx_vs_64 = max(abf_err(a, b) for a, b in zip(rmx, r64))
+    fallbacks = sum(r["info"]["fell_back_fp64"] for r in rmx)
+    iters = [r["info"]["refine_iters"] for r in rmx]
+
+    def stage_means(res):
+        return {k: float(np.mean([r["info"][k] for r in res])) for k in ("ms_assemble", "ms_factor", "ms_solve")}
+
+    # roofline of the dominant kernel: the MFMA trailing-update GEMM of the LU
+    kap = 1.0 ** 2 / 9.81
+    prof_mx = body.profile_lu(kap, bm.MODE_MIXED)
+    prof_64 = body.profile_lu(kap, bm.MODE_FP64)
+    achieved = prof_mx["flops_per_launch"] / (prof_mx["ms_gemm_per_launch"] * 1e-3)
+    roofline = {
+        "kernel": "k_cgemm_sub<float> (LU trailing update, FP32 MFMA 16x16x4)",
+        "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32, "traffic": None,
+        "fp64_gemm": {"achieved": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / 1e12,
+                      "peak": PEAK_FP64 / 1e12,
+                      "frac": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / PEAK_FP64},
+        "lu_ms": {"mixed": prof_mx["ms_lu"], "fp64": prof_64["ms_lu"],
+                  "gemm_share_mixed": prof_mx["ms_gemm"] / prof_mx["ms_lu"],
+                  "gemm_share_fp64": prof_64["ms_gemm"] / prof_64["ms_lu"]},
+    }
+
+    cpu = None
+    err_vs_oracle = None
+    if args.cpu_sample > 0:
+        import oracle
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        oracle.set_threads(nthreads)
+        nodes, elems = mesh.arrays()
+        ob = oracle.Body(nodes, elems)
+        t0 = time.perf_counter()
+        ob.rankine()
+        t_rk = time.perf_counter() - t0
+        sample = [omegas[len(omegas) // 2]][: args.cpu_sample]
+        t0 = time.perf_counter()
+        ref = [ob.solve(w, betas=betas) for w in sample]
+        t_cpu = (time.perf_counter() - t0) / len(sample)
+        gm = [body.solve(w, betas=betas, mode=bm.MODE_MIXED) for w in sample]
+        g64 = [body.solve(w, betas=betas, mode=bm.MODE_FP64) for w in sample]
+        err_vs_oracle = {"mixed": max(abf_err(a, b) for a, b in zip(gm, ref)),
+                         "fp64": max(abf_err(a, b) for a, b in zip(g64, ref))}
+        cpu = {"value": 1.0 / t_cpu, "unit": "frequency-solves/s", "cores": nthreads, "kind": "port",
+               "sample": f"{len(sample)} frequency (omega={sample[0]:.4f} rad/s) of the same {N}-panel workload, "
+                         f"FP64 oracle (OpenMP assembly + blocked partial-pivot LU); Rankine assembly "
+                         f"{t_rk:.1f} s once per body, excluded like on the GPU",
+               "stages_s": ref[0]["timings"]}
+        print(f"[bench] CPU oracle {t_cpu:.2f} s/frequency on {nthreads} threads", file=sys.stderr, flush=True)
+
+    value = args.steps / dtmx
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "frequency-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dtmx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-lu+fp64-refine",
+        "data": "synthetic",
+        "config": {"workload": workload, "panels": N, "frequencies": len(omegas), "headings": len(betas),
+                   "dof": 6, "mode": "mixed (FP64 assembly, FP32-MFMA LU, FP64 iterative refinement)"},
+        "fp64_value": args.steps / dt64,
+        "speedup_vs_fp64": dt64 / dtmx,
+        "max_rel_err_mixed_vs_fp64_gpu": err_mx_vs_64,
+        "max_rel_err_vs_cpu_oracle": err_vs_oracle,
+        "mixed_refine_iters": [int(min(iters)), int(max(iters))],
+        "mixed_fp64_fallbacks": int(fallbacks),
+        "stage_ms": {"mixed": stage_means(rmx), "fp64": stage_means(r64)},
+        "rankine_ms": t_rankine * 1e3,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -146,6 +146,13 @@ int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const
 int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
                     double* X, bem_solve_info* info);
 
+/* Instrumented factorisation for measurement: assembles the system at kappa
+ * (compose WAVE) and factors it in the mode's precision with HIP events
+ * around every trailing-update GEMM launch. out (6 doubles): total LU ms,
+ * summed GEMM ms, useful GEMM flops (unpadded order N), GEMM launches,
+ * mean GEMM ms per launch, mean useful flops per launch. */
+int bem_profile_lu(bem_ctx* ctx, bem_body* b, int mode, double kappa, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 CPU oracle.
+
+Tolerances: FP64 kernels per-matrix relative max-norm <= 1e-11; A, B, F
+<= 1e-8 relative (max-norm of the oracle matrix as denominator) in FP64 mode
+and <= 1e-6 in the mixed FP32-LU + FP64-refinement mode.
+"""
+import numpy as np
+import pytest
+
+import bemuse_amd as bm
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    den = np.abs(b).max()
+    return np.abs(a - b).max() / (den if den > 0 else 1.0)
+
+
+def small_cylinder():
+    return bm.Mesh.half_cylinder(5.0, 8.0, 4, 8, 16)      # 16 tris + 176 quads
+
+
+def small_barge():
+    return bm.Mesh.box(12.0, 6.0, -4.0, 0.0, 8, 4, 3, True)
+
+
+@pytest.fixture(scope="module")
+def cyl(ctx):
+    mesh = small_cylinder()
+    nodes, elems = mesh.arrays()
+    body = bm.Body(ctx, mesh)
+    print(f"\n[gpu] small cylinder N={body.N}", flush=True)
+    body.assemble_rankine()
+    yield mesh, body, oracle.Body(nodes, elems)
+    body.close()
+
+
+def test_havelock_device_vs_oracle(ctx):
+    rng = np.random.default_rng(7)
+    n = 4000
+    h = rng.uniform(0, 60, n) * rng.uniform(0, 1, n) ** 2
+    th = rng.uniform(0, 2 * np.pi, n)
+    X, Y = h * np.cos(th), h * np.sin(th)
+    V = -rng.uniform(0, 40, n) * rng.uniform(0, 1, n) ** 2 - 1e-3
+    gpu = ctx.eval_havelock(X, Y, V)
+    ref = oracle.havelock(X, Y, V)
+    g = gpu[:, 0::2] + 1j * gpu[:, 1::2]
+    for k in range(4):
+        scale = np.maximum(np.abs(ref[:, k]), 1.0)
+        err = (np.abs(g[:, k] - ref[:, k]) / scale).max()
+        assert err < 1e-11, (k, err)
+
+
+def test_rankine_matrices(cyl):
+    mesh, body, ob = cyl
+    R = body.get_rankine()
+    O = ob.rankine()
+    for k in ("Ssrc", "Srefl", "Dsrc", "Drefl"):
+        assert relerr(R[k], O[k]) < 1e-11, k
+
+
+@pytest.mark.parametrize("omega", [0.3, 1.1, 2.0])
+def test_wave_system(cyl, omega):
+    mesh, body, ob = cyl
+    K = omega * omega / 9.81
+    S, D = body.get_system(K)
+    O = ob.rankine()
+    Sw, Dw = ob.wave(K)
+    So = O["Ssrc"] + O["Srefl"] + Sw
+    Do = O["Dsrc"] + O["Drefl"] + Dw
+    assert relerr(S, So) < 1e-11
+    assert relerr(D, Do) < 1e-11
+    # wave part alone (the Rankine part dominates the sum)
+    assert relerr(S - (O["Ssrc"] + O["Srefl"]), Sw) < 1e-10
+    assert relerr(D - (O["Dsrc"] + O["Drefl"]), Dw) < 1e-10
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("n", [5, 64, 130, 300])
+def test_dense_solve_diag_dominant(ctx, n, precision):
+    rng = np.random.default_rng(n)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.3
+    A += np.diag(2.0 + rng.uniform(0, 1, n))
+    B = rng.standard_normal((n, 11)) + 1j * rng.standard_normal((n, 11))
+    X, info = ctx.dense_solve(A, B, precision)
+    Xr = np.linalg.solve(A, B)
+    assert relerr(X, Xr) < (1e-12 if precision == 64 else 1e-11), info
+    assert info["pivot_fallback"] == 0
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("n", [64, 200])
+def test_dense_solve_needs_pivoting(ctx, n, precision):
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    X, info = ctx.dense_solve(A, B, precision)
+    Xr = np.linalg.solve(A, B)
+    assert info["pivot_fallback"] > 0
+    assert relerr(X, Xr) < (1e-10 if precision == 64 else 1e-9), info
+
+
+@pytest.mark.parametrize("mode,tol", [(bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)])
+@pytest.mark.parametrize("omega", [0.4, 1.3])
+def test_solve_frequency_cylinder(cyl, mode, tol, omega):
+    mesh, body, ob = cyl
+    betas = [0.0, np.pi / 4]
+    g = body.solve(omega, betas=betas, mode=mode, want_phi=True)
+    r = ob.solve(omega, betas=betas, want_phi=True)
+    for key in ("A", "B", "F", "FK", "SC"):
+        assert relerr(g[key], r[key]) < tol, (key, relerr(g[key], r[key]), g["info"])
+    assert relerr(g["phiJ"], r["phiJ"]) < tol * 10
+    if mode == bm.MODE_MIXED:
+        assert g["info"]["fell_back_fp64"] == 0
+        assert g["info"]["residual"] < 1e-13
+
+
+def test_solve_barge_both_modes(ctx):
+    mesh = small_barge()
+    nodes, elems = mesh.arrays()
+    body = bm.Body(ctx, mesh)
+    body.assemble_rankine()
+    ob = oracle.Body(nodes, elems)
+    r = ob.solve(0.8, betas=[0.0, 0.5])
+    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+        g = body.solve(0.8, betas=[0.0, 0.5], mode=mode)
+        for key in ("A", "B", "F"):
+            assert relerr(g[key], r[key]) < tol, (mode, key)
+    body.close()
+
+
+def test_compose_cases(cyl):
+    """Omega == 0 (Rankine sum) and the infinite-frequency composition."""
+    mesh, body, ob = cyl
+    for compose in (bm.COMPOSE_ZERO, bm.COMPOSE_INF):
+        g = body.solve(0.0 if compose == bm.COMPOSE_ZERO else -1.0, kappa=0.0, compose=compose, rad_only=True)
+        r = ob.solve(0.0 if compose == bm.COMPOSE_ZERO else -1.0, kappa=0.0, compose=compose, rad_only=True)
+        assert relerr(g["A"], r["A"]) < 1e-8
+        assert relerr(g["B"] + 1.0, r["B"] + 1.0) < 1e-8
