@@ -20,9 +20,12 @@ void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, c
                      double omega, double kappa, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
                      double* DAr, double* DAi);
 int matvec_kslabs(int Np);
+// Y = M X (mode 0) or Y = B - M X (mode 1); with Pabs (nks x Np scratch) and
+// norm, also norm[0] = max row sum of |re| + |im| of M.
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
-                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi);
+                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
+                   double* Pabs = nullptr, double* norm = nullptr);
 // Optional timing of the trailing-update GEMM launches of one factorisation:
 // events bracket every GEMM launch; useful_flops counts 8 M N K over the
 // unpadded part (order N) of each launch.
@@ -34,18 +37,27 @@ struct LuProfile {
     double useful_flops = 0.0;
 };
 
+// Work buffers of one factorisation in precision T: saved diagonal block
+// (NB x NB), saved A21 (Np x NB, ld Np) and the per-panel inverses of the
+// diagonal-block triangles (Np/NB blocks of NB x NB, column-major).
 template <typename T>
-void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv,
-               LuProfile* prof = nullptr);
+struct LuWork {
+    T *sd_r = nullptr, *sd_i = nullptr, *s21_r = nullptr, *s21_i = nullptr;
+    T *Linv_r = nullptr, *Linv_i = nullptr, *Uinv_r = nullptr, *Uinv_i = nullptr;
+};
+
+// s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
+template <typename T>
+void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
+               int* ipiv, LuProfile* prof = nullptr);
 template <typename T, typename TX>
-void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const int* ipiv, const int* flag_any,
-                     TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch);
+void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch);
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb);
 void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb);
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb);
 void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
                    const double* xi, double* out);
-void launch_rowabs(hipStream_t s, int N, int Np, const double* Ar, const double* Ai, double* rowsum);
 void launch_reduce_coeffs(hipStream_t s, int N, int Np, int nb, const double* gen, const double* Xr, const double* Xi,
                           const double* PhiIr, const double* PhiIi, const double* DAr, const double* DAi,
                           double* out);

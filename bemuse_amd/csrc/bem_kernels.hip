@@ -163,47 +163,74 @@ __global__ void k_incident(int N, int Np, int nb, const double* __restrict__ bod
 }
 
 // ---------------------------------------------------------------------------
-// Tall-skinny products: Y = op(M X) for an Np x Np complex matrix and nr <= 16
-// columns, split over K slabs; partials are reduced in a fixed order.
+// Tall-skinny products Y = op(M X) for an Np x Np complex matrix M and
+// nr <= 16 columns X, split over K slabs (partials reduced in a fixed order).
+// A work-group covers 256 rows (one per lane) of one slab; chunks of 64 rows
+// of X are staged in LDS and each lane keeps 8 columns of M in flight.
+// Optionally accumulates the row sums of |re| + |im| of M.
 // ---------------------------------------------------------------------------
-constexpr int MV_ROWS = 64, MV_MAXR = 16;
+constexpr int MV_ROWS = 256, MV_MAXR = 16, MV_KC = 64;
 
-template <typename TM>
+template <typename TM, bool ROWABS>
 __global__ __launch_bounds__(256) void k_matvec_partial(int Np, int nr, int kslab, const TM* __restrict__ Mr,
                                                         const TM* __restrict__ Mi, const double* __restrict__ Xr,
-                                                        const double* __restrict__ Xi, double* Pr, double* Pi) {
-    int r0 = blockIdx.x * MV_ROWS, ks = blockIdx.y;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int row = r0 + lane;
+                                                        const double* __restrict__ Xi, double* Pr, double* Pi,
+                                                        double* Pabs) {
+    __shared__ double xs[MV_KC][MV_MAXR][2];
+    int row = blockIdx.x * MV_ROWS + threadIdx.x, ks = blockIdx.y;
+    bool live = row < Np;
+    if (!live) row = Np - 1;           // keeps the loads in bounds; results discarded
     double ar[MV_MAXR], ai[MV_MAXR];
 #pragma unroll
     for (int j = 0; j < MV_MAXR; j++) { ar[j] = 0; ai[j] = 0; }
+    double sabs = 0.0;
     int kb = ks * kslab, ke = min(Np, kb + kslab);
-    for (int k = kb + wave; k < ke; k += 4) {
-        double mr = (double)Mr[(size_t)k * Np + row], mi = (double)Mi[(size_t)k * Np + row];
+    const TM* pr = Mr + row;
+    const TM* pi = Mi + row;
+    for (int kc = kb; kc < ke; kc += MV_KC) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < MV_KC * MV_MAXR; t += 256) {
+            int k = t % MV_KC, j = t / MV_KC;
+            bool ok = j < nr && kc + k < ke;
+            xs[k][j][0] = ok ? Xr[(size_t)j * Np + kc + k] : 0.0;
+            xs[k][j][1] = ok ? Xi[(size_t)j * Np + kc + k] : 0.0;
+        }
+        __syncthreads();
+        int kn = min(MV_KC, ke - kc);
+        for (int k8 = 0; k8 < kn; k8 += 8) {
+            double mr[8], mi[8];
 #pragma unroll
-        for (int j = 0; j < MV_MAXR; j++) {
-            if (j < nr) {
-                double xr = Xr[(size_t)j * Np + k], xi = Xi[(size_t)j * Np + k];
-                ar[j] += mr * xr - mi * xi;
-                ai[j] += mr * xi + mi * xr;
+            for (int u = 0; u < 8; u++) {
+                size_t o = (size_t)(kc + k8 + u) * Np;
+                mr[u] = (double)pr[o]; mi[u] = (double)pi[o];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (ROWABS) sabs += fabs(mr[u]) + fabs(mi[u]);
+#pragma unroll
+                for (int j = 0; j < MV_MAXR; j++) {
+                    if (j < nr) {
+                        double xr = xs[k8 + u][j][0], xi = xs[k8 + u][j][1];
+                        ar[j] += mr[u] * xr - mi[u] * xi;
+                        ai[j] += mr[u] * xi + mi[u] * xr;
+                    }
+                }
             }
         }
     }
-    __shared__ double red[4][MV_MAXR][2][64];
+    if (!live) return;
 #pragma unroll
-    for (int j = 0; j < MV_MAXR; j++) { red[wave][j][0][lane] = ar[j]; red[wave][j][1][lane] = ai[j]; }
-    __syncthreads();
-    for (int t = threadIdx.x; t < nr * 64; t += 256) {
-        int j = t / 64, l = t % 64;
-        double sr = red[0][j][0][l] + red[1][j][0][l] + red[2][j][0][l] + red[3][j][0][l];
-        double si = red[0][j][1][l] + red[1][j][1][l] + red[2][j][1][l] + red[3][j][1][l];
-        size_t o = ((size_t)ks * MV_MAXR + j) * Np + r0 + l;
-        Pr[o] = sr; Pi[o] = si;
+    for (int j = 0; j < MV_MAXR; j++) {
+        if (j < nr) {
+            size_t o = ((size_t)ks * MV_MAXR + j) * Np + row;
+            Pr[o] = ar[j]; Pi[o] = ai[j];
+        }
     }
+    if (ROWABS) Pabs[(size_t)ks * Np + row] = sabs;
 }
 
-// Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1).
+// Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1); with Pabs, also the max
+// row sum of |re| + |im| into norm[0] (one work-group) -- see k_rownorm.
 __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restrict__ Pr, const double* __restrict__ Pi,
                                 const double* Br, const double* Bi, int mode, double* Yr, double* Yi) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,6 +246,24 @@ __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restric
     Yr[y] = sr; Yi[y] = si;
 }
 
+// norm[0] = max over rows of sum_ks Pabs (the infinity norm with |re|+|im|).
+__global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm) {
+    __shared__ double s[256];
+    double m = 0.0;
+    for (int r = threadIdx.x; r < Np; r += 256) {
+        double v = 0.0;
+        for (int ks = 0; ks < nks; ks++) v += Pabs[(size_t)ks * Np + r];
+        m = fmax(m, v);
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) norm[0] = s[0];
+}
+
 // ---------------------------------------------------------------------------
 // Blocked LU, right-looking, panel width LU_NB, planar complex.
 //
@@ -228,126 +273,341 @@ __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restric
 // that violates it is restored from its saved copy and refactored with row
 // interchanges (k_lu_panel_gepp), and the interchanges are applied to the
 // other columns (k_laswp). Results therefore follow partial pivoting.
+//
+// Per panel p the factorisation also keeps the inverses of the unit-lower and
+// upper triangles of its diagonal block (Linv_p, Uinv_p, NB x NB column-major
+// planar): L21 = A21 Uinv_p and U12 = Linv_p A12 are then small MFMA products,
+// and the triangular solves multiply by them instead of substituting.
 // ---------------------------------------------------------------------------
 constexpr int LU_NB = 64;
+constexpr int NB2 = LU_NB * LU_NB;
 
 template <typename T>
 __device__ inline T cabs2(T r, T i) { return r * r + i * i; }
 
-// Factor the NB x NB diagonal block at (k0, k0) without interchanges, in LDS.
-// Sets flag[panel] bit 1 if a multiplier exceeds 1 in modulus, bit 2 on a zero
-// pivot. Saves the block to `save` (NB x NB planar, ld NB).
+// Broadcast lane `l` (uniform) of v to the wave.
+__device__ inline float bcast(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double bcast(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Inverses of the unit-lower and upper triangles of a factored NB x NB block
+// held in LDS as lu[col][row] (rdr/rdi: LDS scratch of NB reciprocal pivots).
+// Run by waves 0..3 of the work-group; wave w computes columns 16w .. 16w+15
+// one at a time (lane = row) and writes them column-major.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, int* flag) {
-    __shared__ T lr[LU_NB][LU_NB + 1], li[LU_NB][LU_NB + 1];   // [col][row]
-    __shared__ int bad;
-    int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t g = (size_t)(k0 + c) * Np + k0 + r;
-        T vr = Ar[g], vi = Ai[g];
-        lr[c][r] = vr; li[c][r] = vi;
-        save_r[(size_t)c * LU_NB + r] = vr; save_i[(size_t)c * LU_NB + r] = vi;
+__device__ void diag_inverses(const T (*lur)[LU_NB + 1], const T (*lui)[LU_NB + 1], T* rdr, T* rdi, T* Li_r,
+                              T* Li_i, T* Ui_r, T* Ui_i) {
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < LU_NB) {
+        T pr = lur[threadIdx.x][threadIdx.x], pi = lui[threadIdx.x][threadIdx.x];
+        T den = cabs2(pr, pi);
+        rdr[threadIdx.x] = pr / den; rdi[threadIdx.x] = -pi / den;
     }
     __syncthreads();
-    for (int j = 0; j < LU_NB; j++) {
-        T pr = lr[j][j], pi = li[j][j];
-        T den = cabs2(pr, pi);
-        // multipliers l_ij = a_ij / a_jj
-        if (tid > j && tid < LU_NB) {
-            if (den == (T)0) {
-                atomicOr(&bad, 3);
-            } else {
-                T ar = lr[j][tid], ai = li[j][tid];
-                T mr = (ar * pr + ai * pi) / den, mi = (ai * pr - ar * pi) / den;
-                lr[j][tid] = mr; li[j][tid] = mi;
-                if (cabs2(mr, mi) > (T)1) atomicOr(&bad, 1);
+    if (wave >= 4) return;
+    for (int cl = 0; cl < 16; cl++) {
+        int c = 16 * wave + cl;
+        // column c of L^-1: forward substitution on e_c (unit diagonal)
+        T xr = (lane == c) ? (T)1 : (T)0, xi = 0;
+        for (int k = c; k < LU_NB - 1; k++) {
+            T br = bcast(xr, k), bi = bcast(xi, k);
+            if (lane > k) {
+                T lr = lur[k][lane], li = lui[k][lane];
+                xr -= lr * br - li * bi; xi -= lr * bi + li * br;
             }
         }
-        __syncthreads();
-        // rank-1 update of the trailing block
-        int m = LU_NB - j - 1;
-        for (int t = tid; t < m * m; t += 256) {
-            int c = j + 1 + t / m, r = j + 1 + t % m;
-            T ur = lr[c][j], ui = li[c][j];
-            T mr = lr[j][r], mi = li[j][r];
-            lr[c][r] -= mr * ur - mi * ui;
-            li[c][r] -= mr * ui + mi * ur;
+        Li_r[c * LU_NB + lane] = xr; Li_i[c * LU_NB + lane] = xi;
+        // column c of U^-1: back substitution on e_c
+        xr = (lane == c) ? (T)1 : (T)0; xi = 0;
+        for (int k = c; k >= 0; k--) {
+            T b0r = bcast(xr, k), b0i = bcast(xi, k);
+            T rr = rdr[k], ri = rdi[k];
+            T br = b0r * rr - b0i * ri, bi = b0r * ri + b0i * rr;
+            if (lane == k) { xr = br; xi = bi; }
+            if (lane < k) {
+                T ur = lur[k][lane], ui = lui[k][lane];
+                xr -= ur * br - ui * bi; xi -= ur * bi + ui * br;
+            }
+        }
+        Ui_r[c * LU_NB + lane] = xr; Ui_i[c * LU_NB + lane] = xi;
+    }
+}
+
+// Factor the NB x NB diagonal block at (k0, k0) without interchanges. Each of
+// the 256 threads keeps a 4 x 4 tile (rows 4tr.., columns 4tc..) of A and of
+// L^-1 in registers; step j broadcasts row j and column j through LDS (double
+// buffered, one barrier per step), every thread forms the multipliers of its
+// rows and applies the rank-1 update to its A columns right of j and to its
+// L^-1 columns left of (or at) j, since L^-1 = E_63 ... E_0 with E_j the
+// elementary elimination of step j. U^-1 follows from a backward sweep of the
+// same shape (scale row k of the identity by 1/u_kk, subtract u_mk times it
+// from rows m < k). Saves the block, writes L\U, L^-1 and U^-1, and sets
+// flag[panel] bit 1 if a multiplier exceeds 1 in modulus (1|2 on a zero pivot).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
+                                                 T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
+    __shared__ T lur[LU_NB][LU_NB + 1], lui[LU_NB][LU_NB + 1];   // final L\U [col][row]
+    __shared__ T rowr[2][2][LU_NB], rowi[2][2][LU_NB];          // [buf][A or X][col]
+    __shared__ T colr[2][LU_NB], coli[2][LU_NB];                // [buf][row]
+    __shared__ T rdr[LU_NB], rdi[LU_NB];
+    __shared__ int bad;
+    int tid = threadIdx.x;
+    int tr = tid & 15, tc = tid >> 4;
+    int rb = 4 * tr, cb = 4 * tc;
+    if (tid == 0) bad = 0;
+    T ar[4][4], ai[4][4], xr[4][4], xi[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            size_t g = (size_t)(k0 + cb + cc) * Np + k0 + rb + rr;
+            ar[rr][cc] = Ar[g]; ai[rr][cc] = Ai[g];
+            save_r[(cb + cc) * LU_NB + rb + rr] = ar[rr][cc];
+            save_i[(cb + cc) * LU_NB + rb + rr] = ai[rr][cc];
+            xr[rr][cc] = (rb + rr == cb + cc) ? (T)1 : (T)0; xi[rr][cc] = 0;
+        }
+    int mybad = 0;
+    for (int j = 0; j < LU_NB; j++) {
+        int buf = j & 1, jt = j >> 2, jj = j & 3;
+        if (tr == jt) {            // row j: A part and L^-1 part
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                if (rr == jj)
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        rowr[buf][0][cb + cc] = ar[rr][cc]; rowi[buf][0][cb + cc] = ai[rr][cc];
+                        rowr[buf][1][cb + cc] = xr[rr][cc]; rowi[buf][1][cb + cc] = xi[rr][cc];
+                    }
+        }
+        if (tc == jt) {            // column j of A
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++)
+                if (cc == jj)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) { colr[buf][rb + rr] = ar[rr][cc]; coli[buf][rb + rr] = ai[rr][cc]; }
         }
         __syncthreads();
+        T pr = colr[buf][j], pi = coli[buf][j];
+        T den = cabs2(pr, pi);
+        T qr = pr / den, qi = -pi / den;
+        if (tid == 0) { rdr[j] = qr; rdi[j] = qi; if (den == (T)0) mybad |= 3; }
+        T lr[4], li[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            T cr = colr[buf][rb + rr], ci = coli[buf][rb + rr];
+            lr[rr] = cr * qr - ci * qi; li[rr] = cr * qi + ci * qr;
+        }
+        T ur[4], ui[4], vr[4], vi[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            ur[cc] = rowr[buf][0][cb + cc]; ui[cc] = rowi[buf][0][cb + cc];
+            vr[cc] = rowr[buf][1][cb + cc]; vi[cc] = rowi[buf][1][cb + cc];
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            int i = rb + rr;
+            if (i > j) {
+                if (tc == jt) {
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++)
+                        if (cc == jj) {
+                            ar[rr][cc] = lr[rr]; ai[rr][cc] = li[rr];
+                            if (cabs2(lr[rr], li[rr]) > (T)1) mybad |= 1;
+                        }
+                }
+#pragma unroll
+                for (int cc = 0; cc < 4; cc++) {
+                    int c = cb + cc;
+                    if (c > j) {
+                        ar[rr][cc] -= lr[rr] * ur[cc] - li[rr] * ui[cc];
+                        ai[rr][cc] -= lr[rr] * ui[cc] + li[rr] * ur[cc];
+                    } else {
+                        xr[rr][cc] -= lr[rr] * vr[cc] - li[rr] * vi[cc];
+                        xi[rr][cc] -= lr[rr] * vi[cc] + li[rr] * vr[cc];
+                    }
+                }
+            }
+        }
     }
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t g = (size_t)(k0 + c) * Np + k0 + r;
-        Ar[g] = lr[c][r]; Ai[g] = li[c][r];
+    if (mybad) atomicOr(&bad, mybad);
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            size_t g = (size_t)(k0 + cb + cc) * Np + k0 + rb + rr;
+            Ar[g] = ar[rr][cc]; Ai[g] = ai[rr][cc];
+            lur[cb + cc][rb + rr] = ar[rr][cc]; lui[cb + cc][rb + rr] = ai[rr][cc];
+            Li_r[(cb + cc) * LU_NB + rb + rr] = xr[rr][cc]; Li_i[(cb + cc) * LU_NB + rb + rr] = xi[rr][cc];
+            xr[rr][cc] = (rb + rr == cb + cc) ? (T)1 : (T)0; xi[rr][cc] = 0;
+        }
+    __syncthreads();
+    // U^-1: backward sweep on the identity
+    for (int k = LU_NB - 1; k >= 0; k--) {
+        int buf = k & 1, kt = k >> 2, kk = k & 3;
+        if (tr == kt) {
+            T qr = rdr[k], qi = rdi[k];
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++)
+                if (rr == kk)
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        T yr = xr[rr][cc] * qr - xi[rr][cc] * qi, yi = xr[rr][cc] * qi + xi[rr][cc] * qr;
+                        xr[rr][cc] = yr; xi[rr][cc] = yi;
+                        rowr[buf][0][cb + cc] = yr; rowi[buf][0][cb + cc] = yi;
+                    }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            int m = rb + rr;
+            if (m < k) {
+                T u_r = lur[k][m], u_i = lui[k][m];
+#pragma unroll
+                for (int cc = 0; cc < 4; cc++) {
+                    T yr = rowr[buf][0][cb + cc], yi = rowi[buf][0][cb + cc];
+                    xr[rr][cc] -= u_r * yr - u_i * yi;
+                    xi[rr][cc] -= u_r * yi + u_i * yr;
+                }
+            }
+        }
     }
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+            Ui_r[(cb + cc) * LU_NB + rb + rr] = xr[rr][cc]; Ui_i[(cb + cc) * LU_NB + rb + rr] = xi[rr][cc];
+        }
     if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
 }
 
-// L21 = A21 U11^-1 for rows k0+NB .. Np-1 (row-parallel: one wave per row
-// group of 64, lanes over columns). Saves A21 to `save` before overwriting
-// (rows relative to k0+NB, ld = Np). Flags multipliers with |l| > 1.
+// ---------------------------------------------------------------------------
+// MFMA 16x16x4 (lane l = i + 16 g): A supplies A[i][k(g)], B supplies
+// B[k(g)][j]. Accumulator register r of lane l holds D[row][j] with
+// row = 4g + r for the f32 instruction and row = g + 4r for the f64
+// instruction on gfx950 (measured). Within a chunk of 16 k, lane group g
+// covers k = 4g + t at step t, so each lane reads 4 consecutive k of a B
+// column as one vector.
+// ---------------------------------------------------------------------------
+template <typename T> struct MfmaT;
+template <> struct MfmaT<double> {
+    typedef d4 acc;
+    static __device__ inline acc mma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int r) { return g + 4 * r; }
+    static constexpr int WM = 2, WN = 2;   // wave tile 32 x 32
+};
+template <> struct MfmaT<float> {
+    typedef f4 acc;
+    static __device__ inline acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ inline int row(int g, int r) { return 4 * g + r; }
+    static constexpr int WM = 4, WN = 4;   // wave tile 64 x 64
+};
+
+// Complex 16x16 block product step over 16 k: acc += A(16 x 16k) B(16k x 16)
+// with A element (i, k) at a[k * lda + i] and B element (k, j) at b[j * ldb + k].
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, int* flag) {
-    __shared__ T ur[LU_NB][LU_NB + 1], ui[LU_NB][LU_NB + 1];   // U11 [row][col]
-    __shared__ T xr[LU_NB][LU_NB + 1], xi[LU_NB][LU_NB + 1];   // tile rows [row][col]
-    __shared__ int bad;
-    int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    int row0 = k0 + LU_NB + blockIdx.x * LU_NB;
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t g = (size_t)(k0 + c) * Np + k0 + r;
-        ur[r][c] = (r <= c) ? Ar[g] : (T)0;
-        ui[r][c] = (r <= c) ? Ai[g] : (T)0;
-        size_t a = (size_t)(k0 + c) * Np + row0 + r;
-        T vr = Ar[a], vi = Ai[a];
-        xr[r][c] = vr; xi[r][c] = vi;
-        save_r[(size_t)c * Np + row0 - k0 - LU_NB + r] = vr;
-        save_i[(size_t)c * Np + row0 - k0 - LU_NB + r] = vi;
+__device__ inline void cmma16(const T* ar_, const T* ai_, size_t lda, const T* br_, const T* bi_, size_t ldb,
+                              typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci, int i, int g) {
+    typedef MfmaT<T> MT;
+    T br[4], bi[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) { br[t] = br_[(size_t)i * ldb + 4 * g + t]; bi[t] = bi_[(size_t)i * ldb + 4 * g + t]; }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        T a_r = ar_[(size_t)(4 * g + t) * lda + i], a_i = ai_[(size_t)(4 * g + t) * lda + i];
+        cr = MT::mma(a_r, br[t], cr);
+        cr = MT::mma(-a_i, bi[t], cr);
+        ci = MT::mma(a_r, bi[t], ci);
+        ci = MT::mma(a_i, br[t], ci);
     }
-    __syncthreads();
-    // each wave solves 16 rows: x U = a, lanes over columns
-    int lane = tid & 63, wave = tid >> 6;
-    for (int rr = 0; rr < LU_NB / 4; rr++) {
-        int r = wave * (LU_NB / 4) + rr;
-        T ar = xr[r][lane], ai = xi[r][lane];
-        for (int j = 0; j < LU_NB; j++) {
-            // lane j finalises x_j = a_j / u_jj, then broadcast
-            T pr = ur[j][j], pi = ui[j][j];
-            T den = cabs2(pr, pi);
-            T vr = (ar * pr + ai * pi) / den, vi = (ai * pr - ar * pi) / den;
-            T br = __shfl(vr, j), bi = __shfl(vi, j);
-            if (lane == j) { ar = br; ai = bi; if (cabs2(br, bi) > (T)1) bad = 1; }
-            if (lane > j) {
-                T u_r = ur[j][lane], u_i = ui[j][lane];
-                ar -= br * u_r - bi * u_i;
-                ai -= br * u_i + bi * u_r;
-            }
+}
+
+// L21 = A21 Uinv_p in place for rows k0+NB .. Np-1. Each wave owns 16 rows
+// (all NB columns), so reading and overwriting them cannot race. Saves A21
+// (rows relative to k0+NB, ld Np) for the pivoting fallback and flags
+// multipliers with |l| > 1.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
+                                                T* save_r, T* save_i, int* flag) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int rowl = blockIdx.x * 64 + wave * 16;             // relative to k0 + NB
+    int row0 = k0 + LU_NB + rowl;
+    for (int t = lane; t < 16 * LU_NB; t += 64) {
+        int r = t % 16, c = t / 16;
+        size_t g = (size_t)(k0 + c) * Np + row0 + r;
+        save_r[(size_t)c * Np + rowl + r] = Ar[g];
+        save_i[(size_t)c * Np + rowl + r] = Ai[g];
+    }
+    int i = lane & 15, g = lane >> 4;
+    acc_t cr[4], ci[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) { cr[b] = acc_t{0, 0, 0, 0}; ci[b] = acc_t{0, 0, 0, 0}; }
+    const T* A0r = Ar + (size_t)k0 * Np + row0;
+    const T* A0i = Ai + (size_t)k0 * Np + row0;
+    for (int kk = 0; kk < LU_NB; kk += 16)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            cmma16<T>(A0r + (size_t)kk * Np, A0i + (size_t)kk * Np, Np, Ui_r + (16 * b) * LU_NB + kk,
+                      Ui_i + (16 * b) * LU_NB + kk, LU_NB, cr[b], ci[b], i, g);
+    int mybad = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int rr = MT::row(g, r);
+            size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + rr;
+            T lr = cr[b][r], li = ci[b][r];
+            Ar[o] = lr; Ai[o] = li;
+            if (cabs2(lr, li) > (T)1) mybad = 1;
         }
-        xr[r][lane] = ar; xi[r][lane] = ai;
-    }
-    __syncthreads();
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t a = (size_t)(k0 + c) * Np + row0 + r;
-        Ar[a] = xr[r][c]; Ai[a] = xi[r][c];
-    }
-    if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, 1);
+    if (mybad) atomicOr(flag + k0 / LU_NB, 1);
+}
+
+// U12 = Linv_p A12 in place for columns k0+NB .. Np-1. Each wave owns 16
+// columns (all NB rows).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int col0 = k0 + LU_NB + blockIdx.x * 64 + wave * 16;
+    int i = lane & 15, g = lane >> 4;
+    acc_t cr[4], ci[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) { cr[a] = acc_t{0, 0, 0, 0}; ci[a] = acc_t{0, 0, 0, 0}; }
+    const T* B0r = Ar + (size_t)col0 * Np + k0;
+    const T* B0i = Ai + (size_t)col0 * Np + k0;
+    for (int kk = 0; kk < LU_NB; kk += 16)
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            cmma16<T>(Li_r + (size_t)kk * LU_NB + 16 * a, Li_i + (size_t)kk * LU_NB + 16 * a, LU_NB, B0r + kk,
+                      B0i + kk, Np, cr[a], ci[a], i, g);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            size_t o = (size_t)(col0 + i) * Np + k0 + 16 * a + MT::row(g, r);
+            Ar[o] = cr[a][r]; Ai[o] = ci[a][r];
+        }
 }
 
 // Fallback for a rejected panel: restore it and factor it with partial
-// pivoting in one work-group, streaming through global memory. ipiv receives
-// absolute row indices. Runs only if flag[panel] is set.
+// pivoting in one work-group, streaming through global memory; then rebuild
+// the triangle inverses of its diagonal block. ipiv receives absolute row
+// indices. Runs only if flag[panel] is set.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T* Ai, const T* sd_r, const T* sd_i,
-                                                       const T* s21_r, const T* s21_i, int* flag, int* ipiv) {
+                                                       const T* s21_r, const T* s21_i, T* Li_r, T* Li_i, T* Ui_r,
+                                                       T* Ui_i, int* flag, int* ipiv) {
     int f = flag[k0 / LU_NB];
     if (!(f & 1)) return;
     int tid = threadIdx.x;
-    // restore
-    for (int t = tid; t < LU_NB * LU_NB; t += 1024) {
+    for (int t = tid; t < NB2; t += 1024) {
         int c = t / LU_NB, r = t % LU_NB;
         size_t g = (size_t)(k0 + c) * Np + k0 + r;
         Ar[g] = sd_r[(size_t)c * LU_NB + r]; Ai[g] = sd_i[(size_t)c * LU_NB + r];
@@ -363,6 +623,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
     __shared__ int bi[1024];
     __shared__ int piv_s;
     __shared__ T prow_r[LU_NB], prow_i[LU_NB];
+    __shared__ T lur[LU_NB][LU_NB + 1], lui[LU_NB][LU_NB + 1];
+    __shared__ T rdr[LU_NB], rdi[LU_NB];
     for (int j = 0; j < LU_NB; j++) {
         int col = k0 + j;
         T best = (T)-1; int brow = Np;
@@ -383,7 +645,6 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
         if (tid == 0) { piv_s = bi[0]; ipiv[col] = bi[0]; }
         __syncthreads();
         int p = piv_s;
-        // swap rows col <-> p within the panel columns
         if (p != col && tid < LU_NB) {
             size_t a = (size_t)(k0 + tid) * Np;
             T tr = Ar[a + col], ti = Ai[a + col];
@@ -410,6 +671,13 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
         }
         __syncthreads();
     }
+    for (int t = tid; t < NB2; t += 1024) {
+        int c = t / LU_NB, r = t % LU_NB;
+        size_t g = (size_t)(k0 + c) * Np + k0 + r;
+        lur[c][r] = Ar[g]; lui[c][r] = Ai[g];
+    }
+    __syncthreads();
+    diag_inverses<T>(lur, lui, rdr, rdi, Li_r, Li_i, Ui_r, Ui_i);
 }
 
 // Apply the panel's interchanges to all columns outside the panel
@@ -430,64 +698,12 @@ __global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int
     }
 }
 
-// U12 = L11^-1 A12 for columns k0+NB .. Np-1: one wave per column, lanes over
-// the NB rows, forward substitution with unit-lower L11 held in LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai) {
-    __shared__ T lr[LU_NB][LU_NB + 1], li[LU_NB][LU_NB + 1];   // L11 [col][row]
-    int tid = threadIdx.x;
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t g = (size_t)(k0 + c) * Np + k0 + r;
-        lr[c][r] = Ar[g]; li[c][r] = Ai[g];
-    }
-    __syncthreads();
-    int lane = tid & 63, wave = tid >> 6;
-    int col = k0 + LU_NB + blockIdx.x * 16 + wave * 4;
-    for (int cc = 0; cc < 4; cc++) {
-        int c = col + cc;
-        if (c >= Np) break;
-        size_t g = (size_t)c * Np + k0 + lane;
-        T ar = Ar[g], ai = Ai[g];
-        for (int j = 0; j < LU_NB - 1; j++) {
-            T br = __shfl(ar, j), bi = __shfl(ai, j);
-            if (lane > j) {
-                T mr = lr[j][lane], mi = li[j][lane];
-                ar -= mr * br - mi * bi;
-                ai -= mr * bi + mi * br;
-            }
-        }
-        Ar[g] = ar; Ai[g] = ai;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Complex GEMM on MFMA: C -= A B (LU trailing update) for planar complex
 // operands in one column-major array with leading dimension ld.
 // A: M x K at (ra, ca), B: K x N at (rb, cb), C: M x N at (rc, cc); M, N
 // multiples of 64, K multiple of 16.
-//
-// MFMA 16x16x4 operand layout (lane l = i + 16 g): A supplies A[i][k(g)],
-// B supplies B[k(g)][j]. The accumulator register r of lane l holds D[row][j]
-// with row = 4g + r for the f32 instruction and row = g + 4r for the f64
-// instruction on gfx950 (measured). Within a chunk of 16 k, lane group g
-// covers k = 4g + t at step t, so each lane reads 4 consecutive k of a B
-// column as one vector.
 // ---------------------------------------------------------------------------
-template <typename T> struct MfmaT;
-template <> struct MfmaT<double> {
-    typedef d4 acc;
-    static __device__ inline acc mma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    static __device__ inline int row(int g, int r) { return g + 4 * r; }
-    static constexpr int WM = 2, WN = 2;   // wave tile 32 x 32
-};
-template <> struct MfmaT<float> {
-    typedef f4 acc;
-    static __device__ inline acc mma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    static __device__ inline int row(int g, int r) { return 4 * g + r; }
-    static constexpr int WM = 4, WN = 4;   // wave tile 64 x 64
-};
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, T* __restrict__ Pr, T* __restrict__ Pi, int ld,
                                                    int ra, int ca, int rb, int cb, int rc, int cc) {
@@ -558,20 +774,29 @@ __global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, T* __res
 
 // ---------------------------------------------------------------------------
 // Triangular solves with NB-row blocks, one persistent launch per solve.
-// Work-groups take blocks in dependency order from a ticket counter; block b
-// waits for the published results of the blocks it depends on (epoch flags),
-// accumulates their contributions, solves its diagonal block and publishes.
-// lower = 1: L (unit diagonal) forward; lower = 0: U backward.
+// Work-groups take blocks in dependency order from a ticket counter, so a
+// block only ever waits for blocks whose work-groups are already running.
+// Block b accumulates sum_c M_bc X_c over the blocks it depends on (each wave
+// a quarter of every block's columns, waiting on that block's epoch flag),
+// then X_b = Inv_b (B_b - sum), with Inv_b the stored inverse of the
+// diagonal triangle, and publishes X_b.
+// lower = 1: L (unit diagonal) forward with Linv; lower = 0: U backward with Uinv.
 // X is Np x nr planar (ld Np), type TX; the factor has type T.
 // ---------------------------------------------------------------------------
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower, const T* __restrict__ Ar,
-                                                      const T* __restrict__ Ai, TX* Xr, TX* Xi, unsigned* counter,
+                                                      const T* __restrict__ Ai, const T* __restrict__ Inv_r,
+                                                      const T* __restrict__ Inv_i, TX* Xr, TX* Xi, unsigned* counter,
                                                       unsigned* flags, unsigned epoch) {
-    __shared__ TX accr[MV_MAXR][LU_NB], acci[MV_MAXR][LU_NB];
-    __shared__ T mr[LU_NB][LU_NB + 1], mi[LU_NB][LU_NB + 1];   // [col][row]
+    // cross-wave partial sums (after the dependency loop) share storage with
+    // the per-wave staging of X_c rows (during it)
+    __shared__ TX shared_buf[4 * MV_MAXR * 2 * LU_NB];
+    TX (*red)[MV_MAXR][2][LU_NB] = reinterpret_cast<TX (*)[MV_MAXR][2][LU_NB]>(shared_buf);
+    TX (*xw)[16][MV_MAXR][2] = reinterpret_cast<TX (*)[16][MV_MAXR][2]>(shared_buf);
+    __shared__ TX yr[MV_MAXR][LU_NB], yi[MV_MAXR][LU_NB];
+    __shared__ T ivr[LU_NB][LU_NB + 1], ivi[LU_NB][LU_NB + 1];   // Inv_b [col][row]
     __shared__ unsigned ticket;
-    int tid = threadIdx.x;
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int nblk = Np / LU_NB;
     if (tid == 0) {
         ticket = atomicAdd(counter, 1u);
@@ -580,89 +805,96 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
     __syncthreads();
     int b = lower ? (int)ticket : nblk - 1 - (int)ticket;
     int r0 = b * LU_NB;
-    for (int t = tid; t < nr * LU_NB; t += 256) {
-        int j = t / LU_NB, r = t % LU_NB;
-        accr[j][r] = Xr[(size_t)j * Np + r0 + r];
-        acci[j][r] = Xi[(size_t)j * Np + r0 + r];
+    // the inverse of the diagonal triangle is ready before the launch
+    for (int t = tid; t < NB2; t += 256) {
+        int c = t / LU_NB, r = t % LU_NB;
+        ivr[c][r] = Inv_r[(size_t)b * NB2 + t]; ivi[c][r] = Inv_i[(size_t)b * NB2 + t];
     }
+    TX ar[MV_MAXR], ai[MV_MAXR];
+#pragma unroll
+    for (int j = 0; j < MV_MAXR; j++) { ar[j] = 0; ai[j] = 0; }
     int ndep = lower ? b : nblk - 1 - b;
     for (int d = 0; d < ndep; d++) {
         int c = lower ? d : nblk - 1 - d;
-        int c0 = c * LU_NB;
-        // off-diagonal block M(r0.., c0..)
-        for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-            int cc = t / LU_NB, r = t % LU_NB;
-            size_t g = (size_t)(c0 + cc) * Np + r0 + r;
-            mr[cc][r] = Ar[g]; mi[cc][r] = Ai[g];
+        int c0 = c * LU_NB + wave * 16;
+        T mr[16], mi[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            size_t gidx = (size_t)(c0 + u) * Np + r0 + lane;
+            mr[u] = Ar[gidx]; mi[u] = Ai[gidx];
         }
-        if (tid == 0) {
-            while (__hip_atomic_load(flags + c, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch)
+        // Block c is published with device-scope stores of X_c followed by a
+        // release of flag[c]. Poll the flag and read X_c with device-scope
+        // (L2-bypassing) relaxed loads: an acquire here would invalidate the
+        // whole L2 of the XCD for every dependency.
+        if (lane == 0) {
+            while (__hip_atomic_load(flags + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
                 __builtin_amdgcn_s_sleep(1);
         }
-        __syncthreads();
-        // acc -= M X_c (X_c read from global after the acquire)
-        for (int t = tid; t < nr * LU_NB; t += 256) {
-            int j = t / LU_NB, r = t % LU_NB;
-            TX sr = 0, si = 0;
-            const TX* xr = Xr + (size_t)j * Np + c0;
-            const TX* xi = Xi + (size_t)j * Np + c0;
-            for (int k = 0; k < LU_NB; k++) {
-                TX a_r = (TX)mr[k][r], a_i = (TX)mi[k][r];
-                TX vr = __builtin_nontemporal_load(xr + k), vi = __builtin_nontemporal_load(xi + k);
-                sr += a_r * vr - a_i * vi;
-                si += a_r * vi + a_i * vr;
-            }
-            accr[j][r] -= sr; acci[j][r] -= si;
+        __builtin_amdgcn_wave_barrier();
+        for (int t = lane; t < 16 * nr; t += 64) {
+            int u = t % 16, j = t / 16;
+            xw[wave][u][j][0] = __hip_atomic_load(Xr + (size_t)j * Np + c0 + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            xw[wave][u][j][1] = __hip_atomic_load(Xi + (size_t)j * Np + c0 + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __syncthreads();
-    }
-    // diagonal block
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int cc = t / LU_NB, r = t % LU_NB;
-        size_t g = (size_t)(r0 + cc) * Np + r0 + r;
-        mr[cc][r] = Ar[g]; mi[cc][r] = Ai[g];
-    }
-    __syncthreads();
-    // substitution: one wave per right-hand side column, lanes over rows
-    int lane = tid & 63, wave = tid >> 6;
-    for (int j = wave; j < nr; j += 4) {
-        TX vr = accr[j][lane], vi = acci[j][lane];
-        if (lower) {
-            for (int k = 0; k < LU_NB - 1; k++) {
-                TX br = __shfl(vr, k), bi = __shfl(vi, k);
-                if (lane > k) {
-                    TX a_r = (TX)mr[k][lane], a_i = (TX)mi[k][lane];
-                    vr -= a_r * br - a_i * bi;
-                    vi -= a_r * bi + a_i * br;
-                }
-            }
-        } else {
-            for (int k = LU_NB - 1; k >= 0; k--) {
-                TX pr = (TX)mr[k][k], pi = (TX)mi[k][k];
-                TX den = pr * pr + pi * pi;
-                TX qr = (vr * pr + vi * pi) / den, qi = (vi * pr - vr * pi) / den;
-                TX br = __shfl(qr, k), bi = __shfl(qi, k);
-                if (lane == k) { vr = br; vi = bi; }
-                if (lane < k) {
-                    TX a_r = (TX)mr[k][lane], a_i = (TX)mi[k][lane];
-                    vr -= a_r * br - a_i * bi;
-                    vi -= a_r * bi + a_i * br;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            TX m_r = (TX)mr[u], m_i = (TX)mi[u];
+#pragma unroll
+            for (int j = 0; j < MV_MAXR; j++) {
+                if (j < nr) {
+                    TX vr = xw[wave][u][j][0], vi = xw[wave][u][j][1];
+                    ar[j] += m_r * vr - m_i * vi;
+                    ai[j] += m_r * vi + m_i * vr;
                 }
             }
         }
-        Xr[(size_t)j * Np + r0 + lane] = vr;
-        Xi[(size_t)j * Np + r0 + lane] = vi;
+        __builtin_amdgcn_wave_barrier();
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) {
-        __threadfence();
-        __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int j = 0; j < MV_MAXR; j++) { red[wave][j][0][lane] = ar[j]; red[wave][j][1][lane] = ai[j]; }
+    __syncthreads();
+    for (int t = tid; t < nr * LU_NB; t += 256) {
+        int j = t / LU_NB, r = t % LU_NB;
+        size_t x = (size_t)j * Np + r0 + r;
+        yr[j][r] = Xr[x] - (red[0][j][0][r] + red[1][j][0][r] + red[2][j][0][r] + red[3][j][0][r]);
+        yi[j][r] = Xi[x] - (red[0][j][1][r] + red[1][j][1][r] + red[2][j][1][r] + red[3][j][1][r]);
     }
+    __syncthreads();
+    // X_b = Inv_b y: lane = row, wave w takes right-hand sides j = w, w+4, ...
+    TX sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
+#pragma unroll 4
+    for (int k = 0; k < LU_NB; k++) {
+        TX v_r = (TX)ivr[k][lane], v_i = (TX)ivi[k][lane];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            int j = wave + 4 * q;
+            if (j < nr) {
+                TX y_r = yr[j][k], y_i = yi[j][k];
+                sr[q] += v_r * y_r - v_i * y_i;
+                si[q] += v_r * y_i + v_i * y_r;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        int j = wave + 4 * q;
+        if (j < nr) {
+            __hip_atomic_store(Xr + (size_t)j * Np + r0 + lane, sr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(Xi + (size_t)j * Np + r0 + lane, si[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    // every wave waits for its stores; one agent-scope release publishes the block
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Row permutation of the right-hand sides: X(perm order) for the
-// interchanges recorded in ipiv (applied sequentially, one thread per column).
+// Row permutation of the right-hand sides for the interchanges recorded in
+// ipiv (applied sequentially, one thread per column); only if any panel fell
+// back to pivoting.
 template <typename TX>
 __global__ void k_apply_ipiv(int Np, int nr, const int* ipiv, const int* any_flag, TX* Xr, TX* Xi) {
     if (!any_flag[0]) return;
@@ -731,15 +963,6 @@ __global__ void k_colmax(int N, int Np, const double* rr, const double* ri, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) { out[2 * j] = s1[0]; out[2 * j + 1] = s2[0]; }
-}
-
-// Max row sum of |A| (infinity norm), rows < N: per-row sums then a max.
-__global__ void k_rowabs(int N, int Np, const double* Ar, const double* Ai, double* rowsum) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
-    double s = 0;
-    for (int c = 0; c < N; c++) { size_t o = (size_t)c * Np + r; s += sqrt(Ar[o] * Ar[o] + Ai[o] * Ai[o]); }
-    rowsum[r] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -814,82 +1037,119 @@ void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, c
                        Xr, Xi, ldx, PhiIr, PhiIi, DAr, DAi);
 }
 
-int matvec_kslabs(int Np) { return Np >= 4096 ? 16 : (Np >= 1024 ? 8 : 4); }
+int matvec_kslabs(int Np) { return Np >= 4096 ? 32 : (Np >= 1024 ? 16 : 4); }
 
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
-                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi) {
+                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
+                   double* Pabs, double* norm) {
     int nks = matvec_kslabs(Np);
     int kslab = (Np / nks + 63) / 64 * 64;
     nks = (Np + kslab - 1) / kslab;
-    dim3 grid(Np / MV_ROWS, nks);
-    hipLaunchKernelGGL(k_matvec_partial<TM>, grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi);
+    dim3 grid(cdiv(Np, MV_ROWS), nks);
+    if (Pabs)
+        hipLaunchKernelGGL((k_matvec_partial<TM, true>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi,
+                           Pabs);
+    else
+        hipLaunchKernelGGL((k_matvec_partial<TM, false>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr,
+                           Pi, Pabs);
     hipLaunchKernelGGL(k_matvec_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, Np, nr, nks, Pr, Pi, Br, Bi,
                        mode, Yr, Yi);
+    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(256), 0, s, Np, nks, Pabs, norm);
 }
 template void launch_matvec<double>(hipStream_t, int, int, const double*, const double*, const double*, const double*,
-                                    double*, double*, const double*, const double*, int, double*, double*);
+                                    double*, double*, const double*, const double*, int, double*, double*, double*,
+                                    double*);
 
 template <typename T>
-void launch_lu(hipStream_t s, int Np, T* Ar, T* Ai, T* sd_r, T* sd_i, T* s21_r, T* s21_i, int* flag, int* ipiv,
-               LuProfile* prof) {
+static void gemm_sub(hipStream_t s, int Np, int M, int N, T* Ar, T* Ai, int k0, int cc, LuProfile* prof, int Nuse) {
+    constexpr int TM = 32 * MfmaT<T>::WM, TN = 32 * MfmaT<T>::WN;
+    int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
+    int per = (tiles + 7) / 8;
+    bool timed = prof && prof->n < LuProfile::kMax;
+    if (timed) {
+        if (!prof->ev[2 * prof->n]) {
+            (void)hipEventCreate(&prof->ev[2 * prof->n]);
+            (void)hipEventCreate(&prof->ev[2 * prof->n + 1]);
+        }
+        (void)hipEventRecord(prof->ev[2 * prof->n], s);
+    }
+    // C(k0+NB.., cc..) -= A(k0+NB.., k0..k0+NB) B(k0..k0+NB, cc..)
+    hipLaunchKernelGGL(k_cgemm_sub<T>, dim3(per * 8), dim3(256), 0, s, M, N, LU_NB, Ar, Ai, Np, k0 + LU_NB, k0, k0, cc,
+                       k0 + LU_NB, cc);
+    if (timed) {
+        (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
+        double mu = (double)std::max(0, prof->N - (k0 + LU_NB));
+        double nu = (double)std::max(0, std::min(Nuse, prof->N - cc));
+        prof->useful_flops += 8.0 * mu * nu * LU_NB;
+        prof->n++;
+    }
+}
+
+// Right-looking blocked LU with one-panel lookahead: the panel stream s2
+// factors the diagonal block and L21 of panel p+1 while the main stream s
+// applies panel p's update to the columns right of panel p+1. Events ev
+// (2 per panel) order the two streams; fallback pivoting (GEPP + row swaps)
+// of a panel always runs on the main stream after the trailing update.
+template <typename T>
+void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
+               int* ipiv, LuProfile* prof) {
     int npanel = Np / LU_NB;
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv);
-    for (int p = 0; p < npanel; p++) {
-        int k0 = p * LU_NB;
-        int rest = Np - k0 - LU_NB;
-        hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, s, Np, k0, Ar, Ai, sd_r, sd_i, flag);
+    auto panel = [&](hipStream_t st, int p) {
+        int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
+        T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
+        T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
+        hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, w.sd_r, w.sd_i, Lir, Lii, Uir,
+                           Uii, flag);
         if (rest > 0)
-            hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / LU_NB), dim3(256), 0, s, Np, k0, Ar, Ai, s21_r, s21_i, flag);
-        hipLaunchKernelGGL(k_lu_panel_gepp<T>, dim3(1), dim3(1024), 0, s, Np, k0, Ar, Ai, sd_r, sd_i, s21_r, s21_i,
-                           flag, ipiv);
+            hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / 64), dim3(256), 0, st, Np, k0, Ar, Ai, Uir, Uii, w.s21_r,
+                               w.s21_i, flag);
+    };
+    // panel 0 on the main stream
+    panel(s, 0);
+    for (int p = 0; p < npanel; p++) {
+        int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
+        if (p > 0) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);     // panel p factored on s2
+        T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
+        T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
+        hipLaunchKernelGGL(k_lu_panel_gepp<T>, dim3(1), dim3(1024), 0, s, Np, k0, Ar, Ai, w.sd_r, w.sd_i, w.s21_r,
+                           w.s21_i, Lir, Lii, Uir, Uii, flag, ipiv);
         hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(Np, 256)), dim3(256), 0, s, Np, k0, Ar, Ai, flag, ipiv);
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_lu_u12<T>, dim3(cdiv(rest, 16)), dim3(256), 0, s, Np, k0, Ar, Ai);
-            constexpr int TM = 32 * MfmaT<T>::WM, TN = 32 * MfmaT<T>::WN;
-            int tiles = (int)(cdiv(rest, TM) * cdiv(rest, TN));
-            int per = (tiles + 7) / 8;
-            bool timed = prof && prof->n < LuProfile::kMax;
-            if (timed) {
-                if (!prof->ev[2 * prof->n]) {
-                    (void)hipEventCreate(&prof->ev[2 * prof->n]);
-                    (void)hipEventCreate(&prof->ev[2 * prof->n + 1]);
-                }
-                (void)hipEventRecord(prof->ev[2 * prof->n], s);
-            }
-            hipLaunchKernelGGL(k_cgemm_sub<T>, dim3(per * 8), dim3(256), 0, s, rest, rest, LU_NB, Ar, Ai, Np,
-                               k0 + LU_NB, k0, k0, k0 + LU_NB, k0 + LU_NB, k0 + LU_NB);
-            if (timed) {
-                (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
-                double u = (double)std::max(0, prof->N - (k0 + LU_NB));
-                prof->useful_flops += 8.0 * u * u * LU_NB;
-                prof->n++;
-            }
-        }
+        if (rest <= 0) break;
+        hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 64), dim3(256), 0, s, Np, k0, Ar, Ai, Lir, Lii);
+        // columns of panel p+1 first, then hand panel p+1 to the panel stream
+        gemm_sub<T>(s, Np, rest, LU_NB, Ar, Ai, k0, k0 + LU_NB, prof, LU_NB);
+        (void)hipEventRecord(ev[2 * (p + 1)], s);
+        (void)hipStreamWaitEvent(s2, ev[2 * (p + 1)], 0);
+        panel(s2, p + 1);
+        (void)hipEventRecord(ev[2 * (p + 1) + 1], s2);
+        if (rest > LU_NB) gemm_sub<T>(s, Np, rest, rest - LU_NB, Ar, Ai, k0, k0 + 2 * LU_NB, prof, rest - LU_NB);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
 }
-template void launch_lu<double>(hipStream_t, int, double*, double*, double*, double*, double*, double*, int*, int*,
-                                 LuProfile*);
-template void launch_lu<float>(hipStream_t, int, float*, float*, float*, float*, float*, float*, int*, int*,
-                                LuProfile*);
+template void launch_lu<double>(hipStream_t, hipStream_t, hipEvent_t*, int, double*, double*, const LuWork<double>&,
+                                 int*, int*, LuProfile*);
+template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float*, float*, const LuWork<float>&, int*,
+                                int*, LuProfile*);
 
 template <typename T, typename TX>
-void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const int* ipiv, const int* flag_any,
-                     TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch) {
+void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch) {
     hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi);
     int nblk = Np / LU_NB;
     ++epoch;
-    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, Xr, Xi, counter, flags,
-                       epoch);
+    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i, Xr,
+                       Xi, counter, flags, epoch);
     ++epoch;
-    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, Xr, Xi, counter, flags,
-                       epoch);
+    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i, Xr,
+                       Xi, counter, flags, epoch);
 }
-template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*, const int*,
-                                              const int*, double*, double*, unsigned*, unsigned*, unsigned&);
-template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const int*, const int*,
-                                            float*, float*, unsigned*, unsigned*, unsigned&);
+template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*,
+                                              const LuWork<double>&, const int*, const int*, double*, double*,
+                                              unsigned*, unsigned*, unsigned&);
+template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const LuWork<float>&,
+                                            const int*, const int*, float*, float*, unsigned*, unsigned*, unsigned&);
 
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb) {
     hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb);
@@ -903,9 +1163,6 @@ void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, dou
 void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
                    const double* xi, double* out) {
     hipLaunchKernelGGL(k_colmax, dim3(nr), dim3(256), 0, s, N, Np, rr, ri, xr, xi, out);
-}
-void launch_rowabs(hipStream_t s, int N, int Np, const double* Ar, const double* Ai, double* rowsum) {
-    hipLaunchKernelGGL(k_rowabs, dim3(cdiv(N, 256)), dim3(256), 0, s, N, Np, Ar, Ai, rowsum);
 }
 void launch_reduce_coeffs(hipStream_t s, int N, int Np, int nb, const double* gen, const double* Xr, const double* Xi,
                           const double* PhiIr, const double* PhiIi, const double* DAr, const double* DAi,
