@@ -91,6 +91,8 @@ _SIGS = {
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
+    "bem_solve_sweep": (c_int, [c_void_p, c_void_p, c_int, c_int, _PD, _PD, c_int, c_int, _PD, c_int, _PD, _PD,
+                                _PD, _PD, _PD, POINTER(SolveInfo)]),
 }
 
 
@@ -295,6 +297,24 @@ class Body:
         _check(lib().bem_profile_lu(self.ctx._h, self._h, mode, kappa, _dp(out)))
         return dict(ms_lu=out[0], ms_gemm=out[1], gemm_flops=out[2], gemm_launches=int(out[3]),
                     ms_gemm_per_launch=out[4], flops_per_launch=out[5])
+
+    def solve_sweep(self, omegas, betas=(), mode=MODE_FP64, lanes=4, kappas=None, compose=COMPOSE_WAVE):
+        """Frequencies pipelined over `lanes` concurrent solve lanes; returns
+        stacked A (nf,6,6), B, F/FK/SC (nf,6,nb) and a list of info dicts."""
+        omegas = np.ascontiguousarray(omegas, dtype=np.float64)
+        nf = len(omegas)
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        nb = len(betas)
+        kap = None if kappas is None else np.ascontiguousarray(kappas, dtype=np.float64)
+        A = np.zeros((nf, 36)); B = np.zeros((nf, 36))
+        F = np.zeros((nf, 6, max(nb, 1)), dtype=np.complex128)
+        FK = np.zeros_like(F); SC = np.zeros_like(F)
+        infos = (SolveInfo * max(nf, 1))()
+        _check(lib().bem_solve_sweep(self.ctx._h, self._h, mode, nf, _dp(omegas), _dp(kap), compose, nb,
+                                     _dp(betas) if nb else None, lanes, _dp(A), _dp(B), F.ctypes.data_as(_PD),
+                                     FK.ctypes.data_as(_PD), SC.ctypes.data_as(_PD), infos))
+        return dict(A=A.reshape(nf, 6, 6), B=B.reshape(nf, 6, 6), F=F[:, :, :nb], FK=FK[:, :, :nb],
+                    SC=SC[:, :, :nb], info=[infos[i].as_dict() for i in range(nf)])
 
     def solve(self, omega, kappa=None, betas=(), mode=MODE_FP64, compose=COMPOSE_WAVE, rad_only=False,
               want_phi=False):

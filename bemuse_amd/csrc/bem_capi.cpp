@@ -1,5 +1,6 @@
-// C-ABI of the library (include/bemuse_amd.h): meshes, contexts, bodies and
-// the per-frequency solve. Host orchestration only; arithmetic is on the GPU.
+// C-ABI of the library (include/bemuse_amd.h): meshes, contexts, bodies, the
+// per-frequency solve and the pipelined frequency sweep. Host orchestration
+// only; arithmetic is on the GPU.
 #include "../../include/bemuse_amd.h"
 #include "bem_internal.h"
 #include "bem_kernels.h"
@@ -54,14 +55,8 @@ struct bem_mesh {
 
 struct bem_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t panel_stream = nullptr;  // LU lookahead stream (same priority)
-    std::vector<hipEvent_t> lu_events;   // 2 per LU panel, grown on demand
-    hipEvent_t ev[6] = {};
-    unsigned* counter = nullptr;     // ticket counter of the persistent solves
-    unsigned* flags = nullptr;       // per-block epoch flags
-    unsigned epoch = 0;
-    int max_blocks = 0;
+    hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
+    hipEvent_t ev[2] = {};
 };
 
 // Dense system workspace for one padded size Np.
@@ -76,7 +71,7 @@ struct LinSys {
     double *Br = nullptr, *Bi = nullptr, *Xr = nullptr, *Xi = nullptr, *Rr = nullptr, *Ri = nullptr;
     float *X32r = nullptr, *X32i = nullptr;
     double *Pr = nullptr, *Pi = nullptr, *Pabs = nullptr;   // matvec partials
-    double *colmax = nullptr, *anorm = nullptr;
+    double* anorm = nullptr;
     bool have32 = false;
     std::vector<void*> owned;
 
@@ -87,7 +82,6 @@ struct LinSys {
         for (void* p : owned)
             if (p) (void)hipFree(p);
     }
-
     template <typename T>
     int get(T** p, size_t n) {
         int r = dalloc_t(p, n);
@@ -116,7 +110,7 @@ struct LinSys {
             return r;
         size_t nks = (size_t)matvec_kslabs(Np);
         if ((r = get(&Pr, nks * NR)) || (r = get(&Pi, nks * NR)) || (r = get(&Pabs, nks * Np))) return r;
-        if ((r = get(&colmax, 2 * kMaxRhs)) || (r = get(&anorm, 1))) return r;
+        if ((r = get(&anorm, 1))) return r;
         return BEM_OK;
     }
     int init32() {
@@ -132,6 +126,83 @@ struct LinSys {
     }
 };
 
+// Results of one frequency as copied back to the host (pinned).
+struct LaneOut {
+    double coeffs[2 * (36 + 12 * kMaxRhs)];
+    RefineCtrl ctrl;
+    int flags[kMaxPanels + 1];
+};
+
+// One solve lane: the streams, events, solve-chain state and per-frequency
+// buffers needed to run a frequency independently of the other lanes.
+struct Lane {
+    hipStream_t s1 = nullptr, s2 = nullptr;     // main + LU lookahead stream
+    std::vector<hipEvent_t> lu_events;
+    hipEvent_t t[4] = {};                      // timing
+    hipEvent_t done = nullptr;
+    unsigned *counter = nullptr, *flags = nullptr;
+    unsigned epoch = 0;
+    RefineCtrl* ctrl = nullptr;
+    LaneOut* hout = nullptr;                   // pinned
+    LaneOut* dout = nullptr;
+    LinSys sys;
+    double *Sr = nullptr, *Si = nullptr;            // composed SMat
+    double *PhiIr = nullptr, *PhiIi = nullptr, *DAr = nullptr, *DAi = nullptr;
+    double *Xinr = nullptr, *Xini = nullptr;        // RHS input block
+    double* d_betas = nullptr;
+    std::vector<void*> owned;
+    // bookkeeping of the frequency in flight
+    int pending = -1;
+    int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
+    double omega = 0, kappa = 0;
+
+    ~Lane() {
+        if (s1) (void)hipStreamSynchronize(s1);
+        if (s2) (void)hipStreamSynchronize(s2);
+        for (void* p : owned)
+            if (p) (void)hipFree(p);
+        if (hout) (void)hipHostFree(hout);
+        for (auto& e : lu_events) (void)hipEventDestroy(e);
+        for (auto& e : t)
+            if (e) (void)hipEventDestroy(e);
+        if (done) (void)hipEventDestroy(done);
+        if (s2) (void)hipStreamDestroy(s2);
+        if (s1) (void)hipStreamDestroy(s1);
+    }
+    template <typename T>
+    int get(T** p, size_t n) {
+        int r = dalloc_t(p, n);
+        if (!r) owned.push_back(*p);
+        return r;
+    }
+    int init(int N) {
+        int r = BEM_OK;
+        HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        for (auto& e : t) HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+        if ((r = sys.init(N))) return r;
+        int Np = sys.Np;
+        int nblk = Np / kLuNb;
+        if ((r = get(&counter, 1)) || (r = get(&flags, nblk)) || (r = get(&ctrl, 1)) || (r = get(&dout, 1))) return r;
+        HIP_TRY(hipMemset(counter, 0, sizeof(unsigned)));
+        HIP_TRY(hipMemset(flags, 0, nblk * sizeof(unsigned)));
+        HIP_TRY(hipHostMalloc((void**)&hout, sizeof(LaneOut), hipHostMallocDefault));
+        size_t NN = (size_t)Np * Np, NR = (size_t)Np * kMaxRhs;
+        if ((r = get(&Sr, NN)) || (r = get(&Si, NN))) return r;
+        if ((r = get(&PhiIr, NR)) || (r = get(&PhiIi, NR)) || (r = get(&DAr, NR)) || (r = get(&DAi, NR)) ||
+            (r = get(&Xinr, NR)) || (r = get(&Xini, NR)) || (r = get(&d_betas, kMaxRhs)))
+            return r;
+        for (size_t i = 0; i < 2 * (size_t)nblk + 2; i++) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            lu_events.push_back(e);
+        }
+        HIP_TRY(hipDeviceSynchronize());
+        return BEM_OK;
+    }
+};
+
 struct bem_body {
     bem_ctx* ctx = nullptr;
     int N = 0, Np = 0;
@@ -139,15 +210,11 @@ struct bem_body {
     double *d_body = nullptr, *d_refl = nullptr, *d_quad = nullptr, *d_recv = nullptr, *d_gen = nullptr;
     double *Ssrc = nullptr, *Srefl = nullptr, *Dsrc = nullptr, *Drefl = nullptr;
     bool have_rankine = false;
-    double *Sr = nullptr, *Si = nullptr;            // composed SMat (per frequency)
-    double *PhiIr = nullptr, *PhiIi = nullptr, *DAr = nullptr, *DAi = nullptr;
-    double *Xinr = nullptr, *Xini = nullptr;        // RHS input block
-    double *d_betas = nullptr, *d_out = nullptr;
-    LinSys sys;
+    std::vector<Lane*> lanes;
 
     ~bem_body() {
-        void* ps[] = {d_body, d_refl, d_quad, d_recv, d_gen, Ssrc, Srefl, Dsrc, Drefl, Sr, Si,
-                      PhiIr, PhiIi, DAr, DAi, Xinr, Xini, d_betas, d_out};
+        for (Lane* l : lanes) delete l;
+        void* ps[] = {d_body, d_refl, d_quad, d_recv, d_gen, Ssrc, Srefl, Dsrc, Drefl};
         for (void* p : ps)
             if (p) (void)hipFree(p);
     }
@@ -155,99 +222,53 @@ struct bem_body {
 
 namespace {
 
-int lu_events(bem_ctx* c, int Np) {
-    size_t need = 2 * (size_t)(Np / kLuNb) + 2;
-    while (c->lu_events.size() < need) {
-        hipEvent_t e;
-        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        c->lu_events.push_back(e);
-    }
-    return BEM_OK;
+// Device LU in the given precision (FP64 in place, or FP32 on the FP32 copy).
+void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) {
+    LinSys& s = L.sys;
+    if (precision == 64)
+        launch_lu<double>(L.s1, L.s2, L.lu_events.data(), s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof);
+    else
+        launch_lu<float>(L.s1, L.s2, L.lu_events.data(), s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof);
 }
 
-// Device LU in the given precision (FP64, or FP32 on the FP32 copy).
-int factor_system(bem_ctx* c, LinSys& s, int precision, LuProfile* prof = nullptr) {
-    int r = lu_events(c, s.Np);
-    if (r) return r;
-    if (precision == 64) {
-        launch_lu<double>(c->stream, c->panel_stream, c->lu_events.data(), s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv,
-                          prof);
-    } else {
-        if ((r = s.init32())) return r;
-        launch_lu<float>(c->stream, c->panel_stream, c->lu_events.data(), s.Np, s.D32r, s.D32i, s.w32, s.flag32,
-                         s.ipiv32, prof);
-    }
-    HIP_TRY(hipGetLastError());
-    return BEM_OK;
-}
+constexpr int kRefineMaxIter = 4;
+// Refinement stops at a max-norm relative residual of 1e-13: with the
+// system's condition numbers this keeps A, B, F orders of magnitude inside
+// the 1e-6 bar while saving correction sweeps.
+constexpr double kRefineTol = 1e-13;
 
-int count_flags(bem_ctx* c, LinSys& s, int precision, int* fallback) {
-    int np = s.Np / kLuNb;
-    std::vector<int> f(np + 1);
-    HIP_TRY(hipMemcpyAsync(f.data(), precision == 64 ? s.flag : s.flag32, (np + 1) * sizeof(int),
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    int n = 0;
-    for (int i = 0; i < np; i++) n += (f[i] & 1) ? 1 : 0;
-    *fallback = n;
-    return (f[np] & 2) ? 1 : 0;
-}
-
-// Solve D X = B for nr columns (B, X in the workspace). In FP32 mode the
-// FP64 matrix s.Dr/s.Di must hold the unfactored system.
-int solve_system(bem_ctx* c, LinSys& s, int precision, int nr, bem_solve_info* info) {
+// Solve D X = B for nr columns (B in the workspace, X out). FP32 mode:
+// FP32 LU solve followed by FP64 iterative refinement against the FP64
+// matrix s.Dr/s.Di, controlled on the device (no host round trip).
+void solve_system(Lane& L, int precision, int nr) {
+    LinSys& s = L.sys;
     size_t n = (size_t)s.Np * nr;
     if (precision == 64) {
-        launch_copy2(c->stream, n, s.Br, s.Bi, s.Xr, s.Xi);
-        launch_lu_solve<double, double>(c->stream, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr,
-                                        s.Xi, c->counter, c->flags, c->epoch);
-        HIP_TRY(hipGetLastError());
-        if (info) { info->refine_iters = 0; info->residual = 0.0; }
-        return BEM_OK;
+        launch_copy2(L.s1, n, s.Br, s.Bi, s.Xr, s.Xi);
+        launch_lu_solve<double, double>(L.s1, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
+                                        L.counter, L.flags, L.epoch);
+        return;
     }
-    // FP32 LU solve, then refine: r = b - A x (FP64), d = LU32^-1 r, x += d.
     const int* fany = s.flag32 + s.Np / kLuNb;
-    launch_to_f32(c->stream, n, s.Br, s.Bi, s.X32r, s.X32i);
-    launch_lu_solve<float, float>(c->stream, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                  c->counter, c->flags, c->epoch);
-    HIP_TRY(hipMemsetAsync(s.Xr, 0, n * sizeof(double), c->stream));
-    HIP_TRY(hipMemsetAsync(s.Xi, 0, n * sizeof(double), c->stream));
-    launch_add_f32(c->stream, n, s.Xr, s.Xi, s.X32r, s.X32i);
-    // Stop at a max-norm relative residual of 1e-13: with the system's
-    // condition numbers this keeps A, B, F orders of magnitude inside the
-    // 1e-6 bar while saving a refinement sweep.
-    const int max_iter = 12;
-    const double tol = 1e-13;
-    double prev = 1e300, rel = 1e300, anorm = 0.0;
-    int it = 0;
-    std::vector<double> cm(2 * kMaxRhs + 1);
-    for (; it <= max_iter; it++) {
-        // the first residual also yields |A|_inf (row sums of |re| + |im|)
-        launch_matvec<double>(c->stream, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
-                              it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr);
-        launch_colmax(c->stream, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.colmax);
-        HIP_TRY(hipMemcpyAsync(cm.data(), s.colmax, 2 * nr * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        if (it == 0) HIP_TRY(hipMemcpyAsync(&anorm, s.anorm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        rel = 0;
-        for (int j = 0; j < nr; j++) {
-            double den = anorm * cm[2 * j + 1];
-            double v = den > 0 ? cm[2 * j] / den : (cm[2 * j] > 0 ? 1e300 : 0.0);
-            rel = std::max(rel, v);
-        }
-        if (!(rel == rel)) break;                 // NaN
-        if (rel <= tol) break;
-        if (it > 0 && rel > 0.5 * prev) break;    // stagnation
-        if (it == max_iter) break;
-        prev = rel;
-        launch_to_f32(c->stream, n, s.Rr, s.Ri, s.X32r, s.X32i);
-        launch_lu_solve<float, float>(c->stream, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                      c->counter, c->flags, c->epoch);
-        launch_add_f32(c->stream, n, s.Xr, s.Xi, s.X32r, s.X32i);
+    const int* skip = &L.ctrl->done;
+    launch_refine_reset(L.s1, L.ctrl);
+    launch_to_f32(L.s1, n, s.Br, s.Bi, s.X32r, s.X32i);
+    launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, L.counter,
+                                  L.flags, L.epoch);
+    launch_from_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i);
+    for (int it = 0; it <= kRefineMaxIter; it++) {
+        // r = b - A x in FP64; the first residual also yields |A|_inf
+        launch_matvec<double>(L.s1, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
+                              it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
+        launch_refine_check(L.s1, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, L.ctrl, kRefineTol,
+                            kRefineMaxIter);
+        if (it == kRefineMaxIter) break;
+        // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
+        launch_to_f32(L.s1, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
+        launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
+                                      L.counter, L.flags, L.epoch, skip);
+        launch_add_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
     }
-    HIP_TRY(hipGetLastError());
-    if (info) { info->refine_iters = it; info->residual = rel; }
-    return BEM_OK;
 }
 
 void planar_to_interleaved(const std::vector<double>& re, const std::vector<double>& im, int N, int Np, int ncol,
@@ -372,17 +393,6 @@ int bem_create(int device, bem_ctx** out) {
         return fail(BEM_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
-    if (hipStreamCreateWithFlags(&c->panel_stream, hipStreamNonBlocking) != hipSuccess) {
-        bem_destroy(c);
-        return fail(BEM_ERR_HIP, "hipStreamCreate failed");
-    }
-    c->max_blocks = 4096;
-    int r = dalloc_t(&c->counter, 1);
-    if (!r) r = dalloc_t(&c->flags, c->max_blocks);
-    if (r) { bem_destroy(c); return r; }
-    (void)hipMemset(c->counter, 0, sizeof(unsigned));
-    (void)hipMemset(c->flags, 0, c->max_blocks * sizeof(unsigned));
-    (void)hipDeviceSynchronize();
     *out = c;
     return BEM_OK;
 }
@@ -391,13 +401,8 @@ void bem_destroy(bem_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->counter) (void)hipFree(c->counter);
-    if (c->flags) (void)hipFree(c->flags);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->lu_events)
-        if (e) (void)hipEventDestroy(e);
-    if (c->panel_stream) { (void)hipStreamSynchronize(c->panel_stream); (void)hipStreamDestroy(c->panel_stream); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -411,7 +416,7 @@ int bem_body_create(bem_ctx* c, const bem_mesh* m, bem_body** out) {
     if (!build_geometry(m->m, b->geo, err)) { delete b; return fail(BEM_ERR_ARG, err); }
     b->N = b->geo.N;
     b->Np = round_up(b->N, kPad);
-    if (b->Np / kLuNb > c->max_blocks) { delete b; return fail(BEM_ERR_ARG, "body too large"); }
+    if (b->Np / kLuNb > kMaxPanels) { delete b; return fail(BEM_ERR_ARG, "body too large"); }
     int N = b->N;
     int r = BEM_OK;
     if ((r = dalloc_t(&b->d_body, (size_t)N * 32)) || (r = dalloc_t(&b->d_refl, (size_t)N * 32)) ||
@@ -459,73 +464,91 @@ int bem_body_assemble_rankine(bem_ctx* c, bem_body* b) {
     return BEM_OK;
 }
 
-static int ensure_freq_buffers(bem_body* b) {
-    if (b->Sr) return BEM_OK;
-    size_t NN = (size_t)b->Np * b->Np, NR = (size_t)b->Np * kMaxRhs;
-    int r = BEM_OK;
-    if ((r = dalloc_t(&b->Sr, NN)) || (r = dalloc_t(&b->Si, NN))) return r;
-    if ((r = dalloc_t(&b->PhiIr, NR)) || (r = dalloc_t(&b->PhiIi, NR)) || (r = dalloc_t(&b->DAr, NR)) ||
-        (r = dalloc_t(&b->DAi, NR)) || (r = dalloc_t(&b->Xinr, NR)) || (r = dalloc_t(&b->Xini, NR)))
-        return r;
-    if ((r = dalloc_t(&b->d_betas, kMaxRhs)) || (r = dalloc_t(&b->d_out, 2 * (36 + 12 * kMaxRhs)))) return r;
-    return b->sys.init(b->N);
-}
+namespace {
 
-static void assemble(bem_ctx* c, bem_body* b, int mode, double kappa, int compose) {
-    bool f32 = (mode != BEM_MODE_FP64);
-    launch_wave_compose(c->stream, b->N, b->Np, kappa, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc,
-                        b->Drefl, b->Sr, b->Si, b->sys.Dr, b->sys.Di, f32 ? b->sys.D32r : nullptr,
-                        f32 ? b->sys.D32i : nullptr);
-}
-
-int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double kappa, int compose, int rad_only,
-                        int nb, const double* betas, double* A36, double* B36, double* F, double* FK, double* SC,
-                        double* phiJ, double* phiS, bem_solve_info* info) {
-    if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bem_solve_frequency: bad context/body");
-    if (nb < 0 || nb > kMaxRhs - 6 || (nb > 0 && !betas)) return fail(BEM_ERR_ARG, "bem_solve_frequency: nb must be in [0, 10]");
-    if (mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bem_solve_frequency: bad mode");
-    if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bem_solve_frequency: bad compose case");
-    if (!A36 || !B36) return fail(BEM_ERR_ARG, "bem_solve_frequency: A36/B36 required");
-    if (!b->have_rankine) return fail(BEM_ERR_STATE, "bem_solve_frequency: Rankine matrices not assembled");
-    HIP_TRY(hipSetDevice(c->device));
-    int r = ensure_freq_buffers(b);
-    if (r) return r;
-    if (mode != BEM_MODE_FP64 && (r = b->sys.init32())) return r;
-    hipStream_t s = c->stream;
-    LinSys& sys = b->sys;
-    int nr = 6 + nb;
-    bem_solve_info loc{};
-    if (nb > 0) HIP_TRY(hipMemcpyAsync(b->d_betas, betas, nb * sizeof(double), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipEventRecord(c->ev[0], s));
-    assemble(c, b, mode, kappa, compose);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], s));
-    int prec = (mode == BEM_MODE_FP64) ? 64 : 32;
-    if ((r = factor_system(c, sys, prec))) return r;
-    HIP_TRY(hipEventRecord(c->ev[2], s));
-    launch_incident(s, b->N, b->Np, nb, b->d_body, b->d_gen, b->d_betas, omega, kappa, b->Xinr, b->Xini, kMaxRhs,
-                    b->PhiIr, b->PhiIi, b->DAr, b->DAi);
-    launch_matvec<double>(s, b->Np, nr, b->Sr, b->Si, b->Xinr, b->Xini, sys.Pr, sys.Pi, nullptr, nullptr, 0, sys.Br,
-                          sys.Bi);
-    HIP_TRY(hipGetLastError());
-    if ((r = solve_system(c, sys, prec, nr, &loc))) return r;
-    if (prec == 32 && !(loc.residual <= 1e-11)) {
-        // refinement did not reach FP64 accuracy: refactor in FP64
-        assemble(c, b, BEM_MODE_FP64, kappa, compose);
-        if ((r = factor_system(c, sys, 64)) || (r = solve_system(c, sys, 64, nr, nullptr))) return r;
-        loc.fell_back_fp64 = 1;
-        prec = 64;
+int ensure_lanes(bem_body* b, int n, bool need32) {
+    while ((int)b->lanes.size() < n) {
+        Lane* L = new Lane;
+        int r = L->init(b->N);
+        if (r) { delete L; return r; }
+        b->lanes.push_back(L);
     }
-    launch_reduce_coeffs(s, b->N, b->Np, nb, b->d_gen, sys.Xr, sys.Xi, b->PhiIr, b->PhiIi, b->DAr, b->DAi, b->d_out);
+    if (need32)
+        for (int i = 0; i < n; i++) {
+            int r = b->lanes[i]->sys.init32();
+            if (r) return r;
+        }
+    return BEM_OK;
+}
+
+void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
+    bool f32 = (mode != BEM_MODE_FP64);
+    launch_wave_compose(L.s1, b->N, b->Np, kappa, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
+                        L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr);
+}
+
+// Issues one frequency on a lane (all asynchronous on the lane's streams).
+// The betas must already be on the device (L.d_betas).
+int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, double kappa, int compose, int rad_only,
+                    int nb) {
+    LinSys& s = L.sys;
+    hipStream_t st = L.s1;
+    L.pending = index;
+    L.mode = mode; L.prec = (mode == BEM_MODE_FP64) ? 64 : 32;
+    L.omega = omega; L.kappa = kappa; L.compose = compose; L.rad_only = rad_only; L.nb = nb;
+    int nr = 6 + nb;
+    HIP_TRY(hipEventRecord(L.t[0], st));
+    assemble(b, L, mode, kappa, compose);
+    HIP_TRY(hipEventRecord(L.t[1], st));
+    factor_system(L, L.prec);
+    HIP_TRY(hipEventRecord(L.t[2], st));
+    launch_incident(st, b->N, b->Np, nb, b->d_body, b->d_gen, L.d_betas, omega, kappa, L.Xinr, L.Xini, kMaxRhs,
+                    L.PhiIr, L.PhiIi, L.DAr, L.DAi);
+    launch_matvec<double>(st, b->Np, nr, L.Sr, L.Si, L.Xinr, L.Xini, s.Pr, s.Pi, nullptr, nullptr, 0, s.Br, s.Bi);
+    solve_system(L, L.prec, nr);
+    launch_reduce_coeffs(st, b->N, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
+    HIP_TRY(hipEventRecord(L.t[3], st));
+    int np = b->Np / kLuNb;
+    HIP_TRY(hipMemcpyAsync(L.hout->coeffs, L.dout->coeffs, sizeof(double) * 2 * (36 + 12 * nb), hipMemcpyDeviceToHost, st));
+    if (L.prec == 32)
+        HIP_TRY(hipMemcpyAsync(&L.hout->ctrl, L.ctrl, sizeof(RefineCtrl), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(L.hout->flags, L.prec == 64 ? s.flag : s.flag32, sizeof(int) * (np + 1),
+                           hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(L.done, st));
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[3], s));
-    std::vector<double> out(2 * (36 + 12 * nb));
-    HIP_TRY(hipMemcpyAsync(out.data(), b->d_out, out.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    int zero_piv = count_flags(c, sys, prec, &loc.pivot_fallback);
+    return BEM_OK;
+}
+
+// Waits for the lane's frequency and writes its results (FP64 re-solve if
+// the refinement did not converge).
+int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* info) {
+    HIP_TRY(hipEventSynchronize(L.done));
+    bem_solve_info loc{};
+    int fell_back = 0;
+    if (L.prec == 32) {
+        loc.refine_iters = L.hout->ctrl.iters;
+        loc.residual = L.hout->ctrl.rel;
+        if (!L.hout->ctrl.converged) {
+            int r = issue_frequency(b, L, L.pending, BEM_MODE_FP64, L.omega, L.kappa, L.compose, L.rad_only, L.nb);
+            if (r) return r;
+            HIP_TRY(hipEventSynchronize(L.done));
+            fell_back = 1;
+        }
+    }
+    loc.fell_back_fp64 = fell_back;
+    int np = b->Np / kLuNb;
+    for (int i = 0; i < np; i++) loc.pivot_fallback += (L.hout->flags[i] & 1) ? 1 : 0;
+    float t01 = 0, t12 = 0, t23 = 0;
+    (void)hipEventElapsedTime(&t01, L.t[0], L.t[1]);
+    (void)hipEventElapsedTime(&t12, L.t[1], L.t[2]);
+    (void)hipEventElapsedTime(&t23, L.t[2], L.t[3]);
+    loc.ms_assemble = t01; loc.ms_factor = t12; loc.ms_solve = t23;
+    const double* out = L.hout->coeffs;
+    int nb = L.nb;
+    double omega = L.omega;
     // A = Re(M Phi_J), B = Im(M Phi_J) (Hydrodynamic_Solver.cpp:463-465)
     for (int q = 0; q < 36; q++) { A36[q] = out[2 * q]; B36[q] = out[2 * q + 1]; }
-    if (!rad_only && nb > 0) {
+    if (!L.rad_only && nb > 0) {
         typedef std::complex<double> cd;
         const cd Im(0.0, 1.0);
         for (int i = 0; i < 6; i++)
@@ -541,23 +564,83 @@ int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double 
                 if (SC) { SC[o] = sc.real(); SC[o + 1] = sc.imag(); }
             }
     }
+    if (info) *info = loc;
+    L.pending = -1;
+    return BEM_OK;
+}
+
+int check_solve_args(bem_ctx* c, bem_body* b, int mode, int compose, int nb, const double* betas) {
+    if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bad context/body");
+    if (nb < 0 || nb > kMaxRhs - 6 || (nb > 0 && !betas)) return fail(BEM_ERR_ARG, "nb must be in [0, 10]");
+    if (mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bad mode");
+    if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bad compose case");
+    if (!b->have_rankine) return fail(BEM_ERR_STATE, "Rankine matrices not assembled");
+    return BEM_OK;
+}
+
+int upload_betas(Lane& L, int nb, const double* betas) {
+    if (nb > 0) {
+        HIP_TRY(hipMemcpyAsync(L.d_betas, betas, nb * sizeof(double), hipMemcpyHostToDevice, L.s1));
+        HIP_TRY(hipStreamSynchronize(L.s1));
+    }
+    return BEM_OK;
+}
+
+}  // namespace
+
+int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double kappa, int compose, int rad_only,
+                        int nb, const double* betas, double* A36, double* B36, double* F, double* FK, double* SC,
+                        double* phiJ, double* phiS, bem_solve_info* info) {
+    int r = check_solve_args(c, b, mode, compose, nb, betas);
+    if (r) return r;
+    if (!A36 || !B36) return fail(BEM_ERR_ARG, "bem_solve_frequency: A36/B36 required");
+    HIP_TRY(hipSetDevice(c->device));
+    if ((r = ensure_lanes(b, 1, mode != BEM_MODE_FP64))) return r;
+    Lane& L = *b->lanes[0];
+    if ((r = upload_betas(L, nb, betas))) return r;
+    if ((r = issue_frequency(b, L, 0, mode, omega, kappa, compose, rad_only, nb))) return r;
+    if ((r = harvest(b, L, A36, B36, F, FK, SC, info))) return r;
     if (phiJ || phiS) {
-        std::vector<double> xr((size_t)b->Np * nr), xi((size_t)b->Np * nr);
-        HIP_TRY(hipMemcpy(xr.data(), sys.Xr, xr.size() * 8, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(xi.data(), sys.Xi, xi.size() * 8, hipMemcpyDeviceToHost));
-        if (phiJ) planar_to_interleaved(xr, xi, b->N, b->Np, 6, phiJ);
+        int Np = b->Np, nr = 6 + nb;
+        std::vector<double> xr((size_t)Np * nr), xi((size_t)Np * nr);
+        HIP_TRY(hipMemcpy(xr.data(), L.sys.Xr, xr.size() * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(xi.data(), L.sys.Xi, xi.size() * 8, hipMemcpyDeviceToHost));
+        if (phiJ) planar_to_interleaved(xr, xi, b->N, Np, 6, phiJ);
         if (phiS && nb > 0) {
-            std::vector<double> sr(xr.begin() + (size_t)6 * b->Np, xr.end()), si(xi.begin() + (size_t)6 * b->Np, xi.end());
-            planar_to_interleaved(sr, si, b->N, b->Np, nb, phiS);
+            std::vector<double> sr(xr.begin() + (size_t)6 * Np, xr.end()), si(xi.begin() + (size_t)6 * Np, xi.end());
+            planar_to_interleaved(sr, si, b->N, Np, nb, phiS);
         }
     }
-    float t01 = 0, t12 = 0, t23 = 0;
-    (void)hipEventElapsedTime(&t01, c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&t12, c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&t23, c->ev[2], c->ev[3]);
-    loc.ms_assemble = t01; loc.ms_factor = t12; loc.ms_solve = t23;
-    if (info) *info = loc;
-    (void)zero_piv;
+    return BEM_OK;
+}
+
+int bem_solve_sweep(bem_ctx* c, bem_body* b, int mode, int nf, const double* omegas, const double* kappas,
+                    int compose, int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
+                    double* FK, double* SC, bem_solve_info* infos) {
+    int r = check_solve_args(c, b, mode, compose, nb, betas);
+    if (r) return r;
+    if (nf < 0 || !omegas || !A36 || !B36 || lanes < 1 || lanes > 16)
+        return fail(BEM_ERR_ARG, "bem_solve_sweep: bad arguments (lanes in [1, 16])");
+    HIP_TRY(hipSetDevice(c->device));
+    if ((r = ensure_lanes(b, lanes, mode != BEM_MODE_FP64))) return r;
+    for (int l = 0; l < lanes; l++)
+        if ((r = upload_betas(*b->lanes[l], nb, betas))) return r;
+    size_t fs = (size_t)6 * nb * 2;
+    auto collect = [&](Lane& L) -> int {
+        int f = L.pending;
+        return harvest(b, L, A36 + 36 * (size_t)f, B36 + 36 * (size_t)f, F ? F + fs * f : nullptr,
+                       FK ? FK + fs * f : nullptr, SC ? SC + fs * f : nullptr, infos ? infos + f : nullptr);
+    };
+    for (int f = 0; f < nf; f++) {
+        Lane& L = *b->lanes[f % lanes];
+        if (L.pending >= 0 && (r = collect(L))) return r;
+        double kap = kappas ? kappas[f] : omegas[f] * omegas[f] / 9.81;
+        if ((r = issue_frequency(b, L, f, mode, omegas[f], kap, compose, 0, nb))) return r;
+    }
+    for (int l = 0; l < lanes; l++) {
+        Lane& L = *b->lanes[l];
+        if (L.pending >= 0 && (r = collect(L))) return r;
+    }
     return BEM_OK;
 }
 
@@ -585,12 +668,12 @@ int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose,
     if (!c || !b || !b->have_rankine) return fail(BEM_ERR_STATE, "bem_get_system: Rankine matrices not assembled");
     if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bem_get_system: bad compose case");
     HIP_TRY(hipSetDevice(c->device));
-    int r = ensure_freq_buffers(b);
+    int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
-    if (mode != BEM_MODE_FP64 && (r = b->sys.init32())) return r;
-    assemble(c, b, mode, kappa, compose);
+    Lane& L = *b->lanes[0];
+    assemble(b, L, mode, kappa, compose);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(L.s1));
     int N = b->N, Np = b->Np;
     std::vector<double> re((size_t)Np * N), im((size_t)Np * N);
     auto out = [&](const double* dr, const double* di, double* h) -> int {
@@ -599,25 +682,24 @@ int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose,
         planar_to_interleaved(re, im, N, Np, N, h);
         return BEM_OK;
     };
-    if (SMat && (r = out(b->Sr, b->Si, SMat))) return r;
-    if (DMat && (r = out(b->sys.Dr, b->sys.Di, DMat))) return r;
+    if (SMat && (r = out(L.Sr, L.Si, SMat))) return r;
+    if (DMat && (r = out(L.sys.Dr, L.sys.Di, DMat))) return r;
     return BEM_OK;
 }
 
 int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out) {
     if (!c || !b || !b->have_rankine || !out) return fail(BEM_ERR_STATE, "bem_profile_lu: Rankine matrices not assembled");
     HIP_TRY(hipSetDevice(c->device));
-    int r = ensure_freq_buffers(b);
+    int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
-    if (mode != BEM_MODE_FP64 && (r = b->sys.init32())) return r;
-    assemble(c, b, mode, kappa, BEM_COMPOSE_WAVE);
-    LinSys& s = b->sys;
+    Lane& L = *b->lanes[0];
+    assemble(b, L, mode, kappa, BEM_COMPOSE_WAVE);
     LuProfile* prof = new LuProfile;
     prof->N = b->N;
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if ((r = factor_system(c, s, mode == BEM_MODE_FP64 ? 64 : 32, prof))) { delete prof; return r; }
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    hipError_t e = hipStreamSynchronize(c->stream);
+    HIP_TRY(hipEventRecord(L.t[0], L.s1));
+    factor_system(L, mode == BEM_MODE_FP64 ? 64 : 32, prof);
+    HIP_TRY(hipEventRecord(L.t[1], L.s1));
+    hipError_t e = hipStreamSynchronize(L.s1);
     double sum = 0.0;
     for (int i = 0; i < prof->n; i++) {
         float t = 0;
@@ -625,7 +707,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
         sum += t;
     }
     float tot = 0;
-    (void)hipEventElapsedTime(&tot, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&tot, L.t[0], L.t[1]);
     out[0] = tot;
     out[1] = sum;
     out[2] = prof->useful_flops;
@@ -665,10 +747,12 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
                     bem_solve_info* info) {
     if (!c || n <= 0 || !A || !B || !X || nrhs < 1 || nrhs > kMaxRhs || (precision != 64 && precision != 32))
         return fail(BEM_ERR_ARG, "bem_dense_solve: bad arguments (nrhs in [1,16], precision 64 or 32)");
+    if (round_up(n, kPad) / kLuNb > kMaxPanels) return fail(BEM_ERR_ARG, "bem_dense_solve: matrix too large");
     HIP_TRY(hipSetDevice(c->device));
-    LinSys s;
-    int r = s.init(n);
+    Lane L;
+    int r = L.init(n);
     if (r) return r;
+    LinSys& s = L.sys;
     int Np = s.Np;
     {
         std::vector<double> re((size_t)Np * Np, 0.0), im((size_t)Np * Np, 0.0);
@@ -689,18 +773,26 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     bem_solve_info loc{};
     if (precision == 32) {
         if ((r = s.init32())) return r;
-        launch_to_f32(c->stream, (size_t)Np * Np, s.Dr, s.Di, s.D32r, s.D32i);
+        launch_to_f32(L.s1, (size_t)Np * Np, s.Dr, s.Di, s.D32r, s.D32i);
     }
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if ((r = factor_system(c, s, precision))) return r;
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    if ((r = solve_system(c, s, precision, nrhs, &loc))) return r;
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    count_flags(c, s, precision, &loc.pivot_fallback);
+    HIP_TRY(hipEventRecord(L.t[0], L.s1));
+    factor_system(L, precision);
+    HIP_TRY(hipEventRecord(L.t[1], L.s1));
+    solve_system(L, precision, nrhs);
+    HIP_TRY(hipEventRecord(L.t[2], L.s1));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(L.s1));
+    RefineCtrl ctrl{};
+    if (precision == 32) HIP_TRY(hipMemcpy(&ctrl, L.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    std::vector<int> flags(Np / kLuNb + 1);
+    HIP_TRY(hipMemcpy(flags.data(), precision == 64 ? s.flag : s.flag32, flags.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < Np / kLuNb; i++) loc.pivot_fallback += (flags[i] & 1) ? 1 : 0;
+    loc.refine_iters = ctrl.iters;
+    loc.residual = precision == 32 ? ctrl.rel : 0.0;
+    loc.fell_back_fp64 = (precision == 32 && !ctrl.converged) ? 1 : 0;
     float t1 = 0, t2 = 0;
-    (void)hipEventElapsedTime(&t1, c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&t2, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&t1, L.t[0], L.t[1]);
+    (void)hipEventElapsedTime(&t2, L.t[1], L.t[2]);
     loc.ms_factor = t1; loc.ms_solve = t2;
     std::vector<double> xr((size_t)Np * kMaxRhs), xi((size_t)Np * kMaxRhs);
     HIP_TRY(hipMemcpy(xr.data(), s.Xr, xr.size() * 8, hipMemcpyDeviceToHost));

@@ -9,6 +9,7 @@ namespace bemuse {
 constexpr int kPad = 128;       // Np = roundup(N, kPad)
 constexpr int kLuNb = 64;       // LU panel width
 constexpr int kMaxRhs = 16;     // columns of the right-hand-side block (6 + nb <= 16)
+constexpr int kMaxPanels = 1024; // largest padded order / kLuNb (Np <= 65536)
 
 void launch_rankine(hipStream_t s, int N, int Np, const double* body, const double* refl, const double* recv,
                     double* Ssrc, double* Srefl, double* Dsrc, double* Drefl);
@@ -25,7 +26,7 @@ int matvec_kslabs(int Np);
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
                    double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
-                   double* Pabs = nullptr, double* norm = nullptr);
+                   double* Pabs = nullptr, double* norm = nullptr, const int* skip = nullptr);
 // Optional timing of the trailing-update GEMM launches of one factorisation:
 // events bracket every GEMM launch; useful_flops counts 8 M N K over the
 // unpadded part (order N) of each launch.
@@ -52,9 +53,23 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                int* ipiv, LuProfile* prof = nullptr);
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch);
-void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb);
-void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb);
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch,
+                     const int* skip = nullptr);
+void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb,
+                   const int* skip = nullptr);
+void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb,
+                    const int* skip = nullptr);
+
+// Device-resident refinement state (one per solve lane); `done` is the first
+// member so kernels can take &ctrl->done as their skip flag.
+struct RefineCtrl {
+    int done, iters, converged, pad;
+    double rel, prev;
+};
+void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl);
+void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
+                         const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter);
+void launch_from_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb);
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb);
 void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
                    const double* xi, double* out);

@@ -175,7 +175,8 @@ template <typename TM, bool ROWABS>
 __global__ __launch_bounds__(256) void k_matvec_partial(int Np, int nr, int kslab, const TM* __restrict__ Mr,
                                                         const TM* __restrict__ Mi, const double* __restrict__ Xr,
                                                         const double* __restrict__ Xi, double* Pr, double* Pi,
-                                                        double* Pabs) {
+                                                        double* Pabs, const int* skip) {
+    if (skip && *skip) return;
     __shared__ double xs[MV_KC][MV_MAXR][2];
     int row = blockIdx.x * MV_ROWS + threadIdx.x, ks = blockIdx.y;
     bool live = row < Np;
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(256) void k_matvec_partial(int Np, int nr, int ksla
 // Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1); with Pabs, also the max
 // row sum of |re| + |im| into norm[0] (one work-group) -- see k_rownorm.
 __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restrict__ Pr, const double* __restrict__ Pi,
-                                const double* Br, const double* Bi, int mode, double* Yr, double* Yi) {
+                                const double* Br, const double* Bi, int mode, double* Yr, double* Yi, const int* skip) {
+    if (skip && *skip) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nr * Np) return;
     int j = t / Np, r = t % Np;
@@ -247,7 +249,8 @@ __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restric
 }
 
 // norm[0] = max over rows of sum_ks Pabs (the infinity norm with |re|+|im|).
-__global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm) {
+__global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm, const int* skip) {
+    if (skip && *skip) return;
     __shared__ double s[256];
     double m = 0.0;
     for (int r = threadIdx.x; r < Np; r += 256) {
@@ -787,7 +790,8 @@ template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower, const T* __restrict__ Ar,
                                                       const T* __restrict__ Ai, const T* __restrict__ Inv_r,
                                                       const T* __restrict__ Inv_i, TX* Xr, TX* Xi, unsigned* counter,
-                                                      unsigned* flags, unsigned epoch) {
+                                                      unsigned* flags, unsigned epoch, const int* skip) {
+    if (skip && *skip) return;   // uniform: no work-group takes a ticket
     // cross-wave partial sums (after the dependency loop) share storage with
     // the per-wave staging of X_c rows (during it)
     __shared__ TX shared_buf[4 * MV_MAXR * 2 * LU_NB];
@@ -896,7 +900,8 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
 // ipiv (applied sequentially, one thread per column); only if any panel fell
 // back to pivoting.
 template <typename TX>
-__global__ void k_apply_ipiv(int Np, int nr, const int* ipiv, const int* any_flag, TX* Xr, TX* Xi) {
+__global__ void k_apply_ipiv(int Np, int nr, const int* ipiv, const int* any_flag, TX* Xr, TX* Xi, const int* skip) {
+    if (skip && *skip) return;
     if (!any_flag[0]) return;
     int j = threadIdx.x;
     if (j >= nr) return;
@@ -932,13 +937,19 @@ __global__ void k_init_lu(int Np, int npanel, int* flag, int* ipiv) {
 // ---------------------------------------------------------------------------
 // Refinement helpers
 // ---------------------------------------------------------------------------
-__global__ void k_to_f32(size_t n, const double* a, const double* b, float* fa, float* fb) {
+__global__ void k_to_f32(size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
+    if (skip && *skip) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { fa[i] = (float)a[i]; fb[i] = (float)b[i]; }
 }
-__global__ void k_add_f32(size_t n, double* a, double* b, const float* fa, const float* fb) {
+__global__ void k_add_f32(size_t n, double* a, double* b, const float* fa, const float* fb, const int* skip) {
+    if (skip && *skip) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { a[i] += (double)fa[i]; b[i] += (double)fb[i]; }
+}
+__global__ void k_from_f32(size_t n, double* a, double* b, const float* fa, const float* fb) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { a[i] = (double)fa[i]; b[i] = (double)fb[i]; }
 }
 __global__ void k_copy2(size_t n, const double* a, const double* b, double* ca, double* cb) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -963,6 +974,58 @@ __global__ void k_colmax(int N, int Np, const double* rr, const double* ri, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) { out[2 * j] = s1[0]; out[2 * j + 1] = s2[0]; }
+}
+
+// Device-side control of iterative refinement (no host round trip inside a
+// frequency solve). ctrl: done, iterations, converged; rel[0] = last
+// max-norm relative residual, rel[1] = previous one, rel[2] = |A|_inf.
+__global__ void k_refine_reset(RefineCtrl* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl->done = 0; ctrl->iters = 0; ctrl->converged = 0;
+        ctrl->rel = 1e300; ctrl->prev = 1e300;
+    }
+}
+
+// After a residual R = B - A X: rel = max_j max|r_j| / (|A| max|x_j|) over
+// rows < N. Converged when rel <= tol; stops (not converged) on stagnation
+// (rel > 0.5 prev) or after max_iter corrections. One work-group.
+__global__ void k_refine_check(int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
+                               const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
+    if (ctrl->done) return;
+    __shared__ double sm[2][MV_MAXR][256];
+    for (int j = 0; j < nr; j++) {
+        double a = 0, b = 0;
+        for (int r = threadIdx.x; r < N; r += 256) {
+            size_t o = (size_t)j * Np + r;
+            a = fmax(a, sqrt(Rr[o] * Rr[o] + Ri[o] * Ri[o]));
+            b = fmax(b, sqrt(Xr[o] * Xr[o] + Xi[o] * Xi[o]));
+        }
+        sm[0][j][threadIdx.x] = a; sm[1][j][threadIdx.x] = b;
+    }
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int j = 0; j < nr; j++) {
+                sm[0][j][threadIdx.x] = fmax(sm[0][j][threadIdx.x], sm[0][j][threadIdx.x + h]);
+                sm[1][j][threadIdx.x] = fmax(sm[1][j][threadIdx.x], sm[1][j][threadIdx.x + h]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double an = anorm[0], rel = 0.0;
+        for (int j = 0; j < nr; j++) {
+            double den = an * sm[1][j][0];
+            double v = den > 0 ? sm[0][j][0] / den : (sm[0][j][0] > 0 ? 1e300 : 0.0);
+            rel = fmax(rel, v);
+        }
+        if (!(rel == rel)) rel = 1e300;
+        ctrl->prev = ctrl->rel;
+        ctrl->rel = rel;
+        if (rel <= tol) { ctrl->done = 1; ctrl->converged = 1; }
+        else if (ctrl->iters > 0 && rel > 0.5 * ctrl->prev) ctrl->done = 1;
+        else if (ctrl->iters >= max_iter) ctrl->done = 1;
+        else ctrl->iters += 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1042,24 +1105,24 @@ int matvec_kslabs(int Np) { return Np >= 4096 ? 32 : (Np >= 1024 ? 16 : 4); }
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
                    double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
-                   double* Pabs, double* norm) {
+                   double* Pabs, double* norm, const int* skip) {
     int nks = matvec_kslabs(Np);
     int kslab = (Np / nks + 63) / 64 * 64;
     nks = (Np + kslab - 1) / kslab;
     dim3 grid(cdiv(Np, MV_ROWS), nks);
     if (Pabs)
         hipLaunchKernelGGL((k_matvec_partial<TM, true>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi,
-                           Pabs);
+                           Pabs, skip);
     else
         hipLaunchKernelGGL((k_matvec_partial<TM, false>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr,
-                           Pi, Pabs);
+                           Pi, Pabs, skip);
     hipLaunchKernelGGL(k_matvec_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, Np, nr, nks, Pr, Pi, Br, Bi,
-                       mode, Yr, Yi);
-    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(256), 0, s, Np, nks, Pabs, norm);
+                       mode, Yr, Yi, skip);
+    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(256), 0, s, Np, nks, Pabs, norm, skip);
 }
 template void launch_matvec<double>(hipStream_t, int, int, const double*, const double*, const double*, const double*,
                                     double*, double*, const double*, const double*, int, double*, double*, double*,
-                                    double*);
+                                    double*, const int*);
 
 template <typename T>
 static void gemm_sub(hipStream_t s, int Np, int M, int N, T* Ar, T* Ai, int k0, int cc, LuProfile* prof, int Nuse) {
@@ -1135,27 +1198,39 @@ template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch) {
-    hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi);
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch,
+                     const int* skip) {
+    hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
     int nblk = Np / LU_NB;
     ++epoch;
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i, Xr,
-                       Xi, counter, flags, epoch);
+                       Xi, counter, flags, epoch, skip);
     ++epoch;
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i, Xr,
-                       Xi, counter, flags, epoch);
+                       Xi, counter, flags, epoch, skip);
 }
 template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*,
                                               const LuWork<double>&, const int*, const int*, double*, double*,
-                                              unsigned*, unsigned*, unsigned&);
+                                              unsigned*, unsigned*, unsigned&, const int*);
 template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const LuWork<float>&,
-                                            const int*, const int*, float*, float*, unsigned*, unsigned*, unsigned&);
+                                            const int*, const int*, float*, float*, unsigned*, unsigned*, unsigned&,
+                                            const int*);
 
-void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb) {
-    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb);
+void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
+    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);
 }
-void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb) {
-    hipLaunchKernelGGL(k_add_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb);
+void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb, const int* skip) {
+    hipLaunchKernelGGL(k_add_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);
+}
+void launch_from_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb) {
+    hipLaunchKernelGGL(k_from_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb);
+}
+void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl) {
+    hipLaunchKernelGGL(k_refine_reset, dim3(1), dim3(64), 0, s, ctrl);
+}
+void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
+                         const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
+    hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(256), 0, s, N, Np, nr, Rr, Ri, Xr, Xi, anorm, ctrl, tol, max_iter);
 }
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb) {
     hipLaunchKernelGGL(k_copy2, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, ca, cb);

@@ -9,8 +9,9 @@ no input data.
 
 A step is one frequency solve (wave assembly + compose, LU, radiation and
 diffraction solves, refinement, A/B/F reductions); steps cycle through the 40
-frequencies. Rankine assembly (once per body) is outside the timed region and
-reported separately. The headline mode is MIXED (FP32-MFMA LU + FP64
+frequencies and are pipelined over --lanes concurrent solve lanes (one C-ABI
+sweep call covers the timed steps). Rankine assembly (once per body) is
+outside the timed region and reported separately. The headline mode is MIXED (FP32-MFMA LU + FP64
 refinement); the FP64 mode is timed in the same run for the ratio, and every
 MIXED result is compared with the FP64 GPU result at the same frequency and
 with the FP64 CPU oracle at the sampled frequency.
@@ -50,6 +51,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequencies of the CPU oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="1k-panel cylinder instead of the 6k semi-sub (quick check)")
+    ap.add_argument("--lanes", type=int, default=4, help="frequencies in flight (concurrent solve lanes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -76,13 +78,13 @@ def main():
     print(f"[bench] N={N} Rankine assembly {t_rankine * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
     def run(mode, steps, warmup):
-        for i in range(warmup):
-            body.solve(omegas[i % len(omegas)], betas=betas, mode=mode)
-        res = []
+        if warmup:
+            body.solve_sweep(omegas[np.arange(warmup) % len(omegas)], betas=betas, mode=mode, lanes=args.lanes)
+        sel = omegas[np.arange(steps) % len(omegas)]
         t0 = time.perf_counter()
-        for i in range(steps):
-            res.append(body.solve(omegas[i % len(omegas)], betas=betas, mode=mode))
+        sw = body.solve_sweep(sel, betas=betas, mode=mode, lanes=args.lanes)
         dt = time.perf_counter() - t0
+        res = [dict(A=sw["A"][i], B=sw["B"][i], F=sw["F"][i], info=sw["info"][i]) for i in range(steps)]
         return dt, res
 
     dt64, r64 = run(bm.MODE_FP64, args.steps, args.warmup)
@@ -154,7 +156,8 @@ def main():
         "dtype": "fp32-lu+fp64-refine",
         "data": "synthetic",
         "config": {"workload": workload, "panels": N, "frequencies": len(omegas), "headings": len(betas),
-                   "dof": 6, "mode": "mixed (FP64 assembly, FP32-MFMA LU, FP64 iterative refinement)"},
+                   "dof": 6, "lanes": args.lanes,
+                   "mode": "mixed (FP64 assembly, FP32-MFMA LU, FP64 iterative refinement)"},
         "fp64_value": args.steps / dt64,
         "speedup_vs_fp64": dt64 / dtmx,
         "max_rel_err_mixed_vs_fp64_gpu": err_mx_vs_64,

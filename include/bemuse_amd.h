@@ -129,6 +129,18 @@ int bem_solve_frequency(bem_ctx* ctx, bem_body* b, int mode, double omega, doubl
                         double* A36, double* B36, double* F, double* FK, double* SC,
                         double* phiJ, double* phiS, bem_solve_info* info);
 
+/* Several frequencies of one body (the loop of BEMUse_Console::Execute,
+ * src/BEMUser_Console.cpp:252-257, each iteration Solve()), pipelined over
+ * `lanes` (1..16) concurrent solve lanes, each with its own HIP streams and
+ * workspaces, so that one frequency's dense MFMA work overlaps another's
+ * latency-bound panel factorisations and triangular solves. kappas may be
+ * NULL (Omega^2 / 9.81). Outputs per frequency f: A36 + 36 f, B36 + 36 f,
+ * F/FK/SC + 12 nb f (row-major 6 x nb complex), infos + f (may be NULL).
+ * Results are identical to nf calls of bem_solve_frequency. */
+int bem_solve_sweep(bem_ctx* ctx, bem_body* b, int mode, int nf, const double* omegas, const double* kappas,
+                    int compose, int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
+                    double* FK, double* SC, bem_solve_info* infos);
+
 /* ---- kernel-level access (parity checks of individual seams) ---------------- */
 /* Rankine matrices as assembled on the device: SSrc = -s, SSrcRefl = -sr,
  * DSrc = d, DSrcRefl = dr (N x N real column-major; NULL skips). */

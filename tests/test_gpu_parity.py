@@ -139,3 +139,22 @@ def test_compose_cases(cyl):
         r = ob.solve(0.0 if compose == bm.COMPOSE_ZERO else -1.0, kappa=0.0, compose=compose, rad_only=True)
         assert relerr(g["A"], r["A"]) < 1e-8
         assert relerr(g["B"] + 1.0, r["B"] + 1.0) < 1e-8
+
+
+@pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
+def test_sweep_matches_single_solves(cyl, mode):
+    """The pipelined sweep (3 lanes) returns the per-frequency results."""
+    mesh, body, ob = cyl
+    omegas = np.linspace(0.2, 1.9, 7)
+    betas = [0.0, 1.0]
+    sw = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=3)
+    tol = 1e-12 if mode == bm.MODE_FP64 else 1e-9
+    for i, w in enumerate(omegas):
+        g = body.solve(w, betas=betas, mode=mode)
+        assert relerr(sw["A"][i], g["A"]) < tol
+        assert relerr(sw["B"][i], g["B"]) < tol
+        assert relerr(sw["F"][i], g["F"]) < tol
+        assert sw["info"][i]["fell_back_fp64"] == 0
+    r = ob.solve(omegas[3], betas=betas)
+    for key in ("A", "B", "F"):
+        assert relerr(sw[key][3], r[key]) < (1e-8 if mode == bm.MODE_FP64 else 1e-6)
