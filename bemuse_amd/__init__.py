@@ -89,6 +89,7 @@ _SIGS = {
     "bem_get_rankine": (c_int, [c_void_p, c_void_p, _PD, _PD, _PD, _PD]),
     "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
+    "bem_gf_coefficients": (c_int, [_PD, c_int]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_solve_sweep": (c_int, [c_void_p, c_void_p, c_int, c_int, _PD, _PD, c_int, c_int, _PD, c_int, _PD, _PD,
@@ -128,6 +129,13 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(_PI) if a is not None else None
+
+
+def gf_coefficients():
+    """The HavelockGF polynomial tables and EUL the device code is built with (65 values)."""
+    out = np.zeros(65)
+    _check(lib().bem_gf_coefficients(_dp(out), 65))
+    return out
 
 
 # ---------------------------------------------------------------------------

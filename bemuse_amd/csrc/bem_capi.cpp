@@ -2,6 +2,7 @@
 // per-frequency solve and the pipelined frequency sweep. Host orchestration
 // only; arithmetic is on the GPU.
 #include "../../include/bemuse_amd.h"
+#include "bem_device.hpp"
 #include "bem_internal.h"
 #include "bem_kernels.h"
 
@@ -718,6 +719,20 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
         if (ev) (void)hipEventDestroy(ev);
     delete prof;
     if (e != hipSuccess) return fail(BEM_ERR_HIP, std::string("bem_profile_lu: ") + hipGetErrorString(e));
+    return BEM_OK;
+}
+
+int bem_gf_coefficients(double* out, int n) {
+    if (!out || n < 65) return fail(BEM_ERR_ARG, "bem_gf_coefficients: need room for 65 values");
+    int k = 0;
+    for (double v : dev::kGF.A) out[k++] = v;
+    for (double v : dev::kGF.B) out[k++] = v;
+    for (double v : dev::kGF.C) out[k++] = v;
+    for (double v : dev::kGF.D) out[k++] = v;
+    for (double v : dev::kGF.dA) out[k++] = v;
+    for (double v : dev::kGF.dB) out[k++] = v;
+    for (double v : dev::kGF.dC) out[k++] = v;
+    out[k++] = dev::kEUL;
     return BEM_OK;
 }
 

@@ -706,66 +706,150 @@ __global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int
 // operands in one column-major array with leading dimension ld.
 // A: M x K at (ra, ca), B: K x N at (rb, cb), C: M x N at (rc, cc); M, N
 // multiples of 64, K multiple of 16.
+//
+// Chunks of 16 k of the A and B tiles are staged in LDS (double buffered:
+// the global loads of chunk c+1 are in flight while chunk c feeds the MFMAs)
+// with 16-byte coalesced loads; both are stored k-contiguous so every MFMA
+// operand quad (4 consecutive k) is one LDS read. 32 KB (FP32) / 32 KB (FP64)
+// of LDS per stage lets two work-groups share a CU.
 // ---------------------------------------------------------------------------
+constexpr int GK = 16;           // k per LDS chunk
+constexpr int GKP = GK + 4;      // padded row length in LDS (bank spread)
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float4 type; };
+template <> struct Vec4<double> { typedef double4 type; };
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_cgemm_sub(int M, int N, int K, T* __restrict__ Pr, T* __restrict__ Pi, int ld,
-                                                   int ra, int ca, int rb, int cb, int rc, int cc) {
+__global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __restrict__ Pr, T* __restrict__ Pi,
+                                                      int ld, int ra, int ca, int rb, int cb, int rc, int cc) {
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
+    typedef typename Vec4<T>::type v4;
     constexpr int WM = MT::WM, WN = MT::WN;
     constexpr int TM = 32 * WM, TN = 32 * WN;      // work-group tile
-    int tiles_m = M / TM + (M % TM ? 1 : 0);
-    int tiles_n = N / TN + (N % TN ? 1 : 0);
+    constexpr int NLA = TM * GK / 4 / 256;          // 4-element A loads per thread per chunk (re or im)
+    constexpr int NLB = TN * GK / 4 / 256;
+    __shared__ T As[2][2][TM][GKP];                 // [buf][re/im][row][k]
+    __shared__ T Bs[2][2][TN][GKP];                 // [buf][re/im][col][k]
+    int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
     int ntiles = tiles_m * tiles_n;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so
     // give each XCD a contiguous run of tiles (row-major), sharing A rows in L2.
     int bid = blockIdx.x;
     int per = (ntiles + 7) / 8;
-    int xcd = bid % 8, loc = bid / 8;
-    int tile = xcd * per + loc;
+    int tile = (bid % 8) * per + bid / 8;
     if (tile >= ntiles) return;
     int tm = tile / tiles_n, tn = tile % tiles_n;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int wm = wave >> 1, wn = wave & 1;
-    int row0 = tm * TM + wm * 16 * WM, col0 = tn * TN + wn * 16 * WN;
-    if (row0 >= M || col0 >= N) return;
+    int rowt = tm * TM, colt = tn * TN;
+    int mrows = min(TM, M - rowt), ncols = min(TN, N - colt);
+    const size_t ldl = (size_t)ld;
+
+    v4 ga[2][NLA], gb[2][NLB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int l = 0; l < NLA; l++) {
+            int idx = tid + 256 * l;
+            int kk = idx / (TM / 4), rq = idx % (TM / 4);
+            int r = min(4 * rq, mrows - 4);
+            size_t o = (size_t)(ca + k0 + kk) * ldl + ra + rowt + r;
+            ga[0][l] = *reinterpret_cast<const v4*>(Pr + o);
+            ga[1][l] = *reinterpret_cast<const v4*>(Pi + o);
+        }
+#pragma unroll
+        for (int l = 0; l < NLB; l++) {
+            int idx = tid + 256 * l;
+            int c = idx / (GK / 4), kq = idx % (GK / 4);
+            int cl = min(c, ncols - 1);
+            size_t o = (size_t)(cb + colt + cl) * ldl + rb + k0 + 4 * kq;
+            gb[0][l] = *reinterpret_cast<const v4*>(Pr + o);
+            gb[1][l] = *reinterpret_cast<const v4*>(Pi + o);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int l = 0; l < NLA; l++) {
+            int idx = tid + 256 * l;
+            int kk = idx / (TM / 4), rq = idx % (TM / 4);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                As[buf][p][4 * rq + 0][kk] = ga[p][l].x;
+                As[buf][p][4 * rq + 1][kk] = ga[p][l].y;
+                As[buf][p][4 * rq + 2][kk] = ga[p][l].z;
+                As[buf][p][4 * rq + 3][kk] = ga[p][l].w;
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < NLB; l++) {
+            int idx = tid + 256 * l;
+            int c = idx / (GK / 4), kq = idx % (GK / 4);
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                Bs[buf][p][c][4 * kq + 0] = gb[p][l].x;
+                Bs[buf][p][c][4 * kq + 1] = gb[p][l].y;
+                Bs[buf][p][c][4 * kq + 2] = gb[p][l].z;
+                Bs[buf][p][c][4 * kq + 3] = gb[p][l].w;
+            }
+        }
+    };
+
     int i = lane & 15, g = lane >> 4;
+    int row0 = wm * 16 * WM, col0 = wn * 16 * WN;           // within the tile
+    bool active = (row0 < mrows) && (col0 < ncols);
     acc_t cr[WM][WN], ci[WM][WN];
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
         for (int b = 0; b < WN; b++) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
-    const size_t ldl = (size_t)ld;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        T br[WN][4], bi[WN][4];
-#pragma unroll
-        for (int b = 0; b < WN; b++) {
-            size_t off = (size_t)(cb + col0 + 16 * b + i) * ldl + rb + k0 + 4 * g;
-#pragma unroll
-            for (int t = 0; t < 4; t++) { br[b][t] = Pr[off + t]; bi[b][t] = Pi[off + t]; }
-        }
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            T ar[WM], ai[WM];
-            size_t offa = (size_t)(ca + k0 + 4 * g + t) * ldl + ra + row0 + i;
-#pragma unroll
-            for (int a = 0; a < WM; a++) { ar[a] = Pr[offa + 16 * a]; ai[a] = Pi[offa + 16 * a]; }
+
+    int nchunk = K / GK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+        int buf = c & 1;
+        if (c + 1 < nchunk) gload((c + 1) * GK);
+        if (active) {
+            T ar[WM][4], ai[WM][4], br[WN][4], bi[WN][4];
 #pragma unroll
             for (int a = 0; a < WM; a++)
 #pragma unroll
-                for (int b = 0; b < WN; b++) {
-                    cr[a][b] = MT::mma(ar[a], br[b][t], cr[a][b]);
-                    cr[a][b] = MT::mma(-ai[a], bi[b][t], cr[a][b]);
-                    ci[a][b] = MT::mma(ar[a], bi[b][t], ci[a][b]);
-                    ci[a][b] = MT::mma(ai[a], br[b][t], ci[a][b]);
+                for (int t = 0; t < 4; t++) {
+                    ar[a][t] = As[buf][0][row0 + 16 * a + i][4 * g + t];
+                    ai[a][t] = As[buf][1][row0 + 16 * a + i][4 * g + t];
                 }
+#pragma unroll
+            for (int b = 0; b < WN; b++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    br[b][t] = Bs[buf][0][col0 + 16 * b + i][4 * g + t];
+                    bi[b][t] = Bs[buf][1][col0 + 16 * b + i][4 * g + t];
+                }
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) {
+                        cr[a][b] = MT::mma(ar[a][t], br[b][t], cr[a][b]);
+                        cr[a][b] = MT::mma(-ai[a][t], bi[b][t], cr[a][b]);
+                        ci[a][b] = MT::mma(ar[a][t], bi[b][t], ci[a][b]);
+                        ci[a][b] = MT::mma(ai[a][t], br[b][t], ci[a][b]);
+                    }
+        }
+        if (c + 1 < nchunk) {
+            sstore(buf ^ 1);
+            __syncthreads();
         }
     }
+    if (!active) return;
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
         for (int b = 0; b < WN; b++) {
-            size_t off = (size_t)(cc + col0 + 16 * b + i) * ldl + rc + row0 + 16 * a;
+            size_t off = (size_t)(cc + colt + col0 + 16 * b + i) * ldl + rc + rowt + row0 + 16 * a;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int rr = MT::row(g, r);

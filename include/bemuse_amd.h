@@ -149,9 +149,14 @@ int bem_get_rankine(bem_ctx* ctx, bem_body* b, double* Ssrc, double* Srefl, doub
  * as produced by the device assembly in the given mode. */
 int bem_get_system(bem_ctx* ctx, bem_body* b, int mode, double kappa, int compose, double* SMat, double* DMat);
 /* Device evaluation of HavelockGF (Kernels.cpp:342-412) at n points;
- * out: 8 doubles per point (GF, DGFX, DGFY, DGFZ as re, im). precision 64 or 32. */
+ * out: 8 doubles per point (GF, DGFX, DGFY, DGFZ as re, im). precision: 64. */
 int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const double* Y,
                       const double* V, double* out);
+/* The constant tables the device HavelockGF is compiled with, for checking
+ * against the reference literals without a GPU: the coefficients of
+ * GF_FuncA..D and GF_dFuncA..C (Kernels.cpp:120-256) in that order (64
+ * values), then EUL (BEMUse_Math_Types.h:99). n must be >= 65. */
+int bem_gf_coefficients(double* out, int n);
 /* Device LU + solve of a caller matrix (complex n x n, column-major):
  * precision 64 (FP64 LU) or 32 (FP32-MFMA LU + FP64 refinement); X receives
  * the solution of A X = B for nrhs columns. */

@@ -96,7 +96,8 @@ def havelock(X, Y, V):
 
 
 def poly_coeffs():
-    out = np.zeros(64)
+    """The GF polynomial tables cA, cB, cC, cD, cdA, cdB, cdC (64 values) then EUL."""
+    out = np.zeros(65)
     lib().orc_poly_coeffs(_dp(out))
     return out
 
@@ -134,6 +135,9 @@ class Body:
             self._h = None
 
     def geometry(self):
+        """Body panel centroid, normal, area, BC point and frame; the q*
+        fields (nq, qpos, qnrm, qw) are the MIRRORED panel's quadrature nodes
+        (the ones the wave term integrates over)."""
         N = self.N
         cen = np.zeros((N, 3)); nrm = np.zeros((N, 3)); area = np.zeros(N); bc = np.zeros((N, 3))
         nq = np.zeros(N, dtype=np.int32); qpos = np.zeros((N, 4, 3)); qnrm = np.zeros((N, 4, 3)); qw = np.zeros((N, 4))
@@ -143,6 +147,7 @@ class Body:
         return dict(centroid=cen, normal=nrm, area=area, bc=bc, nq=nq, qpos=qpos, qnrm=qnrm, qw=qw, frame=frame)
 
     def body_qnodes(self):
+        """Quadrature nodes of the body panels."""
         N = self.N
         nq = np.zeros(N, dtype=np.int32); qpos = np.zeros((N, 4, 3)); qnrm = np.zeros((N, 4, 3)); qw = np.zeros((N, 4))
         lib().orc_body_qnodes(self._h, nq.ctypes.data_as(_PI), _dp(qpos), _dp(qnrm), _dp(qw))

@@ -471,7 +471,7 @@ void orc_havelock(int n, const double* X, const double* Y, const double* V, doub
     }
 }
 
-void orc_poly_coeffs(double* out /*64*/) {
+void orc_poly_coeffs(double* out /*65: 64 coefficients, then EUL*/) {
     int k = 0;
     for (double c : cA) out[k++] = c;
     for (double c : cB) out[k++] = c;

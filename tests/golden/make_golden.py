@@ -1,0 +1,94 @@
+"""Writes the golden fixtures under tests/golden/ (run in a checkout that has
+the reference sources; the fixtures are committed, the script is not needed
+at test time).
+
+gf_poly_coeffs.json
+    The numeric literals of the seven polynomials of the Havelock global
+    approximation (GF_FuncA..D, GF_dFuncA..C, src/Kernels.cpp:120-256) and
+    the Euler constant literal EUL (src/BEMUse_Math_Types.h:99), parsed from
+    the reference text by a tokeniser (the running-power statements
+    `S = c0` / `S += c*t;`; commented-out lines are skipped). The oracle's
+    hand-typed tables are compared with these.
+
+cylinder_abf.npz
+    A, B, F of the oracle for the small cylinder (r = 5 m, draft 8 m,
+    4 x 8 x 16 panels, N = 192) at omega = 0.4 and 1.3 rad/s, headings 0 and
+    45 deg. A regression fixture of the oracle itself (the checks that pin the
+    oracle live in tests/test_oracle.py); the GPU parity tests compare with
+    it as well as with the live oracle.
+
+Usage: python tests/golden/make_golden.py [--reference /root/reference]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+NUM = r"[-+]?\s*(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?"
+
+
+def parse_poly(text, name):
+    m = re.search(r"inline\s+Real\s+" + name + r"\s*\(const Real &tt\)\s*\{(.*?)\n\}", text, re.S)
+    if not m:
+        raise RuntimeError(f"{name} not found")
+    coeffs = []
+    for line in m.group(1).splitlines():
+        line = line.split("//", 1)[0].strip()
+        if not line:
+            continue
+        a = re.match(r"Real\s+S\s*=\s*(" + NUM + r")\s*,", line)
+        if a:
+            coeffs.append(float(a.group(1).replace(" ", "")))
+            continue
+        b = re.match(r"S\s*\+=\s*(" + NUM + r")\s*\*\s*t\s*;", line)
+        if b:
+            coeffs.append(float(b.group(1).replace(" ", "")))
+    if not coeffs:
+        raise RuntimeError(f"{name}: no coefficients parsed")
+    return coeffs
+
+
+def poly_fixture(ref):
+    text = open(os.path.join(ref, "src", "Kernels.cpp")).read()
+    out = {}
+    for nm in ("GF_FuncA", "GF_FuncB", "GF_FuncC", "GF_FuncD", "GF_dFuncA", "GF_dFuncB", "GF_dFuncC"):
+        out[nm] = parse_poly(text, nm)
+    mt = open(os.path.join(ref, "src", "BEMUse_Math_Types.h")).read()
+    m = re.search(r"EUL\s*=\s*(" + NUM + ")", mt)
+    out["EUL"] = float(m.group(1))
+    out["_source"] = "src/Kernels.cpp:120-256 (GF_Func*/GF_dFunc* literals), src/BEMUse_Math_Types.h:99 (EUL)"
+    with open(os.path.join(HERE, "gf_poly_coeffs.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote gf_poly_coeffs.json:", {k: len(v) for k, v in out.items() if isinstance(v, list)})
+
+
+def cylinder_fixture():
+    sys.path.insert(0, ROOT)
+    import oracle
+    import bemuse_amd as bm
+    mesh = bm.Mesh.half_cylinder(5.0, 8.0, 4, 8, 16)
+    nodes, elems = mesh.arrays()
+    ob = oracle.Body(nodes, elems)
+    omegas = np.array([0.4, 1.3])
+    betas = np.array([0.0, np.pi / 4])
+    A, B, F = [], [], []
+    for w in omegas:
+        r = ob.solve(w, betas=betas)
+        A.append(r["A"]); B.append(r["B"]); F.append(r["F"])
+    np.savez(os.path.join(HERE, "cylinder_abf.npz"), nodes=nodes, elems=elems, omegas=omegas, betas=betas,
+             A=np.array(A), B=np.array(B), F=np.array(F))
+    print("wrote cylinder_abf.npz N =", len(elems))
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reference", default="/root/reference")
+    args = ap.parse_args()
+    poly_fixture(args.reference)
+    cylinder_fixture()

@@ -57,3 +57,47 @@ def test_context_without_gpu_fails_cleanly():
         pytest.skip("GPU present")
     with pytest.raises(bm.BemError):
         bm.Context(0)
+
+
+def test_product_does_not_use_the_oracle():
+    """The library and its Python binding never load the oracle (it is the
+    checker only)."""
+    pkg = os.path.join(ROOT, "bemuse_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h", ".hpp")) or f == "Makefile":
+                text = open(os.path.join(dirpath, f)).read()
+                hit = re.search(r"import\s+oracle|from\s+oracle|liboracle|\borc_\w+\(", text)
+                assert not hit, (os.path.join(dirpath, f), hit.group(0))
+
+
+def test_host_geometry_matches_oracle():
+    """The library's host precompute (centroids, frames, quadrature, BC
+    points, generalised normals) equals the oracle's panel geometry."""
+    import oracle
+    mesh = bm.Mesh.half_cylinder(5.0, 8.0, 4, 8, 16)
+    nodes, elems = mesh.arrays()
+    ob = oracle.Body(nodes, elems)
+    og = ob.geometry()
+    oq = ob.body_qnodes()
+    pg = mesh.geometry()
+    body, refl, quad, recv, gen = pg["body"], pg["refl"], pg["quad"], pg["recv"], pg["gen"]
+    np.testing.assert_allclose(body[:, 0:3], og["centroid"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(body[:, 9:12], og["normal"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(body[:, 29], og["area"], rtol=1e-15)
+    np.testing.assert_allclose(recv[:, 0:3], og["bc"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(body[:, 3:6], og["frame"][:, 0], atol=1e-14)
+    np.testing.assert_allclose(body[:, 6:9], og["frame"][:, 1], atol=1e-14)
+    for e in range(len(elems)):
+        nq = int(quad[e, 28])
+        assert nq == og["nq"][e]
+        for q in range(nq):
+            rec = quad[e, 7 * q: 7 * q + 7]
+            np.testing.assert_allclose(rec[0:3], og["qpos"][e, q], atol=1e-14)
+            np.testing.assert_allclose(rec[3:6], og["qnrm"][e, q], atol=1e-14)
+            np.testing.assert_allclose(rec[6], og["qw"][e, q], rtol=1e-14)
+    # generalised normals: n and x_c x n at centroids, then the area
+    c, n = og["centroid"], og["normal"]
+    np.testing.assert_allclose(gen[:, 0:3], n, atol=1e-14)
+    np.testing.assert_allclose(gen[:, 3:6], np.cross(c, n), atol=1e-13)
+    np.testing.assert_allclose(gen[:, 6], og["area"], rtol=1e-15)

@@ -158,3 +158,17 @@ def test_sweep_matches_single_solves(cyl, mode):
     r = ob.solve(omegas[3], betas=betas)
     for key in ("A", "B", "F"):
         assert relerr(sw[key][3], r[key]) < (1e-8 if mode == bm.MODE_FP64 else 1e-6)
+
+
+def test_solve_vs_committed_fixture(ctx):
+    """A, B, F against the committed oracle fixture (tests/golden/cylinder_abf.npz)."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cylinder_abf.npz"))
+    body = bm.Body(ctx, bm.Mesh.from_arrays(fx["nodes"], fx["elems"]))
+    body.assemble_rankine()
+    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+        sw = body.solve_sweep(fx["omegas"], betas=fx["betas"], mode=mode, lanes=2)
+        for k in ("A", "B", "F"):
+            for i in range(len(fx["omegas"])):
+                assert relerr(sw[k][i], fx[k][i]) < tol, (mode, k, i)
+    body.close()
