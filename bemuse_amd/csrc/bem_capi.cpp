@@ -95,7 +95,8 @@ struct LinSys {
         int r = BEM_OK;
         if ((r = get(&w.sd_r, kLuNb * kLuNb)) || (r = get(&w.sd_i, kLuNb * kLuNb)) || (r = get(&w.s21_r, pan)) ||
             (r = get(&w.s21_i, pan)) || (r = get(&w.Linv_r, pan)) || (r = get(&w.Linv_i, pan)) ||
-            (r = get(&w.Uinv_r, pan)) || (r = get(&w.Uinv_i, pan)))
+            (r = get(&w.Uinv_r, pan)) || (r = get(&w.Uinv_i, pan)) || (r = get(&w.PL_r, pan)) ||
+            (r = get(&w.PL_i, pan)) || (r = get(&w.PU_r, pan)) || (r = get(&w.PU_i, pan)))
             return r;
         return BEM_OK;
     }
@@ -141,7 +142,8 @@ struct Lane {
     std::vector<hipEvent_t> lu_events;
     hipEvent_t t[4] = {};                      // timing
     hipEvent_t done = nullptr;
-    unsigned *counter = nullptr, *flags = nullptr;
+    unsigned* counter = nullptr;
+    unsigned long long* pub = nullptr;         // triangular-solve publication words
     unsigned epoch = 0;
     RefineCtrl* ctrl = nullptr;
     LaneOut* hout = nullptr;                   // pinned
@@ -185,9 +187,10 @@ struct Lane {
         if ((r = sys.init(N))) return r;
         int Np = sys.Np;
         int nblk = Np / kLuNb;
-        if ((r = get(&counter, 1)) || (r = get(&flags, nblk)) || (r = get(&ctrl, 1)) || (r = get(&dout, 1))) return r;
+        size_t npub = lu_solve_pub_words(Np);
+        if ((r = get(&counter, 1)) || (r = get(&pub, npub)) || (r = get(&ctrl, 1)) || (r = get(&dout, 1))) return r;
         HIP_TRY(hipMemset(counter, 0, sizeof(unsigned)));
-        HIP_TRY(hipMemset(flags, 0, nblk * sizeof(unsigned)));
+        HIP_TRY(hipMemset(pub, 0, npub * sizeof(unsigned long long)));
         HIP_TRY(hipHostMalloc((void**)&hout, sizeof(LaneOut), hipHostMallocDefault));
         size_t NN = (size_t)Np * Np, NR = (size_t)Np * kMaxRhs;
         if ((r = get(&Sr, NN)) || (r = get(&Si, NN))) return r;
@@ -247,7 +250,7 @@ void solve_system(Lane& L, int precision, int nr) {
     if (precision == 64) {
         launch_copy2(L.s1, n, s.Br, s.Bi, s.Xr, s.Xi);
         launch_lu_solve<double, double>(L.s1, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
-                                        L.counter, L.flags, L.epoch);
+                                        L.counter, L.pub, L.epoch);
         return;
     }
     const int* fany = s.flag32 + s.Np / kLuNb;
@@ -255,7 +258,7 @@ void solve_system(Lane& L, int precision, int nr) {
     launch_refine_reset(L.s1, L.ctrl);
     launch_to_f32(L.s1, n, s.Br, s.Bi, s.X32r, s.X32i);
     launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, L.counter,
-                                  L.flags, L.epoch);
+                                  L.pub, L.epoch);
     launch_from_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i);
     for (int it = 0; it <= kRefineMaxIter; it++) {
         // r = b - A x in FP64; the first residual also yields |A|_inf
@@ -267,7 +270,7 @@ void solve_system(Lane& L, int precision, int nr) {
         // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
         launch_to_f32(L.s1, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
         launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                      L.counter, L.flags, L.epoch, skip);
+                                      L.counter, L.pub, L.epoch, skip);
         launch_add_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
     }
 }

@@ -45,6 +45,7 @@ template <typename T>
 struct LuWork {
     T *sd_r = nullptr, *sd_i = nullptr, *s21_r = nullptr, *s21_i = nullptr;
     T *Linv_r = nullptr, *Linv_i = nullptr, *Uinv_r = nullptr, *Uinv_i = nullptr;
+    T *PL_r = nullptr, *PL_i = nullptr, *PU_r = nullptr, *PU_i = nullptr;   // block-solve coefficients
 };
 
 // s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
@@ -53,8 +54,10 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                int* ipiv, LuProfile* prof = nullptr);
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch,
-                     const int* skip = nullptr);
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned long long* pub,
+                     unsigned& epoch, const int* skip = nullptr);
+// Words of the triangular-solve publication buffer (enough for FP64).
+inline size_t lu_solve_pub_words(int Np) { return (size_t)(Np / kLuNb) * kMaxRhs * kLuNb * 2 * 2; }
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb,
                    const int* skip = nullptr);
 void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb,

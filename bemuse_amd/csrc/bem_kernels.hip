@@ -861,30 +861,166 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 
 // ---------------------------------------------------------------------------
 // Triangular solves with NB-row blocks, one persistent launch per solve.
-// Work-groups take blocks in dependency order from a ticket counter, so a
-// block only ever waits for blocks whose work-groups are already running.
-// Block b accumulates sum_c M_bc X_c over the blocks it depends on (each wave
-// a quarter of every block's columns, waiting on that block's epoch flag),
-// then X_b = Inv_b (B_b - sum), with Inv_b the stored inverse of the
-// diagonal triangle, and publishes X_b.
-// lower = 1: L (unit diagonal) forward with Linv; lower = 0: U backward with Uinv.
-// X is Np x nr planar (ld Np), type TX; the factor has type T.
+//
+// Block form of the lower solve (the upper one mirrors it with c > b):
+//   X_b = Inv_b (B_b - sum_{c<b} M_bc X_c) = Z_b - P_b X_{b-1},
+//   Z_b = Inv_b (B_b - sum_{c<b-1} M_bc X_c),   P_b = Inv_b M_{b,b-1},
+// with Inv_b the stored inverse of the diagonal triangle. P_b is formed once
+// per factorisation (k_trsm_pcoef), and a work-group computes Z_b while the
+// blocks before it are still being solved, so once X_{b-1} is out a single
+// 64 x 64 by 64 x nr product stands between it and X_b.
+//
+// Publication: X_b goes to X (for the kernels after the solve) and to `pub`,
+// where every 32 bits of a value travel in one 64-bit word tagged with the
+// solve's epoch (value bits | epoch << 32), stored with agent-scope relaxed
+// atomics. Readers poll exactly the words they need with agent-scope relaxed
+// loads until all carry the current epoch, so no flag, release fence or L2
+// write-back is involved, and the buffer never needs clearing (each solve
+// call takes a new epoch). Work-groups take blocks in dependency order from a
+// ticket counter, so a block only waits for blocks whose work-groups are
+// already running.
 // ---------------------------------------------------------------------------
+template <typename TX> struct PubWords;
+template <> struct PubWords<float> { static constexpr int W = 1; };
+template <> struct PubWords<double> { static constexpr int W = 2; };
+
+__device__ inline void pub_store(unsigned long long* p, float v, unsigned epoch) {
+    unsigned long long w = ((unsigned long long)epoch << 32) | (unsigned long long)__float_as_uint(v);
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void pub_store(unsigned long long* p, double v, unsigned epoch) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(v), e = (unsigned long long)epoch << 32;
+    __hip_atomic_store(p, e | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, e | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// One published value; false while any of its words is from an older epoch.
+__device__ inline bool pub_load(unsigned long long* p, unsigned epoch, float& v) {
+    unsigned long long w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __uint_as_float((unsigned)w);
+    return (unsigned)(w >> 32) == epoch;
+}
+__device__ inline bool pub_load(unsigned long long* p, unsigned epoch, double& v) {
+    unsigned long long h = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long l = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __longlong_as_double((long long)((h << 32) | (l & 0xffffffffull)));
+    return (unsigned)(h >> 32) == epoch && (unsigned)(l >> 32) == epoch;
+}
+// Spins (one lane) until the word at p carries the epoch.
+__device__ inline void pub_wait(unsigned long long* p, unsigned epoch) {
+    while ((unsigned)(__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32) != epoch)
+        __builtin_amdgcn_s_sleep(2);
+}
+// word offset of element (row r, rhs j, part p: 0 re / 1 im) of block c
+template <typename TX>
+__device__ inline size_t pub_off(int c, int j, int r, int p) {
+    return ((((size_t)c * MV_MAXR + j) * LU_NB + r) * 2 + p) * PubWords<TX>::W;
+}
+
+// P coefficients of the block solves: PL_b = Linv_b L_{b,b-1} (b >= 1) and
+// PU_b = Uinv_b U_{b,b+1} (b <= nblk-2), 64 x 64 column-major per block,
+// from the factor in place (L strictly below, U on and above the diagonal).
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_pcoef(int Np, const T* __restrict__ Ar, const T* __restrict__ Ai,
+                                                    const T* Li_r, const T* Li_i, const T* Ui_r, const T* Ui_i,
+                                                    T* PL_r, T* PL_i, T* PU_r, T* PU_i) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
+    int b = blockIdx.x, nblk = Np / LU_NB;
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, i = lane & 15, g = lane >> 4;
+    for (int which = 0; which < 2; which++) {
+        int c = which == 0 ? b - 1 : b + 1;
+        if (c < 0 || c >= nblk) continue;
+        const T* Ir = (which == 0 ? Li_r : Ui_r) + (size_t)b * NB2;
+        const T* Ii = (which == 0 ? Li_i : Ui_i) + (size_t)b * NB2;
+        T* Or = (which == 0 ? PL_r : PU_r) + (size_t)b * NB2;
+        T* Oi = (which == 0 ? PL_i : PU_i) + (size_t)b * NB2;
+        for (int tj = 0; tj < 4; tj++) {
+            acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0};
+            for (int kc = 0; kc < 4; kc++) {
+                size_t ao = (size_t)(kc * 16) * LU_NB + wave * 16;
+                size_t bo = (size_t)(c * LU_NB + tj * 16) * Np + b * LU_NB + kc * 16;
+                cmma16<T>(Ir + ao, Ii + ao, LU_NB, Ar + bo, Ai + bo, (size_t)Np, cr, ci, i, g);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                size_t o = (size_t)(tj * 16 + i) * LU_NB + wave * 16 + MT::row(g, r);
+                Or[o] = cr[r];
+                Oi[o] = ci[r];
+            }
+        }
+    }
+}
+
+// lower = 1: L (unit diagonal) forward with Inv = Linv, P = PL; lower = 0: U
+// backward with Inv = Uinv, P = PU. X is Np x nr planar (ld Np); the factor
+// and X have the same type T.
+//
+// All products run on MFMA with wave w owning rows 16w..16w+15 of the block
+// and lane column i = right-hand side: the accumulator starts at B_b, every
+// dependency c other than `prev` subtracts M_bc X_c, then Z = Inv_b y and
+// X_b = Z - P_b X_prev. A block of X is staged once per work-group into LDS
+// (one batched poll: all loads are issued before any epoch is checked) and
+// each lane then reads its 16 operands in one burst.
+template <typename T>
+__device__ inline void cmac(typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci, T ar, T ai, T br, T bi) {
+    typedef MfmaT<T> MT;
+    cr = MT::mma(ar, br, cr);
+    cr = MT::mma(-ai, bi, cr);
+    ci = MT::mma(ar, bi, ci);
+    ci = MT::mma(ai, br, ci);
+}
+
+// Stages block c of the published X (rows 0..63, right-hand sides < nr) into
+// sr/si [rhs][row]; returns once every value carries the epoch.
+template <typename TX>
+__device__ inline void stage_block(unsigned long long* pub, unsigned epoch, int c, int nr, int tid, int lane,
+                                   TX (*sr)[LU_NB + 1], TX (*si)[LU_NB + 1]) {
+    constexpr int W = PubWords<TX>::W;
+    for (;;) {
+        TX v[8];
+        int ok = 1;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {          // value t: rhs t / 128, row (t % 128) / 2, part t % 2
+            int t = tid + 256 * q;
+            v[q] = 0;
+            if (t < 128 * nr) {
+                size_t o = ((size_t)(c * MV_MAXR + (t >> 7)) * LU_NB) * 2 * W + (size_t)(t & 127) * W;
+                ok &= pub_load(pub + o, epoch, v[q]) ? 1 : 0;
+            }
+        }
+        if (__all(ok)) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                int t = tid + 256 * q;
+                if (t < 128 * nr) {
+                    if (t & 1) si[t >> 7][(t & 127) >> 1] = v[q];
+                    else sr[t >> 7][(t & 127) >> 1] = v[q];
+                }
+            }
+            return;
+        }
+        // not out yet: one lane waits on a single word of the block before the
+        // batch is read again (every waiting work-group polling every word of
+        // a block congests its memory channel)
+        if (lane == 0) pub_wait(pub + ((size_t)(c * MV_MAXR) * LU_NB) * 2 * W, epoch);
+    }
+}
+
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower, const T* __restrict__ Ar,
                                                       const T* __restrict__ Ai, const T* __restrict__ Inv_r,
-                                                      const T* __restrict__ Inv_i, TX* Xr, TX* Xi, unsigned* counter,
-                                                      unsigned* flags, unsigned epoch, const int* skip) {
+                                                      const T* __restrict__ Inv_i, const T* __restrict__ P_r,
+                                                      const T* __restrict__ P_i, TX* Xr, TX* Xi,
+                                                      unsigned long long* pub, unsigned* counter, unsigned epoch,
+                                                      const int* skip) {
+    static_assert(sizeof(T) == sizeof(TX), "factor and solution share one precision");
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
     if (skip && *skip) return;   // uniform: no work-group takes a ticket
-    // cross-wave partial sums (after the dependency loop) share storage with
-    // the per-wave staging of X_c rows (during it)
-    __shared__ TX shared_buf[4 * MV_MAXR * 2 * LU_NB];
-    TX (*red)[MV_MAXR][2][LU_NB] = reinterpret_cast<TX (*)[MV_MAXR][2][LU_NB]>(shared_buf);
-    TX (*xw)[16][MV_MAXR][2] = reinterpret_cast<TX (*)[16][MV_MAXR][2]>(shared_buf);
-    __shared__ TX yr[MV_MAXR][LU_NB], yi[MV_MAXR][LU_NB];
-    __shared__ T ivr[LU_NB][LU_NB + 1], ivi[LU_NB][LU_NB + 1];   // Inv_b [col][row]
+    __shared__ TX sr[MV_MAXR][LU_NB + 1], si[MV_MAXR][LU_NB + 1];   // a staged block of X / y  [rhs][row]
     __shared__ unsigned ticket;
     int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int i = lane & 15, g = lane >> 4;
     int nblk = Np / LU_NB;
     if (tid == 0) {
         ticket = atomicAdd(counter, 1u);
@@ -892,92 +1028,90 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
     }
     __syncthreads();
     int b = lower ? (int)ticket : nblk - 1 - (int)ticket;
-    int r0 = b * LU_NB;
-    // the inverse of the diagonal triangle is ready before the launch
-    for (int t = tid; t < NB2; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        ivr[c][r] = Inv_r[(size_t)b * NB2 + t]; ivi[c][r] = Inv_i[(size_t)b * NB2 + t];
-    }
-    TX ar[MV_MAXR], ai[MV_MAXR];
+    int r0 = b * LU_NB, w0 = r0 + wave * 16;
+    int prev = lower ? b - 1 : b + 1;               // the block whose X finishes X_b
+    bool has_prev = prev >= 0 && prev < nblk;
+    // accumulator = B_b (rows of this wave's tile, column i)
+    acc_t cr, ci;
 #pragma unroll
-    for (int j = 0; j < MV_MAXR; j++) { ar[j] = 0; ai[j] = 0; }
-    int ndep = lower ? b : nblk - 1 - b;
+    for (int r = 0; r < 4; r++) {
+        size_t x = (size_t)i * Np + w0 + MT::row(g, r);
+        cr[r] = i < nr ? Xr[x] : (TX)0;
+        ci[r] = i < nr ? Xi[x] : (TX)0;
+    }
+    int ndep = (lower ? b : nblk - 1 - b) - 1;      // all dependencies but `prev`
     for (int d = 0; d < ndep; d++) {
         int c = lower ? d : nblk - 1 - d;
-        int c0 = c * LU_NB + wave * 16;
-        T mr[16], mi[16];
+        T ma_r[16], ma_i[16];                       // -M_bc: A(i, k = 4s + g)
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            size_t gidx = (size_t)(c0 + u) * Np + r0 + lane;
-            mr[u] = Ar[gidx]; mi[u] = Ai[gidx];
+        for (int s = 0; s < 16; s++) {
+            size_t o = (size_t)(c * LU_NB + 4 * s + g) * Np + w0 + i;
+            ma_r[s] = -Ar[o]; ma_i[s] = -Ai[o];
         }
-        // Block c is published with device-scope stores of X_c followed by a
-        // release of flag[c]. Poll the flag and read X_c with device-scope
-        // (L2-bypassing) relaxed loads: an acquire here would invalidate the
-        // whole L2 of the XCD for every dependency.
-        if (lane == 0) {
-            while (__hip_atomic_load(flags + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch)
-                __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_wave_barrier();
-        for (int t = lane; t < 16 * nr; t += 64) {
-            int u = t % 16, j = t / 16;
-            xw[wave][u][j][0] = __hip_atomic_load(Xr + (size_t)j * Np + c0 + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            xw[wave][u][j][1] = __hip_atomic_load(Xi + (size_t)j * Np + c0 + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();                            // the previous block's operands are read
+        stage_block<TX>(pub, epoch, c, nr, tid, lane, sr, si);
+        __syncthreads();
+        T xb_r[16], xb_i[16];
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            TX m_r = (TX)mr[u], m_i = (TX)mi[u];
-#pragma unroll
-            for (int j = 0; j < MV_MAXR; j++) {
-                if (j < nr) {
-                    TX vr = xw[wave][u][j][0], vi = xw[wave][u][j][1];
-                    ar[j] += m_r * vr - m_i * vi;
-                    ai[j] += m_r * vi + m_i * vr;
-                }
-            }
+        for (int s = 0; s < 16; s++) {
+            xb_r[s] = i < nr ? sr[i][4 * s + g] : (TX)0;
+            xb_i[s] = i < nr ? si[i][4 * s + g] : (TX)0;
         }
-        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int s = 0; s < 16; s++) cmac<T>(cr, ci, ma_r[s], ma_i[s], xb_r[s], xb_i[s]);
+    }
+    // y = B_b - sum into LDS as the B operand of Inv_b
+    T va_r[16], va_i[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        size_t o = (size_t)b * NB2 + (size_t)(4 * s + g) * LU_NB + wave * 16 + i;
+        va_r[s] = Inv_r[o]; va_i[s] = Inv_i[o];
     }
     __syncthreads();
+    if (i < nr) {
 #pragma unroll
-    for (int j = 0; j < MV_MAXR; j++) { red[wave][j][0][lane] = ar[j]; red[wave][j][1][lane] = ai[j]; }
-    __syncthreads();
-    for (int t = tid; t < nr * LU_NB; t += 256) {
-        int j = t / LU_NB, r = t % LU_NB;
-        size_t x = (size_t)j * Np + r0 + r;
-        yr[j][r] = Xr[x] - (red[0][j][0][r] + red[1][j][0][r] + red[2][j][0][r] + red[3][j][0][r]);
-        yi[j][r] = Xi[x] - (red[0][j][1][r] + red[1][j][1][r] + red[2][j][1][r] + red[3][j][1][r]);
-    }
-    __syncthreads();
-    // X_b = Inv_b y: lane = row, wave w takes right-hand sides j = w, w+4, ...
-    TX sr[4] = {0, 0, 0, 0}, si[4] = {0, 0, 0, 0};
-#pragma unroll 4
-    for (int k = 0; k < LU_NB; k++) {
-        TX v_r = (TX)ivr[k][lane], v_i = (TX)ivi[k][lane];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            int j = wave + 4 * q;
-            if (j < nr) {
-                TX y_r = yr[j][k], y_i = yi[j][k];
-                sr[q] += v_r * y_r - v_i * y_i;
-                si[q] += v_r * y_i + v_i * y_r;
-            }
+        for (int r = 0; r < 4; r++) {
+            sr[i][wave * 16 + MT::row(g, r)] = cr[r];
+            si[i][wave * 16 + MT::row(g, r)] = ci[r];
         }
     }
+    __syncthreads();
+    T xb_r[16], xb_i[16];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        int j = wave + 4 * q;
-        if (j < nr) {
-            __hip_atomic_store(Xr + (size_t)j * Np + r0 + lane, sr[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(Xi + (size_t)j * Np + r0 + lane, si[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int s = 0; s < 16; s++) {
+        xb_r[s] = i < nr ? sr[i][4 * s + g] : (TX)0;
+        xb_i[s] = i < nr ? si[i][4 * s + g] : (TX)0;
+    }
+    acc_t zr = {0, 0, 0, 0}, zi = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 16; s++) cmac<T>(zr, zi, va_r[s], va_i[s], xb_r[s], xb_i[s]);
+    if (has_prev) {
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            size_t o = (size_t)b * NB2 + (size_t)(4 * s + g) * LU_NB + wave * 16 + i;
+            va_r[s] = -P_r[o]; va_i[s] = -P_i[o];
+        }
+        __syncthreads();                           // y is read
+        stage_block<TX>(pub, epoch, prev, nr, tid, lane, sr, si);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 16; s++) {
+            xb_r[s] = i < nr ? sr[i][4 * s + g] : (TX)0;
+            xb_i[s] = i < nr ? si[i][4 * s + g] : (TX)0;
+        }
+#pragma unroll
+        for (int s = 0; s < 16; s++) cmac<T>(zr, zi, va_r[s], va_i[s], xb_r[s], xb_i[s]);
+    }
+    if (i < nr) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int row = wave * 16 + MT::row(g, r);
+            pub_store(pub + pub_off<TX>(b, i, row, 0), zr[r], epoch);
+            pub_store(pub + pub_off<TX>(b, i, row, 1), zi[r], epoch);
+            Xr[(size_t)i * Np + r0 + row] = zr[r];
+            Xi[(size_t)i * Np + r0 + row] = zi[r];
         }
     }
-    // every wave waits for its stores; one agent-scope release publishes the block
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Row permutation of the right-hand sides for the interchanges recorded in
@@ -1274,6 +1408,8 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         if (rest > LU_NB) gemm_sub<T>(s, Np, rest, rest - LU_NB, Ar, Ai, k0, k0 + 2 * LU_NB, prof, rest - LU_NB);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
+    hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,
+                       w.Uinv_i, w.PL_r, w.PL_i, w.PU_r, w.PU_i);
 }
 template void launch_lu<double>(hipStream_t, hipStream_t, hipEvent_t*, int, double*, double*, const LuWork<double>&,
                                  int*, int*, LuProfile*);
@@ -1282,23 +1418,23 @@ template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned* flags, unsigned& epoch,
-                     const int* skip) {
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned long long* pub,
+                     unsigned& epoch, const int* skip) {
     hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
     int nblk = Np / LU_NB;
     ++epoch;
-    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i, Xr,
-                       Xi, counter, flags, epoch, skip);
+    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i,
+                       w.PL_r, w.PL_i, Xr, Xi, pub, counter, epoch, skip);
     ++epoch;
-    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i, Xr,
-                       Xi, counter, flags, epoch, skip);
+    hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i,
+                       w.PU_r, w.PU_i, Xr, Xi, pub, counter, epoch, skip);
 }
 template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*,
                                               const LuWork<double>&, const int*, const int*, double*, double*,
-                                              unsigned*, unsigned*, unsigned&, const int*);
+                                              unsigned*, unsigned long long*, unsigned&, const int*);
 template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const LuWork<float>&,
-                                            const int*, const int*, float*, float*, unsigned*, unsigned*, unsigned&,
-                                            const int*);
+                                            const int*, const int*, float*, float*, unsigned*, unsigned long long*,
+                                            unsigned&, const int*);
 
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
     hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);
