@@ -711,7 +711,10 @@ __global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int
 // the global loads of chunk c+1 are in flight while chunk c feeds the MFMAs)
 // with 16-byte coalesced loads; both are stored k-contiguous so every MFMA
 // operand quad (4 consecutive k) is one LDS read. 32 KB (FP32) / 32 KB (FP64)
-// of LDS per stage lets two work-groups share a CU.
+// of LDS per stage lets two work-groups share a CU. The accumulators start
+// from C, loaded at the top of the kernel so that its latency overlaps the
+// main loop, and A is staged negated, so the loop accumulates C - A B and the
+// epilogue only stores.
 // ---------------------------------------------------------------------------
 constexpr int GK = 16;           // k per LDS chunk
 constexpr int GKP = GK + 4;      // padded row length in LDS (bank spread)
@@ -775,10 +778,10 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
             int kk = idx / (TM / 4), rq = idx % (TM / 4);
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                As[buf][p][4 * rq + 0][kk] = ga[p][l].x;
-                As[buf][p][4 * rq + 1][kk] = ga[p][l].y;
-                As[buf][p][4 * rq + 2][kk] = ga[p][l].z;
-                As[buf][p][4 * rq + 3][kk] = ga[p][l].w;
+                As[buf][p][4 * rq + 0][kk] = -ga[p][l].x;
+                As[buf][p][4 * rq + 1][kk] = -ga[p][l].y;
+                As[buf][p][4 * rq + 2][kk] = -ga[p][l].z;
+                As[buf][p][4 * rq + 3][kk] = -ga[p][l].w;
             }
         }
 #pragma unroll
@@ -799,13 +802,24 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
     int row0 = wm * 16 * WM, col0 = wn * 16 * WN;           // within the tile
     bool active = (row0 < mrows) && (col0 < ncols);
     acc_t cr[WM][WN], ci[WM][WN];
+    int nchunk = K / GK;
+    gload(0);
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
-        for (int b = 0; b < WN; b++) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
-
-    int nchunk = K / GK;
-    gload(0);
+        for (int b = 0; b < WN; b++) {
+            if (active) {
+                size_t off = (size_t)(cc + colt + col0 + 16 * b + i) * ldl + rc + rowt + row0 + 16 * a;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    cr[a][b][r] = Pr[off + MT::row(g, r)];
+                    ci[a][b][r] = Pi[off + MT::row(g, r)];
+                }
+            } else {
+                cr[a][b] = acc_t{0, 0, 0, 0};
+                ci[a][b] = acc_t{0, 0, 0, 0};
+            }
+        }
     sstore(0);
     __syncthreads();
     for (int c = 0; c < nchunk; c++) {
@@ -853,8 +867,8 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int rr = MT::row(g, r);
-                Pr[off + rr] -= cr[a][b][r];
-                Pi[off + rr] -= ci[a][b][r];
+                Pr[off + rr] = cr[a][b][r];
+                Pi[off + rr] = ci[a][b][r];
             }
         }
 }
