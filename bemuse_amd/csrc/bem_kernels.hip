@@ -345,18 +345,26 @@ __device__ void diag_inverses(const T (*lur)[LU_NB + 1], const T (*lui)[LU_NB + 
 // L^-1 in registers; step j broadcasts row j and column j through LDS (double
 // buffered, one barrier per step), every thread forms the multipliers of its
 // rows and applies the rank-1 update to its A columns right of j and to its
-// L^-1 columns left of (or at) j, since L^-1 = E_63 ... E_0 with E_j the
-// elementary elimination of step j. U^-1 follows from a backward sweep of the
-// same shape (scale row k of the identity by 1/u_kk, subtract u_mk times it
-// from rows m < k). Saves the block, writes L\U, L^-1 and U^-1, and sets
-// flag[panel] bit 1 if a multiplier exceeds 1 in modulus (1|2 on a zero pivot).
+// L^-1 tile, since L^-1 = E_63 ... E_0 with E_j the elementary elimination of
+// step j (row j of the partial L^-1 is zero right of j). U^-1 follows from a
+// backward sweep of the same shape (scale row k of the identity by 1/u_kk,
+// subtract u_mk times it from rows m < k). The updates are masked
+// arithmetically, with no per-element branches, so the step loop stays one
+// basic block and the LDS reads of a step issue together. Saves the block,
+// writes L\U, L^-1 and U^-1, and sets flag[panel] bit 1 if a multiplier
+// exceeds 1 in modulus (1|2 on a zero pivot).
+template <typename T>
+__device__ inline T pick4(const T (&v)[4], int k) {
+    return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
                                                  T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
-    __shared__ T lur[LU_NB][LU_NB + 1], lui[LU_NB][LU_NB + 1];   // final L\U [col][row]
-    __shared__ T rowr[2][2][LU_NB], rowi[2][2][LU_NB];          // [buf][A or X][col]
-    __shared__ T colr[2][LU_NB], coli[2][LU_NB];                // [buf][row]
-    __shared__ T rdr[LU_NB], rdi[LU_NB];
+    constexpr int LP = LU_NB + 4;                                // 16-byte aligned rows
+    __shared__ __align__(16) T lur[LU_NB][LP], lui[LU_NB][LP];   // final L\U [col][row]
+    __shared__ __align__(16) T rowr[2][2][LU_NB], rowi[2][2][LU_NB];   // [buf][A or X][col]
+    __shared__ __align__(16) T colr[2][LU_NB], coli[2][LU_NB];         // [buf][row]
     __shared__ int bad;
     int tid = threadIdx.x;
     int tr = tid & 15, tc = tid >> 4;
@@ -376,63 +384,57 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     int mybad = 0;
     for (int j = 0; j < LU_NB; j++) {
         int buf = j & 1, jt = j >> 2, jj = j & 3;
-        if (tr == jt) {            // row j: A part and L^-1 part
+        if (tr == jt) {            // row j of A and of the partial L^-1
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-                if (rr == jj)
-#pragma unroll
-                    for (int cc = 0; cc < 4; cc++) {
-                        rowr[buf][0][cb + cc] = ar[rr][cc]; rowi[buf][0][cb + cc] = ai[rr][cc];
-                        rowr[buf][1][cb + cc] = xr[rr][cc]; rowi[buf][1][cb + cc] = xi[rr][cc];
-                    }
+            for (int cc = 0; cc < 4; cc++) {
+                T c0r[4] = {ar[0][cc], ar[1][cc], ar[2][cc], ar[3][cc]}, c0i[4] = {ai[0][cc], ai[1][cc], ai[2][cc], ai[3][cc]};
+                T c1r[4] = {xr[0][cc], xr[1][cc], xr[2][cc], xr[3][cc]}, c1i[4] = {xi[0][cc], xi[1][cc], xi[2][cc], xi[3][cc]};
+                rowr[buf][0][cb + cc] = pick4(c0r, jj); rowi[buf][0][cb + cc] = pick4(c0i, jj);
+                rowr[buf][1][cb + cc] = pick4(c1r, jj); rowi[buf][1][cb + cc] = pick4(c1i, jj);
+            }
         }
         if (tc == jt) {            // column j of A
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++)
-                if (cc == jj)
-#pragma unroll
-                    for (int rr = 0; rr < 4; rr++) { colr[buf][rb + rr] = ar[rr][cc]; coli[buf][rb + rr] = ai[rr][cc]; }
+            for (int rr = 0; rr < 4; rr++) {
+                colr[buf][rb + rr] = pick4(ar[rr], jj);
+                coli[buf][rb + rr] = pick4(ai[rr], jj);
+            }
         }
         __syncthreads();
         T pr = colr[buf][j], pi = coli[buf][j];
         T den = cabs2(pr, pi);
-        T qr = pr / den, qi = -pi / den;
-        if (tid == 0) { rdr[j] = qr; rdi[j] = qi; if (den == (T)0) mybad |= 3; }
+        T inv = (T)1 / den;
+        T qr = pr * inv, qi = -pi * inv;
+        if (tid == 0 && den == (T)0) mybad |= 3;
+        T cr4[4], ci4[4], ur[4], ui[4], vr[4], vi[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cr4[k] = colr[buf][rb + k]; ci4[k] = coli[buf][rb + k];
+            ur[k] = rowr[buf][0][cb + k]; ui[k] = rowi[buf][0][cb + k];
+            vr[k] = rowr[buf][1][cb + k]; vi[k] = rowi[buf][1][cb + k];
+        }
         T lr[4], li[4];
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-            T cr = colr[buf][rb + rr], ci = coli[buf][rb + rr];
-            lr[rr] = cr * qr - ci * qi; li[rr] = cr * qi + ci * qr;
+            bool below = rb + rr > j;
+            T l_r = cr4[rr] * qr - ci4[rr] * qi, l_i = cr4[rr] * qi + ci4[rr] * qr;
+            if (below && cabs2(l_r, l_i) > (T)1) mybad |= 1;
+            lr[rr] = below ? l_r : (T)0;
+            li[rr] = below ? l_i : (T)0;
         }
-        T ur[4], ui[4], vr[4], vi[4];
 #pragma unroll
         for (int cc = 0; cc < 4; cc++) {
-            ur[cc] = rowr[buf][0][cb + cc]; ui[cc] = rowi[buf][0][cb + cc];
-            vr[cc] = rowr[buf][1][cb + cc]; vi[cc] = rowi[buf][1][cb + cc];
-        }
+            bool right = cb + cc > j, here = cb + cc == j;
+            T u_r = right ? ur[cc] : (T)0, u_i = right ? ui[cc] : (T)0;
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            int i = rb + rr;
-            if (i > j) {
-                if (tc == jt) {
-#pragma unroll
-                    for (int cc = 0; cc < 4; cc++)
-                        if (cc == jj) {
-                            ar[rr][cc] = lr[rr]; ai[rr][cc] = li[rr];
-                            if (cabs2(lr[rr], li[rr]) > (T)1) mybad |= 1;
-                        }
-                }
-#pragma unroll
-                for (int cc = 0; cc < 4; cc++) {
-                    int c = cb + cc;
-                    if (c > j) {
-                        ar[rr][cc] -= lr[rr] * ur[cc] - li[rr] * ui[cc];
-                        ai[rr][cc] -= lr[rr] * ui[cc] + li[rr] * ur[cc];
-                    } else {
-                        xr[rr][cc] -= lr[rr] * vr[cc] - li[rr] * vi[cc];
-                        xi[rr][cc] -= lr[rr] * vi[cc] + li[rr] * vr[cc];
-                    }
-                }
+            for (int rr = 0; rr < 4; rr++) {
+                T nr_ = ar[rr][cc] - (lr[rr] * u_r - li[rr] * u_i);
+                T ni_ = ai[rr][cc] - (lr[rr] * u_i + li[rr] * u_r);
+                bool setl = here && (rb + rr > j);
+                ar[rr][cc] = setl ? lr[rr] : nr_;
+                ai[rr][cc] = setl ? li[rr] : ni_;
+                xr[rr][cc] -= lr[rr] * vr[cc] - li[rr] * vi[cc];
+                xi[rr][cc] -= lr[rr] * vi[cc] + li[rr] * vr[cc];
             }
         }
     }
@@ -452,31 +454,37 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     for (int k = LU_NB - 1; k >= 0; k--) {
         int buf = k & 1, kt = k >> 2, kk = k & 3;
         if (tr == kt) {
-            T qr = rdr[k], qi = rdi[k];
+            T dr = lur[k][k], di = lui[k][k];
+            T inv = (T)1 / cabs2(dr, di);
+            T qr = dr * inv, qi = -di * inv;
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-                if (rr == kk)
+            for (int cc = 0; cc < 4; cc++) {
 #pragma unroll
-                    for (int cc = 0; cc < 4; cc++) {
+                for (int rr = 0; rr < 4; rr++) {
+                    if (rr == kk) {
                         T yr = xr[rr][cc] * qr - xi[rr][cc] * qi, yi = xr[rr][cc] * qi + xi[rr][cc] * qr;
                         xr[rr][cc] = yr; xi[rr][cc] = yi;
                         rowr[buf][0][cb + cc] = yr; rowi[buf][0][cb + cc] = yi;
                     }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            int m = rb + rr;
-            if (m < k) {
-                T u_r = lur[k][m], u_i = lui[k][m];
-#pragma unroll
-                for (int cc = 0; cc < 4; cc++) {
-                    T yr = rowr[buf][0][cb + cc], yi = rowi[buf][0][cb + cc];
-                    xr[rr][cc] -= u_r * yr - u_i * yi;
-                    xi[rr][cc] -= u_r * yi + u_i * yr;
                 }
             }
         }
+        __syncthreads();
+        T yr[4], yi[4], u_r[4], u_i[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            yr[q] = rowr[buf][0][cb + q]; yi[q] = rowi[buf][0][cb + q];
+            bool above = rb + q < k;
+            u_r[q] = above ? lur[k][rb + q] : (T)0;
+            u_i[q] = above ? lui[k][rb + q] : (T)0;
+        }
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                xr[rr][cc] -= u_r[rr] * yr[cc] - u_i[rr] * yi[cc];
+                xi[rr][cc] -= u_r[rr] * yi[cc] + u_i[rr] * yr[cc];
+            }
     }
 #pragma unroll
     for (int cc = 0; cc < 4; cc++)
@@ -484,6 +492,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
         for (int rr = 0; rr < 4; rr++) {
             Ui_r[(cb + cc) * LU_NB + rb + rr] = xr[rr][cc]; Ui_i[(cb + cc) * LU_NB + rb + rr] = xi[rr][cc];
         }
+    __syncthreads();
     if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
 }
 
