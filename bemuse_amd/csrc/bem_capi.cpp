@@ -56,6 +56,7 @@ struct bem_mesh {
 
 struct bem_ctx {
     int device = 0;
+    void* surrogate = nullptr;         // trained wave-term surrogate (none yet)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
 };
@@ -577,6 +578,8 @@ int check_solve_args(bem_ctx* c, bem_body* b, int mode, int compose, int nb, con
     if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bad context/body");
     if (nb < 0 || nb > kMaxRhs - 6 || (nb > 0 && !betas)) return fail(BEM_ERR_ARG, "nb must be in [0, 10]");
     if (mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bad mode");
+    if (mode == BEM_MODE_MIXED_SURROGATE && !b->ctx->surrogate)
+        return fail(BEM_ERR_STATE, "BEM_MODE_MIXED_SURROGATE needs a trained surrogate on the context");
     if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bad compose case");
     if (!b->have_rankine) return fail(BEM_ERR_STATE, "Rankine matrices not assembled");
     return BEM_OK;

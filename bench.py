@@ -34,6 +34,19 @@ PEAK_FP32 = 157.3e12     # MI355X dense FP32 matrix, flop/s
 METRIC = "frequency-solves/sec at 6k panels (mixed+surrogate vs FP64) at <=1e-6 rel err"
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel, from the rocprof PMC summary
+    committed with the code (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", "cgemm_f32_semisub6018_roofline.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def rel(a, b):
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
@@ -98,15 +111,18 @@ def main():
     def stage_means(res):
         return {k: float(np.mean([r["info"][k] for r in res])) for k in ("ms_assemble", "ms_factor", "ms_solve")}
 
-    # roofline of the dominant kernel: the MFMA trailing-update GEMM of the LU
+    # roofline of the dominant kernel: the MFMA trailing-update GEMM of the LU,
+    # timed with HIP events around each launch of one isolated factorisation
     kap = 1.0 ** 2 / 9.81
     prof_mx = body.profile_lu(kap, bm.MODE_MIXED)
     prof_64 = body.profile_lu(kap, bm.MODE_FP64)
     achieved = prof_mx["flops_per_launch"] / (prof_mx["ms_gemm_per_launch"] * 1e-3)
     roofline = {
-        "kernel": "k_cgemm_sub<float> (LU trailing update, FP32 MFMA 16x16x4)",
+        "kernel": "k_cgemm_sub<float> (LU trailing update, FP32 MFMA 16x16x4, K=64)",
         "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32, "traffic": None,
+        "frac": achieved / PEAK_FP32, "traffic": measured_traffic(),
+        "flops_per_launch": prof_mx["flops_per_launch"], "launches": int(prof_mx["gemm_launches"]),
+        "us_per_launch": prof_mx["ms_gemm_per_launch"] * 1e3,
         "fp64_gemm": {"achieved": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / 1e12,
                       "peak": PEAK_FP64 / 1e12,
                       "frac": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / PEAK_FP64},

@@ -101,3 +101,20 @@ def test_host_geometry_matches_oracle():
     np.testing.assert_allclose(gen[:, 0:3], n, atol=1e-14)
     np.testing.assert_allclose(gen[:, 3:6], np.cross(c, n), atol=1e-13)
     np.testing.assert_allclose(gen[:, 6], og["area"], rtol=1e-15)
+
+
+def test_cpp_dropin_without_gpu_reports_and_returns_nan():
+    """The drop-in follows the reference's error behaviour: no exception, a
+    message on stdout, NaN results (no GPU in this environment)."""
+    import subprocess
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    exe = os.path.join(ROOT, "tests", "cpp", "hydro_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/hydro_dropin not built")
+    p = subprocess.run([exe, "0"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0
+    assert "BEMUse:" in p.stdout
+    a_lines = [l for l in p.stdout.splitlines() if l.startswith("A ")]
+    assert a_lines and all(v == "nan" or v == "-nan" for v in a_lines[0].split()[2:])

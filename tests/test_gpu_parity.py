@@ -172,3 +172,44 @@ def test_solve_vs_committed_fixture(ctx):
             for i in range(len(fx["omegas"])):
                 assert relerr(sw[k][i], fx[k][i]) < tol, (mode, k, i)
     body.close()
+
+
+def _run_dropin(mode):
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "hydro_dropin")
+    assert os.path.exists(exe), "tests/cpp/hydro_dropin is not built (make -C tests/cpp)"
+    out = subprocess.run([exe, str(mode)], capture_output=True, text=True, timeout=300, check=True).stdout
+    res = {"stdout": out}
+    for line in out.splitlines():
+        tok = line.split()
+        if tok[0] == "N":
+            res["N"] = int(tok[1])
+        elif tok[0] == "SETUP_A":
+            res["setup_A"] = np.array(tok[1:], float).reshape(6, 6)
+        elif tok[0] in ("A", "B", "F"):
+            w = float(tok[1])
+            v = np.array(tok[2:], float)
+            if tok[0] == "F":
+                v = (v[0::2] + 1j * v[1::2]).reshape(6, -1)
+            else:
+                v = v.reshape(6, 6)
+            res.setdefault(w, {})[tok[0]] = v
+    return res
+
+
+@pytest.mark.parametrize("mode,tol", [(bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)])
+def test_cpp_dropin_console_flow(cyl, mode, tol):
+    """The C++ drop-in (include/bemuse_hydro.h), driven like the reference
+    console, against the oracle: the FreqInf solve of Setup and two frequencies."""
+    mesh, body, ob = cyl
+    res = _run_dropin(mode)
+    assert res["N"] == body.N, res["stdout"]
+    om = -1.0 / (2 * np.pi)
+    r = ob.solve(om, kappa=om * om / 9.81, compose=0, rad_only=True)
+    assert relerr(res["setup_A"], r["A"]) < tol
+    for w in (0.4, 1.3):
+        g = res[w]
+        r = ob.solve(w, betas=[0.0, np.pi / 4])
+        for key in ("A", "B", "F"):
+            assert relerr(g[key], r[key]) < tol, (w, key, relerr(g[key], r[key]))
