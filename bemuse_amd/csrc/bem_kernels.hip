@@ -349,8 +349,8 @@ __device__ void diag_inverses(const T (*lur)[LU_NB + 1], const T (*lui)[LU_NB + 
 // step j (row j of the partial L^-1 is zero right of j). U^-1 follows from a
 // backward sweep of the same shape (scale row k of the identity by 1/u_kk,
 // subtract u_mk times it from rows m < k). The updates are masked
-// arithmetically, with no per-element branches, so the step loop stays one
-// basic block and the LDS reads of a step issue together. Saves the block,
+// arithmetically (no divergent per-element branches), and the LDS reads of a
+// step issue together. Saves the block,
 // writes L\U, L^-1 and U^-1, and sets flag[panel] bit 1 if a multiplier
 // exceeds 1 in modulus (1|2 on a zero pivot).
 template <typename T>
