@@ -10,7 +10,8 @@ no input data.
 A step is one frequency solve (wave assembly + compose, LU, radiation and
 diffraction solves, refinement, A/B/F reductions); steps cycle through the 40
 frequencies and are pipelined over --lanes concurrent solve lanes (one C-ABI
-sweep call covers the timed steps). Rankine assembly (once per body) is
+sweep call covers the timed steps; GPU_MAX_HW_QUEUES is raised to 32 so that
+the lanes' streams do not share hardware queues). Rankine assembly (once per body) is
 outside the timed region and reported separately. The headline mode is MIXED (FP32-MFMA LU + FP64
 refinement); the FP64 mode is timed in the same run for the ratio, and every
 MIXED result is compared with the FP64 GPU result at the same frequency and
@@ -64,8 +65,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1, help="frequencies of the CPU oracle sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="1k-panel cylinder instead of the 6k semi-sub (quick check)")
-    ap.add_argument("--lanes", type=int, default=4, help="frequencies in flight (concurrent solve lanes)")
+    ap.add_argument("--lanes", type=int, default=8, help="frequencies in flight (concurrent solve lanes)")
     args = ap.parse_args()
+    # Each lane drives two HIP streams. HIP maps streams onto GPU_MAX_HW_QUEUES
+    # hardware queues (default 4); streams sharing a queue run in order, so a
+    # lane's latency-bound panel kernels would wait behind another lane's GEMM.
+    # The runtime reads the variable when it initialises: set it before the
+    # first HIP call (the library is loaded below).
+    want = 32 if args.lanes > 1 else 4
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(frequencies are independent: a multi-GPU run would shard the frequency list).", file=sys.stderr)
