@@ -483,6 +483,14 @@ int ensure_lanes(bem_body* b, int n, bool need32) {
             int r = b->lanes[i]->sys.init32();
             if (r) return r;
         }
+    // GEMM tile autotuning for this padded order (once per device, precision
+    // and order), on lane 0's matrix buffers before any frequency uses them
+    Lane& L0 = *b->lanes[0];
+    if (L0.pending < 0) {
+        tune_lu_gemms<double>(L0.s1, b->Np, L0.sys.Dr, L0.sys.Di);
+        if (need32) tune_lu_gemms<float>(L0.s1, b->Np, L0.sys.D32r, L0.sys.D32i);
+        HIP_TRY(hipStreamSynchronize(L0.s1));
+    }
     return BEM_OK;
 }
 

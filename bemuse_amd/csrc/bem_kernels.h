@@ -52,6 +52,14 @@ struct LuWork {
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
                int* ipiv, LuProfile* prof = nullptr);
+// Times the trailing-update tile shapes on the GEMM shapes of a factorisation
+// of padded order Np (once per device, precision and Np) using the Np x Np
+// scratch matrix Ar/Ai; returns the number of shape classes tuned.
+template <typename T>
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai);
+// Tile chosen for an M x N x K update on the current device: 1000 * rows + cols.
+template <typename T>
+int gemm_tile_choice(int M, int N, int K);
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
                      const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned long long* pub,

@@ -8,6 +8,12 @@
 #include "bem_kernels.h"
 #include "bem_device.hpp"
 
+#include <array>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -732,14 +738,17 @@ template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float4 type; };
 template <> struct Vec4<double> { typedef double4 type; };
 
-template <typename T>
+// WM x WN MFMA tiles of 16 x 16 per wave; the four waves form a 2 x 2 grid, so
+// the work-group tile is (32 WM) x (32 WN). Every element accumulates its k in
+// the same order for any (WM, WN), so all tile shapes give identical results.
+template <typename T, int WM, int WN>
 __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __restrict__ Pr, T* __restrict__ Pi,
                                                       int ld, int ra, int ca, int rb, int cb, int rc, int cc) {
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     typedef typename Vec4<T>::type v4;
-    constexpr int WM = MT::WM, WN = MT::WN;
     constexpr int TM = 32 * WM, TN = 32 * WN;      // work-group tile
+    static_assert(TM >= 64 && TN >= 64, "a chunk must give every thread at least one 4-element load");
     constexpr int NLA = TM * GK / 4 / 256;          // 4-element A loads per thread per chunk (re or im)
     constexpr int NLB = TN * GK / 4 / 256;
     __shared__ T As[2][2][TM][GKP];                 // [buf][re/im][row][k]
@@ -1365,11 +1374,168 @@ template void launch_matvec<double>(hipStream_t, int, int, const double*, const 
                                     double*, double*, const double*, const double*, int, double*, double*, double*,
                                     double*, const int*);
 
-template <typename T>
-static void gemm_sub(hipStream_t s, int Np, int M, int N, T* Ar, T* Ai, int k0, int cc, LuProfile* prof, int Nuse) {
-    constexpr int TM = 32 * MfmaT<T>::WM, TN = 32 * MfmaT<T>::WN;
+// ---------------------------------------------------------------------------
+// GEMM tile autotuner. The tile shapes of k_cgemm_sub are timed on the device
+// per (M class, N class, K) with M, N classes floor(log2(x / 64)), for the
+// shapes a factorisation of a given padded order launches, and the fastest is
+// kept per device and precision. Untuned shapes use a size rule.
+// ---------------------------------------------------------------------------
+namespace {
+struct TileCfg { int wm, wn; };
+constexpr TileCfg kCfgF32[] = {{4, 4}, {4, 2}, {2, 4}, {2, 2}};
+constexpr TileCfg kCfgF64[] = {{2, 2}, {4, 2}, {2, 4}};
+template <typename T> constexpr int n_cfg() { return sizeof(T) == 4 ? 4 : 3; }
+template <typename T> constexpr TileCfg cfg_of(int i) { return sizeof(T) == 4 ? kCfgF32[i] : kCfgF64[i]; }
+
+constexpr int kMaxDev = 16, kClasses = 11;
+struct TuneTable {
+    signed char pick[2][kClasses][kClasses][2];   // [prec][M class][N class][K == 128]
+    bool tuned_np[2][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128)
+    TuneTable() {
+        std::memset(pick, -1, sizeof(pick));
+        std::memset(tuned_np, 0, sizeof(tuned_np));
+    }
+};
+TuneTable g_tune[kMaxDev];
+std::mutex g_tune_mu;
+
+inline int size_class(int x) {
+    int c = 0;
+    while ((64 << (c + 1)) <= x && c < kClasses - 1) c++;
+    return c;
+}
+
+template <typename T, int WM, int WN>
+void launch_tile(hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int ra, int ca, int rb, int cb, int rc,
+                 int cc) {
+    constexpr int TM = 32 * WM, TN = 32 * WN;
     int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
     int per = (tiles + 7) / 8;
+    hipLaunchKernelGGL((k_cgemm_sub<T, WM, WN>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb,
+                       rc, cc);
+}
+
+template <typename T>
+void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int ra, int ca, int rb, int cb,
+                int rc, int cc) {
+    if constexpr (sizeof(T) == 4) {
+        switch (cfg) {
+        case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        case 3: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        default: launch_tile<T, 4, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        }
+    } else {
+        switch (cfg) {
+        case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        default: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+        }
+    }
+}
+
+// Size rule for untuned shapes: the largest tile that still gives 2 work-groups per CU.
+template <typename T>
+int default_cfg(int M, int N) {
+    if constexpr (sizeof(T) == 4) {
+        if (cdiv(M, 128) * cdiv(N, 128) >= 512) return 0;
+        if (cdiv(M, 128) * cdiv(N, 64) >= 512) return 1;
+        return 3;
+    } else {
+        return 0;
+    }
+}
+
+template <typename T>
+int pick_cfg(int dev, int M, int N, int K) {
+    int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].pick[sizeof(T) == 8][size_class(M)][size_class(N)][K >= 128] : -1;
+    return v >= 0 ? v : default_cfg<T>(M, N);
+}
+
+// The (M, N, K) of every trailing-update launch of launch_lu at padded order Np.
+std::vector<std::array<int, 3>> lu_gemm_shapes(int Np) {
+    std::vector<std::array<int, 3>> v;
+    int npanel = Np / LU_NB;
+    for (int p = 0; p < npanel; p += 2) {
+        int rest = Np - p * LU_NB - LU_NB, rest2 = rest - LU_NB;
+        v.push_back({rest, LU_NB, LU_NB});
+        if (rest2 <= 0) break;
+        v.push_back({LU_NB, rest2, LU_NB});
+        v.push_back({rest2, LU_NB, 2 * LU_NB});
+        if (rest2 > LU_NB) v.push_back({rest2, rest2 - LU_NB, 2 * LU_NB});
+    }
+    return v;
+}
+}  // namespace
+
+// Times every tile shape on the shapes a factorisation of order Np launches
+// (one representative per class, the largest of the class) and records the
+// fastest. Ar, Ai: an Np x Np scratch matrix (its contents are overwritten).
+template <typename T>
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 0;
+    int prec = sizeof(T) == 8;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_tune[dev].tuned_np[prec][Np / 128]) return 0;
+    }
+    std::map<std::array<int, 3>, std::array<int, 3>> reps;   // (mc, nc, kb) -> largest (M, N, K)
+    for (auto& sh : lu_gemm_shapes(Np)) {
+        std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), sh[2] >= 128};
+        auto it = reps.find(key);
+        if (it == reps.end() || (double)sh[0] * sh[1] > (double)it->second[0] * it->second[1]) reps[key] = sh;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 0;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 0; }
+    std::vector<std::pair<std::array<int, 3>, int>> picks;
+    for (auto& kv : reps) {
+        const std::array<int, 3>& sh = kv.second;
+        int M = sh[0], N = sh[1], K = sh[2];
+        // A at (128, 0), B at (0, 128), C at (128, 128): inside the Np x Np scratch
+        if (M > Np - 128 || N > Np - 128) { M = std::min(M, Np - 128); N = std::min(N, Np - 128); }
+        if (M < 64 || N < 64) continue;
+        float best = 1e30f;
+        int bi = default_cfg<T>(M, N);
+        for (int c = 0; c < n_cfg<T>(); c++) {
+            launch_cfg<T>(c, s, M, N, K, Ar, Ai, Np, 128, 0, 0, 128, 128, 128);   // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3; r++) launch_cfg<T>(c, s, M, N, K, Ar, Ai, Np, 128, 0, 0, 128, 128, 128);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; bi = c; }
+        }
+        picks.push_back({kv.first, bi});
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& pk : picks) g_tune[dev].pick[prec][pk.first[0]][pk.first[1]][pk.first[2]] = (signed char)pk.second;
+    g_tune[dev].tuned_np[prec][Np / 128] = true;
+    return (int)picks.size();
+}
+template int tune_lu_gemms<float>(hipStream_t, int, float*, float*);
+template int tune_lu_gemms<double>(hipStream_t, int, double*, double*);
+
+template <typename T>
+int gemm_tile_choice(int M, int N, int K) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int c = pick_cfg<T>(dev, M, N, K);
+    TileCfg t = cfg_of<T>(c);
+    return 1000 * (32 * t.wm) + 32 * t.wn;
+}
+template int gemm_tile_choice<float>(int, int, int);
+template int gemm_tile_choice<double>(int, int, int);
+
+// C(rc.., cc..) -= A(ra.., ca..) B(rb.., cb..), M x N x K (profile: HIP events
+// around the launch and the flops of the unpadded part).
+template <typename T>
+static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar, T* Ai, int ra, int ca, int rb,
+                     int cb, int rc, int cc, LuProfile* prof) {
     bool timed = prof && prof->n < LuProfile::kMax;
     if (timed) {
         if (!prof->ev[2 * prof->n]) {
@@ -1378,27 +1544,37 @@ static void gemm_sub(hipStream_t s, int Np, int M, int N, T* Ar, T* Ai, int k0, 
         }
         (void)hipEventRecord(prof->ev[2 * prof->n], s);
     }
-    // C(k0+NB.., cc..) -= A(k0+NB.., k0..k0+NB) B(k0..k0+NB, cc..)
-    hipLaunchKernelGGL(k_cgemm_sub<T>, dim3(per * 8), dim3(256), 0, s, M, N, LU_NB, Ar, Ai, Np, k0 + LU_NB, k0, k0, cc,
-                       k0 + LU_NB, cc);
+    launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc);
     if (timed) {
         (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
-        double mu = (double)std::max(0, prof->N - (k0 + LU_NB));
-        double nu = (double)std::max(0, std::min(Nuse, prof->N - cc));
-        prof->useful_flops += 8.0 * mu * nu * LU_NB;
+        double mu = (double)std::max(0, std::min(M, prof->N - rc));
+        double nu = (double)std::max(0, std::min(N, prof->N - cc));
+        double ku = (double)std::max(0, std::min(K, prof->N - ca));
+        prof->useful_flops += 8.0 * mu * nu * ku;
         prof->n++;
     }
 }
 
-// Right-looking blocked LU with one-panel lookahead: the panel stream s2
-// factors the diagonal block and L21 of panel p+1 while the main stream s
-// applies panel p's update to the columns right of panel p+1. Events ev
-// (2 per panel) order the two streams; fallback pivoting (GEPP + row swaps)
-// of a panel always runs on the main stream after the trailing update.
+// Right-looking blocked LU over pairs of panels, with lookahead. For the pair
+// (p, p+1) at k0 = p NB:
+//   main stream:      fix p (pivot check / fallback + row swaps), U12_p, then
+//                     the K = 64 update of panel p+1's columns;
+//   panel stream s2:  factor panel p+1 (diagonal block, L21);
+//   main stream:      fix p+1, the K = 64 update of panel p+1's rows right of
+//                     it (L21_p rows x U12_p), U12_{p+1}, the K = 128 update of
+//                     panel p+2's columns with [L21_p L21_{p+1}] [U12_p; U12_{p+1}];
+//   panel stream s2:  factor panel p+2, while the main stream applies the same
+//                     K = 128 update to all columns right of panel p+2.
+// Deferring panel p's trailing update to the K = 128 product doubles the flops
+// per byte of C moved. Row interchanges of panel p+1 (fallback pivoting) hit
+// the not yet updated trailing rows and L21_p alike, which commutes with the
+// update. Events ev (2 per panel) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
                int* ipiv, LuProfile* prof) {
-    int npanel = Np / LU_NB;
+    int npanel = Np / LU_NB;    // even: Np is a multiple of 128
+    int dev = 0;
+    (void)hipGetDevice(&dev);
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv);
     auto panel = [&](hipStream_t st, int p) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
@@ -1410,25 +1586,51 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / 64), dim3(256), 0, st, Np, k0, Ar, Ai, Uir, Uii, w.s21_r,
                                w.s21_i, flag);
     };
-    // panel 0 on the main stream
-    panel(s, 0);
-    for (int p = 0; p < npanel; p++) {
-        int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
-        if (p > 0) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);     // panel p factored on s2
+    auto fix = [&](int p) {      // pivot check and fallback of panel p, row swaps elsewhere
+        int k0 = p * LU_NB;
         T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
         T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
         hipLaunchKernelGGL(k_lu_panel_gepp<T>, dim3(1), dim3(1024), 0, s, Np, k0, Ar, Ai, w.sd_r, w.sd_i, w.s21_r,
                            w.s21_i, Lir, Lii, Uir, Uii, flag, ipiv);
         hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(Np, 256)), dim3(256), 0, s, Np, k0, Ar, Ai, flag, ipiv);
-        if (rest <= 0) break;
-        hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 64), dim3(256), 0, s, Np, k0, Ar, Ai, Lir, Lii);
-        // columns of panel p+1 first, then hand panel p+1 to the panel stream
-        gemm_sub<T>(s, Np, rest, LU_NB, Ar, Ai, k0, k0 + LU_NB, prof, LU_NB);
-        (void)hipEventRecord(ev[2 * (p + 1)], s);
-        (void)hipStreamWaitEvent(s2, ev[2 * (p + 1)], 0);
+    };
+    auto u12 = [&](int p) {
+        int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
+        if (rest > 0)
+            hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 64), dim3(256), 0, s, Np, k0, Ar, Ai,
+                               w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2);
+    };
+    const int NB = LU_NB;
+    panel(s, 0);
+    for (int p = 0; p < npanel; p += 2) {
+        int k0 = p * NB, q = p / 2;
+        hipEvent_t evA = ev[4 * q], evB = ev[4 * q + 1], evC = ev[4 * q + 2], evD = ev[4 * q + 3];
+        if (p > 0) (void)hipStreamWaitEvent(s, ev[4 * q - 1], 0);   // panel p factored on s2
+        fix(p);
+        u12(p);
+        int rest = Np - k0 - NB;                                    // >= NB: panel p+1 exists
+        gemm_sub<T>(s, dev, Np, rest, NB, NB, Ar, Ai, k0 + NB, k0, k0, k0 + NB, k0 + NB, k0 + NB, prof);
+        (void)hipEventRecord(evA, s);
+        (void)hipStreamWaitEvent(s2, evA, 0);
         panel(s2, p + 1);
-        (void)hipEventRecord(ev[2 * (p + 1) + 1], s2);
-        if (rest > LU_NB) gemm_sub<T>(s, Np, rest, rest - LU_NB, Ar, Ai, k0, k0 + 2 * LU_NB, prof, rest - LU_NB);
+        (void)hipEventRecord(evB, s2);
+        (void)hipStreamWaitEvent(s, evB, 0);
+        fix(p + 1);
+        int rest2 = rest - NB;
+        if (rest2 <= 0) break;
+        // rows of panel p+1, right of it: A -= L21_p(those rows) U12_p
+        gemm_sub<T>(s, dev, Np, NB, rest2, NB, Ar, Ai, k0 + NB, k0, k0, k0 + 2 * NB, k0 + NB, k0 + 2 * NB, prof);
+        u12(p + 1);
+        // panel p+2's columns, then hand it to the panel stream
+        gemm_sub<T>(s, dev, Np, rest2, NB, 2 * NB, Ar, Ai, k0 + 2 * NB, k0, k0, k0 + 2 * NB, k0 + 2 * NB, k0 + 2 * NB,
+                    prof);
+        (void)hipEventRecord(evC, s);
+        (void)hipStreamWaitEvent(s2, evC, 0);
+        panel(s2, p + 2);
+        (void)hipEventRecord(evD, s2);
+        if (rest2 > NB)
+            gemm_sub<T>(s, dev, Np, rest2, rest2 - NB, 2 * NB, Ar, Ai, k0 + 2 * NB, k0, k0, k0 + 3 * NB, k0 + 2 * NB,
+                        k0 + 3 * NB, prof);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,
