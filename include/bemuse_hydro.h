@@ -20,20 +20,37 @@
  *   - RadSolArray / DiffSolArray receive Phi_J (N x 6) and Phi_S (N x NBeta)
  *     per Solve (:512-513), when Keep_Potentials is set (they are large);
  *   - no exceptions: failures are printed to std::cout and leave NaN results.
+ *   - output files (Hydrodynamic_Solver_IO.cpp:13-645): Setup writes the
+ *     .bem header, .hst and .mmx and clears the per-frequency files; every
+ *     Solve appends its block to .bem and its rows to .1, .2, .2fk, .2sc, in
+ *     the reference's layout and number format (BEMUse_IO.h:58-73). Boundary
+ *     volume, area and centre of buoyancy follow Boundary_Base.cpp:125-228,
+ *     the mass matrix Hydrodynamic_Solver.cpp:300-337.
  * What is added: the device and arithmetic mode (constructor), accessors for
  * A, B, F, FK, SC (private members in the reference, read from its output
- * files), and Solve_Frequencies for a pipelined sweep over many frequencies.
+ * files), Solve_Frequencies for a pipelined sweep over many frequencies, and
+ * Write_Output (default true) to switch the files off.
  * Not provided: IFR lid panels, Kochin functions, exterior free-surface
- * elevation, RAOs and the output files (reported, not computed).
+ * elevation and RAOs. The files .4 (RAOs, which in the reference use an
+ * uninitialised damping matrix) and _KochinRad/_KochinDiff.bem are created
+ * empty. The waterplane moments of the hydrostatic stiffness are integrated
+ * exactly over the waterline polygon (the reference sums them over its lid
+ * panels, which are not generated here).
  */
 #ifndef BEMUSE_HYDRO_H
 #define BEMUSE_HYDRO_H
 
+#include <sys/stat.h>
+#include <sys/types.h>
+
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <complex>
+#include <fstream>
 #include <iostream>
 #include <limits>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -72,8 +89,28 @@ struct Vector3 {
 };
 
 static const Real TwoPIinv = 1.0 / (2.0 * 3.14159265358979323846);
+static const Real R2D = 180.0 / 3.14159265358979323846;
 static const Real Gravity = 9.81;      // BEMUse_Math_Types.h:115 (K and Phi_I use this constant)
 static const Real Rho_wat = 1025.0;    // BEMUse_Math_Types.h:113
+static const Real P_atm = 101325.0;    // BEMUse_Math_Types.h:112
+
+// Fixed-width number and text fields of the output files: default stream
+// formatting (6 significant digits), a leading blank for non-negative
+// numbers, blank-padded to the width.
+inline std::string Field(Real a, int width = 15) {
+    std::ostringstream os;
+    os << a;
+    std::string t = os.str();
+    if (a >= 0) t.insert(0, " ");
+    if ((int)t.size() < width) t.append(width - t.size(), ' ');
+    return t;
+}
+inline std::string Field(const std::string& str, int width = 15) {
+    std::string t = str;
+    if ((int)t.size() < width) t.append(width - t.size(), ' ');
+    return t;
+}
+inline int Make_Directory(const std::string& path) { return mkdir(path.c_str(), 0733); }
 
 // Parameter: a named int, Real or bool (Boundary_Base.h:47-55).
 struct Parameter {
@@ -110,11 +147,76 @@ public:
         if (mesh_) bem_mesh_export(mesh_, nodes.data(), elems.data());
     }
     const bem_mesh* Get_Device_Mesh() const { return mesh_; }
+
+    //--- geometric and kinematic properties (Boundary_Base.h:132-156)
+    Real Volume = 0, Surface_Area = 0, Max_Dim = 0, Mass = 0;
+    Matrix Mass_Mat = Matrix::Zero(6, 6);
+    Matrix Stiffness_Mat = Matrix::Zero(6, 6);
+    Vector3 Centre_Gravity, Centre_Buoyancy;
+    Matrix Radii_Gyration = Matrix::Zero(3, 3);
+    std::string FileInfo;
+    // waterplane area and moments int x, y, x^2, x y, y^2 dA (exact over the waterline polygon)
+    Real WP_Area = 0, WP_Mx = 0, WP_My = 0, WP_Mxx = 0, WP_Mxy = 0, WP_Myy = 0;
+    int N_Aux_Elements() const { return 0; }
+
 protected:
     void Adopt(int rc, bem_mesh* m) {
         if (mesh_) { bem_mesh_free(mesh_); mesh_ = nullptr; }
         if (rc != BEM_OK) { std::cout << "BEMUse: mesh generation failed: " << bem_last_error() << std::endl; return; }
         mesh_ = m;
+        Calculate_Properties();
+    }
+    // Calculate_GridParams, _Volume, _SurfaceArea, _COB (Boundary_Base.cpp:125-228):
+    // sums over the panels with centroid, normal (into the body) and area.
+    void Calculate_Properties() {
+        std::vector<Real> nodes;
+        std::vector<int> elems;
+        Get_Mesh(nodes, elems);
+        int nn = (int)nodes.size() / 3, ne = (int)elems.size() / 4;
+        Real mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int i = 0; i < nn; i++)
+            for (int c = 0; c < 3; c++) { mn[c] = std::min(mn[c], nodes[3 * i + c]); mx[c] = std::max(mx[c], nodes[3 * i + c]); }
+        Max_Dim = 0;
+        for (int c = 0; c < 3; c++) if (nn && mx[c] - mn[c] > Max_Dim) Max_Dim = mx[c] - mn[c];
+        std::vector<double> body(32 * (size_t)ne);
+        if (bem_geometry(mesh_, body.data(), nullptr, nullptr, nullptr, nullptr) != BEM_OK) return;
+        Real vx = 0, vy = 0, vz = 0, cb[3] = {0, 0, 0};
+        Surface_Area = 0;
+        for (int e = 0; e < ne; e++) {
+            const double* P = &body[32 * (size_t)e];
+            Real cx = P[0], cy = P[1], cz = P[2], nx = P[9], ny = P[10], nz = P[11], A = P[29];
+            vx += -cx * nx * A; vy += -cy * ny * A; vz += -cz * nz * A;
+            cb[0] -= nx * cx * cx * A; cb[1] -= ny * cy * cy * A; cb[2] -= nz * cz * cz * A;
+            Surface_Area += A;
+        }
+        Volume = (vx + vy + vz) / 3.0;
+        Real* C[3] = {&Centre_Buoyancy.x, &Centre_Buoyancy.y, &Centre_Buoyancy.z};
+        for (int c = 0; c < 3; c++) {
+            *C[c] = cb[c] * 0.5 / Volume;
+            if (std::fabs(*C[c]) < Max_Dim * 1e-6) *C[c] = 0;
+        }
+        // waterplane: the panel edges lying in z = 0, in panel node order
+        Real a = 0, m1x = 0, m1y = 0, mxx = 0, mxy = 0, myy = 0, tol = 1e-9 * std::max(Max_Dim, (Real)1);
+        for (int e = 0; e < ne; e++) {
+            int nv = elems[4 * e + 3] < 0 ? 3 : 4;
+            for (int k = 0; k < nv; k++) {
+                const Real* p = &nodes[3 * (size_t)elems[4 * e + k]];
+                const Real* q = &nodes[3 * (size_t)elems[4 * e + (k + 1) % nv]];
+                if (std::fabs(p[2]) > tol || std::fabs(q[2]) > tol) continue;
+                Real x1 = p[0], y1 = p[1], x2 = q[0], y2 = q[1], cr = x1 * y2 - x2 * y1;
+                a += cr / 2;
+                m1x += cr * (x1 + x2) / 6;
+                m1y += cr * (y1 + y2) / 6;
+                mxx += cr * (x1 * x1 + x1 * x2 + x2 * x2) / 12;
+                myy += cr * (y1 * y1 + y1 * y2 + y2 * y2) / 12;
+                mxy += cr * (x1 * y2 + 2 * x1 * y1 + 2 * x2 * y2 + x2 * y1) / 24;
+            }
+        }
+        Real sg = a < 0 ? -1 : 1;
+        WP_Area = sg * a; WP_Mx = sg * m1x; WP_My = sg * m1y; WP_Mxx = sg * mxx; WP_Mxy = sg * mxy; WP_Myy = sg * myy;
+        std::cout << "Geometry generated. " << ne << " elements, 0 auxiliary (free surface) elements. "
+                  << "0 external (free surface) elements. \n";
+        std::cout << "Surface area = " << Surface_Area << " m^2. Volume = " << Volume << " m^3" << std::endl;
     }
     bem_mesh* mesh_ = nullptr;
 };
@@ -258,9 +360,13 @@ public:
         F_i = CMatrix::Zero(6, NBeta);
         FK_i = CMatrix::Zero(6, NBeta);
         SC_i = CMatrix::Zero(6, NBeta);
+        Calculate_Mass_Matrix(B);
+        Calculate_Hydrostatic_Stiffness_Matrix(B);
+        if (Write_Output) Generate_Output_File(B);
         // the "infinite frequency" radiation solve (Hydrodynamic_Solver.cpp:295-297)
         Frequency = FreqInf;
         Solve();
+        if (Write_Output) Update_Output_File();
     }
 
     void Solve() override {
@@ -296,6 +402,20 @@ public:
                 FK_i(i, b) = CReal(FK[k], FK[k + 1]);
                 SC_i(i, b) = CReal(SC[k], SC[k + 1]);
             }
+        if (Write_Output) Update_Output_File();   // Hydrodynamic_Solver.cpp:508
+    }
+
+    //--- output files (Hydrodynamic_Solver_IO.cpp:13-31)
+    void Generate_Output_File(Boundary* B) override {
+        if (!B) return;
+        Make_Directory(OutputDirectory);
+        Make_Directory(OutputPath);
+        Generate_Output_File_BEMUse(B);
+        Generate_Output_File_WAMIT(B);
+    }
+    void Update_Output_File() override {
+        Update_Output_File_BEMUse();
+        Update_Output_File_WAMIT();
     }
 
     // Many frequencies at once (the console's loop over Set_Real + Solve),
@@ -332,8 +452,216 @@ public:
     int N_Panels() const { return NPTOT; }
 
     bool Keep_Potentials = false;
+    bool Write_Output = true;
+    const Matrix& Mass_Matrix() const { return M_ij; }
+    const Matrix& Hydrostatic_Stiffness() const { return C_ij; }
 
 private:
+    // Hydrodynamic_Solver.cpp:300-337
+    void Calculate_Mass_Matrix(Boundary* B) {
+        bool zero = true;
+        for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) zero = zero && B->Mass_Mat(i, j) == 0;
+        if (!zero) { M_ij = B->Mass_Mat; return; }
+        if (B->Mass == 0) B->Mass = B->Volume * Rho_wat;
+        M_ij = Matrix::Zero(6, 6);
+        Real m = B->Mass, xg = B->Centre_Gravity.x, yg = B->Centre_Gravity.y, zg = B->Centre_Gravity.z;
+        M_ij(0, 0) = m; M_ij(1, 1) = m; M_ij(2, 2) = m;
+        M_ij(0, 4) = m * zg; M_ij(1, 3) = -m * zg; M_ij(0, 5) = -m * yg;
+        M_ij(2, 3) = m * yg; M_ij(1, 5) = m * xg; M_ij(2, 4) = -m * xg;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) M_ij(3 + i, j) = -M_ij(i, 3 + j);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) M_ij(3 + i, 3 + j) = m * B->Radii_Gyration(i, j) * std::fabs(B->Radii_Gyration(i, j));
+    }
+    // Hydrodynamic_Solver.cpp:339-398 (WAMIT manual formulas), including the
+    // reference's final symmetry assignments C(2,3)=C(3,2), C(2,4)=C(4,2),
+    // C(3,4)=C(4,3); waterplane moments from the boundary.
+    void Calculate_Hydrostatic_Stiffness_Matrix(Boundary* B) {
+        bool zero = true;
+        for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) zero = zero && B->Stiffness_Mat(i, j) == 0;
+        if (!zero) { C_ij = B->Stiffness_Mat; return; }
+        C_ij = Matrix::Zero(6, 6);
+        Real g = Grav, rho = Rho_wat, M = B->Mass, V = B->Volume;
+        Real xb = B->Centre_Buoyancy.x, yb = B->Centre_Buoyancy.y, zb = B->Centre_Buoyancy.z;
+        Real xg = B->Centre_Gravity.x, yg = B->Centre_Gravity.y, zg = B->Centre_Gravity.z;
+        C_ij(2, 2) = Rho * g * B->WP_Area;
+        C_ij(2, 3) = Rho * g * B->WP_My;
+        C_ij(2, 4) = -Rho * g * B->WP_Mx;
+        C_ij(3, 3) = Rho * g * B->WP_Myy + rho * g * V * zb - M * g * zg;
+        C_ij(3, 4) = -Rho * g * B->WP_Mxy;
+        C_ij(3, 5) = -Rho * g * V * xb + M * g * xg;
+        C_ij(4, 4) = Rho * g * B->WP_Mxx + rho * g * V * zb - M * g * zg;
+        C_ij(4, 5) = -Rho * g * V * yb + M * g * yg;
+        C_ij(2, 3) = C_ij(3, 2);
+        C_ij(2, 4) = C_ij(4, 2);
+        C_ij(3, 4) = C_ij(4, 3);
+    }
+
+    std::string beta_row(const char* label) const {
+        std::string t = Field(std::string(label), 30);
+        for (Real b : BetaArray) t += Field(b * R2D, 30);
+        return t;
+    }
+
+    // Hydrodynamic_Solver_IO.cpp:107-225
+    void Generate_Output_File_BEMUse(Boundary* B) {
+        std::ofstream f(OutputPath + ".bem", std::ofstream::out | std::ofstream::trunc);
+        if (!f.is_open()) return;
+        f << "---------------------------------------------------------------\n";
+        f << "------------------------ BEMUse Output ------------------------\n";
+        f << "---------------------------------------------------------------\n";
+        f << "\n";
+        f << "------------ Environmental Parameters ------------\n";
+        f << "Gravity:                 " << Gravity << " m s^{-2}.\n";
+        f << "Density of Water:        " << Rho_wat << " kg m^{-3}.\n";
+        f << "Atmospheric Pressure:    " << P_atm << " Pa.\n";
+        f << "Water Depth:             Infinite.\n";
+        f << "\n";
+        f << "------------ Wave Parameters ------------\n";
+        std::string wa = "Incoming wave angles: (deg)        ";
+        for (Real b : BetaArray) wa += Field(b * R2D);
+        f << wa << std::endl;
+        f << "\n";
+        f << "------------ Grid Summary ------------\n";
+        f << B->FileInfo << std::endl;
+        f << "Input Mesh has:          " << B->N_Nodes() << " nodes.\n";
+        f << "Input Mesh has:          " << B->N_Elements() << " panels.\n";
+        f << "Input Mesh has:          " << B->N_Aux_Elements() << " auxiliary panels.\n";
+        f << "\n";
+        f << "------------ Geometry Summary ------------\n";
+        f << "Nb: This is calculated by BEMUse with the approximation of flat surface panels." << std::endl;
+        f << "With increasing mesh refinement the values are expected to approach those of the continuous geometry.\n";
+        f << "Volume:                  " << B->Volume << " m^3.\n";
+        f << "Surface area:            " << B->Surface_Area << " m^2.\n";
+        f << "Mass:                    " << B->Mass << " kg. (Assuming not pre-specified),\n";
+        f << "Centre of Mass:          [" << B->Centre_Gravity.x << ", " << B->Centre_Gravity.y << ", "
+          << B->Centre_Gravity.z << "] m.\n";
+        f << "Centre of Buoyancy:      [" << B->Centre_Buoyancy.x << ", " << B->Centre_Buoyancy.y << ", "
+          << B->Centre_Buoyancy.z << "] m.\n";
+        f << "\n";
+        f << "------------ Inertial Summary ------------\n";
+        f << "\n";
+        f << "Mass Matrix: \n";
+        for (int i = 0; i < 6; i++) {
+            f << "|";
+            for (int j = 0; j < 6; j++) f << Field(M_ij(i, j));
+            f << " |\n";
+        }
+        f << "\n";
+        f << "Hydrostatic stiffness Matrix: \n";
+        for (int i = 0; i < 6; i++) {
+            f << "|";
+            for (int j = 0; j < 6; j++) f << Field(C_ij(i, j));
+            f << " |\n";
+        }
+        f << "\n";
+        f << "------------ Output of Hydrodynamic Analysis: ------------\n";
+        f << "\n";
+        f << "\n";
+    }
+
+    // Hydrodynamic_Solver_IO.cpp:227-306
+    void Update_Output_File_BEMUse() {
+        std::ofstream f(OutputPath + ".bem", std::ios_base::app);
+        if (!f.is_open()) return;
+        f << "----------\n";
+        f << "Frequency:       " << Omega << " s^{-1}." << std::endl;
+        f << "Period:          " << Period << " s." << std::endl;
+        f << "Wavenumber:      " << Kappa << " m^{-1}." << std::endl;
+        f << "\n";
+        const char* cols[] = {" A_x", " B_x", " A_y", " B_y", " A_z", " B_z", " A_pitch", " B_pitch", " A_roll",
+                              " B_roll", " A_yaw", " B_yaw"};
+        f << Field(std::string("Motion"));
+        for (const char* c : cols) f << Field(std::string(c));
+        f << std::endl;
+        const char* rows[] = {"Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"};
+        for (int i = 0; i < 6; i++) {
+            f << Field(std::string(rows[i]));
+            for (int j = 0; j < 6; j++) f << Field(A_ij(i, j)) << Field(B_ij(i, j));
+            f << std::endl;
+        }
+        f << "\n";
+        f << "\n";
+        // force tables: the reference labels indices 3 and 4 "Pitch" and "Roll" here
+        const char* frows[] = {"Surge: ", "Sway: ", "Heave: ", "Pitch: ", "Roll: ", "Yaw: "};
+        auto table = [&](const CMatrix& X, const char* mag, const char* arg) {
+            std::string hd = Field(std::string("Motion: "), 30);
+            for (int b = 0; b < NBeta; b++) hd += Field(std::string(mag), 15) + Field(std::string(arg), 15);
+            f << beta_row("Wave Heading: (deg) ") << std::endl;
+            f << hd << std::endl;
+            for (int i = 0; i < 6; i++) {
+                std::string r = Field(std::string(frows[i]), 30);
+                for (int b = 0; b < NBeta; b++) r += Field(std::abs(X(i, b)) / Gravity, 15) + Field(std::arg(X(i, b)), 15);
+                f << r << std::endl;
+            }
+            f << "\n";
+            f << "\n";
+        };
+        f << "Froude-Krylov Forces:\n";
+        table(FK_i, " Mag(Fh)", " Arg(Fh)");
+        f << "Excitation Forces:\n";
+        table(F_i, " Mag(Xh)", " Arg(Xh)");
+        f << "----------";
+        f << "\n";
+        f << "\n";
+    }
+
+    // Hydrodynamic_Solver_IO.cpp:430-500 (files cleared; .hst, .mmx written)
+    void Generate_Output_File_WAMIT(Boundary* B) {
+        const char* ext[] = {".1", ".2", ".2fk", ".2sc", ".4", ".6", ".hst", ".mmx", "_KochinRad.bem", "_KochinDiff.bem"};
+        for (const char* e : ext) std::ofstream(OutputPath + e, std::ofstream::out | std::ofstream::trunc).close();
+        {
+            std::ofstream f(OutputPath + ".hst");
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++)
+                    f << "     " << i + 1 << "     " << j + 1 << "   " << Field(C_ij(i, j) / Rho / Gravity) << std::endl;
+        }
+        {
+            std::ofstream f(OutputPath + ".mmx");
+            f << std::endl;
+            f << "Gravity:     " << Grav << "     Length scale:        1.00000" << std::endl;
+            f << std::endl;
+            f << "Volume     " << B->Volume << std::endl;
+            f << "Center of Buoyancy (Xb,Yb,Zb):   " << Field(B->Centre_Buoyancy.x) << Field(B->Centre_Buoyancy.y)
+              << Field(B->Centre_Buoyancy.z) << std::endl;
+            f << "Center of Gravity  (Xg,Yg,Zg):   " << Field(B->Centre_Gravity.x) << Field(B->Centre_Gravity.y)
+              << Field(B->Centre_Gravity.z) << std::endl;
+            f << std::endl;
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++)
+                    f << "     " << i + 1 << "     " << j + 1 << "   " << Field(M_ij(i, j) / Rho) << std::endl;
+        }
+    }
+
+    // Hydrodynamic_Solver_IO.cpp:502-581 (radiation and Haskind force files)
+    void Update_Output_File_WAMIT() {
+        {
+            std::ofstream f(OutputPath + ".1", std::ios_base::app);
+            for (int i = 0; i < 6; i++)
+                for (int j = 0; j < 6; j++) {
+                    if (Omega == 0.0)
+                        f << Field(0.0) << i + 1 << "     " << j + 1 << "     " << Field(A_ij(i, j)) << std::endl;
+                    else if (Frequency == FreqInf)
+                        f << Field(-1.0) << i + 1 << "     " << j + 1 << "     " << Field(A_ij(i, j)) << std::endl;
+                    else
+                        f << Field(Frequency) << i + 1 << "     " << j + 1 << "     " << Field(A_ij(i, j))
+                          << Field(B_ij(i, j)) << std::endl;
+                }
+        }
+        auto forces = [&](const char* ext, const CMatrix& X, Real sg) {
+            std::ofstream f(OutputPath + ext, std::ios_base::app);
+            for (int j = 0; j < NBeta; j++)
+                for (int i = 0; i < 6; i++) {
+                    if (Omega == 0.0 || Frequency == FreqInf) continue;
+                    f << Field(Frequency) << Field(BetaArray[j] * R2D) << i + 1 << "  " << Field(std::abs(X(i, j)) / Gravity)
+                      << Field(sg * std::arg(X(i, j))) << Field(std::real(X(i, j)) / Gravity)
+                      << Field(sg * std::imag(X(i, j)) / Gravity) << std::endl;
+                }
+        };
+        forces(".2", F_i, 1.0);
+        forces(".2fk", FK_i, 1.0);
+        forces(".2sc", SC_i, -1.0);   // the reference writes -arg and -Im for the scattering force
+    }
+
     bool check(int rc) {
         if (rc == BEM_OK) return true;
         std::cout << "BEMUse: " << bem_last_error() << std::endl;
@@ -362,6 +690,7 @@ private:
     std::vector<Real> BetaArray;
     Real FreqInf = -1.0;
     Matrix A_ij = Matrix::Zero(6, 6), B_ij = Matrix::Zero(6, 6);
+    Matrix M_ij = Matrix::Zero(6, 6), C_ij = Matrix::Zero(6, 6);
     CMatrix F_i, FK_i, SC_i;
     bem_solve_info Info = bem_solve_info();
 };

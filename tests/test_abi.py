@@ -103,7 +103,7 @@ def test_host_geometry_matches_oracle():
     np.testing.assert_allclose(gen[:, 6], og["area"], rtol=1e-15)
 
 
-def test_cpp_dropin_without_gpu_reports_and_returns_nan():
+def test_cpp_dropin_without_gpu_reports_and_returns_nan(tmp_path):
     """The drop-in follows the reference's error behaviour: no exception, a
     message on stdout, NaN results (no GPU in this environment)."""
     import subprocess
@@ -113,8 +113,48 @@ def test_cpp_dropin_without_gpu_reports_and_returns_nan():
     exe = os.path.join(ROOT, "tests", "cpp", "hydro_dropin")
     if not os.path.exists(exe):
         pytest.skip("tests/cpp/hydro_dropin not built")
-    p = subprocess.run([exe, "0"], capture_output=True, text=True, timeout=120)
+    p = subprocess.run([exe, "0"], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert p.returncode == 0
     assert "BEMUse:" in p.stdout
     a_lines = [l for l in p.stdout.splitlines() if l.startswith("A ")]
     assert a_lines and all(v == "nan" or v == "-nan" for v in a_lines[0].split()[2:])
+
+
+def test_dropin_boundary_properties_closed_forms():
+    """Volume, area, centre of buoyancy (Boundary_Base.cpp:177-228) and the
+    exact waterplane moments of the drop-in's boundaries, against closed
+    forms (a box; a regular 16-gon prism), and the output field format
+    (BEMUse_IO.h:58-73)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "boundary_props")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/boundary_props not built")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60, check=True).stdout
+    v = {}
+    for line in out.splitlines():
+        tok = line.split(None, 1)
+        if len(tok) == 2 and not line.startswith(("Geometry", "Surface")):
+            v[tok[0]] = tok[1]
+    f = {k: float(x) for k, x in v.items() if not k.startswith("field")}
+    L, W, D = 20.0, 10.0, 5.0
+    assert abs(f["box_volume"] - L * W * D) < 1e-9
+    assert abs(f["box_area"] - (L * W + 2 * (L + W) * D)) < 1e-9
+    assert abs(f["box_cob_z"] + D / 2) < 1e-12 and f["box_cob_x"] == 0 and f["box_cob_y"] == 0
+    assert abs(f["box_wp_area"] - L * W) < 1e-9
+    assert abs(f["box_wp_mxx"] - W * L ** 3 / 12) < 1e-8 and abs(f["box_wp_myy"] - L * W ** 3 / 12) < 1e-8
+    assert abs(f["box_wp_mx"]) < 1e-9 and abs(f["box_wp_my"]) < 1e-9 and abs(f["box_wp_mxy"]) < 1e-9
+    n, R, T = 16, 5.0, 8.0
+    th = 2 * np.pi * np.arange(n) / n
+    x, y = R * np.cos(th), R * np.sin(th)
+    x2, y2 = np.roll(x, -1), np.roll(y, -1)
+    cr = x * y2 - x2 * y
+    area = cr.sum() / 2
+    ixx = (cr * (x * x + x * x2 + x2 * x2)).sum() / 12
+    assert abs(f["cyl_wp_area"] - area) < 1e-10
+    assert abs(f["cyl_volume"] - area * T) < 1e-9
+    assert abs(f["cyl_cob_z"] + T / 2) < 1e-12
+    assert abs(f["cyl_wp_mxx"] - ixx) < 1e-9 and abs(f["cyl_wp_myy"] - ixx) < 1e-9
+    assert v["field_pos"] == "[ 1.5           ]"
+    assert v["field_neg"] == "[-2.25          ]"
+    assert v["field_small"] == "[ 1e-07         ]"
+    assert v["field_txt"] == "[Surge:" + " " * 24 + "]"

@@ -174,12 +174,12 @@ def test_solve_vs_committed_fixture(ctx):
     body.close()
 
 
-def _run_dropin(mode):
+def _run_dropin(mode, cwd=None):
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "hydro_dropin")
     assert os.path.exists(exe), "tests/cpp/hydro_dropin is not built (make -C tests/cpp)"
-    out = subprocess.run([exe, str(mode)], capture_output=True, text=True, timeout=300, check=True).stdout
+    out = subprocess.run([exe, str(mode)], capture_output=True, text=True, timeout=300, check=True, cwd=cwd).stdout
     res = {"stdout": out}
     for line in out.splitlines():
         tok = line.split()
@@ -199,17 +199,48 @@ def _run_dropin(mode):
 
 
 @pytest.mark.parametrize("mode,tol", [(bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)])
-def test_cpp_dropin_console_flow(cyl, mode, tol):
+def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
     """The C++ drop-in (include/bemuse_hydro.h), driven like the reference
-    console, against the oracle: the FreqInf solve of Setup and two frequencies."""
+    console, against the oracle: the FreqInf solve of Setup and two frequencies;
+    and its output files (.1, .2) against the same results to the printed
+    6 significant digits."""
     mesh, body, ob = cyl
-    res = _run_dropin(mode)
+    res = _run_dropin(mode, cwd=str(tmp_path))
     assert res["N"] == body.N, res["stdout"]
     om = -1.0 / (2 * np.pi)
     r = ob.solve(om, kappa=om * om / 9.81, compose=0, rad_only=True)
     assert relerr(res["setup_A"], r["A"]) < tol
+    refs = {}
     for w in (0.4, 1.3):
         g = res[w]
         r = ob.solve(w, betas=[0.0, np.pi / 4])
+        refs[w] = r
         for key in ("A", "B", "F"):
             assert relerr(g[key], r[key]) < tol, (w, key, relerr(g[key], r[key]))
+    # WAMIT-style files: .1 rows "F i j A B" (F = 2 pi omega), FreqInf rows "-1 i j A"
+    out = tmp_path / "Output"
+    rows1 = [l.split() for l in (out / "dropin_test.1").read_text().splitlines() if l.strip()]
+    assert len(rows1) == 36 * 3
+    assert all(r[0] == "-1" for r in rows1[:36])
+    for k, w in enumerate((0.4, 1.3)):
+        blk = rows1[36 * (k + 1): 36 * (k + 2)]
+        assert abs(float(blk[0][0]) - 2 * np.pi * w) < 1e-5 * 2 * np.pi * w
+        A = np.array([float(r[3]) for r in blk]).reshape(6, 6)
+        B = np.array([float(r[4]) for r in blk]).reshape(6, 6)
+        scale = np.abs(refs[w]["A"]).max()
+        assert np.abs(A - refs[w]["A"]).max() < 1e-5 * scale
+        assert np.abs(B - refs[w]["B"]).max() < 1e-5 * np.abs(refs[w]["B"]).max() + 1e-5 * scale
+    rows2 = [l.split() for l in (out / "dropin_test.2").read_text().splitlines() if l.strip()]
+    assert len(rows2) == 2 * 2 * 6            # 2 frequencies x 2 headings x 6 DOF
+    for k, w in enumerate((0.4, 1.3)):
+        for r in rows2[12 * k: 12 * (k + 1)]:
+            beta = np.deg2rad(float(r[1]))
+            b = int(np.argmin(np.abs(np.array([0.0, np.pi / 4]) - beta)))
+            i = int(r[2]) - 1
+            Fref = refs[w]["F"][i, b]
+            mag = abs(Fref) / 9.81
+            assert abs(float(r[3]) - mag) <= 1e-5 * np.abs(refs[w]["F"]).max() / 9.81 + 1e-12
+    for ext in (".bem", ".2fk", ".2sc", ".hst", ".mmx"):
+        assert (out / ("dropin_test" + ext)).exists()
+    bem = (out / "dropin_test.bem").read_text()
+    assert bem.count("Frequency:") == 3 and "Excitation Forces:" in bem
