@@ -90,6 +90,7 @@ _SIGS = {
     "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_gf_coefficients": (c_int, [_PD, c_int]),
+    "bem_mesh_read": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(c_void_p)]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_solve_sweep": (c_int, [c_void_p, c_void_p, c_int, c_int, _PD, _PD, c_int, c_int, _PD, c_int, _PD, _PD,
@@ -163,6 +164,11 @@ class Mesh:
         nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
         elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 4)
         return cls._make(lib().bem_mesh_from_arrays, len(nodes), _dp(nodes), len(elems), _ip(elems))
+
+    @classmethod
+    def read(cls, path, fmt=0):
+        """GDF / MAR / PNL / ASCII STL file (fmt 0: by extension)."""
+        return cls._make(lib().bem_mesh_read, os.fsencode(path), int(fmt))
 
     @classmethod
     def half_cylinder(cls, radius, draft, n_radial, n_vertical, n_azimuthal, cosine=True):

@@ -12,7 +12,9 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cctype>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -307,6 +309,29 @@ int bem_mesh_from_arrays(int n_nodes, const double* nodes, int n_elems, const in
         }
     *out = m;
     return BEM_OK;
+}
+
+static int wrap_mesh(Mesh&& mm, bem_mesh** out);
+
+int bem_mesh_read(const char* path, int format, bem_mesh** out) {
+    if (!path || !out) return fail(BEM_ERR_ARG, "bem_mesh_read: bad arguments");
+    std::string p(path);
+    if (format == BEM_MESH_AUTO) {
+        std::string ext = p.size() >= 4 ? p.substr(p.size() - 4) : "";
+        for (auto& ch : ext) ch = (char)std::tolower((unsigned char)ch);
+        format = ext == ".gdf" ? BEM_MESH_GDF : ext == ".mar" ? BEM_MESH_MAR : ext == ".pnl" ? BEM_MESH_PNL
+               : ext == ".stl" ? BEM_MESH_STL : -1;
+        if (format < 0) return fail(BEM_ERR_ARG, "bem_mesh_read: unknown extension of " + p);
+    }
+    std::ifstream in(p);
+    if (!in.is_open()) return fail(BEM_ERR_ARG, "bem_mesh_read: cannot open " + p);
+    Mesh m;
+    std::string err;
+    bool ok = format == BEM_MESH_GDF ? read_gdf(in, m, err) : format == BEM_MESH_MAR ? read_mar(in, m, err)
+            : format == BEM_MESH_PNL ? read_pnl(in, m, err) : format == BEM_MESH_STL ? read_stl(in, m, err)
+            : (err = "unknown format", false);
+    if (!ok) return fail(BEM_ERR_ARG, "bem_mesh_read: " + p + ": " + err);
+    return wrap_mesh(std::move(m), out);
 }
 
 static int wrap_mesh(Mesh&& mm, bem_mesh** out) {

@@ -4,6 +4,8 @@
 #include <cstdint>
 #include <random>
 #include <string>
+#include <istream>
+#include <string>
 #include <vector>
 
 namespace bemuse {
@@ -20,6 +22,12 @@ struct Mesh {
     void add_tri(int a, int b, int c);
     void append(const Mesh& o, double dx, double dy, double dz);
 };
+
+// Mesh files (GDF, MAR, PNL, ASCII STL), as the reference's readers parse them.
+bool read_gdf(std::istream& in, Mesh& m, std::string& err);
+bool read_mar(std::istream& in, Mesh& m, std::string& err);
+bool read_pnl(std::istream& in, Mesh& m, std::string& err);
+bool read_stl(std::istream& in, Mesh& m, std::string& err);
 
 Mesh mesh_revolution(const std::vector<double>& r, const std::vector<double>& z, int NA);
 Mesh mesh_half_cylinder(double R, double draft, int NR, int NV, int NA, bool cosine);

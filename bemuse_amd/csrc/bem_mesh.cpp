@@ -15,9 +15,13 @@
 //     (src/Boundary/Barge.cpp:34-98).
 #include "bem_internal.h"
 
+#include <array>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
+#include <istream>
 #include <string>
+#include <vector>
 
 namespace bemuse {
 
@@ -219,6 +223,175 @@ Mesh mesh_random_body(uint64_t seed, int index, double* params /*6: type, a, b, 
         params[4] = target; params[5] = m.n_elems();
     }
     return m;
+}
+
+}  // namespace bemuse
+
+// ---------------------------------------------------------------------------
+// Mesh files (src/Boundary/GDF_Geo.cpp, MAR_Geo.cpp, PNL_Geo.cpp, STL_Geo.cpp).
+// Lines are split into whitespace-separated fields (the reference splits on
+// single blanks; tabs and carriage returns are also separators here).
+// ---------------------------------------------------------------------------
+namespace bemuse {
+namespace {
+
+std::vector<std::string> fields(const std::string& line) {
+    std::vector<std::string> t;
+    std::string cur;
+    for (char c : line) {
+        if (c == ' ' || c == '\t' || c == '\r' || c == '\n') {
+            if (!cur.empty()) { t.push_back(cur); cur.clear(); }
+        } else {
+            cur.push_back(c);
+        }
+    }
+    if (!cur.empty()) t.push_back(cur);
+    return t;
+}
+
+bool to_double(const std::string& s, double& v) {
+    char* end = nullptr;
+    v = std::strtod(s.c_str(), &end);
+    return end != s.c_str();
+}
+bool to_int(const std::string& s, int& v) {
+    char* end = nullptr;
+    long x = std::strtol(s.c_str(), &end, 10);
+    v = (int)x;
+    return end != s.c_str();
+}
+
+}  // namespace
+
+// GDF (WAMIT): every line of exactly three numbers is a vertex, four vertices
+// per panel, a fourth vertex equal to the third makes a triangle; nodes are not
+// shared between panels (GDF_Geo.cpp:12-103).
+bool read_gdf(std::istream& in, Mesh& m, std::string& err) {
+    std::string line;
+    double p[4][3];
+    int count = 0;
+    while (std::getline(in, line)) {
+        auto f = fields(line);
+        if (f.size() != 3) continue;
+        double v[3];
+        if (!to_double(f[0], v[0]) || !to_double(f[1], v[1]) || !to_double(f[2], v[2])) continue;
+        for (int c = 0; c < 3; c++) p[count][c] = v[c];
+        if (++count == 4) {
+            int id[4];
+            for (int k = 0; k < 3; k++) id[k] = m.add_node(p[k][0], p[k][1], p[k][2]);
+            bool tri = p[3][0] == p[2][0] && p[3][1] == p[2][1] && p[3][2] == p[2][2];
+            if (tri) {
+                m.add_tri(id[0], id[1], id[2]);
+            } else {
+                id[3] = m.add_node(p[3][0], p[3][1], p[3][2]);
+                m.add_quad(id[0], id[1], id[2], id[3]);
+            }
+            count = 0;
+        }
+    }
+    if (m.n_elems() == 0) { err = "no panels found"; return false; }
+    return true;
+}
+
+// MAR (NEMOH): two-field lines (symmetry) are skipped; four-field lines are
+// nodes "id x y z" until a line with id 0, then connectivity "n1 n2 n3 n4"
+// (1-based, n3 == n4 for a triangle); the symmetry flag is not applied, as
+// in the reference (MAR_Geo.cpp:12-97).
+bool read_mar(std::istream& in, Mesh& m, std::string& err) {
+    std::string line;
+    bool conn = false;
+    std::vector<std::array<int, 4>> el;
+    while (std::getline(in, line)) {
+        auto f = fields(line);
+        if (f.size() != 4) continue;
+        int id;
+        if (!to_int(f[0], id)) continue;
+        if (id == 0 && !conn) { conn = true; continue; }
+        if (conn) {
+            int n[4];
+            for (int k = 0; k < 4; k++)
+                if (!to_int(f[k], n[k])) { err = "bad connectivity line: " + line; return false; }
+            if (n[0] == 0) continue;   // closing "0 0 0 0"
+            el.push_back({n[0] - 1, n[1] - 1, n[2] - 1, f[2] == f[3] ? -1 : n[3] - 1});
+        } else {
+            double v[3];
+            for (int k = 0; k < 3; k++)
+                if (!to_double(f[k + 1], v[k])) { err = "bad node line: " + line; return false; }
+            m.add_node(v[0], v[1], v[2]);
+        }
+    }
+    for (auto& e : el) {
+        for (int k = 0; k < 4; k++)
+            if ((k < 3 || e[3] >= 0) && (e[k] < 0 || e[k] >= m.n_nodes())) { err = "node index out of range"; return false; }
+        if (e[3] < 0) m.add_tri(e[0], e[1], e[2]); else m.add_quad(e[0], e[1], e[2], e[3]);
+    }
+    if (m.n_elems() == 0) { err = "no panels found"; return false; }
+    return true;
+}
+
+// PNL (HAMS): sections marked by "Number of Panels", "Start/End Definition of
+// Node Coordinates" and "... Node Relations"; node lines "id x y z", panel
+// lines "id nv n1 n2 n3 [n4]" (1-based); symmetry flags are read but not
+// applied, as in the reference (PNL_Geo.cpp:16-131).
+bool read_pnl(std::istream& in, Mesh& m, std::string& err) {
+    std::string line;
+    int pos = 0;
+    std::vector<std::array<int, 4>> el;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        if (line.find("Number of Panels") != std::string::npos) { pos = 1; continue; }
+        if (line.find("Start Definition of Node Coordinates") != std::string::npos) { pos = 2; continue; }
+        if (line.find("End Definition of Node Coordinates") != std::string::npos) { pos = 3; continue; }
+        if (line.find("Start Definition of Node Relations") != std::string::npos) { pos = 4; continue; }
+        if (line.find("End Definition of Node Relations") != std::string::npos) { pos = 5; continue; }
+        auto f = fields(line);
+        if (pos == 2 && f.size() >= 4) {
+            double v[3];
+            for (int k = 0; k < 3; k++)
+                if (!to_double(f[k + 1], v[k])) { err = "bad node line: " + line; return false; }
+            m.add_node(v[0], v[1], v[2]);
+        } else if (pos == 4 && f.size() >= 5) {
+            int nv, n[4] = {0, 0, 0, 0};
+            if (!to_int(f[1], nv) || (nv != 3 && nv != 4) || (int)f.size() < 2 + nv) { err = "bad panel line: " + line; return false; }
+            for (int k = 0; k < nv; k++)
+                if (!to_int(f[2 + k], n[k])) { err = "bad panel line: " + line; return false; }
+            el.push_back({n[0] - 1, n[1] - 1, n[2] - 1, nv == 4 ? n[3] - 1 : -1});
+        }
+    }
+    for (auto& e : el) {
+        for (int k = 0; k < 4; k++)
+            if ((k < 3 || e[3] >= 0) && (e[k] < 0 || e[k] >= m.n_nodes())) { err = "node index out of range"; return false; }
+        if (e[3] < 0) m.add_tri(e[0], e[1], e[2]); else m.add_quad(e[0], e[1], e[2], e[3]);
+    }
+    if (m.n_elems() == 0) { err = "no panels found"; return false; }
+    return true;
+}
+
+// ASCII STL: three "vertex x y z" lines per facet. As in the reference the
+// coordinates are scaled by 100 and the vertex order is reversed
+// (STL_Geo.cpp:75, 88-91); nodes are not shared.
+bool read_stl(std::istream& in, Mesh& m, std::string& err) {
+    std::string line;
+    std::vector<std::array<double, 3>> v;
+    while (std::getline(in, line)) {
+        auto f = fields(line);
+        if (f.empty()) continue;
+        if (f[0] == "facet") v.clear();
+        if (f[0] == "vertex" && f.size() >= 4) {
+            std::array<double, 3> p;
+            for (int k = 0; k < 3; k++)
+                if (!to_double(f[k + 1], p[k])) { err = "bad vertex line: " + line; return false; }
+            v.push_back(p);
+            if (v.size() == 3) {
+                int id[3];
+                for (int k = 0; k < 3; k++) id[k] = m.add_node(100.0 * v[k][0], 100.0 * v[k][1], 100.0 * v[k][2]);
+                m.add_tri(id[2], id[1], id[0]);
+                v.clear();
+            }
+        }
+    }
+    if (m.n_elems() == 0) { err = "no facets found"; return false; }
+    return true;
 }
 
 }  // namespace bemuse

@@ -89,6 +89,12 @@ int bem_mesh_cone_spar(double r_top, double r_bot, double draft, double r_plate,
 /* Body `index` of the seeded uneven batch; params (6, optional): type (0 cyl,
  * 1 barge), dim a, dim b, draft, target panels, actual panels. */
 int bem_mesh_random_body(uint64_t seed, int index, double* params, bem_mesh** out);
+/* Mesh files as the reference reads them: GDF (GDF_Geo.cpp:12-103), MAR
+ * (MAR_Geo.cpp:12-97), PNL (PNL_Geo.cpp:16-131), ASCII STL (STL_Geo.cpp:12-96,
+ * coordinates x100 and reversed vertex order, as there). format: BEM_MESH_*;
+ * AUTO picks by extension. */
+enum { BEM_MESH_AUTO = 0, BEM_MESH_GDF = 1, BEM_MESH_MAR = 2, BEM_MESH_PNL = 3, BEM_MESH_STL = 4 };
+int bem_mesh_read(const char* path, int format, bem_mesh** out);
 int bem_mesh_size(const bem_mesh* m, int* n_nodes, int* n_elems);
 int bem_mesh_export(const bem_mesh* m, double* nodes, int* elems);
 void bem_mesh_free(bem_mesh* m);

@@ -26,6 +26,8 @@
  *     the reference's layout and number format (BEMUse_IO.h:58-73). Boundary
  *     volume, area and centre of buoyancy follow Boundary_Base.cpp:125-228,
  *     the mass matrix Hydrodynamic_Solver.cpp:300-337.
+ *   - mesh files through GDF_Geometry, MAR_Geometry, PNL_Geometry and
+ *     STL_Geometry (Read_Input_File + Setup), parsed as the reference does;
  * What is added: the device and arithmetic mode (constructor), accessors for
  * A, B, F, FK, SC (private members in the reference, read from its output
  * files), Solve_Frequencies for a pipelined sweep over many frequencies, and
@@ -230,6 +232,40 @@ public:
         Adopt(rc, m);   // rc first: the call must complete before m is read
     }
 };
+
+// Imported meshes (GDF_Geo.h, MAR_Geo.h, PNL_Geo.h, STL_Geo.h): Read_Input_File
+// then Setup, as in the reference; the file is parsed as the reference's
+// readers parse it (see bem_mesh_read).
+class File_Geometry : public Boundary {
+public:
+    explicit File_Geometry(int format) : format_(format) {}
+    virtual bool Read_Input_File(std::string& FilePath) {
+        bem_mesh* m = nullptr;
+        int rc = bem_mesh_read(FilePath.c_str(), format_, &m);
+        if (rc != BEM_OK) {
+            std::cout << FilePath << " . " << bem_last_error() << std::endl;
+            return false;
+        }
+        if (pending_) bem_mesh_free(pending_);
+        pending_ = m;
+        FileInfo = "Geometry imported from file: " + FilePath;
+        return true;
+    }
+    void Setup() override {
+        if (!pending_) { std::cout << "BEMUse: no geometry file read" << std::endl; return; }
+        bem_mesh* m = pending_;
+        pending_ = nullptr;
+        Adopt(BEM_OK, m);
+    }
+    ~File_Geometry() override { if (pending_) bem_mesh_free(pending_); }
+private:
+    int format_;
+    bem_mesh* pending_ = nullptr;
+};
+class GDF_Geometry : public File_Geometry { public: GDF_Geometry() : File_Geometry(BEM_MESH_GDF) {} };
+class MAR_Geometry : public File_Geometry { public: MAR_Geometry() : File_Geometry(BEM_MESH_MAR) {} };
+class PNL_Geometry : public File_Geometry { public: PNL_Geometry() : File_Geometry(BEM_MESH_PNL) {} };
+class STL_Geometry : public File_Geometry { public: STL_Geometry() : File_Geometry(BEM_MESH_STL) {} };
 
 // Half_Cylinder (Volume_of_Revolution.cpp:167-199): open-top vertical cylinder
 // with a bottom disc (triangle fan at the centre).
