@@ -91,6 +91,8 @@ _SIGS = {
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_gf_coefficients": (c_int, [_PD, c_int]),
     "bem_mesh_read": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(c_void_p)]),
+    "bem_solve_batch": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), _PI, _PD, c_int, c_int, _PD, c_int, _PD,
+                                _PD, _PD, c_void_p]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_solve_sweep": (c_int, [c_void_p, c_void_p, c_int, c_int, _PD, _PD, c_int, c_int, _PD, c_int, _PD, _PD,
@@ -265,6 +267,33 @@ class Context:
         _check(lib().bem_dense_solve(self._h, precision, n, A.ctypes.data_as(_PD), nrhs, B.ctypes.data_as(_PD),
                                      X.ctypes.data_as(_PD), ctypes.byref(info)))
         return X, info.as_dict()
+
+    def solve_batch(self, bodies, omegas, betas=(), mode=1, lanes=4):
+        """Frequencies of many bodies (omegas: one array per body) on the
+        context's shared lanes; returns one dict (A, B, F, info per
+        frequency) per body, in order."""
+        bodies = list(bodies)
+        omegas = [np.ascontiguousarray(w, dtype=np.float64).ravel() for w in omegas]
+        assert len(omegas) == len(bodies)
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        nb = len(betas)
+        nf = np.array([len(w) for w in omegas], dtype=np.int32)
+        allw = np.concatenate(omegas) if len(omegas) else np.zeros(0)
+        tot = int(nf.sum())
+        A = np.zeros((tot, 36)); B = np.zeros((tot, 36))
+        F = np.zeros((tot, 6, max(nb, 1)), dtype=np.complex128)
+        infos = (SolveInfo * max(tot, 1))()
+        hs = (c_void_p * len(bodies))(*[b._h.value for b in bodies])
+        _check(lib().bem_solve_batch(self._h, len(bodies), hs, nf.ctypes.data_as(_PI), _dp(allw), mode, nb,
+                                     _dp(betas) if nb else None, lanes, _dp(A), _dp(B),
+                                     F.ctypes.data_as(_PD), infos))
+        out, o = [], 0
+        for k in range(len(bodies)):
+            sl = slice(o, o + nf[k])
+            out.append(dict(A=A[sl].reshape(-1, 6, 6), B=B[sl].reshape(-1, 6, 6), F=F[sl][:, :, :nb],
+                            info=[infos[i].as_dict() for i in range(o, o + nf[k])]))
+            o += nf[k]
+        return out
 
 
 class Body:

@@ -61,11 +61,14 @@ struct bem_ctx {
     void* surrogate = nullptr;         // trained wave-term surrogate (none yet)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
+    std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch
 };
 
-// Dense system workspace for one padded size Np.
+// Dense system workspace. Buffers are allocated for a capacity capNp; N and
+// Np are those of the body currently bound (Np <= capNp), and every kernel
+// uses Np as the leading dimension, so one workspace serves any smaller body.
 struct LinSys {
-    int N = 0, Np = 0;
+    int N = 0, Np = 0, capNp = 0;
     double *Dr = nullptr, *Di = nullptr;          // FP64 system matrix (LU in place in FP64 mode)
     float *D32r = nullptr, *D32i = nullptr;       // FP32 copy factored in mixed mode
     int *ipiv = nullptr, *flag = nullptr;
@@ -94,7 +97,7 @@ struct LinSys {
     }
     template <typename T>
     int work(LuWork<T>& w) {
-        size_t pan = (size_t)Np * kLuNb;
+        size_t pan = (size_t)capNp * kLuNb;
         int r = BEM_OK;
         if ((r = get(&w.sd_r, kLuNb * kLuNb)) || (r = get(&w.sd_i, kLuNb * kLuNb)) || (r = get(&w.s21_r, pan)) ||
             (r = get(&w.s21_i, pan)) || (r = get(&w.Linv_r, pan)) || (r = get(&w.Linv_i, pan)) ||
@@ -104,7 +107,7 @@ struct LinSys {
         return BEM_OK;
     }
     int init(int n) {
-        N = n; Np = round_up(n, kPad);
+        N = n; Np = round_up(n, kPad); capNp = Np;
         size_t NN = (size_t)Np * Np, NR = (size_t)Np * kMaxRhs;
         int r = BEM_OK;
         if ((r = get(&Dr, NN)) || (r = get(&Di, NN))) return r;
@@ -120,10 +123,10 @@ struct LinSys {
     }
     int init32() {
         if (have32) return BEM_OK;
-        size_t NN = (size_t)Np * Np, NR = (size_t)Np * kMaxRhs;
+        size_t NN = (size_t)capNp * capNp, NR = (size_t)capNp * kMaxRhs;
         int r = BEM_OK;
         if ((r = get(&D32r, NN)) || (r = get(&D32i, NN))) return r;
-        if ((r = get(&ipiv32, Np)) || (r = get(&flag32, Np / kLuNb + 1))) return r;
+        if ((r = get(&ipiv32, capNp)) || (r = get(&flag32, capNp / kLuNb + 1))) return r;
         if ((r = work(w32))) return r;
         if ((r = get(&X32r, NR)) || (r = get(&X32i, NR))) return r;
         have32 = true;
@@ -430,6 +433,8 @@ int bem_create(int device, bem_ctx** out) {
 void bem_destroy(bem_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    for (Lane* l : c->pool) delete l;
+    c->pool.clear();
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -529,7 +534,10 @@ void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
 // The betas must already be on the device (L.d_betas).
 int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, double kappa, int compose, int rad_only,
                     int nb) {
+    if (b->Np > L.sys.capNp) return fail(BEM_ERR_STATE, "lane workspace smaller than the body");
     LinSys& s = L.sys;
+    s.N = b->N;
+    s.Np = b->Np;
     hipStream_t st = L.s1;
     L.pending = index;
     L.mode = mode; L.prec = (mode == BEM_MODE_FP64) ? 64 : 32;
@@ -618,6 +626,55 @@ int check_solve_args(bem_ctx* c, bem_body* b, int mode, int compose, int nb, con
     return BEM_OK;
 }
 
+// One (body, frequency) of a sweep or batch; `out` is its result slot.
+struct Task {
+    bem_body* b;
+    int out;
+    double omega, kappa;
+};
+
+// Runs the tasks in order on the lanes: each goes to a free lane, else to the
+// first lane whose previous task has completed, else (all busy) to the lane
+// that has been busy longest, after that lane's results are collected.
+int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, int mode, int compose, int nb,
+              double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* infos) {
+    size_t fs = (size_t)6 * nb * 2;
+    int nl = (int)lanes.size();
+    std::vector<int> busy(nl, -1);
+    std::vector<unsigned long long> since(nl, 0);
+    unsigned long long seq = 0;
+    size_t next = 0;
+    for (;;) {
+        int pick = -1;
+        if (next < tasks.size())
+            for (int l = 0; l < nl && pick < 0; l++)
+                if (busy[l] < 0) pick = l;
+        for (int l = 0; l < nl && pick < 0; l++)
+            if (busy[l] >= 0 && hipEventQuery(lanes[l]->done) == hipSuccess) pick = l;
+        if (pick < 0)   // every lane busy and none finished: the one busy longest
+            for (int l = 0; l < nl; l++)
+                if (busy[l] >= 0 && (pick < 0 || since[l] < since[pick])) pick = l;
+        if (pick < 0) break;   // nothing in flight and nothing left
+        int r;
+        if (busy[pick] >= 0) {
+            const Task& t = tasks[busy[pick]];
+            size_t f = (size_t)t.out;
+            r = harvest(t.b, *lanes[pick], A36 + 36 * f, B36 + 36 * f, F ? F + fs * f : nullptr,
+                        FK ? FK + fs * f : nullptr, SC ? SC + fs * f : nullptr, infos ? infos + f : nullptr);
+            if (r) return r;
+            busy[pick] = -1;
+        }
+        if (next < tasks.size()) {
+            const Task& t = tasks[next];
+            if ((r = issue_frequency(t.b, *lanes[pick], (int)next, mode, t.omega, t.kappa, compose, 0, nb))) return r;
+            busy[pick] = (int)next;
+            since[pick] = seq++;
+            next++;
+        }
+    }
+    return BEM_OK;
+}
+
 int upload_betas(Lane& L, int nb, const double* betas) {
     if (nb > 0) {
         HIP_TRY(hipMemcpyAsync(L.d_betas, betas, nb * sizeof(double), hipMemcpyHostToDevice, L.s1));
@@ -665,22 +722,58 @@ int bem_solve_sweep(bem_ctx* c, bem_body* b, int mode, int nf, const double* ome
     if ((r = ensure_lanes(b, lanes, mode != BEM_MODE_FP64))) return r;
     for (int l = 0; l < lanes; l++)
         if ((r = upload_betas(*b->lanes[l], nb, betas))) return r;
-    size_t fs = (size_t)6 * nb * 2;
-    auto collect = [&](Lane& L) -> int {
-        int f = L.pending;
-        return harvest(b, L, A36 + 36 * (size_t)f, B36 + 36 * (size_t)f, F ? F + fs * f : nullptr,
-                       FK ? FK + fs * f : nullptr, SC ? SC + fs * f : nullptr, infos ? infos + f : nullptr);
-    };
-    for (int f = 0; f < nf; f++) {
-        Lane& L = *b->lanes[f % lanes];
-        if (L.pending >= 0 && (r = collect(L))) return r;
-        double kap = kappas ? kappas[f] : omegas[f] * omegas[f] / 9.81;
-        if ((r = issue_frequency(b, L, f, mode, omegas[f], kap, compose, 0, nb))) return r;
+    std::vector<Task> tasks(nf);
+    for (int f = 0; f < nf; f++) tasks[f] = {b, f, omegas[f], kappas ? kappas[f] : omegas[f] * omegas[f] / 9.81};
+    std::vector<Lane*> ls(b->lanes.begin(), b->lanes.begin() + lanes);
+    return run_tasks(ls, tasks, mode, compose, nb, A36, B36, F, FK, SC, infos);
+}
+
+int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* nfreq, const double* omegas, int mode,
+                    int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
+                    bem_solve_info* infos) {
+    if (!c || nbody < 1 || !bodies || !nfreq || !omegas || !A36 || !B36 || lanes < 1 || lanes > 16)
+        return fail(BEM_ERR_ARG, "bem_solve_batch: bad arguments (lanes in [1, 16])");
+    int capNp = 0, total = 0, r = BEM_OK;
+    for (int i = 0; i < nbody; i++) {
+        if ((r = check_solve_args(c, bodies[i], mode, BEM_COMPOSE_WAVE, nb, betas))) return r;
+        if (nfreq[i] < 0) return fail(BEM_ERR_ARG, "bem_solve_batch: negative frequency count");
+        capNp = std::max(capNp, bodies[i]->Np);
+        total += nfreq[i];
     }
-    for (int l = 0; l < lanes; l++) {
-        Lane& L = *b->lanes[l];
-        if (L.pending >= 0 && (r = collect(L))) return r;
+    HIP_TRY(hipSetDevice(c->device));
+    bool need32 = mode != BEM_MODE_FP64;
+    // lane pool of the context, sized for the largest body of the batch
+    if ((int)c->pool.size() < lanes || (!c->pool.empty() && c->pool[0]->sys.capNp < capNp)) {
+        for (Lane* l : c->pool) delete l;
+        c->pool.clear();
+        while ((int)c->pool.size() < lanes) {
+            Lane* L = new Lane;
+            if ((r = L->init(capNp))) { delete L; return r; }
+            c->pool.push_back(L);
+        }
     }
+    std::vector<Lane*> ls(c->pool.begin(), c->pool.begin() + lanes);
+    for (Lane* L : ls) {
+        if (need32 && (r = L->sys.init32())) return r;
+        if ((r = upload_betas(*L, nb, betas))) return r;
+    }
+    // tile tuning for every padded order of the batch (cached per device)
+    std::vector<int> nps;
+    for (int i = 0; i < nbody; i++) nps.push_back(bodies[i]->Np);
+    std::sort(nps.begin(), nps.end());
+    nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
+    for (int np : nps) {
+        tune_lu_gemms<double>(ls[0]->s1, np, ls[0]->sys.Dr, ls[0]->sys.Di);
+        if (need32) tune_lu_gemms<float>(ls[0]->s1, np, ls[0]->sys.D32r, ls[0]->sys.D32i);
+    }
+    HIP_TRY(hipStreamSynchronize(ls[0]->s1));
+    // tasks, largest body first (the long factorisations start early)
+    std::vector<Task> tasks;
+    tasks.reserve(total);
+    for (int i = 0, o = 0; i < nbody; i++)
+        for (int k = 0; k < nfreq[i]; k++, o++) tasks.push_back({bodies[i], o, omegas[o], omegas[o] * omegas[o] / 9.81});
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task& x, const Task& y) { return x.b->Np > y.b->Np; });
+    return run_tasks(ls, tasks, mode, BEM_COMPOSE_WAVE, nb, A36, B36, F, nullptr, nullptr, infos);
     return BEM_OK;
 }
 

@@ -147,6 +147,17 @@ int bem_solve_sweep(bem_ctx* ctx, bem_body* b, int mode, int nf, const double* o
                     int compose, int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
                     double* FK, double* SC, bem_solve_info* infos);
 
+/* Many bodies of uneven size, each with its own frequency list (the reference
+ * runs one Solver per body, BEMUser_Console.cpp:227-260): nfreq[i] frequencies
+ * of body i, concatenated in omegas; results in the same order (A36 + 36 k,
+ * B36 + 36 k, F + 12 nb k). All tasks share a pool of `lanes` solve lanes
+ * owned by the context, sized for the largest body; larger bodies are started
+ * first, and each task goes to the first lane that is free. Same headings for
+ * all bodies; compose WAVE. Results equal those of per-body sweeps. */
+int bem_solve_batch(bem_ctx* ctx, int nbody, bem_body* const* bodies, const int* nfreq, const double* omegas,
+                    int mode, int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
+                    bem_solve_info* infos);
+
 /* ---- kernel-level access (parity checks of individual seams) ---------------- */
 /* Rankine matrices as assembled on the device: SSrc = -s, SSrcRefl = -sr,
  * DSrc = d, DSrcRefl = dr (N x N real column-major; NULL skips). */

@@ -244,3 +244,57 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
         assert (out / ("dropin_test" + ext)).exists()
     bem = (out / "dropin_test.bem").read_text()
     assert bem.count("Frequency:") == 3 and "Excitation Forces:" in bem
+
+
+@pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
+def test_batch_matches_per_body_sweeps(ctx, mode):
+    """Uneven bodies sharing the context's lanes (bem_solve_batch) give the
+    per-body sweep results; one body is also checked against the oracle."""
+    meshes = [bm.Mesh.half_cylinder(5.0, 8.0, 4, 8, 16), bm.Mesh.box(12.0, 6.0, -4.0, 0.0, 8, 4, 3, True),
+              bm.Mesh.half_cylinder(3.0, 6.0, 3, 6, 20)]
+    bodies = [bm.Body(ctx, m) for m in meshes]
+    for b in bodies:
+        b.assemble_rankine()
+    omegas = [np.array([0.3, 1.1, 1.7]), np.array([0.6, 2.0]), np.array([0.45, 0.9, 1.35, 1.8])]
+    betas = [0.0, 0.8]
+    got = ctx.solve_batch(bodies, omegas, betas=betas, mode=mode, lanes=3)
+    for b, w, g in zip(bodies, omegas, got):
+        ref = b.solve_sweep(w, betas=betas, mode=mode, lanes=2)
+        for k in range(len(w)):
+            assert relerr(g["A"][k], ref["A"][k]) < 1e-12
+            assert relerr(g["B"][k], ref["B"][k]) < 1e-12
+            assert relerr(g["F"][k], ref["F"][k]) < 1e-12
+    nodes, elems = meshes[1].arrays()
+    r = oracle.Body(nodes, elems).solve(2.0, betas=betas)
+    tol = 1e-8 if mode == bm.MODE_FP64 else 1e-6
+    for key in ("A", "B", "F"):
+        assert relerr(got[1][key][1], r[key]) < tol
+    for b in bodies:
+        b.close()
+
+
+def test_spar_20k_panels(ctx):
+    """Truncated-cone spar with heave plate, 20736 panels, 3 headings: mixed
+    (FP32-MFMA LU + FP64 refinement) against the FP64 GPU path, refinement
+    convergence, axisymmetry and reciprocity of A and B, heading symmetry of
+    F, and Newman's damping-excitation relation for heave."""
+    mesh = bm.Mesh.cone_spar(6.0, 9.0, 60.0, 14.0, 2.0, 128, 24, 4, 24, 110)
+    body = bm.Body(ctx, mesh)
+    print(f"\n[gpu] spar N={body.N}", flush=True)
+    assert body.N == 20736
+    body.assemble_rankine()
+    w, betas = 0.6, [0.0, np.pi / 2, np.pi]
+    g64 = body.solve(w, betas=betas, mode=bm.MODE_FP64)
+    gmx = body.solve(w, betas=betas, mode=bm.MODE_MIXED)
+    for key in ("A", "B", "F"):
+        assert relerr(gmx[key], g64[key]) < 1e-6, (key, relerr(gmx[key], g64[key]))
+    assert gmx["info"]["fell_back_fp64"] == 0 and gmx["info"]["residual"] < 1e-13
+    A, B, F = g64["A"], g64["B"], g64["F"]
+    sa, sb = np.abs(A).max(), np.abs(B).max()
+    assert abs(A[0, 0] - A[1, 1]) < 1e-6 * sa and abs(A[3, 3] - A[4, 4]) < 1e-6 * sa
+    assert np.abs(A - A.T).max() < 1e-4 * sa and np.abs(B - B.T).max() < 1e-4 * sb
+    assert abs(abs(F[0, 0]) - abs(F[1, 1])) < 1e-6 * np.abs(F).max()     # surge at 0 = sway at 90 deg
+    assert abs(abs(F[2, 0]) - abs(F[2, 2])) < 1e-6 * np.abs(F).max()     # heave at 0 = heave at 180 deg
+    K = w * w / 9.81
+    assert abs(B[2, 2] / (K * abs(F[2, 0]) ** 2 / (2 * 9.81 ** 2)) - 1) < 1e-2
+    body.close()
