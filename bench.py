@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--small", action="store_true", help="1k-panel cylinder instead of the 6k semi-sub (quick check)")
     ap.add_argument("--lanes", type=int, default=8, help="frequencies in flight (concurrent solve lanes)")
+    ap.add_argument("--workload", default="semisub6k", choices=["semisub6k", "batch64", "spar20k"],
+                    help="semisub6k: the headline line; batch64: 64 seeded uneven bodies x 16 frequencies; "
+                         "spar20k: 20736-panel cone spar x 24 frequencies (extra lines, not the headline)")
     args = ap.parse_args()
     # Each lane drives two HIP streams. HIP maps streams onto GPU_MAX_HW_QUEUES
     # hardware queues (default 4); streams sharing a queue run in order, so a
@@ -81,6 +84,9 @@ def main():
         sys.exit(2)
 
     import bemuse_amd as bm
+
+    if args.workload != "semisub6k":
+        return other_workload(args, bm)
 
     if args.small:
         mesh = bm.Mesh.half_cylinder(10.0, 20.0, 8, 24, 32)
@@ -194,6 +200,70 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def other_workload(args, bm):
+    """Extra measurement lines (the uneven batch and the 20k-panel spar), same
+    JSON shape; mixed is the value, FP64 timed in the same run."""
+    ctx = bm.Context(0)
+    if args.workload == "batch64":
+        seed = 20261203
+        meshes = [bm.Mesh.random_body(seed, i) for i in range(64)]
+        omegas = np.linspace(0.1, 2.0, 16)
+        betas = np.array([0.0])
+        bodies = [bm.Body(ctx, m) for m in meshes]
+        t0 = time.perf_counter()
+        for b in bodies:
+            b.assemble_rankine()
+        t_rk = time.perf_counter() - t0
+        ns = [b.N for b in bodies]
+        ws = [omegas] * len(bodies)
+        print(f"[bench] 64 bodies, N {min(ns)}..{max(ns)} (mean {np.mean(ns):.0f}); Rankine {t_rk:.2f} s",
+              file=sys.stderr, flush=True)
+        res = {}
+        for mode in (bm.MODE_FP64, bm.MODE_MIXED):
+            ctx.solve_batch(bodies[:2], [omegas[:1]] * 2, betas=betas, mode=mode, lanes=args.lanes)   # warm-up, tuning
+            ctx.solve_batch(bodies, [omegas[:1]] * len(bodies), betas=betas, mode=mode, lanes=args.lanes)
+            t0 = time.perf_counter()
+            out = ctx.solve_batch(bodies, ws, betas=betas, mode=mode, lanes=args.lanes)
+            res[mode] = (time.perf_counter() - t0, out)
+            print(f"[bench] mode {mode}: {16 * 64} solves in {res[mode][0]:.2f} s", file=sys.stderr, flush=True)
+        err = max(rel(a, b) for om, o64 in zip(res[bm.MODE_MIXED][1], res[bm.MODE_FP64][1])
+                  for a, b in zip(om["A"], o64["A"]))
+        ntask = 16 * 64
+        cfg = {"workload": "64 seeded uneven bodies (cylinders/barges 5-40 m, drafts 5-25 m, seed 20261203)",
+               "panels_min": int(min(ns)), "panels_max": int(max(ns)), "panels_mean": float(np.mean(ns)),
+               "frequencies_per_body": 16, "headings": 1, "lanes": args.lanes}
+        t_mx, t_64 = res[bm.MODE_MIXED][0], res[bm.MODE_FP64][0]
+    else:
+        mesh = bm.Mesh.cone_spar(6.0, 9.0, 60.0, 14.0, 2.0, 128, 24, 4, 24, 110)
+        body = bm.Body(ctx, mesh)
+        t0 = time.perf_counter()
+        body.assemble_rankine()
+        t_rk = time.perf_counter() - t0
+        omegas = np.linspace(0.05, 1.5, 24)
+        betas = np.deg2rad([0.0, 90.0, 180.0])
+        lanes = min(args.lanes, 4)
+        res = {}
+        for mode in (bm.MODE_FP64, bm.MODE_MIXED):
+            body.solve_sweep(omegas[:1], betas=betas, mode=mode, lanes=lanes)
+            t0 = time.perf_counter()
+            res[mode] = (time.perf_counter() - t0, None)
+            sw = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=lanes)
+            res[mode] = (time.perf_counter() - t0, sw)
+            print(f"[bench] mode {mode}: 24 solves in {res[mode][0]:.2f} s", file=sys.stderr, flush=True)
+        err = max(max(rel(res[bm.MODE_MIXED][1][k][i], res[bm.MODE_FP64][1][k][i]) for k in ("A", "B", "F"))
+                  for i in range(24))
+        ntask = 24
+        cfg = {"workload": "truncated-cone spar with heave plate", "panels": body.N, "frequencies": 24,
+               "headings": 3, "lanes": lanes}
+        t_mx, t_64 = res[bm.MODE_MIXED][0], res[bm.MODE_FP64][0]
+    line = {"metric": f"frequency-solves/sec ({args.workload})", "value": ntask / t_mx, "unit": "frequency-solves/s",
+            "n_gpus": 1, "steps": ntask, "warmup": 1, "ms_per_step": t_mx / ntask * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu+fp64-refine", "data": "synthetic",
+            "config": cfg, "fp64_value": ntask / t_64, "speedup_vs_fp64": t_64 / t_mx,
+            "max_rel_err_mixed_vs_fp64_gpu": err, "rankine_s": t_rk}
     print(json.dumps(line), flush=True)
 
 
