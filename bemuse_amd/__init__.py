@@ -52,17 +52,54 @@ class SolveInfo(ctypes.Structure):
         ("ms_assemble", c_double),
         ("ms_factor", c_double),
         ("ms_solve", c_double),
-        ("surrogate_points", c_int),
-        ("exact_points", c_int),
+        ("surrogate_points", ctypes.c_int64),
+        ("exact_points", ctypes.c_int64),
+        ("surrogate_err", c_double),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SurrogateConfig(ctypes.Structure):
+    _fields_ = [
+        ("seed", c_uint64),
+        ("n_samples", ctypes.c_int64),
+        ("batch", c_int),
+        ("steps", c_int),
+        ("lr", c_double),
+        ("lr_final", c_double),
+        ("n_valid", ctypes.c_int64),
+        ("sampling", c_int),
+    ]
+
+
+class SurrogateReport(ctypes.Structure):
+    _fields_ = [
+        ("loss_initial", c_double),
+        ("loss_final", c_double),
+        ("valid_rms", c_double * 2),
+        ("valid_max", c_double * 2),
+        ("rel_rms", c_double),
+        ("rel_p99", c_double),
+        ("rel_max", c_double),
+        ("ms_sample", c_double),
+        ("ms_train", c_double),
+        ("ms_valid", c_double),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = list(v) if k.startswith("valid_") else v
+        return out
+
+
 _PD = POINTER(c_double)
 _PI = POINTER(c_int)
 _PP = POINTER(c_void_p)
+_PF = POINTER(ctypes.c_float)
 
 _SIGS = {
     "bem_last_error": (ctypes.c_char_p, []),
@@ -95,6 +132,15 @@ _SIGS = {
                                 _PD, _PD, c_void_p]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
+    "bem_surrogate_default_config": (c_int, [POINTER(SurrogateConfig)]),
+    "bem_surrogate_param_count": (c_int, []),
+    "bem_surrogate_train": (c_int, [c_void_p, POINTER(SurrogateConfig), POINTER(SurrogateReport)]),
+    "bem_surrogate_get": (c_int, [c_void_p, _PF, c_int]),
+    "bem_surrogate_set": (c_int, [c_void_p, _PF, c_int]),
+    "bem_surrogate_sample": (c_int, [c_void_p, c_uint64, c_int, ctypes.c_int64, c_int, _PF, _PF]),
+    "bem_surrogate_forward": (c_int, [c_void_p, _PF, c_int, _PF, _PF]),
+    "bem_surrogate_grad": (c_int, [c_void_p, _PF, c_int, _PF, _PF, _PF, _PD]),
+    "bem_surrogate_eval": (c_int, [c_void_p, c_int, _PD, _PD, _PD, _PI]),
     "bem_solve_sweep": (c_int, [c_void_p, c_void_p, c_int, c_int, _PD, _PD, c_int, c_int, _PD, c_int, _PD, _PD,
                                 _PD, _PD, _PD, POINTER(SolveInfo)]),
 }
@@ -132,6 +178,28 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(_PI) if a is not None else None
+
+
+def _fp(a):
+    return a.ctypes.data_as(_PF) if a is not None else None
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def surrogate_param_count():
+    return int(lib().bem_surrogate_param_count())
+
+
+def surrogate_default_config():
+    """Default training configuration (dict) of the wave-term surrogate."""
+    c = SurrogateConfig()
+    _check(lib().bem_surrogate_default_config(ctypes.byref(c)))
+    return {k: getattr(c, k) for k, _ in c._fields_}
 
 
 def gf_coefficients():
@@ -255,6 +323,60 @@ class Context:
         out = np.zeros((len(X), 8))
         _check(lib().bem_eval_havelock(self._h, 64, len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
         return out
+
+    # ---- wave-term surrogate ----
+    def surrogate_train(self, **overrides):
+        """Samples, trains and validates the wave-term surrogate on this
+        context's device (keys of surrogate_default_config() override the
+        defaults); returns the training report as a dict."""
+        c = SurrogateConfig()
+        _check(lib().bem_surrogate_default_config(ctypes.byref(c)))
+        for k, v in overrides.items():
+            if not hasattr(c, k):
+                raise KeyError(k)
+            setattr(c, k, v)
+        rep = SurrogateReport()
+        _check(lib().bem_surrogate_train(self._h, ctypes.byref(c), ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def surrogate_get(self):
+        p = np.zeros(surrogate_param_count(), dtype=np.float32)
+        _check(lib().bem_surrogate_get(self._h, _fp(p), len(p)))
+        return p
+
+    def surrogate_set(self, params):
+        p = _f32(params)
+        _check(lib().bem_surrogate_set(self._h, _fp(p), len(p)))
+
+    def surrogate_sample(self, seed, first, n, sampling=1):
+        X = np.zeros((n, 4), dtype=np.float32)
+        T = np.zeros((n, 2), dtype=np.float32)
+        _check(lib().bem_surrogate_sample(self._h, seed, sampling, first, n, _fp(X), _fp(T)))
+        return X, T
+
+    def surrogate_forward(self, X, params=None):
+        X = _f32(X, (-1, 4))
+        Y = np.zeros((len(X), 2), dtype=np.float32)
+        p = _f32(params) if params is not None else None
+        _check(lib().bem_surrogate_forward(self._h, _fp(p), len(X), _fp(X), _fp(Y)))
+        return Y
+
+    def surrogate_grad(self, params, X, T):
+        p = _f32(params)
+        X = _f32(X, (-1, 4))
+        T = _f32(T, (-1, 2))
+        g = np.zeros(surrogate_param_count(), dtype=np.float32)
+        loss = c_double()
+        _check(lib().bem_surrogate_grad(self._h, _fp(p), len(X), _fp(X), _fp(T), _fp(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def surrogate_eval(self, h, v):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros((len(h), 4))
+        used = np.zeros(len(h), dtype=np.int32)
+        _check(lib().bem_surrogate_eval(self._h, len(h), _dp(h), _dp(v), _dp(out), _ip(used)))
+        return out, used
 
     def dense_solve(self, A, B, precision=64):
         A = np.asfortranarray(A, dtype=np.complex128)

@@ -5,6 +5,7 @@
 #include "bem_device.hpp"
 #include "bem_internal.h"
 #include "bem_kernels.h"
+#include "bem_surrogate.h"
 
 #include <hip/hip_runtime.h>
 
@@ -56,9 +57,19 @@ struct bem_mesh {
     Mesh m;
 };
 
+// Wave-term surrogate held by a context: its network parameters on the
+// device and the report of the training that produced them.
+struct SurState {
+    float* params = nullptr;
+    bem_surrogate_report rep{};
+    ~SurState() {
+        if (params) (void)hipFree(params);
+    }
+};
+
 struct bem_ctx {
     int device = 0;
-    void* surrogate = nullptr;         // trained wave-term surrogate (none yet)
+    SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
     std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch
@@ -138,6 +149,7 @@ struct LinSys {
 struct LaneOut {
     double coeffs[2 * (36 + 12 * kMaxRhs)];
     RefineCtrl ctrl;
+    SurCounters sur;
     int flags[kMaxPanels + 1];
 };
 
@@ -163,6 +175,7 @@ struct Lane {
     // bookkeeping of the frequency in flight
     int pending = -1;
     int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
+    bool sur_used = false;
     double omega = 0, kappa = 0;
 
     ~Lane() {
@@ -220,6 +233,7 @@ struct bem_body {
     double *d_body = nullptr, *d_refl = nullptr, *d_quad = nullptr, *d_recv = nullptr, *d_gen = nullptr;
     double *Ssrc = nullptr, *Srefl = nullptr, *Dsrc = nullptr, *Drefl = nullptr;
     bool have_rankine = false;
+    long long nq_total = 0;             // quadrature nodes over all panels
     std::vector<Lane*> lanes;
 
     ~bem_body() {
@@ -435,6 +449,8 @@ void bem_destroy(bem_ctx* c) {
     (void)hipSetDevice(c->device);
     for (Lane* l : c->pool) delete l;
     c->pool.clear();
+    delete c->surrogate;
+    c->surrogate = nullptr;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -451,6 +467,7 @@ int bem_body_create(bem_ctx* c, const bem_mesh* m, bem_body** out) {
     if (!build_geometry(m->m, b->geo, err)) { delete b; return fail(BEM_ERR_ARG, err); }
     b->N = b->geo.N;
     b->Np = round_up(b->N, kPad);
+    for (int i = 0; i < b->N; i++) b->nq_total += (long long)b->geo.quad[(size_t)i * QF_STRIDE + QF_NQ];
     if (b->Np / kLuNb > kMaxPanels) { delete b; return fail(BEM_ERR_ARG, "body too large"); }
     int N = b->N;
     int r = BEM_OK;
@@ -526,6 +543,14 @@ int ensure_lanes(bem_body* b, int n, bool need32) {
 
 void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
     bool f32 = (mode != BEM_MODE_FP64);
+    L.sur_used = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
+    if (L.sur_used) {
+        launch_sur_counters_clear(L.s1, &L.dout->sur);
+        launch_wave_compose_surr(L.s1, b->ctx->surrogate->params, b->N, b->Np, kappa, b->d_quad, b->d_recv, b->Ssrc,
+                                 b->Srefl, b->Dsrc, b->Drefl, L.Sr, L.Si, L.sys.Dr, L.sys.Di, L.sys.D32r, L.sys.D32i,
+                                 &L.dout->sur);
+        return;
+    }
     launch_wave_compose(L.s1, b->N, b->Np, kappa, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
                         L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr);
 }
@@ -558,6 +583,8 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     HIP_TRY(hipMemcpyAsync(L.hout->coeffs, L.dout->coeffs, sizeof(double) * 2 * (36 + 12 * nb), hipMemcpyDeviceToHost, st));
     if (L.prec == 32)
         HIP_TRY(hipMemcpyAsync(&L.hout->ctrl, L.ctrl, sizeof(RefineCtrl), hipMemcpyDeviceToHost, st));
+    if (L.sur_used)
+        HIP_TRY(hipMemcpyAsync(&L.hout->sur, &L.dout->sur, sizeof(SurCounters), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(L.hout->flags, L.prec == 64 ? s.flag : s.flag32, sizeof(int) * (np + 1),
                            hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(L.done, st));
@@ -582,6 +609,15 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
         }
     }
     loc.fell_back_fp64 = fell_back;
+    if (L.sur_used) {
+        loc.surrogate_points = (int64_t)L.hout->sur.n_sur;
+        loc.exact_points = (int64_t)L.hout->sur.n_exact;
+        float e = 0.0f;
+        std::memcpy(&e, &L.hout->sur.err_bits, sizeof(e));
+        loc.surrogate_err = e;
+    } else if (L.compose == BEM_COMPOSE_WAVE && L.kappa != 0.0) {
+        loc.exact_points = (int64_t)b->N * b->nq_total;
+    }
     int np = b->Np / kLuNb;
     for (int i = 0; i < np; i++) loc.pivot_fallback += (L.hout->flags[i] & 1) ? 1 : 0;
     float t01 = 0, t12 = 0, t23 = 0;
@@ -619,7 +655,7 @@ int check_solve_args(bem_ctx* c, bem_body* b, int mode, int compose, int nb, con
     if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bad context/body");
     if (nb < 0 || nb > kMaxRhs - 6 || (nb > 0 && !betas)) return fail(BEM_ERR_ARG, "nb must be in [0, 10]");
     if (mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bad mode");
-    if (mode == BEM_MODE_MIXED_SURROGATE && !b->ctx->surrogate)
+    if (mode == BEM_MODE_MIXED_SURROGATE && (!b->ctx->surrogate || !b->ctx->surrogate->params))
         return fail(BEM_ERR_STATE, "BEM_MODE_MIXED_SURROGATE needs a trained surrogate on the context");
     if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bad compose case");
     if (!b->have_rankine) return fail(BEM_ERR_STATE, "Rankine matrices not assembled");
@@ -800,6 +836,9 @@ int bem_get_rankine(bem_ctx* c, bem_body* b, double* Ssrc, double* Srefl, double
 int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose, double* SMat, double* DMat) {
     if (!c || !b || !b->have_rankine) return fail(BEM_ERR_STATE, "bem_get_system: Rankine matrices not assembled");
     if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bem_get_system: bad compose case");
+    if (b->ctx != c || mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bem_get_system: bad context or mode");
+    if (mode == BEM_MODE_MIXED_SURROGATE && (!c->surrogate || !c->surrogate->params))
+        return fail(BEM_ERR_STATE, "bem_get_system: BEM_MODE_MIXED_SURROGATE needs a surrogate on the context");
     HIP_TRY(hipSetDevice(c->device));
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
@@ -822,6 +861,9 @@ int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose,
 
 int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out) {
     if (!c || !b || !b->have_rankine || !out) return fail(BEM_ERR_STATE, "bem_profile_lu: Rankine matrices not assembled");
+    if (b->ctx != c || mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bem_profile_lu: bad context or mode");
+    if (mode == BEM_MODE_MIXED_SURROGATE && (!c->surrogate || !c->surrogate->params))
+        return fail(BEM_ERR_STATE, "bem_profile_lu: BEM_MODE_MIXED_SURROGATE needs a surrogate on the context");
     HIP_TRY(hipSetDevice(c->device));
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
@@ -887,6 +929,285 @@ int bem_eval_havelock(bem_ctx* c, int precision, int n, const double* X, const d
     if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * (size_t)n * 8, hipMemcpyDeviceToHost);
     (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dv); (void)hipFree(dout);
     if (e != hipSuccess) return fail(BEM_ERR_HIP, std::string("bem_eval_havelock: ") + hipGetErrorString(e));
+    return BEM_OK;
+}
+
+}  // extern "C"
+
+// ---- wave-term surrogate -----------------------------------------------------
+namespace {
+
+// 16 (a, c) pairs of the per-pass affine shuffle row = (a q + c) mod pool:
+// primes a above the pool (so coprime with it) and seeded offsets c.
+std::vector<unsigned long long> shuffle_table(uint64_t seed, unsigned long long pool) {
+    std::vector<unsigned long long> t;
+    unsigned long long a = std::max<unsigned long long>(pool, 1ULL << 31) + 1;
+    while (t.size() < 32) {
+        bool prime = a > 1;
+        for (unsigned long long d = 2; d * d <= a && prime; d++) prime = (a % d) != 0;
+        if (prime) {
+            t.push_back(a);
+            t.push_back(splitmix64(seed ^ (0xA5A5A5A5ULL + t.size())) % pool);
+        }
+        a += 2;
+    }
+    return t;
+}
+
+// Device buffers of one training run, freed on every exit path.
+struct SurBuffers {
+    std::vector<void*> ptrs;
+    ~SurBuffers() {
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(T** p, size_t n) {
+        int r = dalloc_t(p, n);
+        if (!r) ptrs.push_back(*p);
+        return r;
+    }
+};
+
+int ensure_surrogate(bem_ctx* c) {
+    if (!c->surrogate) c->surrogate = new SurState;
+    if (!c->surrogate->params) return dalloc_t(&c->surrogate->params, kSurParams);
+    return BEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bem_surrogate_default_config(bem_surrogate_config* cfg) {
+    if (!cfg) return fail(BEM_ERR_ARG, "bem_surrogate_default_config: null config");
+    cfg->seed = 20261203ULL;
+    cfg->n_samples = 20000000;
+    cfg->batch = 8192;
+    cfg->steps = 24000;
+    cfg->lr = 2e-3;
+    cfg->lr_final = 2e-5;
+    cfg->n_valid = 1000000;
+    cfg->sampling = 1;
+    return BEM_OK;
+}
+
+int bem_surrogate_param_count(void) { return kSurParams; }
+
+int bem_surrogate_train(bem_ctx* c, const bem_surrogate_config* cfg, bem_surrogate_report* rep) {
+    if (!c || !cfg) return fail(BEM_ERR_ARG, "bem_surrogate_train: null argument");
+    const long long n = cfg->n_samples, nv = cfg->n_valid;
+    if (n < 1 || n > (1LL << 30) || cfg->batch < 1 || cfg->batch > (1 << 20) || cfg->steps < 0 || nv < 0 ||
+        nv > (1LL << 26) || !(cfg->lr >= 0.0) || !(cfg->lr_final >= 0.0) || cfg->sampling < 0 || cfg->sampling > 1)
+        return fail(BEM_ERR_ARG, "bem_surrogate_train: n_samples in [1, 2^30], batch in [1, 2^20], steps >= 0, "
+                                 "n_valid in [0, 2^26], learning rates >= 0, sampling 0 or 1");
+    HIP_TRY(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int nwg = sur_grad_groups(cfg->batch);
+    SurBuffers B;
+    float *X = nullptr, *T = nullptr, *P = nullptr, *m = nullptr, *v = nullptr, *gpart = nullptr, *lpart = nullptr;
+    float *Xv = nullptr, *Tv = nullptr, *Yv = nullptr;
+    double *loss = nullptr, *hv = nullptr;
+    unsigned long long* perm = nullptr;
+    int r = BEM_OK;
+    if ((r = B.get(&X, 4 * (size_t)n)) || (r = B.get(&T, kSurOut * (size_t)n)) || (r = B.get(&P, kSurParams)) ||
+        (r = B.get(&m, kSurParams)) || (r = B.get(&v, kSurParams)) || (r = B.get(&gpart, (size_t)nwg * kSurParams)) ||
+        (r = B.get(&lpart, nwg)) || (r = B.get(&loss, std::max(cfg->steps, 1))) || (r = B.get(&perm, 32)))
+        return r;
+    if (nv > 0 && ((r = B.get(&Xv, 4 * (size_t)nv)) || (r = B.get(&Tv, kSurOut * (size_t)nv)) ||
+                   (r = B.get(&Yv, kSurOut * (size_t)nv)) || (r = B.get(&hv, 2 * (size_t)nv))))
+        return r;
+    std::vector<float> p0;
+    surrogate_init_params(cfg->seed ^ 0x5EED5EEDULL, p0);
+    std::vector<unsigned long long> tab = shuffle_table(cfg->seed, (unsigned long long)n);
+    HIP_TRY(hipMemcpyAsync(P, p0.data(), sizeof(float) * kSurParams, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(perm, tab.data(), sizeof(unsigned long long) * 32, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(m, 0, sizeof(float) * kSurParams, st));
+    HIP_TRY(hipMemsetAsync(v, 0, sizeof(float) * kSurParams, st));
+    hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
+    bem_surrogate_report out{};
+    float ms = 0.0f;
+    HIP_TRY(hipEventRecord(e0, st));
+    launch_sur_sample(st, cfg->seed, cfg->sampling, 0, n, X, T, nullptr);
+    if (nv > 0) launch_sur_sample(st, cfg->seed, cfg->sampling, n, nv, Xv, Tv, hv);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    out.ms_sample = ms;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int t = 0; t < cfg->steps; t++) {
+        const double lr = cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(M_PI * t / cfg->steps));
+        const double lr_t = lr * std::sqrt(1.0 - std::pow((double)b2, t + 1)) / (1.0 - std::pow((double)b1, t + 1));
+        launch_sur_grad(st, P, X, T, n, (long long)t * cfg->batch, cfg->batch, perm, gpart, lpart);
+        launch_sur_adam(st, nwg, gpart, lpart, P, m, v, nullptr, loss + t, (float)lr_t, b1, b2, eps);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    out.ms_train = ms;
+    if (cfg->steps > 0) {
+        std::vector<double> lh(cfg->steps);
+        HIP_TRY(hipMemcpy(lh.data(), loss, sizeof(double) * cfg->steps, hipMemcpyDeviceToHost));
+        const double norm = 1.0 / ((double)kSurOut * cfg->batch);
+        int k0 = std::min(cfg->steps, 10), k1 = std::min(cfg->steps, 100);
+        for (int k = 0; k < k0; k++) out.loss_initial += lh[k] * norm / k0;
+        for (int k = cfg->steps - k1; k < cfg->steps; k++) out.loss_final += lh[k] * norm / k1;
+    }
+    if (nv > 0) {
+        HIP_TRY(hipEventRecord(e0, st));
+        launch_sur_forward(st, P, nv, Xv, Yv);
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        out.ms_valid = ms;
+        std::vector<float> y(kSurOut * (size_t)nv), tg(kSurOut * (size_t)nv);
+        std::vector<double> pts(2 * (size_t)nv);
+        HIP_TRY(hipMemcpy(y.data(), Yv, y.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(tg.data(), Tv, tg.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(pts.data(), hv, pts.size() * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<double> rel((size_t)nv);
+        double ss[kSurOut] = {}, rs = 0.0;
+        for (long long j = 0; j < nv; j++) {
+            const double h = pts[2 * j], vv = pts[2 * j + 1];
+            const SurPoint sp = sur_point(h, vv);
+            double num = 0.0, g[kSurOut];
+            for (int ch = 0; ch < kSurOut; ch++) {
+                const double e = std::fabs((double)y[kSurOut * j + ch] - (double)tg[kSurOut * j + ch]);
+                ss[ch] += e * e;
+                out.valid_max[ch] = std::max(out.valid_max[ch], e);
+                num = std::max(num, e * sp.s);
+                g[ch] = sp.s * tg[kSurOut * j + ch];
+            }
+            g[0] += sp.sg;
+            g[1] += sp.sgh;
+            // relative to the largest component of (GF, GFh), imaginary parts exact
+            const double E = 2.0 * M_PI * std::exp(vv);
+            const double den = std::max(std::max(std::fabs(g[0]), std::fabs(E * ::j0(h))),
+                                        std::max(std::fabs(g[1]), std::fabs(E * ::j1(h))));
+            rel[j] = den > 0.0 ? num / den : 0.0;
+            rs += rel[j] * rel[j];
+            out.rel_max = std::max(out.rel_max, rel[j]);
+        }
+        for (int ch = 0; ch < kSurOut; ch++) out.valid_rms[ch] = std::sqrt(ss[ch] / nv);
+        out.rel_rms = std::sqrt(rs / nv);
+        size_t k99 = (size_t)(0.99 * (nv - 1));
+        std::nth_element(rel.begin(), rel.begin() + k99, rel.end());
+        out.rel_p99 = rel[k99];
+    }
+    if ((r = ensure_surrogate(c))) return r;
+    HIP_TRY(hipMemcpyAsync(c->surrogate->params, P, sizeof(float) * kSurParams, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    c->surrogate->rep = out;
+    if (rep) *rep = out;
+    return BEM_OK;
+}
+
+int bem_surrogate_get(bem_ctx* c, float* params, int n) {
+    if (!c || !params || n != kSurParams) return fail(BEM_ERR_ARG, "bem_surrogate_get: need bem_surrogate_param_count() floats");
+    if (!c->surrogate || !c->surrogate->params) return fail(BEM_ERR_STATE, "bem_surrogate_get: no surrogate on the context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(params, c->surrogate->params, sizeof(float) * kSurParams, hipMemcpyDeviceToHost));
+    return BEM_OK;
+}
+
+int bem_surrogate_set(bem_ctx* c, const float* params, int n) {
+    if (!c || !params || n != kSurParams) return fail(BEM_ERR_ARG, "bem_surrogate_set: need bem_surrogate_param_count() floats");
+    for (int i = 0; i < n; i++)
+        if (!std::isfinite(params[i])) return fail(BEM_ERR_ARG, "bem_surrogate_set: non-finite parameter");
+    HIP_TRY(hipSetDevice(c->device));
+    int r = ensure_surrogate(c);
+    if (r) return r;
+    HIP_TRY(hipMemcpy(c->surrogate->params, params, sizeof(float) * kSurParams, hipMemcpyHostToDevice));
+    c->surrogate->rep = bem_surrogate_report{};
+    return BEM_OK;
+}
+
+int bem_surrogate_sample(bem_ctx* c, uint64_t seed, int sampling, int64_t first, int n, float* X, float* T) {
+    if (!c || n < 1 || first < 0 || !X || !T || sampling < 0 || sampling > 1)
+        return fail(BEM_ERR_ARG, "bem_surrogate_sample: bad arguments");
+    HIP_TRY(hipSetDevice(c->device));
+    SurBuffers B;
+    float *dx = nullptr, *dt = nullptr;
+    int r = BEM_OK;
+    if ((r = B.get(&dx, 4 * (size_t)n)) || (r = B.get(&dt, kSurOut * (size_t)n))) return r;
+    launch_sur_sample(c->stream, seed, sampling, first, n, dx, dt, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(X, dx, 16 * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(T, dt, sizeof(float) * kSurOut * (size_t)n, hipMemcpyDeviceToHost));
+    return BEM_OK;
+}
+
+int bem_surrogate_forward(bem_ctx* c, const float* params, int n, const float* X, float* Y) {
+    if (!c || n < 1 || !X || !Y) return fail(BEM_ERR_ARG, "bem_surrogate_forward: bad arguments");
+    if (!params && (!c->surrogate || !c->surrogate->params))
+        return fail(BEM_ERR_STATE, "bem_surrogate_forward: no parameters given and none on the context");
+    HIP_TRY(hipSetDevice(c->device));
+    SurBuffers B;
+    float *dp = nullptr, *dx = nullptr, *dy = nullptr;
+    int r = BEM_OK;
+    if ((r = B.get(&dx, 4 * (size_t)n)) || (r = B.get(&dy, kSurOut * (size_t)n))) return r;
+    if (params) {
+        if ((r = B.get(&dp, kSurParams))) return r;
+        HIP_TRY(hipMemcpy(dp, params, sizeof(float) * kSurParams, hipMemcpyHostToDevice));
+    } else {
+        dp = c->surrogate->params;
+    }
+    HIP_TRY(hipMemcpy(dx, X, 16 * (size_t)n, hipMemcpyHostToDevice));
+    launch_sur_forward(c->stream, dp, n, dx, dy);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(Y, dy, sizeof(float) * kSurOut * (size_t)n, hipMemcpyDeviceToHost));
+    return BEM_OK;
+}
+
+int bem_surrogate_grad(bem_ctx* c, const float* params, int n, const float* X, const float* T, float* grad,
+                       double* loss) {
+    if (!c || !params || n < 1 || n > (1 << 20) || !X || !T || !grad)
+        return fail(BEM_ERR_ARG, "bem_surrogate_grad: bad arguments (n in [1, 2^20])");
+    HIP_TRY(hipSetDevice(c->device));
+    const int nwg = sur_grad_groups(n);
+    SurBuffers B;
+    float *dp = nullptr, *dx = nullptr, *dt = nullptr, *gpart = nullptr, *lpart = nullptr, *dg = nullptr;
+    double* dl = nullptr;
+    int r = BEM_OK;
+    if ((r = B.get(&dp, kSurParams)) || (r = B.get(&dx, 4 * (size_t)n)) || (r = B.get(&dt, kSurOut * (size_t)n)) ||
+        (r = B.get(&gpart, (size_t)nwg * kSurParams)) || (r = B.get(&lpart, nwg)) || (r = B.get(&dg, kSurParams)) ||
+        (r = B.get(&dl, 1)))
+        return r;
+    HIP_TRY(hipMemcpy(dp, params, sizeof(float) * kSurParams, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dx, X, 16 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dt, T, sizeof(float) * kSurOut * (size_t)n, hipMemcpyHostToDevice));
+    launch_sur_grad(c->stream, dp, dx, dt, n, 0, n, nullptr, gpart, lpart);
+    launch_sur_adam(c->stream, nwg, gpart, lpart, dp, nullptr, nullptr, dg, dl, 0.0f, 0.0f, 0.0f, 0.0f);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(grad, dg, sizeof(float) * kSurParams, hipMemcpyDeviceToHost));
+    double l = 0.0;
+    HIP_TRY(hipMemcpy(&l, dl, sizeof(double), hipMemcpyDeviceToHost));
+    if (loss) *loss = l / ((double)kSurOut * n);
+    return BEM_OK;
+}
+
+int bem_surrogate_eval(bem_ctx* c, int n, const double* h, const double* v, double* out, int* used) {
+    if (!c || n < 1 || !h || !v || !out) return fail(BEM_ERR_ARG, "bem_surrogate_eval: bad arguments");
+    if (!c->surrogate || !c->surrogate->params) return fail(BEM_ERR_STATE, "bem_surrogate_eval: no surrogate on the context");
+    HIP_TRY(hipSetDevice(c->device));
+    SurBuffers B;
+    double *dh = nullptr, *dv = nullptr, *dout = nullptr;
+    int* du = nullptr;
+    int r = BEM_OK;
+    if ((r = B.get(&dh, n)) || (r = B.get(&dv, n)) || (r = B.get(&dout, 4 * (size_t)n)) || (r = B.get(&du, n))) return r;
+    HIP_TRY(hipMemcpy(dh, h, 8 * (size_t)n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dv, v, 8 * (size_t)n, hipMemcpyHostToDevice));
+    launch_sur_eval(c->stream, c->surrogate->params, n, dh, dv, dout, du);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(out, dout, 32 * (size_t)n, hipMemcpyDeviceToHost));
+    if (used) HIP_TRY(hipMemcpy(used, du, 4 * (size_t)n, hipMemcpyDeviceToHost));
     return BEM_OK;
 }
 

@@ -62,8 +62,10 @@ typedef struct bem_solve_info {
     double ms_assemble;      /* device time of wave assembly + compose [ms] */
     double ms_factor;        /* device time of the LU [ms] */
     double ms_solve;         /* device time of RHS, solves, refinement, reductions [ms] */
-    int surrogate_points;    /* wave-kernel evaluations served by the surrogate */
-    int exact_points;        /* wave-kernel evaluations served by the exact kernel */
+    int64_t surrogate_points; /* wave-kernel evaluations served by the surrogate */
+    int64_t exact_points;     /* wave-kernel evaluations served by the exact kernel */
+    double surrogate_err;     /* largest relative deviation of the surrogate from the exact
+                                 kernel over the pairs judged during assembly (one in 61) */
 } bem_solve_info;
 
 const char* bem_last_error(void);
@@ -179,6 +181,68 @@ int bem_gf_coefficients(double* out, int n);
  * the solution of A X = B for nrhs columns. */
 int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
                     double* X, bem_solve_info* info);
+
+/* ---- wave-term surrogate ------------------------------------------------------
+ * No counterpart in the reference: an MLP trained on the device from the exact
+ * kernel stands in for HavelockGF (Kernels.cpp:342-412) inside
+ * Prepare_Linear_System_Wave_Terms (Hydrodynamic_Solver.cpp:177-203) when a
+ * solve runs in BEM_MODE_MIXED_SURROGATE. The exact kernel remains the
+ * fallback outside the trained domain (K R in [0, 200], K (z + zeta) in
+ * [-50, 0]) and the judge of the surrogate's error. Network, features and
+ * targets: bemuse_amd/csrc/bem_surrogate.h. */
+typedef struct bem_surrogate_config {
+    uint64_t seed;       /* sampling and initialisation seed */
+    int64_t n_samples;   /* training samples drawn from the exact kernel on the device */
+    int batch;           /* samples per Adam step */
+    int steps;           /* Adam steps (a new affine shuffle of the pool every pass) */
+    double lr;           /* peak learning rate; cosine decay to lr_final */
+    double lr_final;
+    int64_t n_valid;     /* held-out samples (indices after the training pool) judged exactly */
+    int sampling;        /* 0: uniform in (K R, K (z + zeta)); 1: uniform in the log-scaled
+                            features log1p(K R), log1p(-K (z + zeta)) (default) */
+} bem_surrogate_config;
+
+typedef struct bem_surrogate_report {
+    double loss_initial;    /* mean squared error of the normalised targets, first 10 steps */
+    double loss_final;      /* same, last 100 steps */
+    double valid_rms[2];    /* validation RMS error per output channel (normalised) */
+    double valid_max[2];    /* validation max error per output channel (normalised) */
+    double rel_rms;         /* validation relative error of (GF, GFh): RMS over samples */
+    double rel_p99;         /* 99th percentile */
+    double rel_max;         /* maximum */
+    double ms_sample;       /* device time: drawing the pools from the exact kernel */
+    double ms_train;        /* device time: all Adam steps */
+    double ms_valid;        /* device time: validation forward pass */
+} bem_surrogate_report;
+
+/* Defaults: seed 20261203, 20,000,000 samples, batch 8192, 24,000 steps,
+ * lr 2e-3 -> 2e-5, 1,000,000 validation samples, log-scaled sampling. */
+int bem_surrogate_default_config(bem_surrogate_config* cfg);
+/* Number of network parameters (floats). */
+int bem_surrogate_param_count(void);
+/* Samples, trains and validates a surrogate on the context's device and
+ * keeps it on the context (replacing any earlier one). rep may be NULL. */
+int bem_surrogate_train(bem_ctx* ctx, const bem_surrogate_config* cfg, bem_surrogate_report* rep);
+/* Copies the context's surrogate parameters out / loads parameters into the
+ * context (n = bem_surrogate_param_count()). */
+int bem_surrogate_get(bem_ctx* ctx, float* params, int n);
+int bem_surrogate_set(bem_ctx* ctx, const float* params, int n);
+/* Kernel-level access for checks. Samples `first` .. first+n-1 of the
+ * sampler with this seed and sampling: features X (n x 4) and normalised
+ * targets T (n x 2). */
+int bem_surrogate_sample(bem_ctx* ctx, uint64_t seed, int sampling, int64_t first, int n, float* X, float* T);
+/* Network outputs Y (n x 2) for features X (n x 4); params NULL uses the
+ * context's. */
+int bem_surrogate_forward(bem_ctx* ctx, const float* params, int n, const float* X, float* Y);
+/* Loss (sum of squared errors / (2 n)) and its gradient with respect to
+ * params (host, kSurParams floats) over the batch (X, T), by the training
+ * kernels. */
+int bem_surrogate_grad(bem_ctx* ctx, const float* params, int n, const float* X, const float* T, float* grad,
+                       double* loss);
+/* The context's surrogate at n points (h, v): out n x 4 doubles
+ * (Re GF, Im GF, Re GFh, Im GFh); used[i] = 1 where the network served the
+ * point, 0 where the exact kernel did (outside the domain). used may be NULL. */
+int bem_surrogate_eval(bem_ctx* ctx, int n, const double* h, const double* v, double* out, int* used);
 
 /* Instrumented factorisation for measurement: assembles the system at kappa
  * (compose WAVE) and factors it in the mode's precision with HIP events
