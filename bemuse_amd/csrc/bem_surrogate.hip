@@ -27,6 +27,11 @@ __device__ inline bool sur_in_domain(double h, double v) {
 // tanh(x) = 1 - 2 / (e^{2x} + 1); exact limits +-1 for large |x|.
 __device__ inline float sur_tanh(float x) { return 1.0f - 2.0f / (__expf(2.0f * x) + 1.0f); }
 
+// J0(x), J1(x) out of line: inlined into the fused assembly they would raise
+// its register count above what two work-groups per CU allow.
+struct J01 { double j0, j1; };
+__device__ __attribute__((noinline)) J01 bessel_j01(double x) { return J01{j0(x), j1(x)}; }
+
 __device__ inline f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
 // Aout[m][n] = tanh(sum_k Ain[m][k] W[n][k] + bias[n]), m < M, n < 128.
@@ -39,7 +44,7 @@ __device__ inline void hidden_fwd(const float* __restrict__ W, const float* __re
     f4 acc[MB][2];
 #pragma unroll
     for (int a = 0; a < MB; a++) { acc[a][0] = f4{0, 0, 0, 0}; acc[a][1] = f4{0, 0, 0, 0}; }
-#pragma unroll 2
+#pragma unroll 1
     for (int t = 0; t < kSurHid / 16; t++) {
         const int k = 16 * t + 4 * g;
         const f4 w0 = *reinterpret_cast<const f4*>(W + (n0 + i) * kSurHid + k);
@@ -168,6 +173,7 @@ __device__ inline void stage_small(const float* __restrict__ P, SmallLayers& L, 
 // Input layer: A[m][n] = tanh(W1[n] . x[m] + b1[n]) for the M rows.
 template <int M>
 __device__ inline void input_fwd(const SmallLayers& L, const float (*x)[4], float* A, int tid) {
+#pragma unroll 2
     for (int idx = tid; idx < M * kSurHid; idx += 256) {
         const int m = idx >> 7, n = idx & 127;
         const float* w = L.w1 + n * kSurIn;
@@ -185,13 +191,14 @@ __device__ inline float output_unit(const SmallLayers& L, const float* A, int m,
     const float* a = A + m * LDA;
     const float* w = L.w5 + c * kSurHid;
     float y = L.b5[c];
+#pragma unroll 8
     for (int o = 0; o < kSurHid; o++) y += a[o] * w[o];
     return y;
 }
 
 // Forward pass of 64 rows: features x -> outputs y (both LDS). Uses act0
 // and act1 as ping-pong buffers. Ends synchronised.
-__device__ inline void forward64(const float* __restrict__ P, const SmallLayers& L, const float (*x)[4], float* act0,
+__device__ __attribute__((always_inline)) inline void forward64(const float* __restrict__ P, const SmallLayers& L, const float (*x)[4], float* act0,
                                  float* act1, float (*y)[kSurOut], int tid, int lane, int wave) {
     input_fwd<64>(L, x, act0, tid);
     __syncthreads();
@@ -428,7 +435,7 @@ __device__ inline float wave_max_f(float x) {
 // waves. One pair in kSurJudgeStride is also evaluated by the exact kernel,
 // which records the largest relative deviation.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_wave_compose_surr(
+__global__ __launch_bounds__(256, 2) void k_wave_compose_surr(
     const float* __restrict__ P, int N, int Np, double K, const double* __restrict__ quad,
     const double* __restrict__ recv, const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
     const double* __restrict__ Dsrc, const double* __restrict__ Drefl, double* Sr, double* Si, double* Dr,
@@ -480,24 +487,28 @@ __global__ __launch_bounds__(256) void k_wave_compose_surr(
             __syncthreads();
             forward64(P, L, sx, act[0], act[1], sy, tid, lane, wave);
             if (wave == 0 && R < N) {
-                double gr, gi, hr, hi;
+                double gr = 0.0, gi = 0.0, hr = 0.0, hi = 0.0;
                 if (in) {
                     const double E = 2.0 * kPI * exp(V);
+                    const J01 jj = bessel_j01(h);
                     gr = sp.sg + sp.s * (double)sy[lane][0];
-                    gi = -E * j0(h);
+                    gi = -E * jj.j0;
                     hr = sp.sgh + sp.s * (double)sy[lane][1];
-                    hi = E * j1(h);
+                    hi = E * jj.j1;
                     nsur++;
-                    if ((unsigned)(R * 7 + S * 13 + k) % kSurJudgeStride == 0) {
-                        const GFVal g = havelock_gf(X, Y, V);
+                } else {
+                    nexact++;
+                }
+                const bool judge = in && (unsigned)(R * 7 + S * 13 + k) % kSurJudgeStride == 0;
+                if (!in || judge) {   // one call site of the exact kernel
+                    const GFVal g = havelock_gf(X, Y, V);
+                    if (judge) {
                         const double den = fmax(fmax(fabs(g.gr), fabs(g.gi)), fmax(fabs(g.hr), fabs(g.hi)));
                         const double num = fmax(fmax(fabs(gr - g.gr), fabs(gi - g.gi)), fmax(fabs(hr - g.hr), fabs(hi - g.hi)));
                         if (den > 0.0) errmax = fmaxf(errmax, (float)(num / den));
+                    } else {
+                        gr = g.gr; gi = g.gi; hr = g.hr; hi = g.hi;
                     }
-                } else {
-                    const GFVal g = havelock_gf(X, Y, V);
-                    gr = g.gr; gi = g.gi; hr = g.hr; hi = g.hi;
-                    nexact++;
                 }
                 // one quadrature node of wave_panel (Panel.cpp:601-636,776-794)
                 const double w = p[6];
