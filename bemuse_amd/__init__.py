@@ -132,6 +132,7 @@ _SIGS = {
                                 _PD, _PD, c_void_p]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
+    "bem_set_graphs": (c_int, [c_void_p, c_int]),
     "bem_surrogate_default_config": (c_int, [POINTER(SurrogateConfig)]),
     "bem_surrogate_param_count": (c_int, []),
     "bem_surrogate_train": (c_int, [c_void_p, POINTER(SurrogateConfig), POINTER(SurrogateReport)]),
@@ -323,6 +324,10 @@ class Context:
         out = np.zeros((len(X), 8))
         _check(lib().bem_eval_havelock(self._h, 64, len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
         return out
+
+    def set_graphs(self, enable):
+        """Capture and replay per-frequency HIP graphs on the solve lanes (default on)."""
+        _check(lib().bem_set_graphs(self._h, int(bool(enable))))
 
     # ---- wave-term surrogate ----
     def surrogate_train(self, **overrides):

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -70,6 +71,7 @@ struct SurState {
 struct bem_ctx {
     int device = 0;
     SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
+    bool graphs = true;                // replay captured frequency graphs on the lanes
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
     std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch
@@ -145,11 +147,24 @@ struct LinSys {
     }
 };
 
+// Configuration a captured frequency graph is valid for: the body (by id,
+// never reused), its padded order, mode, headings, rad_only, compose case and
+// surrogate use.
+struct GraphKey {
+    unsigned long long body = 0;
+    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1;
+    bool operator==(const GraphKey& o) const {
+        return body == o.body && Np == o.Np && mode == o.mode && nb == o.nb && rad_only == o.rad_only &&
+               compose == o.compose && sur == o.sur;
+    }
+};
+
 // Results of one frequency as copied back to the host (pinned).
 struct LaneOut {
     double coeffs[2 * (36 + 12 * kMaxRhs)];
     RefineCtrl ctrl;
     SurCounters sur;
+    double freq[2];          // (omega, kappa) of the frequency, host -> device
     int flags[kMaxPanels + 1];
 };
 
@@ -160,9 +175,8 @@ struct Lane {
     std::vector<hipEvent_t> lu_events;
     hipEvent_t t[4] = {};                      // timing
     hipEvent_t done = nullptr;
-    unsigned* counter = nullptr;
+    unsigned long long* counter = nullptr;     // triangular-solve ticket | epoch << 32
     unsigned long long* pub = nullptr;         // triangular-solve publication words
-    unsigned epoch = 0;
     RefineCtrl* ctrl = nullptr;
     LaneOut* hout = nullptr;                   // pinned
     LaneOut* dout = nullptr;
@@ -177,10 +191,15 @@ struct Lane {
     int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
     bool sur_used = false;
     double omega = 0, kappa = 0;
+    // captured frequency graph and the configuration it was captured for
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey{}, last_key{};
+    bool graphed = false;
 
     ~Lane() {
         if (s1) (void)hipStreamSynchronize(s1);
         if (s2) (void)hipStreamSynchronize(s2);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         for (void* p : owned)
             if (p) (void)hipFree(p);
         if (hout) (void)hipHostFree(hout);
@@ -208,7 +227,8 @@ struct Lane {
         int nblk = Np / kLuNb;
         size_t npub = lu_solve_pub_words(Np);
         if ((r = get(&counter, 1)) || (r = get(&pub, npub)) || (r = get(&ctrl, 1)) || (r = get(&dout, 1))) return r;
-        HIP_TRY(hipMemset(counter, 0, sizeof(unsigned)));
+        const unsigned long long first_epoch = 1ULL << 32;
+        HIP_TRY(hipMemcpy(counter, &first_epoch, sizeof(first_epoch), hipMemcpyHostToDevice));
         HIP_TRY(hipMemset(pub, 0, npub * sizeof(unsigned long long)));
         HIP_TRY(hipHostMalloc((void**)&hout, sizeof(LaneOut), hipHostMallocDefault));
         size_t NN = (size_t)Np * Np, NR = (size_t)Np * kMaxRhs;
@@ -228,6 +248,7 @@ struct Lane {
 
 struct bem_body {
     bem_ctx* ctx = nullptr;
+    unsigned long long id = 0;          // unique per body (graph keys)
     int N = 0, Np = 0;
     BodyGeometry geo;
     double *d_body = nullptr, *d_refl = nullptr, *d_quad = nullptr, *d_recv = nullptr, *d_gen = nullptr;
@@ -270,7 +291,7 @@ void solve_system(Lane& L, int precision, int nr) {
     if (precision == 64) {
         launch_copy2(L.s1, n, s.Br, s.Bi, s.Xr, s.Xi);
         launch_lu_solve<double, double>(L.s1, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
-                                        L.counter, L.pub, L.epoch);
+                                        L.counter, L.pub);
         return;
     }
     const int* fany = s.flag32 + s.Np / kLuNb;
@@ -278,7 +299,7 @@ void solve_system(Lane& L, int precision, int nr) {
     launch_refine_reset(L.s1, L.ctrl);
     launch_to_f32(L.s1, n, s.Br, s.Bi, s.X32r, s.X32i);
     launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, L.counter,
-                                  L.pub, L.epoch);
+                                  L.pub);
     launch_from_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i);
     for (int it = 0; it <= kRefineMaxIter; it++) {
         // r = b - A x in FP64; the first residual also yields |A|_inf
@@ -290,7 +311,7 @@ void solve_system(Lane& L, int precision, int nr) {
         // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
         launch_to_f32(L.s1, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
         launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                      L.counter, L.pub, L.epoch, skip);
+                                      L.counter, L.pub, skip);
         launch_add_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
     }
 }
@@ -462,6 +483,8 @@ int bem_body_create(bem_ctx* c, const bem_mesh* m, bem_body** out) {
     if (!c || !m || !out) return fail(BEM_ERR_ARG, "bem_body_create: null argument");
     HIP_TRY(hipSetDevice(c->device));
     bem_body* b = new bem_body;
+    static std::atomic<unsigned long long> next_body_id{1};
+    b->id = next_body_id++;
     b->ctx = c;
     std::string err;
     if (!build_geometry(m->m, b->geo, err)) { delete b; return fail(BEM_ERR_ARG, err); }
@@ -541,22 +564,69 @@ int ensure_lanes(bem_body* b, int n, bool need32) {
     return BEM_OK;
 }
 
+// Copies the frequency (omega, kappa) to the lane's device block; kernels
+// read it there. The host value must stay until the copy has run (the lane's
+// next frequency is set only after its previous one completed).
+void set_frequency(Lane& L, double omega, double kappa) {
+    L.hout->freq[0] = omega;
+    L.hout->freq[1] = kappa;
+    (void)hipMemcpyAsync(L.dout->freq, L.hout->freq, sizeof(L.hout->freq), hipMemcpyHostToDevice, L.s1);
+}
+
+// Wave assembly + compose at the lane's device frequency; kappa (the same
+// value) selects the surrogate path on the host.
 void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
     bool f32 = (mode != BEM_MODE_FP64);
     L.sur_used = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
     if (L.sur_used) {
         launch_sur_counters_clear(L.s1, &L.dout->sur);
-        launch_wave_compose_surr(L.s1, b->ctx->surrogate->params, b->N, b->Np, kappa, b->d_quad, b->d_recv, b->Ssrc,
+        launch_wave_compose_surr(L.s1, b->ctx->surrogate->params, b->N, b->Np, L.dout->freq, b->d_quad, b->d_recv, b->Ssrc,
                                  b->Srefl, b->Dsrc, b->Drefl, L.Sr, L.Si, L.sys.Dr, L.sys.Di, L.sys.D32r, L.sys.D32i,
                                  &L.dout->sur);
         return;
     }
-    launch_wave_compose(L.s1, b->N, b->Np, kappa, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
+    launch_wave_compose(L.s1, b->N, b->Np, L.dout->freq, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
                         L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr);
 }
 
 // Issues one frequency on a lane (all asynchronous on the lane's streams).
 // The betas must already be on the device (L.d_betas).
+// Everything one frequency puts on the lane's streams, from the frequency
+// copy to the result copies. timing: per-stage events (not inside a graph).
+int enqueue_frequency(bem_body* b, Lane& L, bool timing) {
+    LinSys& s = L.sys;
+    hipStream_t st = L.s1;
+    const int nb = L.nb, nr = 6 + nb;
+    (void)hipMemcpyAsync(L.dout->freq, L.hout->freq, sizeof(L.hout->freq), hipMemcpyHostToDevice, st);
+    if (timing) HIP_TRY(hipEventRecord(L.t[0], st));
+    assemble(b, L, L.mode, L.kappa, L.compose);
+    if (timing) HIP_TRY(hipEventRecord(L.t[1], st));
+    factor_system(L, L.prec);
+    if (timing) HIP_TRY(hipEventRecord(L.t[2], st));
+    launch_incident(st, b->N, b->Np, nb, b->d_body, b->d_gen, L.d_betas, L.dout->freq, L.Xinr, L.Xini, kMaxRhs,
+                    L.PhiIr, L.PhiIi, L.DAr, L.DAi);
+    launch_matvec<double>(st, b->Np, nr, L.Sr, L.Si, L.Xinr, L.Xini, s.Pr, s.Pi, nullptr, nullptr, 0, s.Br, s.Bi);
+    solve_system(L, L.prec, nr);
+    launch_reduce_coeffs(st, b->N, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
+    if (timing) HIP_TRY(hipEventRecord(L.t[3], st));
+    int np = b->Np / kLuNb;
+    HIP_TRY(hipMemcpyAsync(L.hout->coeffs, L.dout->coeffs, sizeof(double) * 2 * (36 + 12 * nb), hipMemcpyDeviceToHost, st));
+    if (L.prec == 32)
+        HIP_TRY(hipMemcpyAsync(&L.hout->ctrl, L.ctrl, sizeof(RefineCtrl), hipMemcpyDeviceToHost, st));
+    if (L.sur_used)
+        HIP_TRY(hipMemcpyAsync(&L.hout->sur, &L.dout->sur, sizeof(SurCounters), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(L.hout->flags, L.prec == 64 ? s.flag : s.flag32, sizeof(int) * (np + 1),
+                           hipMemcpyDeviceToHost, st));
+    return BEM_OK;
+}
+
+// Issues one frequency on a lane (all asynchronous on the lane's streams).
+// The betas must already be on the device (L.d_betas). With graphs enabled
+// on the context, the second consecutive frequency of the same configuration
+// (body, mode, headings, compose case, surrogate use) on a lane captures the
+// enqueue sequence into a graph, and later ones replay it: the frequency
+// reaches the kernels through the captured host-to-device copy, and every
+// other input is at a fixed device address.
 int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, double kappa, int compose, int rad_only,
                     int nb) {
     if (b->Np > L.sys.capNp) return fail(BEM_ERR_STATE, "lane workspace smaller than the body");
@@ -567,26 +637,35 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     L.pending = index;
     L.mode = mode; L.prec = (mode == BEM_MODE_FP64) ? 64 : 32;
     L.omega = omega; L.kappa = kappa; L.compose = compose; L.rad_only = rad_only; L.nb = nb;
-    int nr = 6 + nb;
-    HIP_TRY(hipEventRecord(L.t[0], st));
-    assemble(b, L, mode, kappa, compose);
-    HIP_TRY(hipEventRecord(L.t[1], st));
-    factor_system(L, L.prec);
-    HIP_TRY(hipEventRecord(L.t[2], st));
-    launch_incident(st, b->N, b->Np, nb, b->d_body, b->d_gen, L.d_betas, omega, kappa, L.Xinr, L.Xini, kMaxRhs,
-                    L.PhiIr, L.PhiIi, L.DAr, L.DAi);
-    launch_matvec<double>(st, b->Np, nr, L.Sr, L.Si, L.Xinr, L.Xini, s.Pr, s.Pi, nullptr, nullptr, 0, s.Br, s.Bi);
-    solve_system(L, L.prec, nr);
-    launch_reduce_coeffs(st, b->N, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
-    HIP_TRY(hipEventRecord(L.t[3], st));
-    int np = b->Np / kLuNb;
-    HIP_TRY(hipMemcpyAsync(L.hout->coeffs, L.dout->coeffs, sizeof(double) * 2 * (36 + 12 * nb), hipMemcpyDeviceToHost, st));
-    if (L.prec == 32)
-        HIP_TRY(hipMemcpyAsync(&L.hout->ctrl, L.ctrl, sizeof(RefineCtrl), hipMemcpyDeviceToHost, st));
-    if (L.sur_used)
-        HIP_TRY(hipMemcpyAsync(&L.hout->sur, &L.dout->sur, sizeof(SurCounters), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(L.hout->flags, L.prec == 64 ? s.flag : s.flag32, sizeof(int) * (np + 1),
-                           hipMemcpyDeviceToHost, st));
+    L.hout->freq[0] = omega;
+    L.hout->freq[1] = kappa;
+    const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
+    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0};
+    const bool use_graph = b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
+    L.last_key = key;
+    L.graphed = use_graph;
+    if (!use_graph) {
+        int r = enqueue_frequency(b, L, true);
+        if (r) return r;
+    } else {
+        if (!(L.gexec && L.gkey == key)) {
+            if (L.gexec) { (void)hipGraphExecDestroy(L.gexec); L.gexec = nullptr; }
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            int r = enqueue_frequency(b, L, false);
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+            if (e != hipSuccess) return fail(BEM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&L.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { L.gexec = nullptr; return fail(BEM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+            L.gkey = key;
+        }
+        L.sur_used = sur;
+        HIP_TRY(hipEventRecord(L.t[0], st));
+        HIP_TRY(hipGraphLaunch(L.gexec, st));
+        HIP_TRY(hipEventRecord(L.t[3], st));
+    }
     HIP_TRY(hipEventRecord(L.done, st));
     HIP_TRY(hipGetLastError());
     return BEM_OK;
@@ -621,9 +700,13 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
     int np = b->Np / kLuNb;
     for (int i = 0; i < np; i++) loc.pivot_fallback += (L.hout->flags[i] & 1) ? 1 : 0;
     float t01 = 0, t12 = 0, t23 = 0;
-    (void)hipEventElapsedTime(&t01, L.t[0], L.t[1]);
-    (void)hipEventElapsedTime(&t12, L.t[1], L.t[2]);
-    (void)hipEventElapsedTime(&t23, L.t[2], L.t[3]);
+    if (L.graphed) {   // stages are not timed inside a graph: the whole frequency is in ms_solve
+        (void)hipEventElapsedTime(&t23, L.t[0], L.t[3]);
+    } else {
+        (void)hipEventElapsedTime(&t01, L.t[0], L.t[1]);
+        (void)hipEventElapsedTime(&t12, L.t[1], L.t[2]);
+        (void)hipEventElapsedTime(&t23, L.t[2], L.t[3]);
+    }
     loc.ms_assemble = t01; loc.ms_factor = t12; loc.ms_solve = t23;
     const double* out = L.hout->coeffs;
     int nb = L.nb;
@@ -843,6 +926,7 @@ int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose,
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
     Lane& L = *b->lanes[0];
+    set_frequency(L, 0.0, kappa);
     assemble(b, L, mode, kappa, compose);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(L.s1));
@@ -868,6 +952,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
     Lane& L = *b->lanes[0];
+    set_frequency(L, 0.0, kappa);
     assemble(b, L, mode, kappa, BEM_COMPOSE_WAVE);
     LuProfile* prof = new LuProfile;
     prof->N = b->N;
@@ -993,6 +1078,12 @@ int bem_surrogate_default_config(bem_surrogate_config* cfg) {
 }
 
 int bem_surrogate_param_count(void) { return kSurParams; }
+
+int bem_set_graphs(bem_ctx* c, int enable) {
+    if (!c) return fail(BEM_ERR_ARG, "bem_set_graphs: null context");
+    c->graphs = enable != 0;
+    return BEM_OK;
+}
 
 int bem_surrogate_train(bem_ctx* c, const bem_surrogate_config* cfg, bem_surrogate_report* rep) {
     if (!c || !cfg) return fail(BEM_ERR_ARG, "bem_surrogate_train: null argument");

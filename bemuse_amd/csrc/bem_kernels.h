@@ -13,12 +13,14 @@ constexpr int kMaxPanels = 1024; // largest padded order / kLuNb (Np <= 65536)
 
 void launch_rankine(hipStream_t s, int N, int Np, const double* body, const double* refl, const double* recv,
                     double* Ssrc, double* Srefl, double* Dsrc, double* Drefl);
-void launch_wave_compose(hipStream_t s, int N, int Np, double K, int compose, const double* quad, const double* recv,
-                         const double* Ssrc, const double* Srefl, const double* Dsrc, const double* Drefl,
-                         double* Sr, double* Si, double* Dr, double* Di, float* D32r, float* D32i);
+// freq: device pointer to (omega, kappa) of the frequency.
+void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
+                         const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
+                         const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
+                         float* D32i);
 void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out);
 void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,
-                     double omega, double kappa, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
+                     const double* freq, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
                      double* DAr, double* DAi);
 int matvec_kslabs(int Np);
 // Y = M X (mode 0) or Y = B - M X (mode 1); with Pabs (nks x Np scratch) and
@@ -62,8 +64,8 @@ template <typename T>
 int gemm_tile_choice(int M, int N, int K);
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned long long* pub,
-                     unsigned& epoch, const int* skip = nullptr);
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,
+                     const int* skip = nullptr);
 // Words of the triangular-solve publication buffer (enough for FP64).
 inline size_t lu_solve_pub_words(int Np) { return (size_t)(Np / kLuNb) * kMaxRhs * kLuNb * 2 * 2; }
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb,

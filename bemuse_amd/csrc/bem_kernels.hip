@@ -70,13 +70,16 @@ __global__ __launch_bounds__(256) void k_rankine(int N, int Np, const double* __
 // ZERO: S = Ssrc + Srefl, D = Dsrc + Drefl; INF: S = Ssrc - Srefl, D = Dsrc - Drefl.
 // Writes planar FP64 S and D, and optionally an FP32 copy of D. Padding: S = 0,
 // D = identity.
-__global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, double K, int compose,
+// freq: device (omega, kappa) of the frequency (read on the device, so a
+// captured graph replays with the frequency written before each launch).
+__global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const double* __restrict__ freq, int compose,
                                                       const double* __restrict__ quad, const double* __restrict__ recv,
                                                       const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
                                                       const double* __restrict__ Dsrc, const double* __restrict__ Drefl,
                                                       double* Sr, double* Si, double* Dr, double* Di,
                                                       float* D32r, float* D32i) {
     __shared__ double sq[AS_TS][32];
+    const double K = freq[1];
     int r0 = blockIdx.x * AS_TR, s0 = blockIdx.y * AS_TS;
     for (int i = threadIdx.x; i < AS_TS * 32; i += 256) {
         int s = s0 + i / 32;
@@ -132,8 +135,9 @@ __global__ void k_havelock_eval(int n, const double* X, const double* Y, const d
 // block Xin = [N_k (6 columns, :117-138) | -dPhi_I/dn * area (nb columns, :445)].
 // PhiI, DPhiIA (= dPhi_I/dn * area) are N x nb complex planar with ld Np.
 __global__ void k_incident(int N, int Np, int nb, const double* __restrict__ body, const double* __restrict__ gen,
-                           const double* __restrict__ betas, double omega, double kappa,
+                           const double* __restrict__ betas, const double* __restrict__ freq,
                            double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi, double* DAr, double* DAi) {
+    const double omega = freq[0], kappa = freq[1];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Np) return;
     const double G = 9.81;
@@ -907,10 +911,15 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 // solve's epoch (value bits | epoch << 32), stored with agent-scope relaxed
 // atomics. Readers poll exactly the words they need with agent-scope relaxed
 // loads until all carry the current epoch, so no flag, release fence or L2
-// write-back is involved, and the buffer never needs clearing (each solve
-// call takes a new epoch). Work-groups take blocks in dependency order from a
+// write-back is involved, and the buffer never needs clearing (each launch
+// takes a new epoch). Work-groups take blocks in dependency order from a
 // ticket counter, so a block only waits for blocks whose work-groups are
-// already running.
+// already running. Ticket and epoch share one 64-bit counter word
+// (epoch << 32 | ticket): every work-group of a launch reads the same epoch
+// with its ticket, and the work-group taking the last ticket starts the next
+// epoch at ticket 0. The epoch therefore lives on the device, and a captured
+// graph replays the solve unchanged. The word starts at epoch 1 (the
+// publication buffer starts zeroed, epoch 0).
 // ---------------------------------------------------------------------------
 template <typename TX> struct PubWords;
 template <> struct PubWords<float> { static constexpr int W = 1; };
@@ -1043,22 +1052,25 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
                                                       const T* __restrict__ Ai, const T* __restrict__ Inv_r,
                                                       const T* __restrict__ Inv_i, const T* __restrict__ P_r,
                                                       const T* __restrict__ P_i, TX* Xr, TX* Xi,
-                                                      unsigned long long* pub, unsigned* counter, unsigned epoch,
+                                                      unsigned long long* pub, unsigned long long* counter,
                                                       const int* skip) {
     static_assert(sizeof(T) == sizeof(TX), "factor and solution share one precision");
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     if (skip && *skip) return;   // uniform: no work-group takes a ticket
     __shared__ TX sr[MV_MAXR][LU_NB + 1], si[MV_MAXR][LU_NB + 1];   // a staged block of X / y  [rhs][row]
-    __shared__ unsigned ticket;
+    __shared__ unsigned ticket, epoch_s;
     int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int i = lane & 15, g = lane >> 4;
     int nblk = Np / LU_NB;
     if (tid == 0) {
-        ticket = atomicAdd(counter, 1u);
-        if (ticket == (unsigned)nblk - 1) atomicExch(counter, 0u);
+        const unsigned long long w = atomicAdd(counter, 1ULL);
+        ticket = (unsigned)(w & 0xffffffffULL);
+        epoch_s = (unsigned)(w >> 32);
+        if (ticket == (unsigned)nblk - 1) atomicExch(counter, (unsigned long long)(epoch_s + 1u) << 32);
     }
     __syncthreads();
+    const unsigned epoch = epoch_s;
     int b = lower ? (int)ticket : nblk - 1 - (int)ticket;
     int r0 = b * LU_NB, w0 = r0 + wave * 16;
     int prev = lower ? b - 1 : b + 1;               // the block whose X finishes X_b
@@ -1331,12 +1343,13 @@ void launch_rankine(hipStream_t s, int N, int Np, const double* body, const doub
     hipLaunchKernelGGL(k_rankine, grid, dim3(256), 0, s, N, Np, body, refl, recv, Ssrc, Srefl, Dsrc, Drefl);
 }
 
-void launch_wave_compose(hipStream_t s, int N, int Np, double K, int compose, const double* quad, const double* recv,
-                         const double* Ssrc, const double* Srefl, const double* Dsrc, const double* Drefl,
-                         double* Sr, double* Si, double* Dr, double* Di, float* D32r, float* D32i) {
+void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
+                         const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
+                         const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
+                         float* D32i) {
     dim3 grid(Np / AS_TR, Np / AS_TS);
-    hipLaunchKernelGGL(k_wave_compose, grid, dim3(256), 0, s, N, Np, K, compose, quad, recv, Ssrc, Srefl, Dsrc, Drefl,
-                       Sr, Si, Dr, Di, D32r, D32i);
+    hipLaunchKernelGGL(k_wave_compose, grid, dim3(256), 0, s, N, Np, freq, compose, quad, recv, Ssrc, Srefl, Dsrc,
+                       Drefl, Sr, Si, Dr, Di, D32r, D32i);
 }
 
 void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out) {
@@ -1344,9 +1357,9 @@ void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y
 }
 
 void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,
-                     double omega, double kappa, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
+                     const double* freq, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
                      double* DAr, double* DAi) {
-    hipLaunchKernelGGL(k_incident, dim3(cdiv(Np, 256)), dim3(256), 0, s, N, Np, nb, body, gen, betas, omega, kappa,
+    hipLaunchKernelGGL(k_incident, dim3(cdiv(Np, 256)), dim3(256), 0, s, N, Np, nb, body, gen, betas, freq,
                        Xr, Xi, ldx, PhiIr, PhiIi, DAr, DAi);
 }
 
@@ -1643,23 +1656,21 @@ template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
-                     const int* flag_any, TX* Xr, TX* Xi, unsigned* counter, unsigned long long* pub,
-                     unsigned& epoch, const int* skip) {
+                     const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,
+                     const int* skip) {
     hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
     int nblk = Np / LU_NB;
-    ++epoch;
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i,
-                       w.PL_r, w.PL_i, Xr, Xi, pub, counter, epoch, skip);
-    ++epoch;
+                       w.PL_r, w.PL_i, Xr, Xi, pub, counter, skip);
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i,
-                       w.PU_r, w.PU_i, Xr, Xi, pub, counter, epoch, skip);
+                       w.PU_r, w.PU_i, Xr, Xi, pub, counter, skip);
 }
 template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*,
                                               const LuWork<double>&, const int*, const int*, double*, double*,
-                                              unsigned*, unsigned long long*, unsigned&, const int*);
+                                              unsigned long long*, unsigned long long*, const int*);
 template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const LuWork<float>&,
-                                            const int*, const int*, float*, float*, unsigned*, unsigned long long*,
-                                            unsigned&, const int*);
+                                            const int*, const int*, float*, float*, unsigned long long*,
+                                            unsigned long long*, const int*);
 
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
     hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);

@@ -118,7 +118,7 @@ void launch_sur_eval(hipStream_t s, const float* params, int n, const double* h,
                      int* used);
 // Wave assembly fused with compose (as launch_wave_compose with compose =
 // wave), the regular Green function from the network inside its domain.
-void launch_wave_compose_surr(hipStream_t s, const float* params, int N, int Np, double K, const double* quad,
+void launch_wave_compose_surr(hipStream_t s, const float* params, int N, int Np, const double* freq, const double* quad,
                               const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                               const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
                               float* D32i, SurCounters* cnt);

@@ -436,7 +436,7 @@ __device__ inline float wave_max_f(float x) {
 // which records the largest relative deviation.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256, 2) void k_wave_compose_surr(
-    const float* __restrict__ P, int N, int Np, double K, const double* __restrict__ quad,
+    const float* __restrict__ P, int N, int Np, const double* __restrict__ freq, const double* __restrict__ quad,
     const double* __restrict__ recv, const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
     const double* __restrict__ Dsrc, const double* __restrict__ Drefl, double* Sr, double* Si, double* Dr,
     double* Di, float* D32r, float* D32i, SurCounters* cnt) {
@@ -446,6 +446,7 @@ __global__ __launch_bounds__(256, 2) void k_wave_compose_surr(
     __shared__ __align__(16) float act[2][64 * LDA];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = blockIdx.x * 64, s0 = blockIdx.y * SA_TS;
+    const double K = freq[1];
     stage_small(P, L, tid);
     for (int k = tid; k < SA_TS * 32; k += 256) {
         const int s = s0 + k / 32;
@@ -601,12 +602,12 @@ void launch_sur_eval(hipStream_t s, const float* params, int n, const double* h,
     hipLaunchKernelGGL(k_sur_eval, dim3(cdivll(n, 64)), dim3(256), 0, s, params, n, h, v, out, used);
 }
 
-void launch_wave_compose_surr(hipStream_t s, const float* params, int N, int Np, double K, const double* quad,
+void launch_wave_compose_surr(hipStream_t s, const float* params, int N, int Np, const double* freq, const double* quad,
                               const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                               const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
                               float* D32i, SurCounters* cnt) {
     dim3 grid(Np / 64, Np / SA_TS);
-    hipLaunchKernelGGL(k_wave_compose_surr, grid, dim3(256), 0, s, params, N, Np, K, quad, recv, Ssrc, Srefl, Dsrc,
+    hipLaunchKernelGGL(k_wave_compose_surr, grid, dim3(256), 0, s, params, N, Np, freq, quad, recv, Ssrc, Srefl, Dsrc,
                        Drefl, Sr, Si, Dr, Di, D32r, D32i, cnt);
 }
 

@@ -160,6 +160,39 @@ def test_sweep_matches_single_solves(cyl, mode):
         assert relerr(sw[key][3], r[key]) < (1e-8 if mode == bm.MODE_FP64 else 1e-6)
 
 
+@pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
+def test_graph_replays_match_eager(ctx, cyl, mode):
+    """Captured frequency graphs: 10 frequencies on 2 lanes give each lane
+    one eager frequency, one capture and three replays. Every replayed result
+    equals the eager (graphs off) result bit for bit, also across repeated
+    sweeps with eager work and a mode change in between."""
+    mesh, body, ob = cyl
+    omegas = np.linspace(0.2, 1.9, 10)
+    betas = [0.0, 0.7]
+    ctx.set_graphs(False)
+    try:
+        eager = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
+    finally:
+        ctx.set_graphs(True)
+    for rep in range(3):
+        gr = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
+        for key in ("A", "B", "F", "FK", "SC"):
+            assert np.array_equal(gr[key], eager[key]), (rep, key)
+        # graphed frequencies report the whole frequency time in ms_solve
+        assert any(i["ms_assemble"] == 0.0 and i["ms_solve"] > 0.0 for i in gr["info"])
+        # eager work and another mode between the replays
+        body.solve(omegas[1], betas=betas, mode=bm.MODE_FP64 if mode == bm.MODE_MIXED else bm.MODE_MIXED)
+    # the same configuration on fewer headings (a different graph) still agrees
+    g1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
+    ctx.set_graphs(False)
+    try:
+        e1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
+    finally:
+        ctx.set_graphs(True)
+    for key in ("A", "B", "F"):
+        assert np.array_equal(g1[key], e1[key]), key
+
+
 def test_solve_vs_committed_fixture(ctx):
     """A, B, F against the committed oracle fixture (tests/golden/cylinder_abf.npz)."""
     import os
