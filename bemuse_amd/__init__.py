@@ -326,7 +326,7 @@ class Context:
         return out
 
     def set_graphs(self, enable):
-        """Capture and replay per-frequency HIP graphs on the solve lanes (default on)."""
+        """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""
         _check(lib().bem_set_graphs(self._h, int(bool(enable))))
 
     # ---- wave-term surrogate ----

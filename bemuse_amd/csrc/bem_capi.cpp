@@ -71,7 +71,7 @@ struct SurState {
 struct bem_ctx {
     int device = 0;
     SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
-    bool graphs = true;                // replay captured frequency graphs on the lanes
+    bool graphs = false;               // replay captured frequency graphs on the lanes (bem_set_graphs)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
     std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch

@@ -113,14 +113,18 @@ int bem_geometry(const bem_mesh* m, double* body, double* refl, double* quad, do
 int bem_create(int device, bem_ctx** out);
 void bem_destroy(bem_ctx* ctx);
 
-/* Frequency graphs (on by default): from the second consecutive frequency of
- * one configuration (body, mode, headings, compose case) on a solve lane,
+/* Frequency graphs (off by default): from the second consecutive frequency
+ * of one configuration (body, mode, headings, compose case) on a solve lane,
  * the lane's whole per-frequency sequence (assembly, factorisation, solves,
  * refinement, reductions, result copies) is captured once as a HIP graph and
  * replayed for the following frequencies. Results are identical either way.
- * Inside a graph the stages are not timed separately: bem_solve_info then
- * reports the whole frequency in ms_solve (ms_assemble = ms_factor = 0).
- * No counterpart in the reference (its solves are host loops). */
+ * They pay off when hardware queues are scarce (GPU_MAX_HW_QUEUES = 4:
+ * 38 -> 44 frequency-solves/s at 6k panels, 8 lanes) and lose when the lanes
+ * have their own queues (32 queues: 60 eager vs 36 graphed), which is why
+ * they are off by default. Inside a graph the stages are not timed
+ * separately: bem_solve_info then reports the whole frequency in ms_solve
+ * (ms_assemble = ms_factor = 0). No counterpart in the reference (its
+ * solves are host loops). */
 int bem_set_graphs(bem_ctx* ctx, int enable);
 
 /* Uploads the geometry of a mesh (Hydrodynamic_Radiation_Solver::Setup,

@@ -170,27 +170,24 @@ def test_graph_replays_match_eager(ctx, cyl, mode):
     omegas = np.linspace(0.2, 1.9, 10)
     betas = [0.0, 0.7]
     ctx.set_graphs(False)
+    eager = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
+    e1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
+    ctx.set_graphs(True)
     try:
-        eager = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
+        for rep in range(3):
+            gr = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
+            for key in ("A", "B", "F", "FK", "SC"):
+                assert np.array_equal(gr[key], eager[key]), (rep, key)
+            # graphed frequencies report the whole frequency time in ms_solve
+            assert any(i["ms_assemble"] == 0.0 and i["ms_solve"] > 0.0 for i in gr["info"])
+            # eager work and another mode between the replays
+            body.solve(omegas[1], betas=betas, mode=bm.MODE_FP64 if mode == bm.MODE_MIXED else bm.MODE_MIXED)
+        # the same configuration on fewer headings (a different graph) still agrees
+        g1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
+        for key in ("A", "B", "F"):
+            assert np.array_equal(g1[key], e1[key]), key
     finally:
-        ctx.set_graphs(True)
-    for rep in range(3):
-        gr = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=2)
-        for key in ("A", "B", "F", "FK", "SC"):
-            assert np.array_equal(gr[key], eager[key]), (rep, key)
-        # graphed frequencies report the whole frequency time in ms_solve
-        assert any(i["ms_assemble"] == 0.0 and i["ms_solve"] > 0.0 for i in gr["info"])
-        # eager work and another mode between the replays
-        body.solve(omegas[1], betas=betas, mode=bm.MODE_FP64 if mode == bm.MODE_MIXED else bm.MODE_MIXED)
-    # the same configuration on fewer headings (a different graph) still agrees
-    g1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
-    ctx.set_graphs(False)
-    try:
-        e1 = body.solve_sweep(omegas[:6], betas=betas[:1], mode=mode, lanes=2)
-    finally:
-        ctx.set_graphs(True)
-    for key in ("A", "B", "F"):
-        assert np.array_equal(g1[key], e1[key]), key
+        ctx.set_graphs(False)
 
 
 def test_solve_vs_committed_fixture(ctx):
