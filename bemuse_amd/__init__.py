@@ -127,6 +127,7 @@ _SIGS = {
     "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_gf_coefficients": (c_int, [_PD, c_int]),
+    "bem_bessel01": (c_int, [c_int, _PD, _PD]),
     "bem_mesh_read": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(c_void_p)]),
     "bem_solve_batch": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), _PI, _PD, c_int, c_int, _PD, c_int, _PD,
                                 _PD, _PD, c_void_p]),
@@ -201,6 +202,14 @@ def surrogate_default_config():
     c = SurrogateConfig()
     _check(lib().bem_surrogate_default_config(ctypes.byref(c)))
     return {k: getattr(c, k) for k, _ in c._fields_}
+
+
+def bessel01(x):
+    """(n, 4) J0, J1, Y0, Y1 from the device code's Bessel evaluator, run on the host (Y = 0 for x <= 3)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros((len(x), 4))
+    _check(lib().bem_bessel01(len(x), _dp(x), _dp(out)))
+    return out
 
 
 def gf_coefficients():

@@ -981,6 +981,15 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     return BEM_OK;
 }
 
+int bem_bessel01(int n, const double* x, double* out) {
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(BEM_ERR_ARG, "bem_bessel01: bad arguments");
+    for (int i = 0; i < n; i++) {
+        if (!(x[i] >= 0.0)) return fail(BEM_ERR_ARG, "bem_bessel01: x must be >= 0");
+        dev::bessel01(x[i], out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+    }
+    return BEM_OK;
+}
+
 int bem_gf_coefficients(double* out, int n) {
     if (!out || n < 65) return fail(BEM_ERR_ARG, "bem_gf_coefficients: need room for 65 values");
     int k = 0;
@@ -1175,8 +1184,10 @@ int bem_surrogate_train(bem_ctx* c, const bem_surrogate_config* cfg, bem_surroga
             g[1] += sp.sgh;
             // relative to the largest component of (GF, GFh), imaginary parts exact
             const double E = 2.0 * M_PI * std::exp(vv);
-            const double den = std::max(std::max(std::fabs(g[0]), std::fabs(E * ::j0(h))),
-                                        std::max(std::fabs(g[1]), std::fabs(E * ::j1(h))));
+            double J0, J1, Y0, Y1;
+            dev::bessel01(h, J0, J1, Y0, Y1);
+            const double den = std::max(std::max(std::fabs(g[0]), std::fabs(E * J0)),
+                                        std::max(std::fabs(g[1]), std::fabs(E * J1)));
             rel[j] = den > 0.0 ? num / den : 0.0;
             rs += rel[j] * rel[j];
             out.rel_max = std::max(out.rel_max, rel[j]);

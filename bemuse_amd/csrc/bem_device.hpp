@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bem_bessel_coeffs.h"
+
 namespace bemuse {
 namespace dev {
 
@@ -33,6 +35,72 @@ __host__ __device__ inline double struve_h1(double x, double y1) {
     double c1 = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
     double c2 = kPI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
     return c1 / c2 + y1;
+}
+
+// Bessel J0(x), J1(x) for x >= 0 and, for x > 3, Y0(x), Y1(x) (0 below):
+// what HavelockGF needs (Kernels.cpp:31-34, 284-295, 327-338; Y only above
+// 3, :56-74). x <= 3: power series. x > 3: Hankel form with P0, Q0, P1, Q1
+// as degree-23 Chebyshev series in t = 6/x - 1 (coefficients and method in
+// gen_bessel.py) and one shared sin/cos of x (three-piece pi/2 reduction,
+// Taylor series on |r| <= pi/4). Beyond x = 1.6e6 the reduction is no longer
+// exact and the device library functions are used.
+__host__ __device__ inline void bessel01(double x, double& J0, double& J1, double& Y0, double& Y1) {
+    if (x <= 3.0) {
+        const double t = -0.25 * x * x;
+        double a = kBesJ0S[15], b = kBesJ1S[15];
+#pragma unroll
+        for (int k = 14; k >= 0; k--) {
+            a = fma(a, t, kBesJ0S[k]);
+            b = fma(b, t, kBesJ1S[k]);
+        }
+        J0 = a;
+        J1 = 0.5 * x * b;
+        Y0 = 0.0;
+        Y1 = 0.0;
+        return;
+    }
+    if (x > 1.6e6) {
+        J0 = j0(x); J1 = j1(x); Y0 = y0(x); Y1 = y1(x);
+        return;
+    }
+    const double u = 3.0 / x, t = 2.0 * u - 1.0, t2 = 2.0 * t;
+    // Clenshaw for the four series at once
+    double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, r1 = 0.0, r2 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 23; k >= 1; k--) {
+        double tp = fma(t2, p1, kBesP0[k] - p2); p2 = p1; p1 = tp;
+        double tq = fma(t2, q1, kBesQ0[k] - q2); q2 = q1; q1 = tq;
+        double tr = fma(t2, r1, kBesP1[k] - r2); r2 = r1; r1 = tr;
+        double ts = fma(t2, s1, kBesQ1[k] - s2); s2 = s1; s1 = ts;
+    }
+    const double P0 = fma(t, p1, kBesP0[0] - p2), Q0 = fma(t, q1, kBesQ0[0] - q2);
+    const double P1 = fma(t, r1, kBesP1[0] - r2), Q1 = fma(t, s1, kBesQ1[0] - s2);
+    // x = n pi/2 + r, |r| <= pi/4 (n pi/2 pieces exact for n < 2^21)
+    const double n = rint(x * kTwoOverPi);
+    double r = fma(-n, kPio2A, x);
+    r = fma(-n, kPio2B, r);
+    r = fma(-n, kPio2C, r);
+    const double rr = r * r;
+    double sp = kSinT[8], cp = kCosT[9];
+#pragma unroll
+    for (int k = 7; k >= 0; k--) sp = fma(sp, rr, kSinT[k]);
+#pragma unroll
+    for (int k = 8; k >= 0; k--) cp = fma(cp, rr, kCosT[k]);
+    const double sr = r * sp, cr = cp;
+    double sx, cx;
+    switch (((long long)n) & 3) {
+        case 0: sx = sr; cx = cr; break;
+        case 1: sx = cr; cx = -sr; break;
+        case 2: sx = -sr; cx = -cr; break;
+        default: sx = -cr; cx = sr; break;
+    }
+    // chi0 = x - pi/4: cos chi0 = (cos x + sin x)/sqrt 2, sin chi0 = (sin x - cos x)/sqrt 2
+    const double cc = (cx + sx) * kSqrtHalf, sc = (sx - cx) * kSqrtHalf;
+    const double A = sqrt(kTwoOver3Pi * u);       // sqrt(2 / (pi x))
+    J0 = A * (P0 * cc - Q0 * sc);
+    Y0 = A * (P0 * sc + Q0 * cc);
+    J1 = A * (P1 * sc + Q1 * cc);                 // chi1 = chi0 - pi/2
+    Y1 = A * (Q1 * sc - P1 * cc);
 }
 
 // Polynomials A..D (local flow) and dA..dC (its horizontal derivative) of the
@@ -87,9 +155,8 @@ __host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     double R2 = be * dA - (1.0 - al) * dB + be * (1.0 - be) * rho * (1.0 - 2.0 * rho) * dC;
     double Ls = 2.0 * PS * inv1 - 4.0 * QS + 2.0 * rw * R2;
     // wave parts W, Wh (GF_Func_W / GF_Func_Wh, :284-295, :327-338)
-    double j0v = j0(hh), j1v = j1(hh);
-    double y0v = 0.0, y1v = 0.0;
-    if (hh > 3.0) { y0v = y0(hh); y1v = y1(hh); }
+    double j0v, j1v, y0v, y1v;
+    bessel01(hh, j0v, j1v, y0v, y1v);
     double E = 2.0 * kPI * ev;
     GFVal o;
     o.gr = L0 + E * struve_h0(hh, y0v);

@@ -30,7 +30,12 @@ __device__ inline float sur_tanh(float x) { return 1.0f - 2.0f / (__expf(2.0f * 
 // J0(x), J1(x) out of line: inlined into the fused assembly they would raise
 // its register count above what two work-groups per CU allow.
 struct J01 { double j0, j1; };
-__device__ __attribute__((noinline)) J01 bessel_j01(double x) { return J01{j0(x), j1(x)}; }
+__device__ __attribute__((noinline)) J01 bessel_j01(double x) {
+    J01 o;
+    double y0v, y1v;
+    bessel01(x, o.j0, o.j1, y0v, y1v);
+    return o;
+}
 
 __device__ inline f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
@@ -406,10 +411,11 @@ __global__ __launch_bounds__(256) void k_sur_eval(const float* __restrict__ P, i
         double* o = out + 4 * (size_t)j;
         if (in) {
             const double E = 2.0 * kPI * exp(v);
+            const J01 jj = bessel_j01(h);
             o[0] = sp.sg + sp.s * (double)sy[lane][0];
-            o[1] = -E * j0(h);
+            o[1] = -E * jj.j0;
             o[2] = sp.sgh + sp.s * (double)sy[lane][1];
-            o[3] = E * j1(h);
+            o[3] = E * jj.j1;
         } else {
             const GFVal g = havelock_gf(h, 0.0, v);
             o[0] = g.gr; o[1] = g.gi; o[2] = g.hr; o[3] = g.hi;

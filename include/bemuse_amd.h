@@ -190,6 +190,11 @@ int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const
  * GF_FuncA..D and GF_dFuncA..C (Kernels.cpp:120-256) in that order (64
  * values), then EUL (BEMUse_Math_Types.h:99). n must be >= 65. */
 int bem_gf_coefficients(double* out, int n);
+/* The Bessel evaluator of the device HavelockGF (the reference calls
+ * std::tr1::cyl_bessel_j / cyl_neumann, Kernels.cpp:31-34), run on the host
+ * for checking without a GPU: out[4 i .. 4 i + 3] = J0, J1, Y0, Y1 at x[i] >= 0,
+ * with Y0 = Y1 = 0 for x <= 3 (HavelockGF uses them only above 3). */
+int bem_bessel01(int n, const double* x, double* out);
 /* Device LU + solve of a caller matrix (complex n x n, column-major):
  * precision 64 (FP64 LU) or 32 (FP32-MFMA LU + FP64 refinement); X receives
  * the solution of A X = B for nrhs columns. */
