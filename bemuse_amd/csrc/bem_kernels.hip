@@ -755,7 +755,11 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
     static_assert(TM >= 64 && TN >= 64, "a chunk must give every thread at least one 4-element load");
     constexpr int NLA = TM * GK / 4 / 256;          // 4-element A loads per thread per chunk (re or im)
     constexpr int NLB = TN * GK / 4 / 256;
-    __shared__ T As[2][2][TM][GKP];                 // [buf][re/im][row][k]
+    // A is kept k-major as in global memory (one vector store per load,
+    // conflict-free); rows padded by 4 so the four lane groups of an MFMA
+    // operand read hit different banks. B is kept column-major.
+    constexpr int TMP = TM + 4;
+    __shared__ T As[2][2][GK][TMP];                 // [buf][re/im][k][row]
     __shared__ T Bs[2][2][TN][GKP];                 // [buf][re/im][col][k]
     int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
     int ntiles = tiles_m * tiles_n;
@@ -800,10 +804,9 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
             int kk = idx / (TM / 4), rq = idx % (TM / 4);
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                As[buf][p][4 * rq + 0][kk] = -ga[p][l].x;
-                As[buf][p][4 * rq + 1][kk] = -ga[p][l].y;
-                As[buf][p][4 * rq + 2][kk] = -ga[p][l].z;
-                As[buf][p][4 * rq + 3][kk] = -ga[p][l].w;
+                v4 v = ga[p][l];
+                v.x = -v.x; v.y = -v.y; v.z = -v.z; v.w = -v.w;
+                *reinterpret_cast<v4*>(&As[buf][p][kk][4 * rq]) = v;
             }
         }
 #pragma unroll
@@ -853,8 +856,8 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
             for (int a = 0; a < WM; a++)
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    ar[a][t] = As[buf][0][row0 + 16 * a + i][4 * g + t];
-                    ai[a][t] = As[buf][1][row0 + 16 * a + i][4 * g + t];
+                    ar[a][t] = As[buf][0][4 * g + t][row0 + 16 * a + i];
+                    ai[a][t] = As[buf][1][4 * g + t][row0 + 16 * a + i];
                 }
 #pragma unroll
             for (int b = 0; b < WN; b++)
@@ -863,17 +866,28 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
                     br[b][t] = Bs[buf][0][col0 + 16 * b + i][4 * g + t];
                     bi[b][t] = Bs[buf][1][col0 + 16 * b + i][4 * g + t];
                 }
+            // four passes over the blocks per k step, so consecutive MFMAs
+            // write different accumulators (each element still accumulates
+            // re*re, -im*im, re*im, im*re in the same order)
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int t = 0; t < 4; t++) {
 #pragma unroll
                 for (int a = 0; a < WM; a++)
 #pragma unroll
-                    for (int b = 0; b < WN; b++) {
-                        cr[a][b] = MT::mma(ar[a][t], br[b][t], cr[a][b]);
-                        cr[a][b] = MT::mma(-ai[a][t], bi[b][t], cr[a][b]);
-                        ci[a][b] = MT::mma(ar[a][t], bi[b][t], ci[a][b]);
-                        ci[a][b] = MT::mma(ai[a][t], br[b][t], ci[a][b]);
-                    }
+                    for (int b = 0; b < WN; b++) cr[a][b] = MT::mma(ar[a][t], br[b][t], cr[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) ci[a][b] = MT::mma(ar[a][t], bi[b][t], ci[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) cr[a][b] = MT::mma(-ai[a][t], bi[b][t], cr[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) ci[a][b] = MT::mma(ai[a][t], br[b][t], ci[a][b]);
+            }
         }
         if (c + 1 < nchunk) {
             sstore(buf ^ 1);
