@@ -134,6 +134,9 @@ _SIGS = {
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
+    "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
+    "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
+                                 _PD]),
     "bem_surrogate_default_config": (c_int, [POINTER(SurrogateConfig)]),
     "bem_surrogate_param_count": (c_int, []),
     "bem_surrogate_train": (c_int, [c_void_p, POINTER(SurrogateConfig), POINTER(SurrogateReport)]),
@@ -494,6 +497,40 @@ class Body:
                                      FK.ctypes.data_as(_PD), SC.ctypes.data_as(_PD), infos))
         return dict(A=A.reshape(nf, 6, 6), B=B.reshape(nf, 6, 6), F=F[:, :, :nb], FK=FK[:, :, :nb],
                     SC=SC[:, :, :nb], info=[infos[i].as_dict() for i in range(nf)])
+
+    def _potentials(self, phiJ, phiS, nb):
+        pj = np.asfortranarray(phiJ, dtype=np.complex128).reshape(self.N, 6, order="F")
+        ps = np.asfortranarray(phiS, dtype=np.complex128).reshape(self.N, nb, order="F") if nb else None
+        return pj, ps
+
+    def kochin(self, omega, phiJ, phiS=None, betas=(), nkoch=36, kappa=None):
+        """Kochin functions of a solved frequency (potentials as solve(..., want_phi=True)
+        returns them): (KochRad nkoch x 6, KochDiff nkoch x nb) complex."""
+        kappa = omega * omega / 9.81 if kappa is None else kappa
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        nb = len(betas)
+        pj, ps = self._potentials(phiJ, phiS, nb)
+        rad = np.zeros((nkoch, 6), dtype=np.complex128)
+        diff = np.zeros((nkoch, max(nb, 1)), dtype=np.complex128)
+        _check(lib().bem_kochin(self.ctx._h, self._h, omega, kappa, nkoch, nb, _dp(betas) if nb else None,
+                                pj.ctypes.data_as(_PD), ps.ctypes.data_as(_PD) if nb else None,
+                                rad.ctypes.data_as(_PD), diff.ctypes.data_as(_PD)))
+        return rad, diff[:, :nb]
+
+    def fs_elevation(self, omega, points, phiJ, phiS=None, betas=(), kappa=None):
+        """Radiated (npts x 6) and diffracted (npts x nb) free-surface elevation at exterior points."""
+        kappa = omega * omega / 9.81 if kappa is None else kappa
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        betas = np.ascontiguousarray(betas, dtype=np.float64)
+        nb = len(betas)
+        pj, ps = self._potentials(phiJ, phiS, nb)
+        rad = np.zeros((len(pts), 6), dtype=np.complex128)
+        diff = np.zeros((len(pts), max(nb, 1)), dtype=np.complex128)
+        _check(lib().bem_fs_elevation(self.ctx._h, self._h, omega, kappa, len(pts), _dp(pts), nb,
+                                      _dp(betas) if nb else None, pj.ctypes.data_as(_PD),
+                                      ps.ctypes.data_as(_PD) if nb else None, rad.ctypes.data_as(_PD),
+                                      diff.ctypes.data_as(_PD)))
+        return rad, diff[:, :nb]
 
     def solve(self, omega, kappa=None, betas=(), mode=MODE_FP64, compose=COMPOSE_WAVE, rad_only=False,
               want_phi=False):

@@ -52,6 +52,21 @@ int dalloc_t(T** p, size_t n) { return dalloc(reinterpret_cast<void**>(p), n * s
 
 inline int round_up(int n, int m) { return (n + m - 1) / m * m; }
 
+// Temporary device buffers of one call, freed on every exit path.
+struct SurBuffers {
+    std::vector<void*> ptrs;
+    ~SurBuffers() {
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(T** p, size_t n) {
+        int r = dalloc_t(p, n);
+        if (!r) ptrs.push_back(*p);
+        return r;
+    }
+};
+
 }  // namespace
 
 struct bem_mesh {
@@ -981,6 +996,89 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     return BEM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Uploads the potentials (interleaved complex, N x 6 and N x nb) as one planar
+// N x (6 + nb) block, and the headings.
+int upload_potentials(bem_body* b, int nb, const double* betas, const double* phiJ, const double* phiS,
+                      SurBuffers& B, double** phr, double** phi, double** dbetas) {
+    const int N = b->N, nc = 6 + nb;
+    std::vector<double> hr((size_t)N * nc), hi((size_t)N * nc);
+    for (int c = 0; c < nc; c++) {
+        const double* src = c < 6 ? phiJ + 2 * (size_t)c * N : phiS + 2 * (size_t)(c - 6) * N;
+        for (int i = 0; i < N; i++) { hr[(size_t)c * N + i] = src[2 * i]; hi[(size_t)c * N + i] = src[2 * i + 1]; }
+    }
+    int r = BEM_OK;
+    if ((r = B.get(phr, hr.size())) || (r = B.get(phi, hi.size())) || (r = B.get(dbetas, std::max(nb, 1)))) return r;
+    HIP_TRY(hipMemcpy(*phr, hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(*phi, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
+    if (nb > 0) HIP_TRY(hipMemcpy(*dbetas, betas, nb * 8, hipMemcpyHostToDevice));
+    return BEM_OK;
+}
+
+// Splits an n x (6 + nb) complex block into the 6-column and nb-column outputs.
+void split_columns(const std::vector<double>& h, int n, int nb, double* A, double* B) {
+    const int nc = 6 + nb;
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < nc; c++) {
+            double* dst = c < 6 ? (A ? A + 2 * ((size_t)i * 6 + c) : nullptr)
+                                : (B ? B + 2 * ((size_t)i * nb + (c - 6)) : nullptr);
+            if (dst) { dst[0] = h[2 * ((size_t)i * nc + c)]; dst[1] = h[2 * ((size_t)i * nc + c) + 1]; }
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bem_kochin(bem_ctx* c, bem_body* b, double omega, double kappa, int nkoch, int nb, const double* betas,
+               const double* phiJ, const double* phiS, double* KochRad, double* KochDiff) {
+    if (!c || !b || b->ctx != c || nkoch < 1 || nkoch > 65536 || nb < 0 || nb > kMaxRhs - 6 || !phiJ ||
+        (nb > 0 && (!betas || !phiS)))
+        return fail(BEM_ERR_ARG, "bem_kochin: bad arguments (nkoch in [1, 65536], nb in [0, 10])");
+    if (nb > 0 && omega == 0.0) return fail(BEM_ERR_ARG, "bem_kochin: diffraction needs omega != 0");
+    HIP_TRY(hipSetDevice(c->device));
+    SurBuffers B;
+    double *phr = nullptr, *phi = nullptr, *dbetas = nullptr, *dout = nullptr;
+    int r = upload_potentials(b, nb, betas, phiJ, phiS, B, &phr, &phi, &dbetas);
+    if (r) return r;
+    const int nc = 6 + nb;
+    if ((r = B.get(&dout, 2 * (size_t)nkoch * nc))) return r;
+    launch_kochin(c->stream, b->N, nkoch, nb, b->d_body, b->d_gen, dbetas, omega, kappa, phr, phi, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<double> h(2 * (size_t)nkoch * nc);
+    HIP_TRY(hipMemcpy(h.data(), dout, h.size() * 8, hipMemcpyDeviceToHost));
+    split_columns(h, nkoch, nb, KochRad, KochDiff);
+    return BEM_OK;
+}
+
+int bem_fs_elevation(bem_ctx* c, bem_body* b, double omega, double kappa, int npts, const double* pts, int nb,
+                     const double* betas, const double* phiJ, const double* phiS, double* ExtRad, double* ExtDiff) {
+    if (!c || !b || b->ctx != c || npts < 1 || !pts || nb < 0 || nb > kMaxRhs - 6 || !phiJ ||
+        (nb > 0 && (!betas || !phiS)))
+        return fail(BEM_ERR_ARG, "bem_fs_elevation: bad arguments (npts >= 1, nb in [0, 10])");
+    if (nb > 0 && omega == 0.0) return fail(BEM_ERR_ARG, "bem_fs_elevation: diffraction needs omega != 0");
+    HIP_TRY(hipSetDevice(c->device));
+    SurBuffers B;
+    double *phr = nullptr, *phi = nullptr, *dbetas = nullptr, *dpts = nullptr, *dout = nullptr;
+    int r = upload_potentials(b, nb, betas, phiJ, phiS, B, &phr, &phi, &dbetas);
+    if (r) return r;
+    const int nc = 6 + nb;
+    if ((r = B.get(&dpts, 3 * (size_t)npts)) || (r = B.get(&dout, 2 * (size_t)npts * nc))) return r;
+    HIP_TRY(hipMemcpy(dpts, pts, 24 * (size_t)npts, hipMemcpyHostToDevice));
+    launch_fs_field(c->stream, b->N, npts, dpts, nb, dbetas, omega, kappa, b->d_body, b->d_refl, b->d_quad, b->d_gen,
+                    phr, phi, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::vector<double> h(2 * (size_t)npts * nc);
+    HIP_TRY(hipMemcpy(h.data(), dout, h.size() * 8, hipMemcpyDeviceToHost));
+    split_columns(h, npts, nb, ExtRad, ExtDiff);
+    return BEM_OK;
+}
+
 int bem_bessel01(int n, const double* x, double* out) {
     if (n < 0 || (n > 0 && (!x || !out))) return fail(BEM_ERR_ARG, "bem_bessel01: bad arguments");
     for (int i = 0; i < n; i++) {
@@ -1047,21 +1145,6 @@ std::vector<unsigned long long> shuffle_table(uint64_t seed, unsigned long long 
     }
     return t;
 }
-
-// Device buffers of one training run, freed on every exit path.
-struct SurBuffers {
-    std::vector<void*> ptrs;
-    ~SurBuffers() {
-        for (void* p : ptrs)
-            if (p) (void)hipFree(p);
-    }
-    template <typename T>
-    int get(T** p, size_t n) {
-        int r = dalloc_t(p, n);
-        if (!r) ptrs.push_back(*p);
-        return r;
-    }
-};
 
 int ensure_surrogate(bem_ctx* c) {
     if (!c->surrogate) c->surrogate = new SurState;

@@ -86,6 +86,15 @@ void launch_from_f32(hipStream_t s, size_t n, double* a, double* b, const float*
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb);
 void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
                    const double* xi, double* out);
+// Kochin functions (nkoch x (6 + nb) complex) and exterior-point potentials
+// as free-surface elevation (npts x (6 + nb) complex) from the potentials
+// phr/phi (N x (6 + nb), planar, ld N); see bem_postproc.hip.
+void launch_kochin(hipStream_t s, int N, int nkoch, int nb, const double* body, const double* gen,
+                   const double* betas, double omega, double kappa, const double* phr, const double* phi,
+                   double* out);
+void launch_fs_field(hipStream_t s, int N, int npts, const double* pts, int nb, const double* betas, double omega,
+                     double kappa, const double* body, const double* refl, const double* quad, const double* gen,
+                     const double* phr, const double* phi, double* out);
 void launch_reduce_coeffs(hipStream_t s, int N, int Np, int nb, const double* gen, const double* Xr, const double* Xi,
                           const double* PhiIr, const double* PhiIi, const double* DAr, const double* DAi,
                           double* out);

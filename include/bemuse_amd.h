@@ -201,6 +201,34 @@ int bem_bessel01(int n, const double* x, double* out);
 int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
                     double* X, bem_solve_info* info);
 
+/* ---- post-processing of a solved frequency -------------------------------------
+ * Both take the potentials bem_solve_frequency returns (phiJ N x 6, phiS
+ * N x nb, complex column-major) at the same omega, kappa and headings. */
+
+/* Kochin functions (Set_Kochin_Mats, Hydrodynamic_Solver.cpp:598-642, used at
+ * :492-494) at theta_i = (i + 1/2) 2 pi / nkoch:
+ *   H(theta) = sum_P A_P (E dphi/dn - (n . grad E) phi),
+ *   E = exp(K z + i K (x cos theta + y sin theta)) at the centroids,
+ * with (dphi/dn, phi) = (N_k, Phi_J) for radiation and (dPhi_I/dn, Phi_S) for
+ * diffraction, as the reference combines them. The reference builds E from
+ * an uninitialised value (:626-627); E here is the intended Newman (1967)
+ * factor. Outputs KochRad nkoch x 6 and KochDiff nkoch x nb, complex
+ * row-major (either may be NULL). */
+int bem_kochin(bem_ctx* ctx, bem_body* b, double omega, double kappa, int nkoch, int nb, const double* betas,
+               const double* phiJ, const double* phiS, double* KochRad, double* KochDiff);
+
+/* Free-surface elevation of the radiated and diffracted waves at npts
+ * exterior points pts (npts x 3) (Prepare_FS_Linear_System :205-237,
+ * Prepare_FS_Linear_System_Wave_Terms :239-266, ExtRadMat / ExtDiffMat
+ * :499-505): Rankine, image and wave influences of every panel at the points,
+ *   ExtRad  = -i Omega / g (WS N_k - WD Phi_J),
+ *   ExtDiff = -i Omega / g (WS dPhi_I/dn + WD Phi_S).
+ * Outputs npts x 6 and npts x nb complex row-major (either may be NULL).
+ * The reference takes the points from Boundary::Get_FreeSurface_Nodes (the
+ * numerical wave tank boundaries); here the caller passes them. */
+int bem_fs_elevation(bem_ctx* ctx, bem_body* b, double omega, double kappa, int npts, const double* pts, int nb,
+                     const double* betas, const double* phiJ, const double* phiS, double* ExtRad, double* ExtDiff);
+
 /* ---- wave-term surrogate ------------------------------------------------------
  * No counterpart in the reference: an MLP trained on the device from the exact
  * kernel stands in for HavelockGF (Kernels.cpp:342-412) inside

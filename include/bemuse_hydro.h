@@ -32,12 +32,20 @@
  * A, B, F, FK, SC (private members in the reference, read from its output
  * files), Solve_Frequencies for a pipelined sweep over many frequencies, and
  * Write_Output (default true) to switch the files off.
- * Not provided: IFR lid panels, Kochin functions, exterior free-surface
- * elevation and RAOs. The files .4 (RAOs, which in the reference use an
- * uninitialised damping matrix) and _KochinRad/_KochinDiff.bem are created
- * empty. The waterplane moments of the hydrostatic stiffness are integrated
- * exactly over the waterline polygon (the reference sums them over its lid
- * panels, which are not generated here).
+ *   - Kochin functions (Hydrodynamic_Solver.cpp:489-494, 598-642) at NKochin
+ *     angles after every wave Solve, appended to _KochinRad.bem and
+ *     _KochinDiff.bem (Hydrodynamic_Solver_IO.cpp:589-621); the exponential E
+ *     is the Newman factor exp(K z + i K (x cos + y sin)) (the reference
+ *     builds it from an uninitialised value, :626-627); NKochin = 0 skips them;
+ *   - exterior free-surface elevation (:499-505) at the boundary's
+ *     FreeSurface_Nodes (filled by the caller; the reference takes them from
+ *     its numerical-wave-tank boundaries), kept in FS_Rad_SolArray /
+ *     FS_Scat_SolArray (:514-515).
+ * Not provided: IFR lid panels, RAOs and the .vtp wave-height export. The file
+ * .4 (RAOs, which in the reference use an uninitialised damping matrix) is
+ * created empty. The waterplane moments of the hydrostatic stiffness are
+ * integrated exactly over the waterline polygon (the reference sums them over
+ * its lid panels, which are not generated here).
  */
 #ifndef BEMUSE_HYDRO_H
 #define BEMUSE_HYDRO_H
@@ -90,6 +98,7 @@ struct Vector3 {
     Vector3(Real a, Real b, Real c) : x(a), y(b), z(c) {}
 };
 
+static const Real TwoPI = 2.0 * 3.14159265358979323846;
 static const Real TwoPIinv = 1.0 / (2.0 * 3.14159265358979323846);
 static const Real R2D = 180.0 / 3.14159265358979323846;
 static const Real Gravity = 9.81;      // BEMUse_Math_Types.h:115 (K and Phi_I use this constant)
@@ -160,6 +169,9 @@ public:
     // waterplane area and moments int x, y, x^2, x y, y^2 dA (exact over the waterline polygon)
     Real WP_Area = 0, WP_Mx = 0, WP_My = 0, WP_Mxx = 0, WP_Mxy = 0, WP_Myy = 0;
     int N_Aux_Elements() const { return 0; }
+    // exterior free-surface points for the wave elevation (Get_FreeSurface_Nodes)
+    std::vector<Vector3> FreeSurface_Nodes;
+    void Get_FreeSurface_Nodes(std::vector<Vector3>& N) const { N = FreeSurface_Nodes; }
 
 protected:
     void Adopt(int rc, bem_mesh* m) {
@@ -390,6 +402,7 @@ public:
         if (body_) { bem_body_destroy(body_); body_ = nullptr; }
         if (!check(bem_body_create(ctx_, B->Get_Device_Mesh(), &body_))) return;
         NPTOT = bem_body_n(body_);
+        B->Get_FreeSurface_Nodes(Wave_Nodes);
         if (!check(bem_body_assemble_rankine(ctx_, body_))) return;
         A_ij = Matrix::Zero(6, 6);
         B_ij = Matrix::Zero(6, 6);
@@ -415,12 +428,14 @@ public:
         int rad_only = (Frequency == FreqInf) ? 1 : 0;
         std::vector<double> A(36), Bm(36), F(12 * (size_t)NBeta), FK(12 * (size_t)NBeta), SC(12 * (size_t)NBeta);
         std::vector<double> phiJ, phiS;
-        if (Keep_Potentials) { phiJ.resize(2 * (size_t)NPTOT * 6); phiS.resize(2 * (size_t)NPTOT * NBeta); }
+        const bool post = !rad_only && Omega != 0.0 && (NKoch > 0 || !Wave_Nodes.empty());
+        const bool want_phi = Keep_Potentials || post;
+        if (want_phi) { phiJ.resize(2 * (size_t)NPTOT * 6); phiS.resize(2 * (size_t)NPTOT * NBeta); }
         if (!check(bem_solve_frequency(ctx_, body_, mode_, Omega, Kappa, compose, rad_only, NBeta,
                                        NBeta ? BetaArray.data() : nullptr, A.data(), Bm.data(),
                                        NBeta ? F.data() : nullptr, NBeta ? FK.data() : nullptr,
-                                       NBeta ? SC.data() : nullptr, Keep_Potentials ? phiJ.data() : nullptr,
-                                       Keep_Potentials && NBeta ? phiS.data() : nullptr, &Info))) {
+                                       NBeta ? SC.data() : nullptr, want_phi ? phiJ.data() : nullptr,
+                                       want_phi && NBeta ? phiS.data() : nullptr, &Info))) {
             set_nan();
             return;
         }
@@ -438,6 +453,27 @@ public:
                 FK_i(i, b) = CReal(FK[k], FK[k + 1]);
                 SC_i(i, b) = CReal(SC[k], SC[k + 1]);
             }
+        KochinRad = CMatrix::Zero(std::max(NKoch, 0), 6);
+        KochinDiff = CMatrix::Zero(std::max(NKoch, 0), NBeta);
+        if (post && NKoch > 0) {   // Hydrodynamic_Solver.cpp:492-494
+            std::vector<double> kr(2 * (size_t)NKoch * 6), kd(2 * (size_t)NKoch * NBeta);
+            if (check(bem_kochin(ctx_, body_, Omega, Kappa, NKoch, NBeta, NBeta ? BetaArray.data() : nullptr,
+                                 phiJ.data(), NBeta ? phiS.data() : nullptr, kr.data(), NBeta ? kd.data() : nullptr))) {
+                KochinRad = unpack_rows(kr, NKoch, 6);
+                KochinDiff = unpack_rows(kd, NKoch, NBeta);
+            }
+        }
+        if (post && !Wave_Nodes.empty()) {   // :499-505
+            int nw = (int)Wave_Nodes.size();
+            std::vector<double> pts(3 * (size_t)nw), er(2 * (size_t)nw * 6), ed(2 * (size_t)nw * NBeta);
+            for (int i = 0; i < nw; i++) { pts[3 * i] = Wave_Nodes[i].x; pts[3 * i + 1] = Wave_Nodes[i].y; pts[3 * i + 2] = Wave_Nodes[i].z; }
+            if (check(bem_fs_elevation(ctx_, body_, Omega, Kappa, nw, pts.data(), NBeta,
+                                       NBeta ? BetaArray.data() : nullptr, phiJ.data(),
+                                       NBeta ? phiS.data() : nullptr, er.data(), NBeta ? ed.data() : nullptr))) {
+                FS_Rad_SolArray.push_back(unpack_rows(er, nw, 6));
+                FS_Scat_SolArray.push_back(unpack_rows(ed, nw, NBeta));
+            }
+        }
         if (Write_Output) Update_Output_File();   // Hydrodynamic_Solver.cpp:508
     }
 
@@ -485,6 +521,8 @@ public:
     const CMatrix& Froude_Krylov_Force() const { return FK_i; }
     const CMatrix& Scattering_Force() const { return SC_i; }
     const bem_solve_info& Solve_Info() const { return Info; }
+    const CMatrix& Kochin_Radiation() const { return KochinRad; }    // NKochin x 6
+    const CMatrix& Kochin_Diffraction() const { return KochinDiff; } // NKochin x NBeta
     int N_Panels() const { return NPTOT; }
 
     bool Keep_Potentials = false;
@@ -696,6 +734,28 @@ private:
         forces(".2", F_i, 1.0);
         forces(".2fk", FK_i, 1.0);
         forces(".2sc", SC_i, -1.0);   // the reference writes -arg and -Im for the scattering force
+        // Kochin functions (Hydrodynamic_Solver_IO.cpp:589-621)
+        const bool kochin = Omega != 0.0 && Frequency != FreqInf && KochinRad.rows() == NKoch;
+        {
+            std::ofstream f(OutputPath + "_KochinRad.bem", std::ios_base::app);
+            for (int i = 0; kochin && i < NKoch; i++) {
+                Real th = (0.5 + i) * TwoPI / NKoch;
+                for (int d = 0; d < 6; d++)
+                    f << Field(Frequency) << d + 1 << "  " << Field(th * R2D) << Field(std::abs(KochinRad(i, d)))
+                      << Field(std::arg(KochinRad(i, d))) << Field(std::real(KochinRad(i, d)))
+                      << Field(std::imag(KochinRad(i, d))) << std::endl;
+            }
+        }
+        {
+            std::ofstream f(OutputPath + "_KochinDiff.bem", std::ios_base::app);
+            for (int d = 0; kochin && d < NBeta; d++)
+                for (int i = 0; i < NKoch; i++) {
+                    Real th = (0.5 + i) * TwoPI / NKoch;
+                    f << Field(Frequency) << Field(BetaArray[d] * R2D) << "  " << Field(th * R2D)
+                      << Field(std::abs(KochinDiff(i, d))) << Field(std::arg(KochinDiff(i, d)))
+                      << Field(std::real(KochinDiff(i, d))) << Field(std::imag(KochinDiff(i, d))) << std::endl;
+                }
+        }
     }
 
     bool check(int rc) {
@@ -707,6 +767,13 @@ private:
         const Real n = std::numeric_limits<Real>::quiet_NaN();
         A_ij = Matrix(6, 6, n); B_ij = Matrix(6, 6, n);
         F_i = CMatrix(6, NBeta, CReal(n, n)); FK_i = F_i; SC_i = F_i;
+    }
+    // r x c complex, row-major interleaved
+    static CMatrix unpack_rows(const std::vector<double>& v, int r, int c) {
+        CMatrix M = CMatrix::Zero(r, c);
+        for (int i = 0; i < r; i++)
+            for (int j = 0; j < c; j++) M(i, j) = CReal(v[2 * ((size_t)i * c + j)], v[2 * ((size_t)i * c + j) + 1]);
+        return M;
     }
     static CMatrix unpack(const std::vector<double>& v, int r, int c) {
         CMatrix M(r, c);
@@ -728,6 +795,8 @@ private:
     Matrix A_ij = Matrix::Zero(6, 6), B_ij = Matrix::Zero(6, 6);
     Matrix M_ij = Matrix::Zero(6, 6), C_ij = Matrix::Zero(6, 6);
     CMatrix F_i, FK_i, SC_i;
+    CMatrix KochinRad, KochinDiff;
+    std::vector<Vector3> Wave_Nodes;
     bem_solve_info Info = bem_solve_info();
 };
 

@@ -274,6 +274,22 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
         assert (out / ("dropin_test" + ext)).exists()
     bem = (out / "dropin_test.bem").read_text()
     assert bem.count("Frequency:") == 3 and "Excitation Forces:" in bem
+    # Kochin files (Hydrodynamic_Solver_IO.cpp:589-621), NKochin = 36 by default:
+    # "F dof theta |H| arg Re Im" and "F beta theta |H| arg Re Im"
+    import oracle.postproc as opp
+    rowsk = [l.split() for l in (out / "dropin_test_KochinRad.bem").read_text().splitlines() if l.strip()]
+    rowsd = [l.split() for l in (out / "dropin_test_KochinDiff.bem").read_text().splitlines() if l.strip()]
+    assert len(rowsk) == 2 * 36 * 6 and len(rowsd) == 2 * 2 * 36
+    for k, w in enumerate((0.4, 1.3)):
+        K = w * w / 9.81
+        r = ob.solve(w, kappa=K, betas=[0.0, np.pi / 4], want_phi=True)
+        kr, kd = opp.kochin(ob.geometry(), w, K, 36, [0.0, np.pi / 4], r["phiJ"], r["phiS"])
+        blk = rowsk[216 * k: 216 * (k + 1)]
+        got = np.array([float(x[3]) for x in blk]).reshape(36, 6)
+        assert np.abs(got - np.abs(kr)).max() < 1e-5 * np.abs(kr).max() + tol * np.abs(kr).max()
+        blkd = rowsd[72 * k: 72 * (k + 1)]
+        gotd = np.array([float(x[3]) for x in blkd]).reshape(2, 36).T
+        assert np.abs(gotd - np.abs(kd)).max() < 1e-5 * np.abs(kd).max() + tol * np.abs(kd).max()
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
