@@ -120,6 +120,11 @@ _SIGS = {
     "bem_body_create": (c_int, [c_void_p, c_void_p, _PP]),
     "bem_body_destroy": (None, [c_void_p]),
     "bem_body_n": (c_int, [c_void_p]),
+    "bem_body_n_system": (c_int, [c_void_p]),
+    "bem_body_create_lid": (c_int, [c_void_p, c_void_p, c_void_p, _PP]),
+    "bem_mesh_lid_disc": (c_int, [c_double, c_int, c_int, c_int, _PP]),
+    "bem_mesh_lid_rect": (c_int, [c_double, c_double, c_int, c_int, c_int, _PP]),
+    "bem_mesh_append": (c_int, [c_void_p, c_void_p, c_double, c_double, c_double]),
     "bem_body_assemble_rankine": (c_int, [c_void_p, c_void_p]),
     "bem_solve_frequency": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, c_int, c_int, c_int, _PD,
                                     _PD, _PD, _PD, _PD, _PD, _PD, _PD, POINTER(SolveInfo)]),
@@ -260,6 +265,20 @@ class Mesh:
     @classmethod
     def box(cls, L, W, z0, z1, nx, ny, nz, open_top=True):
         return cls._make(lib().bem_mesh_box, L, W, z0, z1, nx, ny, nz, int(open_top))
+
+    @classmethod
+    def lid_disc(cls, radius, n_radial, n_azimuthal, cosine=True):
+        """Interior free-surface lid of a volume of revolution (irregular-frequency removal)."""
+        return cls._make(lib().bem_mesh_lid_disc, radius, n_radial, n_azimuthal, int(cosine))
+
+    @classmethod
+    def lid_rect(cls, L, W, nx, ny, cosine=False):
+        """Interior free-surface lid of a barge."""
+        return cls._make(lib().bem_mesh_lid_rect, L, W, nx, ny, int(cosine))
+
+    def append(self, other, dx=0.0, dy=0.0, dz=0.0):
+        _check(lib().bem_mesh_append(self._h, other._h, dx, dy, dz))
+        return self
 
     @classmethod
     def semisub3(cls, col_radius=6.0, draft=20.0, spacing=50.0, pontoon_width=6.0, pontoon_height=4.0, gap=0.5,
@@ -436,12 +455,14 @@ class Context:
 
 
 class Body:
-    def __init__(self, ctx: Context, mesh: Mesh):
+    def __init__(self, ctx: Context, mesh: Mesh, lid: Mesh = None):
+        """lid: interior free-surface panels for irregular-frequency removal."""
         h = c_void_p()
-        _check(lib().bem_body_create(ctx._h, mesh._h, ctypes.byref(h)))
+        _check(lib().bem_body_create_lid(ctx._h, mesh._h, lid._h if lid is not None else None, ctypes.byref(h)))
         self._h = h
         self.ctx = ctx
-        self.N = lib().bem_body_n(h)
+        self.N = lib().bem_body_n(h)                # body panels (rows of the potentials)
+        self.N_system = lib().bem_body_n_system(h)  # body + lid panels
 
     def close(self):
         if self._h:
@@ -460,13 +481,13 @@ class Body:
         _check(lib().bem_body_assemble_rankine(self.ctx._h, self._h))
 
     def get_rankine(self):
-        N = self.N
+        N = self.N_system
         out = [np.zeros((N, N), order="F") for _ in range(4)]
         _check(lib().bem_get_rankine(self.ctx._h, self._h, *[_dp(o) for o in out]))
         return dict(Ssrc=out[0], Srefl=out[1], Dsrc=out[2], Drefl=out[3])
 
     def get_system(self, kappa, mode=MODE_FP64, compose=COMPOSE_WAVE):
-        N = self.N
+        N = self.N_system
         S = np.zeros((N, N), dtype=np.complex128, order="F")
         D = np.zeros((N, N), dtype=np.complex128, order="F")
         _check(lib().bem_get_system(self.ctx._h, self._h, mode, kappa, compose, S.ctypes.data_as(_PD),

@@ -263,6 +263,7 @@ struct Lane {
 
 struct bem_body {
     bem_ctx* ctx = nullptr;
+    int NPA = 0;                        // body panels; N - NPA lid panels follow them
     unsigned long long id = 0;          // unique per body (graph keys)
     int N = 0, Np = 0;
     BodyGeometry geo;
@@ -348,7 +349,10 @@ const char* bem_last_error(void) { return g_err.c_str(); }
 
 // ---- meshes -----------------------------------------------------------------
 int bem_mesh_from_arrays(int n_nodes, const double* nodes, int n_elems, const int* elems, bem_mesh** out) {
-    if (!out || n_nodes <= 0 || n_elems <= 0 || !nodes || !elems) return fail(BEM_ERR_ARG, "bem_mesh_from_arrays: bad arguments");
+    // an empty mesh (0 nodes, 0 elements) is allowed as a start for bem_mesh_append
+    if (!out || n_nodes < 0 || n_elems < 0 || (n_nodes > 0 && !nodes) || (n_elems > 0 && !elems) ||
+        (n_elems > 0 && n_nodes == 0))
+        return fail(BEM_ERR_ARG, "bem_mesh_from_arrays: bad arguments");
     bem_mesh* m = new bem_mesh;
     m->m.nodes.assign(nodes, nodes + 3 * (size_t)n_nodes);
     m->m.elems.assign(elems, elems + 4 * (size_t)n_elems);
@@ -401,6 +405,22 @@ int bem_mesh_half_cylinder(double radius, double draft, int n_radial, int n_vert
     if (!(radius > 0) || !(draft > 0) || n_radial < 1 || n_vertical < 1 || n_azimuthal < 3)
         return fail(BEM_ERR_ARG, "bem_mesh_half_cylinder: bad dimensions");
     return wrap_mesh(mesh_half_cylinder(radius, draft, n_radial, n_vertical, n_azimuthal, cosine != 0), out);
+}
+
+int bem_mesh_lid_disc(double radius, int n_radial, int n_azimuthal, int cosine, bem_mesh** out) {
+    if (!out || !(radius > 0) || n_radial < 1 || n_azimuthal < 3) return fail(BEM_ERR_ARG, "bem_mesh_lid_disc: bad arguments");
+    return wrap_mesh(mesh_lid_disc(radius, n_radial, n_azimuthal, cosine != 0), out);
+}
+
+int bem_mesh_lid_rect(double L, double W, int nx, int ny, int cosine, bem_mesh** out) {
+    if (!out || !(L > 0) || !(W > 0) || nx < 1 || ny < 1) return fail(BEM_ERR_ARG, "bem_mesh_lid_rect: bad arguments");
+    return wrap_mesh(mesh_lid_rect(L, W, nx, ny, cosine != 0), out);
+}
+
+int bem_mesh_append(bem_mesh* dst, const bem_mesh* src, double dx, double dy, double dz) {
+    if (!dst || !src || dst == src) return fail(BEM_ERR_ARG, "bem_mesh_append: bad arguments");
+    dst->m.append(src->m, dx, dy, dz);
+    return BEM_OK;
 }
 
 int bem_mesh_box(double L, double W, double z0, double z1, int nx, int ny, int nz, int open_top, bem_mesh** out) {
@@ -494,16 +514,31 @@ void bem_destroy(bem_ctx* c) {
     delete c;
 }
 
-int bem_body_create(bem_ctx* c, const bem_mesh* m, bem_body** out) {
+int bem_body_create(bem_ctx* c, const bem_mesh* m, bem_body** out) { return bem_body_create_lid(c, m, nullptr, out); }
+
+int bem_body_create_lid(bem_ctx* c, const bem_mesh* m, const bem_mesh* lid, bem_body** out) {
     if (!c || !m || !out) return fail(BEM_ERR_ARG, "bem_body_create: null argument");
+    if (m->m.n_elems() < 1) return fail(BEM_ERR_ARG, "bem_body_create: empty mesh");
     HIP_TRY(hipSetDevice(c->device));
     bem_body* b = new bem_body;
     static std::atomic<unsigned long long> next_body_id{1};
     b->id = next_body_id++;
     b->ctx = c;
     std::string err;
-    if (!build_geometry(m->m, b->geo, err)) { delete b; return fail(BEM_ERR_ARG, err); }
+    // Create_Panels (Hydrodynamic_Solver.cpp:30-81): body panels first, then
+    // the auxiliary (lid) panels when irregular-frequency removal is on
+    Mesh all = m->m;
+    if (lid) {
+        for (int k = 0; k < lid->m.n_nodes(); k++)
+            if (std::fabs(lid->m.nodes[3 * (size_t)k + 2]) > 1e-9) {
+                delete b;
+                return fail(BEM_ERR_ARG, "bem_body_create_lid: lid nodes must lie in z = 0");
+            }
+        all.append(lid->m, 0.0, 0.0, 0.0);
+    }
+    if (!build_geometry(all, b->geo, err)) { delete b; return fail(BEM_ERR_ARG, err); }
     b->N = b->geo.N;
+    b->NPA = m->m.n_elems();
     b->Np = round_up(b->N, kPad);
     for (int i = 0; i < b->N; i++) b->nq_total += (long long)b->geo.quad[(size_t)i * QF_STRIDE + QF_NQ];
     if (b->Np / kLuNb > kMaxPanels) { delete b; return fail(BEM_ERR_ARG, "body too large"); }
@@ -535,7 +570,8 @@ void bem_body_destroy(bem_body* b) {
     delete b;
 }
 
-int bem_body_n(const bem_body* b) { return b ? b->N : -1; }
+int bem_body_n(const bem_body* b) { return b ? b->NPA : -1; }
+int bem_body_n_system(const bem_body* b) { return b ? b->N : -1; }
 
 int bem_body_assemble_rankine(bem_ctx* c, bem_body* b) {
     if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bem_body_assemble_rankine: bad arguments");
@@ -622,7 +658,8 @@ int enqueue_frequency(bem_body* b, Lane& L, bool timing) {
                     L.PhiIr, L.PhiIi, L.DAr, L.DAi);
     launch_matvec<double>(st, b->Np, nr, L.Sr, L.Si, L.Xinr, L.Xini, s.Pr, s.Pi, nullptr, nullptr, 0, s.Br, s.Bi);
     solve_system(L, L.prec, nr);
-    launch_reduce_coeffs(st, b->N, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
+    // reductions over the body panels only (Prepare_PostProcessing_Mats :145-165, potentials cut to NPA :450-458)
+    launch_reduce_coeffs(st, b->NPA, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
     if (timing) HIP_TRY(hipEventRecord(L.t[3], st));
     int np = b->Np / kLuNb;
     HIP_TRY(hipMemcpyAsync(L.hout->coeffs, L.dout->coeffs, sizeof(double) * 2 * (36 + 12 * nb), hipMemcpyDeviceToHost, st));
@@ -836,10 +873,10 @@ int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double 
         std::vector<double> xr((size_t)Np * nr), xi((size_t)Np * nr);
         HIP_TRY(hipMemcpy(xr.data(), L.sys.Xr, xr.size() * 8, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(xi.data(), L.sys.Xi, xi.size() * 8, hipMemcpyDeviceToHost));
-        if (phiJ) planar_to_interleaved(xr, xi, b->N, Np, 6, phiJ);
+        if (phiJ) planar_to_interleaved(xr, xi, b->NPA, Np, 6, phiJ);
         if (phiS && nb > 0) {
             std::vector<double> sr(xr.begin() + (size_t)6 * Np, xr.end()), si(xi.begin() + (size_t)6 * Np, xi.end());
-            planar_to_interleaved(sr, si, b->N, Np, nb, phiS);
+            planar_to_interleaved(sr, si, b->NPA, Np, nb, phiS);
         }
     }
     return BEM_OK;
@@ -1004,7 +1041,7 @@ namespace {
 // N x (6 + nb) block, and the headings.
 int upload_potentials(bem_body* b, int nb, const double* betas, const double* phiJ, const double* phiS,
                       SurBuffers& B, double** phr, double** phi, double** dbetas) {
-    const int N = b->N, nc = 6 + nb;
+    const int N = b->NPA, nc = 6 + nb;
     std::vector<double> hr((size_t)N * nc), hi((size_t)N * nc);
     for (int c = 0; c < nc; c++) {
         const double* src = c < 6 ? phiJ + 2 * (size_t)c * N : phiS + 2 * (size_t)(c - 6) * N;
@@ -1046,7 +1083,7 @@ int bem_kochin(bem_ctx* c, bem_body* b, double omega, double kappa, int nkoch, i
     if (r) return r;
     const int nc = 6 + nb;
     if ((r = B.get(&dout, 2 * (size_t)nkoch * nc))) return r;
-    launch_kochin(c->stream, b->N, nkoch, nb, b->d_body, b->d_gen, dbetas, omega, kappa, phr, phi, dout);
+    launch_kochin(c->stream, b->NPA, nkoch, nb, b->d_body, b->d_gen, dbetas, omega, kappa, phr, phi, dout);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));
     std::vector<double> h(2 * (size_t)nkoch * nc);
@@ -1069,7 +1106,7 @@ int bem_fs_elevation(bem_ctx* c, bem_body* b, double omega, double kappa, int np
     const int nc = 6 + nb;
     if ((r = B.get(&dpts, 3 * (size_t)npts)) || (r = B.get(&dout, 2 * (size_t)npts * nc))) return r;
     HIP_TRY(hipMemcpy(dpts, pts, 24 * (size_t)npts, hipMemcpyHostToDevice));
-    launch_fs_field(c->stream, b->N, npts, dpts, nb, dbetas, omega, kappa, b->d_body, b->d_refl, b->d_quad, b->d_gen,
+    launch_fs_field(c->stream, b->NPA, npts, dpts, nb, dbetas, omega, kappa, b->d_body, b->d_refl, b->d_quad, b->d_gen,
                     phr, phi, dout);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(c->stream));

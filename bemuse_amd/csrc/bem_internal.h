@@ -31,6 +31,8 @@ bool read_stl(std::istream& in, Mesh& m, std::string& err);
 
 Mesh mesh_revolution(const std::vector<double>& r, const std::vector<double>& z, int NA);
 Mesh mesh_half_cylinder(double R, double draft, int NR, int NV, int NA, bool cosine);
+Mesh mesh_lid_disc(double R, int NRFS, int NA, bool cosine);
+Mesh mesh_lid_rect(double L, double W, int NX, int NY, bool cosine);
 Mesh mesh_box(double L, double W, double z0, double z1, int NX, int NY, int NZ, bool open_top);
 Mesh mesh_semisub3(double rc, double T, double spacing, double pw, double ph, double gap,
                    int NA, int NV, int NR, int NX, int NY, int NZ);

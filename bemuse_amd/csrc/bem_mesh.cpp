@@ -67,6 +67,52 @@ Mesh mesh_revolution(const std::vector<double>& r, const std::vector<double>& z,
     return m;
 }
 
+// Interior free-surface lid of a volume of revolution
+// (Volume_of_Revolution::Generate_Aux_Nodes / Generate_Aux_Elements,
+// Volume_of_Revolution.cpp:43-65, 111-130): rings at r = R i / NRFS (or the
+// cosine spacing) in z = 0, a triangle fan round the centre and quads
+// outside it, in the reference's node order.
+Mesh mesh_lid_disc(double R, int NRFS, int NA, bool cosine) {
+    Mesh m;
+    for (int i = 0; i <= NRFS; i++) {
+        double r = cosine ? R * pi_cos_fac(i, NRFS + 1) : R * i / NRFS;
+        int NAR = (i == 0) ? 1 : NA;
+        for (int j = 0; j < NAR; j++) {
+            double u = j * (2.0 * M_PI) / NAR;
+            m.add_node(r * std::cos(u), r * std::sin(u), 0.0);
+        }
+    }
+    auto id = [NA](int A, int Z) -> int {
+        if (Z == 0) return 0;
+        if (A >= NA) return 1 + (Z - 1) * NA + (A - NA);
+        if (A < 0) return 1 + (Z - 1) * NA + (A + NA);
+        return 1 + (Z - 1) * NA + A;
+    };
+    for (int i = 0; i < NA; i++) m.add_tri(id(0, 0), id(i, 1), id(i + 1, 1));
+    for (int z = 1; z < NRFS; z++)
+        for (int i = 0; i < NA; i++) m.add_quad(id(i, z + 1), id(i + 1, z + 1), id(i + 1, z), id(i, z));
+    return m;
+}
+
+// Interior free-surface lid of a barge (Barge::Generate_Aux_Nodes /
+// Generate_Aux_Elements, Barge.cpp:103-123, 193-204), an NX x NY grid of
+// quads in z = 0. The reference computes y from the x index (y = -W/2 +
+// W i / NY, :115), which makes every lid quad degenerate; the grid here uses
+// the y index.
+Mesh mesh_lid_rect(double L, double W, int NX, int NY, bool cosine) {
+    Mesh m;
+    for (int i = 0; i <= NX; i++)
+        for (int j = 0; j <= NY; j++) {
+            double x = cosine ? -0.5 * L + L * pi_cos_fac(i, NX + 1) : -0.5 * L + L * i / NX;
+            double y = cosine ? -0.5 * W + W * pi_cos_fac(j, NY + 1) : -0.5 * W + W * j / NY;
+            m.add_node(x, y, 0.0);
+        }
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            m.add_quad(i * (NY + 1) + j, (i + 1) * (NY + 1) + j, (i + 1) * (NY + 1) + j + 1, i * (NY + 1) + j + 1);
+    return m;
+}
+
 // Half_Cylinder::Generate_Nodes profile (Volume_of_Revolution.cpp:185-199).
 Mesh mesh_half_cylinder(double R, double draft, int NR, int NV, int NA, bool cosine) {
     double Z = -draft;

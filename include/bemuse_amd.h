@@ -75,9 +75,20 @@ const char* bem_last_error(void);
  * (src/Boundary/Boundary_Base.h:168-172): nodes n x 3, elements e x 4 with
  * elems[4e+3] = -1 for a triangle. */
 int bem_mesh_from_arrays(int n_nodes, const double* nodes, int n_elems, const int* elems, bem_mesh** out);
+/* (n_nodes = n_elems = 0 makes an empty mesh to build on with bem_mesh_append.) */
 /* Half_Cylinder (src/Boundary/Volume_of_Revolution.cpp:167-199). */
 int bem_mesh_half_cylinder(double radius, double draft, int n_radial, int n_vertical,
                            int n_azimuthal, int cosine, bem_mesh** out);
+/* Interior free-surface lids for irregular-frequency removal (the auxiliary
+ * panels of Boundary::Generate_Aux_Nodes/_Elements): a disc of radius R in
+ * z = 0 (Volume_of_Revolution.cpp:43-65, 111-130; n_radial rings, a triangle
+ * fan at the centre) and an L x W rectangle (Barge.cpp:103-123, 193-204; the
+ * reference computes the lid's y from the x index, :115, which makes its
+ * quads degenerate; here y follows its own index). */
+int bem_mesh_lid_disc(double radius, int n_radial, int n_azimuthal, int cosine, bem_mesh** out);
+int bem_mesh_lid_rect(double L, double W, int nx, int ny, int cosine, bem_mesh** out);
+/* Appends src, translated by (dx, dy, dz), to dst (e.g. one lid per column). */
+int bem_mesh_append(bem_mesh* dst, const bem_mesh* src, double dx, double dy, double dz);
 /* Rectangular box x in [-L/2,L/2], y in [-W/2,W/2], z in [z0,z1]
  * (cf. src/Boundary/Barge.cpp:34-191); open_top = 1 for a surface-piercing barge. */
 int bem_mesh_box(double L, double W, double z0, double z1, int nx, int ny, int nz, int open_top, bem_mesh** out);
@@ -130,8 +141,18 @@ int bem_set_graphs(bem_ctx* ctx, int enable);
 /* Uploads the geometry of a mesh (Hydrodynamic_Radiation_Solver::Setup,
  * Hydrodynamic_Solver.cpp:269-279, geometry part). */
 int bem_body_create(bem_ctx* ctx, const bem_mesh* m, bem_body** out);
+/* As bem_body_create with irregular-frequency removal (IFR, Create_Panels
+ * Hydrodynamic_Solver.cpp:45-55): the lid panels (z = 0) join the system
+ * after the body panels, so assembly, factorisation and solves run over all
+ * of them, while potentials, A, B, F and the post-processing use the body
+ * panels only (:145-165, 450-458). lid may be NULL. */
+int bem_body_create_lid(bem_ctx* ctx, const bem_mesh* m, const bem_mesh* lid, bem_body** out);
 void bem_body_destroy(bem_body* b);
+/* Body panels (the rows of the returned potentials). */
 int bem_body_n(const bem_body* b);
+/* Order of the linear system: body plus lid panels (the size of the matrices
+ * bem_get_rankine and bem_get_system return). */
+int bem_body_n_system(const bem_body* b);
 
 /* Rankine source/dipole influence of the body and its image
  * (Prepare_Linear_System, Hydrodynamic_Solver.cpp:83-115), once per body. */

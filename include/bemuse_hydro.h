@@ -41,7 +41,13 @@
  *     FreeSurface_Nodes (filled by the caller; the reference takes them from
  *     its numerical-wave-tank boundaries), kept in FS_Rad_SolArray /
  *     FS_Scat_SolArray (:514-515).
- * Not provided: IFR lid panels, RAOs and the .vtp wave-height export. The file
+ *   - irregular-frequency removal ("IFR"): the boundary's interior lid
+ *     (Half_Cylinder and Triple_Column_SemiSub with
+ *     NPanels_FreeSurface_Int_Radial > 0, Barge always) joins the system as
+ *     in Create_Panels (:45-55); Barge's lid y-index fix is noted at
+ *     bem_mesh_lid_rect. Half_Cylinder also makes the exterior free-surface
+ *     rings of FreeSurface_Radius / NPanels_FreeSurface_Radial.
+ * Not provided: RAOs and the .vtp wave-height export. The file
  * .4 (RAOs, which in the reference use an uninitialised damping matrix) is
  * created empty. The waterplane moments of the hydrostatic stiffness are
  * integrated exactly over the waterline polygon (the reference sums them over
@@ -146,7 +152,10 @@ template <> inline bool Parameter::Get_Param<bool>() const { return b_; }
 // ---------------------------------------------------------------------------
 class Boundary {
 public:
-    virtual ~Boundary() { if (mesh_) bem_mesh_free(mesh_); }
+    virtual ~Boundary() {
+        if (mesh_) bem_mesh_free(mesh_);
+        if (lid_) bem_mesh_free(lid_);
+    }
     virtual void Set_Parameters(std::vector<Parameter>& Params) { (void)Params; }
     virtual void Setup() {}
     int N_Nodes() const { int n = 0, e = 0; if (mesh_) bem_mesh_size(mesh_, &n, &e); return n; }
@@ -158,6 +167,8 @@ public:
         if (mesh_) bem_mesh_export(mesh_, nodes.data(), elems.data());
     }
     const bem_mesh* Get_Device_Mesh() const { return mesh_; }
+    // auxiliary (interior free-surface lid) panels for IFR, if the boundary has them
+    const bem_mesh* Get_Device_Lid() const { return lid_; }
 
     //--- geometric and kinematic properties (Boundary_Base.h:132-156)
     Real Volume = 0, Surface_Area = 0, Max_Dim = 0, Mass = 0;
@@ -168,12 +179,19 @@ public:
     std::string FileInfo;
     // waterplane area and moments int x, y, x^2, x y, y^2 dA (exact over the waterline polygon)
     Real WP_Area = 0, WP_Mx = 0, WP_My = 0, WP_Mxx = 0, WP_Mxy = 0, WP_Myy = 0;
-    int N_Aux_Elements() const { return 0; }
+    int N_Aux_Elements() const { return N_Aux_Panels(); }
     // exterior free-surface points for the wave elevation (Get_FreeSurface_Nodes)
     std::vector<Vector3> FreeSurface_Nodes;
     void Get_FreeSurface_Nodes(std::vector<Vector3>& N) const { N = FreeSurface_Nodes; }
+    int N_Aux_Panels() const { int n = 0, e = 0; if (lid_) bem_mesh_size(lid_, &n, &e); return e; }
 
 protected:
+    void Adopt_Lid(int rc, bem_mesh* m) {
+        if (lid_) { bem_mesh_free(lid_); lid_ = nullptr; }
+        if (rc != BEM_OK) { std::cout << "BEMUse: lid generation failed: " << bem_last_error() << std::endl; return; }
+        lid_ = m;
+    }
+    bem_mesh* lid_ = nullptr;
     void Adopt(int rc, bem_mesh* m) {
         if (mesh_) { bem_mesh_free(mesh_); mesh_ = nullptr; }
         if (rc != BEM_OK) { std::cout << "BEMUse: mesh generation failed: " << bem_last_error() << std::endl; return; }
@@ -228,8 +246,13 @@ protected:
         }
         Real sg = a < 0 ? -1 : 1;
         WP_Area = sg * a; WP_Mx = sg * m1x; WP_My = sg * m1y; WP_Mxx = sg * mxx; WP_Mxy = sg * mxy; WP_Myy = sg * myy;
-        std::cout << "Geometry generated. " << ne << " elements, 0 auxiliary (free surface) elements. "
-                  << "0 external (free surface) elements. \n";
+    }
+    // the summary line of Boundary_Base::Setup, printed once the lid and the
+    // exterior free-surface points are known
+    void Report() const {
+        std::cout << "Geometry generated. " << N_Elements() << " elements, " << N_Aux_Panels()
+                  << " auxiliary (free surface) elements. " << FreeSurface_Nodes.size()
+                  << " external (free surface) elements. \n";
         std::cout << "Surface area = " << Surface_Area << " m^2. Volume = " << Volume << " m^3" << std::endl;
     }
     bem_mesh* mesh_ = nullptr;
@@ -268,6 +291,7 @@ public:
         bem_mesh* m = pending_;
         pending_ = nullptr;
         Adopt(BEM_OK, m);
+        Report();
     }
     ~File_Geometry() override { if (pending_) bem_mesh_free(pending_); }
 private:
@@ -291,16 +315,37 @@ public:
             if (P.myNameis("NPanels_Vertical")) NV = P.Get_Param<int>();
             if (P.myNameis("NPanels_Azimuthal")) NA = P.Get_Param<int>();
             if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
+            if (P.myNameis("FreeSurface_Radius")) RFS = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_FreeSurface_Radial")) NRES = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FreeSurface_Int_Radial")) NRFS = P.Get_Param<int>();
         }
     }
     void Setup() override {
         bem_mesh* m = nullptr;
         int rc = bem_mesh_half_cylinder(R, D, NR, NV, NA, Cosine ? 1 : 0, &m);
         Adopt(rc, m);
+        // interior lid (Volume_of_Revolution.cpp:43-65, 111-130)
+        if (NRFS > 0) {
+            bem_mesh* l = nullptr;
+            int rl = bem_mesh_lid_disc(R, NRFS, NA, Cosine ? 1 : 0, &l);
+            Adopt_Lid(rl, l);
+        }
+        // exterior free-surface rings r = 1.001 R + RFS R i / NRES
+        // (Volume_of_Revolution.cpp:69-89; with NRES = 0 the reference divides
+        // by zero there, so no rings are made then)
+        FreeSurface_Nodes.clear();
+        for (int i = 0; NRES > 0 && i <= NRES; i++) {
+            Real r = R * 1.001 + RFS * R * i / NRES;
+            for (int j = 0; j < NA; j++) {
+                Real u = j * 2.0 * 3.14159265358979323846 / NA;
+                FreeSurface_Nodes.push_back(Vector3(r * std::cos(u), r * std::sin(u), 0.0));
+            }
+        }
+        Report();
     }
 private:
-    Real R = 10, D = 20;
-    int NR = 8, NV = 24, NA = 32;
+    Real R = 10, D = 20, RFS = 0;
+    int NR = 8, NV = 24, NA = 32, NRES = 0, NRFS = 0;
     bool Cosine = true;
 };
 
@@ -321,6 +366,11 @@ public:
         bem_mesh* m = nullptr;
         int rc = bem_mesh_box(L, W, -D, 0.0, NX, NY, NZ, 1, &m);
         Adopt(rc, m);
+        // interior lid (Barge.cpp:103-123, 193-204; y from its own index)
+        bem_mesh* l = nullptr;
+        int rl = bem_mesh_lid_rect(L, W, NX, NY, 0, &l);
+        Adopt_Lid(rl, l);
+        Report();
     }
 private:
     Real L = 20, W = 10, D = 5;
@@ -337,15 +387,31 @@ public:
             if (P.myNameis("Column_Spacing")) S = P.Get_Param<Real>();
             if (P.myNameis("Pontoon_Width")) PW = P.Get_Param<Real>();
             if (P.myNameis("Pontoon_Height")) PH = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_FreeSurface_Int_Radial")) NRFS = P.Get_Param<int>();
         }
     }
     void Setup() override {
         bem_mesh* m = nullptr;
         int rc = bem_mesh_semisub3(Rc, D, S, PW, PH, 0.5, 44, 24, 8, 31, 5, 4, &m);
         Adopt(rc, m);
+        if (NRFS > 0) {   // one disc lid per column, at the columns' centres
+            bem_mesh *disc = nullptr, *all = nullptr;
+            int r1 = bem_mesh_lid_disc(Rc, NRFS, 44, 1, &disc);
+            if (r1 == BEM_OK) r1 = bem_mesh_from_arrays(0, nullptr, 0, nullptr, &all);
+            const Real Rv = S / std::sqrt(3.0);
+            for (int k = 0; r1 == BEM_OK && k < 3; k++) {
+                Real a = 3.14159265358979323846 / 2 + k * 2.0 * 3.14159265358979323846 / 3.0;
+                r1 = bem_mesh_append(all, disc, Rv * std::cos(a), Rv * std::sin(a), 0.0);
+            }
+            if (disc) bem_mesh_free(disc);
+            if (r1 != BEM_OK && all) { bem_mesh_free(all); all = nullptr; }
+            Adopt_Lid(r1, all);
+        }
+        Report();
     }
 private:
     Real Rc = 6, D = 20, S = 50, PW = 6, PH = 4;
+    int NRFS = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -392,7 +458,6 @@ public:
             if (P.myNameis("Gravity")) Grav = P.Get_Param<Real>();
         }
         NBeta = (int)BetaArray.size();
-        if (IFR) std::cout << "BEMUse: irregular-frequency removal (IFR) is not provided; solving without lid panels" << std::endl;
     }
     void Set_Real(Real D) override { Frequency = D; }
 
@@ -400,7 +465,10 @@ public:
         if (!B || !B->Get_Device_Mesh()) { std::cout << "BEMUse: Setup without a generated boundary" << std::endl; return; }
         if (!ctx_ && !check(bem_create(device_, &ctx_))) return;
         if (body_) { bem_body_destroy(body_); body_ = nullptr; }
-        if (!check(bem_body_create(ctx_, B->Get_Device_Mesh(), &body_))) return;
+        // Create_Panels (Hydrodynamic_Solver.cpp:30-81): with IFR the lid panels join the system
+        const bem_mesh* lid = IFR ? B->Get_Device_Lid() : nullptr;
+        if (IFR && !lid) std::cout << "BEMUse: IFR requested but the boundary has no lid panels; solving without" << std::endl;
+        if (!check(bem_body_create_lid(ctx_, B->Get_Device_Mesh(), lid, &body_))) return;
         NPTOT = bem_body_n(body_);
         B->Get_FreeSurface_Nodes(Wave_Nodes);
         if (!check(bem_body_assemble_rankine(ctx_, body_))) return;

@@ -37,6 +37,7 @@ def lib():
         L.orc_body_create.restype = c_void_p
         L.orc_body_create.argtypes = [c_int, _PD, c_int, _PI]
         L.orc_body_destroy.argtypes = [c_void_p]
+        L.orc_body_set_npa.argtypes = [c_void_p, c_int]
         L.orc_body_n.argtypes = [c_void_p]
         L.orc_body_geometry.argtypes = [c_void_p, _PD, _PD, _PD, _PD, _PI, _PD, _PD, _PD, _PD]
         L.orc_body_qnodes.argtypes = [c_void_p, _PI, _PD, _PD, _PD]
@@ -122,12 +123,17 @@ def lu_solve(LU, piv, B):
 class Body:
     """Oracle body built from a node/element mesh (same arrays as the GPU path)."""
 
-    def __init__(self, nodes, elems):
+    def __init__(self, nodes, elems, n_lid=0):
+        """n_lid: the last n_lid elements are interior free-surface lid panels
+        (irregular-frequency removal); potentials and A, B, F use the others."""
         nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
         elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 4)
         self._keep = (nodes, elems)
         self._h = c_void_p(lib().orc_body_create(len(nodes), _dp(nodes), len(elems), elems.ctypes.data_as(_PI)))
-        self.N = len(elems)
+        self.N_system = len(elems)
+        self.N = len(elems) - n_lid
+        if n_lid:
+            lib().orc_body_set_npa(self._h, self.N)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -138,7 +144,7 @@ class Body:
         """Body panel centroid, normal, area, BC point and frame; the q*
         fields (nq, qpos, qnrm, qw) are the MIRRORED panel's quadrature nodes
         (the ones the wave term integrates over)."""
-        N = self.N
+        N = self.N_system
         cen = np.zeros((N, 3)); nrm = np.zeros((N, 3)); area = np.zeros(N); bc = np.zeros((N, 3))
         nq = np.zeros(N, dtype=np.int32); qpos = np.zeros((N, 4, 3)); qnrm = np.zeros((N, 4, 3)); qw = np.zeros((N, 4))
         frame = np.zeros((N, 3, 3))
@@ -148,7 +154,7 @@ class Body:
 
     def body_qnodes(self):
         """Quadrature nodes of the body panels."""
-        N = self.N
+        N = self.N_system
         nq = np.zeros(N, dtype=np.int32); qpos = np.zeros((N, 4, 3)); qnrm = np.zeros((N, 4, 3)); qw = np.zeros((N, 4))
         lib().orc_body_qnodes(self._h, nq.ctypes.data_as(_PI), _dp(qpos), _dp(qnrm), _dp(qw))
         return dict(nq=nq, qpos=qpos, qnrm=qnrm, qw=qw)
@@ -166,14 +172,14 @@ class Body:
         return S[0::2] + 1j * S[1::2], D[0::2] + 1j * D[1::2]
 
     def rankine(self):
-        N = self.N
+        N = self.N_system
         out = [np.zeros((N, N), order="F") for _ in range(4)]
         rn = np.zeros(N)
         lib().orc_rankine(self._h, *[_dp(o) for o in out], _dp(rn))
         return dict(Ssrc=out[0], Srefl=out[1], Dsrc=out[2], Drefl=out[3], reflnorm=rn)
 
     def wave(self, K):
-        N = self.N
+        N = self.N_system
         S = np.zeros((N, N), dtype=np.complex128, order="F")
         D = np.zeros((N, N), dtype=np.complex128, order="F")
         lib().orc_wave(self._h, K, S.ctypes.data_as(_PD), D.ctypes.data_as(_PD))

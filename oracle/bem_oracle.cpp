@@ -366,6 +366,7 @@ inline void wave_sd(const Panel& p, V3 P, double K, cd& S, cd& D) {
 
 struct Body {
     int N;
+    int NPA = 0;                    // body panels; N - NPA lid panels (IFR) follow them
     std::vector<Panel> body, refl;
     std::vector<V3> bc;             // BC positions C + BCF*n
     std::vector<double> Ssrc, Srefl, Dsrc, Drefl;   // N x N col-major (R,S)
@@ -488,6 +489,7 @@ void orc_poly_coeffs(double* out /*65: 64 coefficients, then EUL*/) {
 void* orc_body_create(int nn, const double* nodes, int ne, const int* elems) {
     Body* b = new Body;
     b->N = ne;
+    b->NPA = ne;
     b->body.resize(ne); b->refl.resize(ne); b->bc.resize(ne);
     for (int e = 0; e < ne; e++) {
         const int* el = elems + 4 * e;
@@ -510,6 +512,13 @@ void* orc_body_create(int nn, const double* nodes, int ne, const int* elems) {
 }
 
 void orc_body_destroy(void* h) { delete static_cast<Body*>(h); }
+// The last N - npa elements are the auxiliary lid panels of irregular-frequency
+// removal (Create_Panels, Hydrodynamic_Solver.cpp:45-55): they join the
+// system; post-processing uses the npa body panels (:145-165, 450-458).
+void orc_body_set_npa(void* h, int npa) {
+    Body* b = static_cast<Body*>(h);
+    if (npa > 0 && npa <= b->N) b->NPA = npa;
+}
 int orc_body_n(void* h) { return static_cast<Body*>(h)->N; }
 
 // centroid(3) normal(3) area bc(3) for body panels; qnodes of the MIRRORED panels.
@@ -716,16 +725,17 @@ int orc_solve(void* h, double omega, double kappa, int compose, int rad_only,
     lu_solve(N, DMat.data(), N, piv.data(), RHS.data(), N, nr);
     const cd* PJ = RHS.data();
     const cd* PS = RHS.data() + (size_t)6 * N;
-    // A = Re(M PhiJ), B = Im(M PhiJ) (:463-465)
+    // A = Re(M PhiJ), B = Im(M PhiJ) (:463-465), over the body panels
+    const int NPA = b->NPA;
     for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) {
         cd s(0, 0);
-        for (int k = 0; k < N; k++) s += M[(size_t)i * N + k] * PJ[(size_t)j * N + k];
+        for (int k = 0; k < NPA; k++) s += M[(size_t)i * N + k] * PJ[(size_t)j * N + k];
         A36[6 * i + j] = s.real(); B36[6 * i + j] = s.imag();
     }
     if (!rad_only) {
         for (int i = 0; i < 6; i++) for (int bb = 0; bb < nb; bb++) {
             cd f1(0, 0), f2(0, 0);
-            for (int k = 0; k < N; k++) {
+            for (int k = 0; k < NPA; k++) {
                 f1 += M[(size_t)i * N + k] * PhiI[(size_t)bb * N + k];
                 f2 += PJ[(size_t)i * N + k] * (DPhiI[(size_t)bb * N + k] * b->body[k].area);
             }
@@ -738,8 +748,9 @@ int orc_solve(void* h, double omega, double kappa, int compose, int rad_only,
             if (SC) { SC[o] = sc.real(); SC[o + 1] = sc.imag(); }
         }
     }
-    if (PhiJ) std::memcpy(PhiJ, PJ, (size_t)N * 6 * 16);
-    if (PhiS) std::memcpy(PhiS, PS, (size_t)N * nb * 16);
+    // potentials cut to the body panels (:450-458), NPA x 6 and NPA x nb
+    for (int c = 0; c < 6 && PhiJ; c++) std::memcpy(PhiJ + 2 * (size_t)c * NPA, PJ + (size_t)c * N, (size_t)NPA * 16);
+    for (int c = 0; c < nb && PhiS; c++) std::memcpy(PhiS + 2 * (size_t)c * NPA, PS + (size_t)c * N, (size_t)NPA * 16);
     double t3 = omp_get_wtime();
     if (timings) { timings[0] = t1 - t0; timings[1] = t2 - t1; timings[2] = t3 - t2; }
     return 0;

@@ -35,6 +35,12 @@ import numpy as np
 GRAVITY = 9.81
 
 
+def body_panels(geom, n):
+    """The first n panels' entries of a geometry dict (the body panels; lid
+    panels of irregular-frequency removal follow them)."""
+    return {k: v[:n] for k, v in geom.items()}
+
+
 def generalised_normals(geom):
     c, n = geom["centroid"], geom["normal"]
     return np.concatenate([n, np.cross(c, n)], axis=1)
@@ -49,7 +55,8 @@ def incident_dn(geom, beta, omega, kappa):
 
 
 def kochin(geom, omega, kappa, nkoch, betas, phiJ, phiS):
-    """(KochRad nkoch x 6, KochDiff nkoch x nb)."""
+    """(KochRad nkoch x 6, KochDiff nkoch x nb), over the body panels (the rows of phiJ)."""
+    geom = body_panels(geom, len(phiJ))
     c, n, A = geom["centroid"], geom["normal"], geom["area"]
     th = (0.5 + np.arange(nkoch)) * 2.0 * np.pi / nkoch
     ct, st = np.cos(th)[:, None], np.sin(th)[:, None]
@@ -67,7 +74,7 @@ def kochin(geom, omega, kappa, nkoch, betas, phiJ, phiS):
 def exterior_influence(ob, kappa, pts):
     """(WS, WD) complex npts x N of the oracle body ob at the points."""
     pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
-    geom = ob.geometry()
+    geom = body_panels(ob.geometry(), ob.N)
     reflnorm = 2.0 * geom["normal"][:, 2]
     WS = np.zeros((len(pts), ob.N), dtype=np.complex128)
     WD = np.zeros_like(WS)
@@ -82,7 +89,7 @@ def exterior_influence(ob, kappa, pts):
 
 def fs_elevation(ob, omega, kappa, pts, betas, phiJ, phiS):
     """(ExtRad npts x 6, ExtDiff npts x nb)."""
-    geom = ob.geometry()
+    geom = body_panels(ob.geometry(), ob.N)
     WS, WD = exterior_influence(ob, kappa, pts)
     f = -1j * omega / GRAVITY
     rad = f * (WS @ generalised_normals(geom) - WD @ phiJ)

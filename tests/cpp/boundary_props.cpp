@@ -28,6 +28,12 @@ int main() {
     cyl.Set_Parameters(cp);
     cyl.Setup();
     dump("cyl", cyl);
+    std::printf("box_aux %d\ncyl_aux %d\n", box.N_Aux_Panels(), cyl.N_Aux_Panels());
+    Half_Cylinder lidded;
+    cp.push_back(Parameter("NPanels_FreeSurface_Int_Radial", 3));
+    lidded.Set_Parameters(cp);
+    lidded.Setup();
+    std::printf("lidded_aux %d\n", lidded.N_Aux_Panels());
     std::printf("field_pos [%s]\nfield_neg [%s]\nfield_small [%s]\nfield_txt [%s]\n", Field(1.5).c_str(),
                 Field(-2.25).c_str(), Field(1e-7).c_str(), Field(std::string("Surge: "), 30).c_str());
     return 0;

@@ -5,7 +5,10 @@
 // result for the test to compare with the oracle:
 //   SETUP A <36>        A after Setup (Frequency = FreqInf)
 //   A <omega> <36>      B <omega> <36>      F <omega> <12 nb>
-// Usage: hydro_dropin <mode: 0 FP64 | 1 mixed>
+// Usage: hydro_dropin <mode: 0 FP64 | 1 mixed> [ifr: 0 | 1]
+// With ifr = 1 the cylinder gets a 3-ring interior lid
+// (NPanels_FreeSurface_Int_Radial) and the solver the "IFR" key, and the
+// frequencies are 1.0 and 2.2 (the irregular frequency of this cylinder).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -23,21 +26,25 @@ static void print_real(const char* tag, double w, const Matrix& M, bool with_w) 
 
 int main(int argc, char** argv) {
     int mode = argc > 1 ? std::atoi(argv[1]) : BEM_MODE_FP64;
+    bool ifr = argc > 2 && std::atoi(argv[2]) != 0;
     Half_Cylinder* B = new Half_Cylinder;
     std::vector<Parameter> bp = {Parameter("Radius", 5.0), Parameter("Draft", 8.0), Parameter("NPanels_Radial", 4),
                                  Parameter("NPanels_Vertical", 8), Parameter("NPanels_Azimuthal", 16),
                                  Parameter("Cosine_Disc", true)};
+    if (ifr) bp.push_back(Parameter("NPanels_FreeSurface_Int_Radial", 3));
     B->Set_Parameters(bp);
     B->Setup();
     Hydrodynamic_Radiation_Solver* S = new Hydrodynamic_Radiation_Solver(0, mode);
     std::vector<Parameter> sp = {Parameter("WaveAngle", 0.0), Parameter("WaveAngle", 0.7853981633974483)};
+    if (ifr) sp.push_back(Parameter("IFR", true));
     S->Set_Parameters(sp);
     std::string out = "dropin_test";
     S->Set_OutputFilePath(out);
     S->Setup(B);
     std::printf("N %d\n", S->N_Panels());
     print_real("SETUP_A", 0, S->Added_Mass(), false);
-    for (double w : {0.4, 1.3}) {
+    std::vector<double> ws = ifr ? std::vector<double>{1.0, 2.2} : std::vector<double>{0.4, 1.3};
+    for (double w : ws) {
         S->Set_Real(w * 2.0 * 3.14159265358979323846);   // the console passes F = 2 pi omega
         S->Solve();
         print_real("A", w, S->Added_Mass(), true);

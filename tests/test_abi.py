@@ -154,6 +154,10 @@ def test_dropin_boundary_properties_closed_forms():
     assert abs(f["cyl_volume"] - area * T) < 1e-9
     assert abs(f["cyl_cob_z"] + T / 2) < 1e-12
     assert abs(f["cyl_wp_mxx"] - ixx) < 1e-9 and abs(f["cyl_wp_myy"] - ixx) < 1e-9
+    # interior lids for irregular-frequency removal: the barge always has one
+    # (NX x NY quads), the cylinder only with NPanels_FreeSurface_Int_Radial
+    # (3 rings of 16: a fan of triangles and two rings of quads)
+    assert int(v["box_aux"]) == 10 * 5 and int(v["cyl_aux"]) == 0 and int(v["lidded_aux"]) == 48
     assert v["field_pos"] == "[ 1.5           ]"
     assert v["field_neg"] == "[-2.25          ]"
     assert v["field_small"] == "[ 1e-07         ]"

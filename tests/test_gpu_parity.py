@@ -204,12 +204,13 @@ def test_solve_vs_committed_fixture(ctx):
     body.close()
 
 
-def _run_dropin(mode, cwd=None):
+def _run_dropin(mode, cwd=None, ifr=False):
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "hydro_dropin")
     assert os.path.exists(exe), "tests/cpp/hydro_dropin is not built (make -C tests/cpp)"
-    out = subprocess.run([exe, str(mode)], capture_output=True, text=True, timeout=300, check=True, cwd=cwd).stdout
+    out = subprocess.run([exe, str(mode), str(int(ifr))], capture_output=True, text=True, timeout=300, check=True,
+                         cwd=cwd).stdout
     res = {"stdout": out}
     for line in out.splitlines():
         tok = line.split()
@@ -290,6 +291,25 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
         blkd = rowsd[72 * k: 72 * (k + 1)]
         gotd = np.array([float(x[3]) for x in blkd]).reshape(2, 36).T
         assert np.abs(gotd - np.abs(kd)).max() < 1e-5 * np.abs(kd).max() + tol * np.abs(kd).max()
+
+
+@pytest.mark.parametrize("mode,tol", [(bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)])
+def test_cpp_dropin_ifr(mode, tol, tmp_path):
+    """The drop-in with the "IFR" key and a cylinder lid
+    (NPanels_FreeSurface_Int_Radial = 3) against the oracle with the same lid,
+    at 1.0 and at the irregular frequency 2.2 rad/s."""
+    res = _run_dropin(mode, cwd=str(tmp_path), ifr=True)
+    assert res["N"] == 192, res["stdout"]
+    body = bm.Mesh.half_cylinder(5.0, 8.0, 4, 8, 16)
+    lid = bm.Mesh.lid_disc(5.0, 3, 16)
+    n1, e1 = body.arrays()
+    n2, e2 = lid.arrays()
+    ob = oracle.Body(np.vstack([n1, n2]), np.vstack([e1, np.where(e2 >= 0, e2 + len(n1), -1)]), n_lid=len(e2))
+    for w in (1.0, 2.2):
+        r = ob.solve(w, betas=[0.0, np.pi / 4])
+        for key in ("A", "B", "F"):
+            assert relerr(res[w][key], r[key]) < tol, (w, key, relerr(res[w][key], r[key]))
+        assert r["B"][2, 2] > 0.0
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
