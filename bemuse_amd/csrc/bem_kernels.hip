@@ -730,10 +730,23 @@ __global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int
 // the global loads of chunk c+1 are in flight while chunk c feeds the MFMAs)
 // with 16-byte coalesced loads; both are stored k-contiguous so every MFMA
 // operand quad (4 consecutive k) is one LDS read. 32 KB (FP32) / 32 KB (FP64)
-// of LDS per stage lets two work-groups share a CU. The accumulators start
-// from C, loaded at the top of the kernel so that its latency overlaps the
-// main loop, and A is staged negated, so the loop accumulates C - A B and the
-// epilogue only stores.
+// of LDS per stage lets two work-groups share a CU.
+//
+// FP32 (the factorisation that FP64 refinement corrects) takes three real
+// products per k instead of four (Gauss; Higham, "Stability of a method for
+// multiplying complex matrices with three real matrix multiplications",
+// SIAM J. Matrix Anal. Appl. 13, 1992): with T1 = Ar Br, T2 = Ai Bi,
+// T3 = (Ar + Ai)(Br + Bi),  Re(A B) = T1 - T2,  Im(A B) = T3 - T1 - T2.
+// It is normwise as accurate as the four-product form but not componentwise:
+// the error of Im(A B) scales with |Ar| |Br|, and at low frequency the
+// system's imaginary part (the wave term, O(K)) is orders of magnitude
+// below its real part, which is where the damping comes from. FP64, the
+// unrefined factorisation, therefore keeps the four-product form.
+// A is staged negated and the accumulators start from C, loaded at the top
+// of the kernel so that its latency overlaps the main loop:
+//   three products: P = -T1,  Q = Cr + T2,  R = (Ci - Cr) - T3, and
+//                   Cr - Re(A B) = Q + P,  Ci - Im(A B) = R - P + Q;
+//   four products:  Q = Cr - Re(A B),  R = Ci - Im(A B).
 // ---------------------------------------------------------------------------
 constexpr int GK = 16;           // k per LDS chunk
 constexpr int GKP = GK + 4;      // padded row length in LDS (bank spread)
@@ -826,23 +839,26 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
     int i = lane & 15, g = lane >> 4;
     int row0 = wm * 16 * WM, col0 = wn * 16 * WN;           // within the tile
     bool active = (row0 < mrows) && (col0 < ncols);
-    acc_t cr[WM][WN], ci[WM][WN];
+    constexpr bool kGauss = sizeof(T) == 4;
+    acc_t p1[WM][WN], q2[WM][WN], r3[WM][WN];
     int nchunk = K / GK;
     gload(0);
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
         for (int b = 0; b < WN; b++) {
+            p1[a][b] = acc_t{0, 0, 0, 0};
             if (active) {
                 size_t off = (size_t)(cc + colt + col0 + 16 * b + i) * ldl + rc + rowt + row0 + 16 * a;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    cr[a][b][r] = Pr[off + MT::row(g, r)];
-                    ci[a][b][r] = Pi[off + MT::row(g, r)];
+                    T c_r = Pr[off + MT::row(g, r)], c_i = Pi[off + MT::row(g, r)];
+                    q2[a][b][r] = c_r;
+                    r3[a][b][r] = kGauss ? c_i - c_r : c_i;
                 }
             } else {
-                cr[a][b] = acc_t{0, 0, 0, 0};
-                ci[a][b] = acc_t{0, 0, 0, 0};
+                q2[a][b] = acc_t{0, 0, 0, 0};
+                r3[a][b] = acc_t{0, 0, 0, 0};
             }
         }
     sstore(0);
@@ -866,27 +882,47 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
                     br[b][t] = Bs[buf][0][col0 + 16 * b + i][4 * g + t];
                     bi[b][t] = Bs[buf][1][col0 + 16 * b + i][4 * g + t];
                 }
-            // four passes over the blocks per k step, so consecutive MFMAs
-            // write different accumulators (each element still accumulates
-            // re*re, -im*im, re*im, im*re in the same order)
+            // three (four) passes over the blocks per k step, so consecutive
+            // MFMAs write different accumulators (each element accumulates
+            // its products in the same order for every tile shape)
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-#pragma unroll
-                for (int a = 0; a < WM; a++)
-#pragma unroll
-                    for (int b = 0; b < WN; b++) cr[a][b] = MT::mma(ar[a][t], br[b][t], cr[a][b]);
+            for (int t = 0; t < 4 && !kGauss; t++) {
 #pragma unroll
                 for (int a = 0; a < WM; a++)
 #pragma unroll
-                    for (int b = 0; b < WN; b++) ci[a][b] = MT::mma(ar[a][t], bi[b][t], ci[a][b]);
+                    for (int b = 0; b < WN; b++) q2[a][b] = MT::mma(ar[a][t], br[b][t], q2[a][b]);
 #pragma unroll
                 for (int a = 0; a < WM; a++)
 #pragma unroll
-                    for (int b = 0; b < WN; b++) cr[a][b] = MT::mma(-ai[a][t], bi[b][t], cr[a][b]);
+                    for (int b = 0; b < WN; b++) r3[a][b] = MT::mma(ar[a][t], bi[b][t], r3[a][b]);
 #pragma unroll
                 for (int a = 0; a < WM; a++)
 #pragma unroll
-                    for (int b = 0; b < WN; b++) ci[a][b] = MT::mma(ai[a][t], br[b][t], ci[a][b]);
+                    for (int b = 0; b < WN; b++) q2[a][b] = MT::mma(-ai[a][t], bi[b][t], q2[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) r3[a][b] = MT::mma(ai[a][t], br[b][t], r3[a][b]);
+            }
+#pragma unroll
+            for (int t = 0; t < 4 && kGauss; t++) {
+                T sa[WM], sb[WN];
+#pragma unroll
+                for (int a = 0; a < WM; a++) sa[a] = ar[a][t] + ai[a][t];   // -(Ar + Ai)
+#pragma unroll
+                for (int b = 0; b < WN; b++) sb[b] = br[b][t] + bi[b][t];
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) p1[a][b] = MT::mma(ar[a][t], br[b][t], p1[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) q2[a][b] = MT::mma(-ai[a][t], bi[b][t], q2[a][b]);
+#pragma unroll
+                for (int a = 0; a < WM; a++)
+#pragma unroll
+                    for (int b = 0; b < WN; b++) r3[a][b] = MT::mma(sa[a], sb[b], r3[a][b]);
             }
         }
         if (c + 1 < nchunk) {
@@ -903,8 +939,13 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 int rr = MT::row(g, r);
-                Pr[off + rr] = cr[a][b][r];
-                Pi[off + rr] = ci[a][b][r];
+                if (kGauss) {
+                    Pr[off + rr] = q2[a][b][r] + p1[a][b][r];
+                    Pi[off + rr] = (r3[a][b][r] - p1[a][b][r]) + q2[a][b][r];
+                } else {
+                    Pr[off + rr] = q2[a][b][r];
+                    Pi[off + rr] = r3[a][b][r];
+                }
             }
         }
 }
@@ -1409,10 +1450,11 @@ template void launch_matvec<double>(hipStream_t, int, int, const double*, const 
 // ---------------------------------------------------------------------------
 namespace {
 struct TileCfg { int wm, wn; };
-constexpr TileCfg kCfgF32[] = {{4, 4}, {4, 2}, {2, 4}, {2, 2}};
-constexpr TileCfg kCfgF64[] = {{2, 2}, {4, 2}, {2, 4}};
-template <typename T> constexpr int n_cfg() { return sizeof(T) == 4 ? 4 : 3; }
-template <typename T> constexpr TileCfg cfg_of(int i) { return sizeof(T) == 4 ? kCfgF32[i] : kCfgF64[i]; }
+// 128 x 128 tiles would need 3 x 16 accumulator blocks per wave, more
+// registers than two waves per SIMD allow, so the shapes stop at 128 x 64.
+constexpr TileCfg kCfg[] = {{2, 2}, {4, 2}, {2, 4}};
+template <typename T> constexpr int n_cfg() { return 3; }
+template <typename T> constexpr TileCfg cfg_of(int i) { return kCfg[i]; }
 
 constexpr int kMaxDev = 16, kClasses = 11;
 struct TuneTable {
@@ -1445,32 +1487,22 @@ void launch_tile(hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int r
 template <typename T>
 void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int ra, int ca, int rb, int cb,
                 int rc, int cc) {
-    if constexpr (sizeof(T) == 4) {
-        switch (cfg) {
-        case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        case 3: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        default: launch_tile<T, 4, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        }
-    } else {
-        switch (cfg) {
-        case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        default: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-        }
+    switch (cfg) {
+    case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+    case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+    default: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
     }
 }
 
-// Size rule for untuned shapes: the largest tile that still gives 2 work-groups per CU.
+// Size rule for untuned shapes: the larger FP32 tile where it still gives 2
+// work-groups per CU, else 64 x 64.
 template <typename T>
 int default_cfg(int M, int N) {
     if constexpr (sizeof(T) == 4) {
-        if (cdiv(M, 128) * cdiv(N, 128) >= 512) return 0;
         if (cdiv(M, 128) * cdiv(N, 64) >= 512) return 1;
-        return 3;
-    } else {
-        return 0;
     }
+    (void)M; (void)N;
+    return 0;
 }
 
 template <typename T>
