@@ -125,6 +125,14 @@ _SIGS = {
     "bem_mesh_lid_disc": (c_int, [c_double, c_int, c_int, c_int, _PP]),
     "bem_mesh_lid_rect": (c_int, [c_double, c_double, c_int, c_int, c_int, _PP]),
     "bem_mesh_append": (c_int, [c_void_p, c_void_p, c_double, c_double, c_double]),
+    "bem_mesh_lid_ellipse": (c_int, [c_double, c_double, c_int, c_int, c_int, _PP]),
+    "bem_mesh_revolution": (c_int, [c_int, _PD, _PD, c_int, _PP]),
+    "bem_mesh_tapered_spar": (c_int, [c_double] * 5 + [c_int] * 6 + [_PP]),
+    "bem_mesh_spar_leg": (c_int, [c_double] * 4 + [c_int] * 6 + [_PP]),
+    "bem_mesh_barge": (c_int, [c_double] * 3 + [c_int] * 4 + [_PP]),
+    "bem_mesh_ellipsoid": (c_int, [c_double] * 4 + [c_int] * 4 + [_PP]),
+    "bem_mesh_semi_ellipsoid": (c_int, [c_double] * 3 + [c_int] * 4 + [_PP]),
+    "bem_mesh_wigley": (c_int, [c_double] * 3 + [c_int] * 5 + [_PP]),
     "bem_body_assemble_rankine": (c_int, [c_void_p, c_void_p]),
     "bem_solve_frequency": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, c_int, c_int, c_int, _PD,
                                     _PD, _PD, _PD, _PD, _PD, _PD, _PD, POINTER(SolveInfo)]),
@@ -275,6 +283,47 @@ class Mesh:
     def lid_rect(cls, L, W, nx, ny, cosine=False):
         """Interior free-surface lid of a barge."""
         return cls._make(lib().bem_mesh_lid_rect, L, W, nx, ny, int(cosine))
+
+    @classmethod
+    def lid_ellipse(cls, a, b, n_radial, n_azimuthal, cosine=True):
+        """Interior free-surface lid of a semi-ellipsoid."""
+        return cls._make(lib().bem_mesh_lid_ellipse, a, b, n_radial, n_azimuthal, int(cosine))
+
+    @classmethod
+    def revolution(cls, r, z, n_azimuthal):
+        """Volume of revolution from a profile (r_i, z_i) with r_0 = 0 (the reference's Volume_of_Revolution)."""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        return cls._make(lib().bem_mesh_revolution, len(r), _dp(r), _dp(z), n_azimuthal)
+
+    @classmethod
+    def tapered_spar(cls, r_lower, r_upper, draft, h_taper_begin, h_taper, n_radial, n_vert1, n_vert2, n_vert3,
+                     n_azimuthal, cosine=True):
+        return cls._make(lib().bem_mesh_tapered_spar, r_lower, r_upper, draft, h_taper_begin, h_taper, n_radial,
+                         n_vert1, n_vert2, n_vert3, n_azimuthal, int(cosine))
+
+    @classmethod
+    def spar_leg(cls, r_lower, r_upper, draft, h_plate, n_base, n_plate_vert, n_plate_horiz, n_vert, n_azimuthal,
+                 cosine=True):
+        return cls._make(lib().bem_mesh_spar_leg, r_lower, r_upper, draft, h_plate, n_base, n_plate_vert,
+                         n_plate_horiz, n_vert, n_azimuthal, int(cosine))
+
+    @classmethod
+    def barge(cls, L, W, D, nx, ny, nz, cosine=True):
+        """The reference Barge (open-top box, cosine spacing by default)."""
+        return cls._make(lib().bem_mesh_barge, L, W, D, nx, ny, nz, int(cosine))
+
+    @classmethod
+    def ellipsoid(cls, a, b, c, depth, n_axial, n_azimuthal, cosine=True, triangles=False):
+        return cls._make(lib().bem_mesh_ellipsoid, a, b, c, depth, n_axial, n_azimuthal, int(cosine), int(triangles))
+
+    @classmethod
+    def semi_ellipsoid(cls, a, b, c, n_axial, n_azimuthal, cosine=True, triangles=False):
+        return cls._make(lib().bem_mesh_semi_ellipsoid, a, b, c, n_axial, n_azimuthal, int(cosine), int(triangles))
+
+    @classmethod
+    def wigley(cls, L, B, T, nx, nz, cosine=True, triangles=False, outward=False):
+        return cls._make(lib().bem_mesh_wigley, L, B, T, nx, nz, int(cosine), int(triangles), int(outward))
 
     def append(self, other, dx=0.0, dy=0.0, dz=0.0):
         _check(lib().bem_mesh_append(self._h, other._h, dx, dy, dz))

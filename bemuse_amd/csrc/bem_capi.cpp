@@ -417,6 +417,65 @@ int bem_mesh_lid_rect(double L, double W, int nx, int ny, int cosine, bem_mesh**
     return wrap_mesh(mesh_lid_rect(L, W, nx, ny, cosine != 0), out);
 }
 
+int bem_mesh_lid_ellipse(double a, double b, int n_radial, int n_azimuthal, int cosine, bem_mesh** out) {
+    if (!out || !(a > 0) || !(b > 0) || n_radial < 1 || n_azimuthal < 3)
+        return fail(BEM_ERR_ARG, "bem_mesh_lid_ellipse: bad arguments");
+    return wrap_mesh(mesh_lid_ellipse(a, b, n_radial, n_azimuthal, cosine != 0), out);
+}
+
+int bem_mesh_revolution(int n, const double* r, const double* z, int n_azimuthal, bem_mesh** out) {
+    if (!out || !r || !z || n < 2 || n_azimuthal < 3 || r[0] != 0.0)
+        return fail(BEM_ERR_ARG, "bem_mesh_revolution: need >= 2 profile points starting on the axis (r = 0)");
+    return wrap_mesh(mesh_revolution(std::vector<double>(r, r + n), std::vector<double>(z, z + n), n_azimuthal), out);
+}
+
+int bem_mesh_tapered_spar(double r_lower, double r_upper, double draft, double h_taper_begin, double h_taper,
+                          int n_radial, int n_vert1, int n_vert2, int n_vert3, int n_azimuthal, int cosine,
+                          bem_mesh** out) {
+    if (!out || !(r_lower > 0) || !(r_upper > 0) || !(draft > 0) || h_taper_begin < 0 || h_taper < 0 ||
+        h_taper_begin + h_taper > draft || n_radial < (cosine ? 2 : 1) || n_vert1 < 0 || n_vert2 < 0 || n_vert3 < 1 ||
+        n_azimuthal < 3)
+        return fail(BEM_ERR_ARG, "bem_mesh_tapered_spar: bad dimensions");
+    return wrap_mesh(mesh_tapered_spar(r_lower, r_upper, draft, h_taper_begin, h_taper, n_radial, n_vert1, n_vert2,
+                                       n_vert3, n_azimuthal, cosine != 0), out);
+}
+
+int bem_mesh_spar_leg(double r_lower, double r_upper, double draft, double h_plate, int n_base, int n_plate_vert,
+                      int n_plate_horiz, int n_vert, int n_azimuthal, int cosine, bem_mesh** out) {
+    if (!out || !(r_lower > 0) || !(r_upper > 0) || !(draft > 0) || h_plate < 0 || h_plate >= draft || n_base < 1 ||
+        n_plate_vert < 0 || n_plate_horiz < 0 || n_vert < 1 || n_azimuthal < 3)
+        return fail(BEM_ERR_ARG, "bem_mesh_spar_leg: bad dimensions");
+    return wrap_mesh(mesh_spar_leg(r_lower, r_upper, draft, h_plate, n_base, n_plate_vert, n_plate_horiz, n_vert,
+                                   n_azimuthal, cosine != 0), out);
+}
+
+int bem_mesh_barge(double L, double W, double D, int nx, int ny, int nz, int cosine, bem_mesh** out) {
+    if (!out || !(L > 0) || !(W > 0) || !(D > 0) || nx < 1 || ny < 1 || nz < 1)
+        return fail(BEM_ERR_ARG, "bem_mesh_barge: bad dimensions");
+    return wrap_mesh(mesh_barge(L, W, D, nx, ny, nz, cosine != 0), out);
+}
+
+int bem_mesh_ellipsoid(double a, double b, double c, double depth, int n_axial, int n_azimuthal, int cosine,
+                       int triangles, bem_mesh** out) {
+    if (!out || !(a > 0) || !(b > 0) || !(c > 0) || !(depth > c) || n_axial < 2 || n_azimuthal < 3)
+        return fail(BEM_ERR_ARG, "bem_mesh_ellipsoid: bad dimensions (the ellipsoid must lie below z = 0)");
+    return wrap_mesh(mesh_ellipsoid(a, b, c, -depth, n_axial, n_azimuthal, cosine != 0, triangles != 0), out);
+}
+
+int bem_mesh_semi_ellipsoid(double a, double b, double c, int n_axial, int n_azimuthal, int cosine, int triangles,
+                            bem_mesh** out) {
+    if (!out || !(a > 0) || !(b > 0) || !(c > 0) || n_axial < 1 || n_azimuthal < 3)
+        return fail(BEM_ERR_ARG, "bem_mesh_semi_ellipsoid: bad dimensions");
+    return wrap_mesh(mesh_semi_ellipsoid(a, b, c, n_axial, n_azimuthal, cosine != 0, triangles != 0), out);
+}
+
+int bem_mesh_wigley(double L, double B, double T, int nx, int nz, int cosine, int triangles, int outward,
+                    bem_mesh** out) {
+    if (!out || !(L > 0) || !(B > 0) || !(T > 0) || nx < 1 || nz < 1)
+        return fail(BEM_ERR_ARG, "bem_mesh_wigley: bad dimensions");
+    return wrap_mesh(mesh_wigley(L, B, T, nx, nz, cosine != 0, triangles != 0, outward != 0), out);
+}
+
 int bem_mesh_append(bem_mesh* dst, const bem_mesh* src, double dx, double dy, double dz) {
     if (!dst || !src || dst == src) return fail(BEM_ERR_ARG, "bem_mesh_append: bad arguments");
     dst->m.append(src->m, dx, dy, dz);

@@ -33,6 +33,14 @@ Mesh mesh_revolution(const std::vector<double>& r, const std::vector<double>& z,
 Mesh mesh_half_cylinder(double R, double draft, int NR, int NV, int NA, bool cosine);
 Mesh mesh_lid_disc(double R, int NRFS, int NA, bool cosine);
 Mesh mesh_lid_rect(double L, double W, int NX, int NY, bool cosine);
+Mesh mesh_lid_ellipse(double a, double b, int NRFS, int NA, bool cosine);
+Mesh mesh_tapered_spar(double RB, double RT, double draft, double H1, double H2, int NR, int NV1, int NV2, int NV3,
+                       int NA, bool cosine);
+Mesh mesh_spar_leg(double RB, double RT, double draft, double H1, int N1, int N2, int N3, int N4, int NA, bool cosine);
+Mesh mesh_barge(double L, double W, double D, int NX, int NY, int NZ, bool cosine);
+Mesh mesh_ellipsoid(double a, double b, double c, double zc, int NZ, int NA, bool cosine, bool tri);
+Mesh mesh_semi_ellipsoid(double a, double b, double c, int NZ, int NA, bool cosine, bool tri);
+Mesh mesh_wigley(double L, double B, double T, int NX, int NZ, bool cosine, bool tri, bool outward);
 Mesh mesh_box(double L, double W, double z0, double z1, int NX, int NY, int NZ, bool open_top);
 Mesh mesh_semisub3(double rc, double T, double spacing, double pw, double ph, double gap,
                    int NA, int NV, int NR, int NX, int NY, int NZ);

@@ -72,14 +72,18 @@ Mesh mesh_revolution(const std::vector<double>& r, const std::vector<double>& z,
 // Volume_of_Revolution.cpp:43-65, 111-130): rings at r = R i / NRFS (or the
 // cosine spacing) in z = 0, a triangle fan round the centre and quads
 // outside it, in the reference's node order.
-Mesh mesh_lid_disc(double R, int NRFS, int NA, bool cosine) {
+Mesh mesh_lid_disc(double R, int NRFS, int NA, bool cosine) { return mesh_lid_ellipse(R, R, NRFS, NA, cosine); }
+
+// The elliptic lid of a semi-ellipsoid (Semi_Ellipsoid::Generate_Aux_Nodes /
+// _Elements, Ellipsoid.cpp:326-362): the disc's pattern, scaled by (a, b).
+Mesh mesh_lid_ellipse(double a, double b, int NRFS, int NA, bool cosine) {
     Mesh m;
     for (int i = 0; i <= NRFS; i++) {
-        double r = cosine ? R * pi_cos_fac(i, NRFS + 1) : R * i / NRFS;
+        double f = cosine ? pi_cos_fac(i, NRFS + 1) : 1.0 * i / NRFS;
         int NAR = (i == 0) ? 1 : NA;
         for (int j = 0; j < NAR; j++) {
             double u = j * (2.0 * M_PI) / NAR;
-            m.add_node(r * std::cos(u), r * std::sin(u), 0.0);
+            m.add_node(a * f * std::cos(u), b * f * std::sin(u), 0.0);
         }
     }
     auto id = [NA](int A, int Z) -> int {
@@ -125,6 +129,227 @@ Mesh mesh_half_cylinder(double R, double draft, int NR, int NV, int NA, bool cos
         for (int i = 0; i < NV + 1; i++) { r.push_back(R); z.push_back(Z * (1.0 - 1.0 * i / NV)); }
     }
     return mesh_revolution(r, z, NA);
+}
+
+// Tapered_SparBuoy::Generate_Nodes profile (Volume_of_Revolution.cpp:232-251):
+// bottom disc, lower column (H1), taper (H2), upper column to the waterline.
+// With cosine spacing the disc has NR - 1 rings, as there.
+Mesh mesh_tapered_spar(double RB, double RT, double draft, double H1, double H2, int NR, int NV1, int NV2, int NV3,
+                       int NA, bool cosine) {
+    double Z = -draft, H3 = Z + H1 + H2;
+    std::vector<double> r, z;
+    auto pt = [&](double rr, double zz) { r.push_back(rr); z.push_back(zz); };
+    if (cosine) {
+        for (int i = 0; i < NR - 1; i++) pt(RB * pi_cos_fac(i, NR), Z);
+        for (int i = 0; i < NV1; i++) pt(RB, Z + H1 * pi_cos_fac(i, NV1 + 1));
+        for (int i = 0; i < NV2; i++) { double f = pi_cos_fac(i, NV2 + 1); pt(RB - (RB - RT) * f, Z + H1 + H2 * f); }
+        for (int i = 0; i < NV3 + 1; i++) pt(RT, H3 * (1.0 - pi_cos_fac(i, NV3 + 1)));
+    } else {
+        for (int i = 0; i < NR; i++) pt(RB * i / NR, Z);
+        for (int i = 0; i < NV1; i++) pt(RB, Z + H1 * i / NV1);
+        for (int i = 0; i < NV2; i++) pt(RB - (RB - RT) * 1.0 * i / NV2, Z + H1 + H2 * 1.0 * i / NV2);
+        for (int i = 0; i < NV3 + 1; i++) pt(RT, H3 * (1.0 - 1.0 * i / NV3));
+    }
+    return mesh_revolution(r, z, NA);
+}
+
+// Spar_Leg::Generate_Nodes profile (Volume_of_Revolution.cpp:278-297): base
+// disc (N1), heave-plate rim of height H1 (N2), plate top from RB in to RT
+// (N3), column to the waterline (N4).
+Mesh mesh_spar_leg(double RB, double RT, double draft, double H1, int N1, int N2, int N3, int N4, int NA, bool cosine) {
+    double Z = -draft, H2 = Z + H1;
+    std::vector<double> r, z;
+    auto pt = [&](double rr, double zz) { r.push_back(rr); z.push_back(zz); };
+    if (cosine) {
+        for (int i = 0; i < N1; i++) pt(RB * pi_cos_fac(i, N1 + 1), Z);
+        for (int i = 0; i < N2; i++) pt(RB, Z + H1 * pi_cos_fac(i, N2 + 1));
+        for (int i = 0; i < N3; i++) pt(RB - (RB - RT) * pi_cos_fac(i, N3 + 1), Z + H1);
+        for (int i = 0; i < N4 + 1; i++) pt(RT, H2 * (1.0 - pi_cos_fac(i, N4 + 1)));
+    } else {
+        for (int i = 0; i < N1; i++) pt(RB * i / N1, Z);
+        for (int i = 0; i < N2; i++) pt(RB, Z + H1 * i / N2);
+        for (int i = 0; i < N3; i++) pt(RB - (RB - RT) * 1.0 * i / N3, Z + H1);
+        for (int i = 0; i < N4 + 1; i++) pt(RT, H2 * (1.0 - 1.0 * i / N4));
+    }
+    return mesh_revolution(r, z, NA);
+}
+
+// Barge::Generate_Nodes / Generate_Elements (Barge.cpp:33-191): an open-top
+// box, x in [-L/2, L/2], y in [-W/2, W/2], z in [-D, 0], each face with its
+// own nodes (x faces, then y faces, then the bottom); the two x faces' panels
+// alternate, then the two y faces', then the bottom's; cosine or uniform
+// spacing in every direction.
+Mesh mesh_barge(double L, double W, double D, int NX, int NY, int NZ, bool cosine) {
+    Mesh m;
+    auto sx = [&](int i) { return cosine ? -0.5 * L + L * pi_cos_fac(i, NX + 1) : -0.5 * L + L * i / NX; };
+    auto sy = [&](int j) { return cosine ? -0.5 * W + W * pi_cos_fac(j, NY + 1) : -0.5 * W + W * j / NY; };
+    auto sz = [&](int k) { return cosine ? -D + D * pi_cos_fac(k, NZ + 1) : -D + D * k / NZ; };
+    const int nxf = (NY + 1) * (NZ + 1), nyf = (NX + 1) * (NZ + 1);
+    const int X1 = 0, X2 = nxf, Y1 = 2 * nxf, Y2 = 2 * nxf + nyf, ZF = 2 * nxf + 2 * nyf;
+    for (int side = 0; side < 2; side++)
+        for (int j = 0; j <= NY; j++)
+            for (int k = 0; k <= NZ; k++) m.add_node(side ? 0.5 * L : -0.5 * L, sy(j), sz(k));
+    for (int side = 0; side < 2; side++)
+        for (int i = 0; i <= NX; i++)
+            for (int k = 0; k <= NZ; k++) m.add_node(sx(i), side ? 0.5 * W : -0.5 * W, sz(k));
+    for (int i = 0; i <= NX; i++)
+        for (int j = 0; j <= NY; j++) m.add_node(sx(i), sy(j), -D);
+    auto xf = [NZ](int j, int k) { return j * (NZ + 1) + k; };
+    auto zf = [NY](int i, int j) { return i * (NY + 1) + j; };
+    for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+            m.add_quad(X1 + xf(j, k), X1 + xf(j, k + 1), X1 + xf(j + 1, k + 1), X1 + xf(j + 1, k));
+            m.add_quad(X2 + xf(j, k), X2 + xf(j + 1, k), X2 + xf(j + 1, k + 1), X2 + xf(j, k + 1));
+        }
+    for (int i = 0; i < NX; i++)
+        for (int k = 0; k < NZ; k++) {
+            m.add_quad(Y1 + xf(i, k), Y1 + xf(i + 1, k), Y1 + xf(i + 1, k + 1), Y1 + xf(i, k + 1));
+            m.add_quad(Y2 + xf(i, k), Y2 + xf(i, k + 1), Y2 + xf(i + 1, k + 1), Y2 + xf(i + 1, k));
+        }
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++) m.add_quad(ZF + zf(i, j), ZF + zf(i, j + 1), ZF + zf(i + 1, j + 1), ZF + zf(i + 1, j));
+    return m;
+}
+
+// Ellipsoid rings: ring 0 and (closed ellipsoid) ring NZ are single nodes on
+// the axis, the others NA nodes; with triangular panels the even rings turn by
+// half an azimuthal step. Node index of (azimuth A, ring Z) as
+// Ellipsoid::Node_ID / Semi_Ellipsoid::Node_ID (Ellipsoid.cpp:150-161, 444-454).
+static int ring_id(int A, int Z, int NA, int NZ, bool closed_top) {
+    if (closed_top && Z == NZ) return 1 + NA * (NZ - 1);
+    if (Z == 0) return 0;
+    if (A >= NA) return 1 + (Z - 1) * NA + (A - NA);
+    if (A < 0) return 1 + (Z - 1) * NA + (A + NA);
+    return 1 + (Z - 1) * NA + A;
+}
+
+// Triangles between rings z and z + 1 (two per azimuthal step), the pattern of
+// Ellipsoid.cpp:114-137 / 270-293.
+static void ring_triangles(Mesh& m, int z, int NA, int NZ, bool closed_top) {
+    auto id = [&](int A, int Z) { return ring_id(A, Z, NA, NZ, closed_top); };
+    for (int i = 0; i < NA; i++) {
+        if (z % 2 == 0) {
+            m.add_tri(id(i, z), id(i + 1, z), id(i + 1, z + 1));
+            m.add_tri(id(i + 1, z), id(i + 2, z + 1), id(i + 1, z + 1));
+        } else {
+            m.add_tri(id(i, z), id(i + 1, z), id(i, z + 1));
+            m.add_tri(id(i + 1, z), id(i + 1, z + 1), id(i, z + 1));
+        }
+    }
+}
+
+// Ellipsoid::Generate_Nodes / Generate_Elements (Ellipsoid.cpp:31-148): a
+// closed ellipsoid of semi-axes a, b, c centred at (0, 0, zc), bottom pole
+// first; zenith angle v = pi (1 - PiCosFac) (cosine) or pi (1 - i/NZ).
+Mesh mesh_ellipsoid(double a, double b, double c, double zc, int NZ, int NA, bool cosine, bool tri) {
+    Mesh m;
+    for (int i = 0; i <= NZ; i++) {
+        double v = cosine ? M_PI * (1.0 - pi_cos_fac(i, NZ + 1)) : M_PI - M_PI * i / NZ;
+        double sv = std::sin(v), cv = std::cos(v);
+        if (i == 0) { sv = 0.0; cv = -1.0; }
+        else if (i == NZ) { sv = 0.0; cv = 1.0; }
+        int NAR = (i == 0 || i == NZ) ? 1 : NA;
+        for (int j = 0; j < NAR; j++) {
+            double u = j * (2.0 * M_PI) / NAR;
+            if (tri && i % 2 == 0) u += 0.5 * (2.0 * M_PI) / NAR;
+            m.add_node(a * std::cos(u) * sv, b * std::sin(u) * sv, c * cv + zc);
+        }
+    }
+    auto id = [&](int A, int Z) { return ring_id(A, Z, NA, NZ, true); };
+    for (int i = 0; i < NA; i++) m.add_tri(id(0, 0), id(i + 1, 1), id(i, 1));
+    for (int z = 1; z < NZ - 1; z++) {
+        if (tri) ring_triangles(m, z, NA, NZ, true);
+        else
+            for (int i = 0; i < NA; i++) m.add_quad(id(i, z), id(i + 1, z), id(i + 1, z + 1), id(i, z + 1));
+    }
+    for (int i = 0; i < NA; i++) m.add_tri(id(i + 1, NZ - 1), id(i, NZ), id(i, NZ - 1));
+    return m;
+}
+
+// Semi_Ellipsoid::Generate_Nodes / Generate_Elements (Ellipsoid.cpp:205-310):
+// the lower half (z <= 0) of the ellipsoid, open at the waterline;
+// v = pi (1 - PiCosFac / 2) (cosine) or pi (1 - i / (2 NZ)).
+Mesh mesh_semi_ellipsoid(double a, double b, double c, int NZ, int NA, bool cosine, bool tri) {
+    Mesh m;
+    for (int i = 0; i <= NZ; i++) {
+        double v = cosine ? M_PI * (1.0 - 0.5 * pi_cos_fac(i, NZ + 1)) : M_PI - 0.5 * M_PI * i / NZ;
+        double sv = std::sin(v), cv = std::cos(v);
+        if (i == 0) { sv = 0.0; cv = -1.0; }
+        else if (i == NZ) { sv = 1.0; cv = 0.0; }
+        int NAR = (i == 0) ? 1 : NA;
+        for (int j = 0; j < NAR; j++) {
+            double u = j * (2.0 * M_PI) / NAR;
+            if (tri && i % 2 == 0) u += 0.5 * (2.0 * M_PI) / NAR;
+            m.add_node(a * std::cos(u) * sv, b * std::sin(u) * sv, c * cv);
+        }
+    }
+    auto id = [&](int A, int Z) { return ring_id(A, Z, NA, NZ, false); };
+    for (int i = 0; i < NA; i++) m.add_tri(id(0, 0), id(i + 1, 1), id(i, 1));
+    for (int z = 1; z < NZ; z++) {
+        if (tri) ring_triangles(m, z, NA, NZ, false);
+        else
+            for (int i = 0; i < NA; i++) m.add_quad(id(i, z), id(i + 1, z), id(i + 1, z + 1), id(i, z + 1));
+    }
+    return m;
+}
+
+// Wigley_Hull::Hull_Node (Ship_Hulls.cpp:130-142): x = L xn - L/2,
+// y = 2 B xn (1 - xn) (1 - zn^2), z = T zn for xn in [0, 1], zn in [-1, 0].
+static void wigley_point(double L, double B, double T, double xn, double zn, double p[3]) {
+    p[0] = L * xn - 0.5 * L;
+    p[1] = 2.0 * B * xn * (1.0 - xn) * (1.0 - zn * zn);
+    p[2] = T * zn;
+}
+
+// Wigley_Hull::Generate_Nodes / Generate_*_Elements (Ship_Hulls.cpp:12-90,
+// 144-200): port-side nodes, then starboard (y mirrored), then, with
+// triangular panels, one centre node per cell on each side; port panels
+// first, then starboard. Each cell is a quad, or four triangles round its
+// centre node. outward reverses every panel (Panels_Outward, Geo_Elements.h:128-159).
+Mesh mesh_wigley(double L, double B, double T, int NX, int NZ, bool cosine, bool tri, bool outward) {
+    Mesh m;
+    double p[3];
+    const int side = (NX + 1) * (NZ + 1);
+    for (int s = 0; s < 2; s++)
+        for (int i = 0; i <= NX; i++) {
+            double xf = cosine ? pi_cos_fac(i, NX + 1) : 1.0 * i / NX;
+            for (int k = 0; k <= NZ; k++) {
+                double zf = cosine ? pi_cos_fac(k, NZ + 1) - 1.0 : -1.0 + 1.0 * k / NZ;
+                wigley_point(L, B, T, xf, zf, p);
+                m.add_node(p[0], s ? -p[1] : p[1], p[2]);
+            }
+        }
+    if (tri) {
+        auto shift = [](int i, int itot) { return 0.5 * (1.0 - std::cos((i + 0.5) * M_PI / (itot - 1))); };
+        for (int s = 0; s < 2; s++)
+            for (int i = 0; i < NX; i++) {
+                double xf = cosine ? shift(i, NX + 1) : (i + 0.5) / NX;
+                for (int k = 0; k < NZ; k++) {
+                    double zf = cosine ? shift(k, NZ + 1) - 1.0 : -1.0 + (k + 0.5) / NZ;
+                    wigley_point(L, B, T, xf, zf, p);
+                    m.add_node(p[0], s ? -p[1] : p[1], p[2]);
+                }
+            }
+    }
+    auto Lf = [NZ](int i, int j) { return i * (NZ + 1) + j; };
+    auto Rf = [NZ, side](int i, int j) { return side + i * (NZ + 1) + j; };
+    const int cL = 2 * side, cR = 2 * side + NX * NZ;
+    auto quad = [&m, outward](int a, int b, int c, int d) {
+        if (outward) m.add_quad(d, c, b, a); else m.add_quad(a, b, c, d);
+    };
+    auto trig = [&m, outward](int a, int b, int c) {
+        if (outward) m.add_tri(c, b, a); else m.add_tri(a, b, c);
+    };
+    for (int s = 0; s < 2; s++)
+        for (int i = 0; i < NX; i++)
+            for (int j = 0; j < NZ; j++) {
+                int n1, n2, n3, n4, c;
+                if (s == 0) { n1 = Lf(i, j); n2 = Lf(i, j + 1); n3 = Lf(i + 1, j + 1); n4 = Lf(i + 1, j); c = cL + i * NZ + j; }
+                else { n1 = Rf(i + 1, j); n2 = Rf(i + 1, j + 1); n3 = Rf(i, j + 1); n4 = Rf(i, j); c = cR + i * NZ + j; }
+                if (tri) { trig(n1, n2, c); trig(n2, n3, c); trig(n3, n4, c); trig(n4, n1, c); }
+                else quad(n1, n2, n3, n4);
+            }
+    return m;
 }
 
 // Rectangular box centred on the z axis, x in [-L/2, L/2], y in [-W/2, W/2],

@@ -87,6 +87,37 @@ int bem_mesh_half_cylinder(double radius, double draft, int n_radial, int n_vert
  * quads degenerate; here y follows its own index). */
 int bem_mesh_lid_disc(double radius, int n_radial, int n_azimuthal, int cosine, bem_mesh** out);
 int bem_mesh_lid_rect(double L, double W, int nx, int ny, int cosine, bem_mesh** out);
+/* The elliptic lid of a semi-ellipsoid (Ellipsoid.cpp:326-362); a = b = R is
+ * bem_mesh_lid_disc. */
+int bem_mesh_lid_ellipse(double a, double b, int n_radial, int n_azimuthal, int cosine, bem_mesh** out);
+/* Reference parametric bodies, node for node (same node and panel order):
+ *  - a volume of revolution from a profile (r_i, z_i), r_0 = 0: one axis node,
+ *    n_azimuthal nodes per further point, a triangle fan round the axis node,
+ *    quads between rings (Volume_of_Revolution.cpp:12-37, 92-109);
+ *  - Tapered_SparBuoy (Volume_of_Revolution.cpp:203-251; keys Lower_Radius,
+ *    Upper_Radius, Draft, Height_Taper_Begin, Height_Taper_End, NPanels_Radial,
+ *    NPanels_Vertical_Section1..3, NPanels_Azimuthal, Cosine_Disc);
+ *  - Spar_Leg (Volume_of_Revolution.cpp:253-297; Leg_Lower_Radius,
+ *    Leg_Upper_Radius, Draft, Heave_Plate_Height, NPanels_Radial_Base,
+ *    NPanels_Vertical_HeavePlate, NPanels_Horizontal_HeavePlate,
+ *    NPanels_Vertical, NPanels_Azimuthal, Cosine_Disc);
+ *  - Barge (Barge.cpp:12-191; cosine spacing is the reference default);
+ *  - Ellipsoid, centre at z = -depth, and Semi_Ellipsoid (Ellipsoid.cpp:12-310;
+ *    triangles = Triangular_Panels);
+ *  - Wigley_Hull (Ship_Hulls.cpp:12-200; outward = Panels_Outward). */
+int bem_mesh_revolution(int n, const double* r, const double* z, int n_azimuthal, bem_mesh** out);
+int bem_mesh_tapered_spar(double r_lower, double r_upper, double draft, double h_taper_begin, double h_taper,
+                          int n_radial, int n_vert1, int n_vert2, int n_vert3, int n_azimuthal, int cosine,
+                          bem_mesh** out);
+int bem_mesh_spar_leg(double r_lower, double r_upper, double draft, double h_plate, int n_base, int n_plate_vert,
+                      int n_plate_horiz, int n_vert, int n_azimuthal, int cosine, bem_mesh** out);
+int bem_mesh_barge(double L, double W, double D, int nx, int ny, int nz, int cosine, bem_mesh** out);
+int bem_mesh_ellipsoid(double a, double b, double c, double depth, int n_axial, int n_azimuthal, int cosine,
+                       int triangles, bem_mesh** out);
+int bem_mesh_semi_ellipsoid(double a, double b, double c, int n_axial, int n_azimuthal, int cosine, int triangles,
+                            bem_mesh** out);
+int bem_mesh_wigley(double L, double B, double T, int nx, int nz, int cosine, int triangles, int outward,
+                    bem_mesh** out);
 /* Appends src, translated by (dx, dy, dz), to dst (e.g. one lid per column). */
 int bem_mesh_append(bem_mesh* dst, const bem_mesh* src, double dx, double dy, double dz);
 /* Rectangular box x in [-L/2,L/2], y in [-W/2,W/2], z in [z0,z1]

@@ -186,6 +186,15 @@ public:
     int N_Aux_Panels() const { int n = 0, e = 0; if (lid_) bem_mesh_size(lid_, &n, &e); return e; }
 
 protected:
+    // Boundary::Set_Kin_Params (Boundary_Base.h:163): bodies with known mass
+    // properties set them here
+    virtual void Set_Kin_Params() {}
+    // the end of Boundary::Setup (Boundary_Base.cpp:35-46)
+    void Finish() {
+        Set_Kin_Params();
+        Report();
+    }
+    bool Cosine = true;   // Cosine_Disc (Boundary_Base.h:86)
     void Adopt_Lid(int rc, bem_mesh* m) {
         if (lid_) { bem_mesh_free(lid_); lid_ = nullptr; }
         if (rc != BEM_OK) { std::cout << "BEMUse: lid generation failed: " << bem_last_error() << std::endl; return; }
@@ -291,7 +300,7 @@ public:
         bem_mesh* m = pending_;
         pending_ = nullptr;
         Adopt(BEM_OK, m);
-        Report();
+        Finish();
     }
     ~File_Geometry() override { if (pending_) bem_mesh_free(pending_); }
 private:
@@ -303,9 +312,58 @@ class MAR_Geometry : public File_Geometry { public: MAR_Geometry() : File_Geomet
 class PNL_Geometry : public File_Geometry { public: PNL_Geometry() : File_Geometry(BEM_MESH_PNL) {} };
 class STL_Geometry : public File_Geometry { public: STL_Geometry() : File_Geometry(BEM_MESH_STL) {} };
 
+// Bodies of revolution (Volume_of_Revolution.cpp): a profile (r, z) swept
+// round the z axis, the interior lid of NPanels_FreeSurface_Int_Radial rings
+// (:43-65, 111-130) and exterior free-surface rings r = 1.001 R + RFS R i / NRES
+// (:69-89; with NRES = 0 the reference divides by zero there, so no rings are
+// made then). Subclasses give the hull.
+class Volume_of_Revolution : public Boundary {
+public:
+    void Setup() override {
+        bem_mesh *hull = nullptr, *lid = nullptr;
+        std::vector<Vector3> fs;
+        int rc = Generate_Pieces(&hull, &lid, fs);
+        Adopt(rc, hull);
+        if (lid) Adopt_Lid(BEM_OK, lid);
+        FreeSurface_Nodes = fs;
+        Finish();
+    }
+    // Hull, lid (null without NRFS) and exterior points in the body's own
+    // coordinates; composite bodies place them.
+    int Generate_Pieces(bem_mesh** hull, bem_mesh** lid, std::vector<Vector3>& fs) {
+        int rc = Generate_Hull(hull);
+        if (rc != BEM_OK) return rc;
+        if (NRFS > 0) {
+            rc = bem_mesh_lid_disc(R, NRFS, NA, Cosine ? 1 : 0, lid);
+            if (rc != BEM_OK) std::cout << "BEMUse: lid generation failed: " << bem_last_error() << std::endl;
+        }
+        fs.clear();
+        for (int i = 0; NRES > 0 && i <= NRES; i++) {
+            Real r = R * 1.001 + RFS * R * i / NRES;
+            for (int j = 0; j < NA; j++) {
+                Real u = j * TwoPI / NA;
+                fs.push_back(Vector3(r * std::cos(u), r * std::sin(u), 0.0));
+            }
+        }
+        return BEM_OK;
+    }
+protected:
+    virtual int Generate_Hull(bem_mesh** m) = 0;
+    // keys every body of revolution reads
+    void Common_Parameter(const Parameter& P) {
+        if (P.myNameis("FreeSurface_Radius")) RFS = P.Get_Param<Real>();
+        if (P.myNameis("NPanels_Azimuthal")) NA = P.Get_Param<int>();
+        if (P.myNameis("NPanels_FreeSurface_Radial")) NRES = P.Get_Param<int>();
+        if (P.myNameis("NPanels_FreeSurface_Int_Radial")) NRFS = P.Get_Param<int>();
+        if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
+    }
+    Real R = 10, RFS = 0;   // waterline (lid) radius; exterior ring extent in multiples of R
+    int NA = 32, NRES = 0, NRFS = 0;
+};
+
 // Half_Cylinder (Volume_of_Revolution.cpp:167-199): open-top vertical cylinder
 // with a bottom disc (triangle fan at the centre).
-class Half_Cylinder : public Boundary {
+class Half_Cylinder : public Volume_of_Revolution {
 public:
     void Set_Parameters(std::vector<Parameter>& Params) override {
         for (const Parameter& P : Params) {
@@ -313,43 +371,228 @@ public:
             if (P.myNameis("Draft")) D = P.Get_Param<Real>();
             if (P.myNameis("NPanels_Radial")) NR = P.Get_Param<int>();
             if (P.myNameis("NPanels_Vertical")) NV = P.Get_Param<int>();
-            if (P.myNameis("NPanels_Azimuthal")) NA = P.Get_Param<int>();
-            if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
-            if (P.myNameis("FreeSurface_Radius")) RFS = P.Get_Param<Real>();
-            if (P.myNameis("NPanels_FreeSurface_Radial")) NRES = P.Get_Param<int>();
-            if (P.myNameis("NPanels_FreeSurface_Int_Radial")) NRFS = P.Get_Param<int>();
+            Common_Parameter(P);
+        }
+    }
+protected:
+    int Generate_Hull(bem_mesh** m) override { return bem_mesh_half_cylinder(R, D, NR, NV, NA, Cosine ? 1 : 0, m); }
+    Real D = 20;
+    int NR = 8, NV = 24;
+};
+
+// Tapered_SparBuoy (Volume_of_Revolution.cpp:203-251): lower column of radius
+// RB up to Height_Taper_Begin above the keel, a taper over Height_Taper_End to
+// RT, the upper column to the waterline; the lid radius is RT.
+class Tapered_SparBuoy : public Volume_of_Revolution {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Lower_Radius")) RB = P.Get_Param<Real>();
+            if (P.myNameis("Upper_Radius")) RT = P.Get_Param<Real>();
+            if (P.myNameis("Draft")) D = P.Get_Param<Real>();
+            if (P.myNameis("Height_Taper_Begin")) H1 = P.Get_Param<Real>();
+            if (P.myNameis("Height_Taper_End")) H2 = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_Radial")) NR = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Vertical_Section1")) NV1 = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Vertical_Section2")) NV2 = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Vertical_Section3")) NV3 = P.Get_Param<int>();
+            Common_Parameter(P);
+        }
+        R = RT;
+    }
+protected:
+    int Generate_Hull(bem_mesh** m) override {
+        return bem_mesh_tapered_spar(RB, RT, D, H1, H2, NR, NV1, NV2, NV3, NA, Cosine ? 1 : 0, m);
+    }
+    Real RB = 4.7, RT = 3.25, D = 120, H1 = 108, H2 = 8;
+    int NR = 8, NV1 = 24, NV2 = 4, NV3 = 8;
+};
+
+// OC3_SparBuoy (FOWT_Platforms.h:33-119): the OC3 Hywind spar, a tapered spar
+// of fixed dimensions times Scaling_Factor, with its mass and stiffness.
+class OC3_SparBuoy : public Tapered_SparBuoy {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params)
+            if (P.myNameis("Scaling_Factor")) ScaleFac = P.Get_Param<Real>();
+        Tapered_SparBuoy::Set_Parameters(Params);
+        Set_Dimensions();
+    }
+    void Set_Dimensions(std::vector<Real>& Dm) { ScaleFac = Dm[0]; Set_Dimensions(); }
+    void Set_External_Dimensions(std::vector<Real>& Dm) { RFS = Dm[0]; R = RT; }
+protected:
+    void Set_Dimensions() {
+        RB = 4.7 * ScaleFac; RT = 3.25 * ScaleFac; D = 120 * ScaleFac; H1 = 108 * ScaleFac; H2 = 8 * ScaleFac;
+        R = RT;
+    }
+    void Set_Kin_Params() override {
+        Real SF3 = ScaleFac * ScaleFac * ScaleFac, SF5 = SF3 * ScaleFac * ScaleFac;
+        Centre_Gravity = Vector3(0, 0, -89.9155 * ScaleFac);
+        Mass = 7.46633e+06 * SF3;
+        Mass_Mat = Matrix::Zero(6, 6);
+        Stiffness_Mat = Matrix::Zero(6, 6);
+        for (int i = 0; i < 3; i++) Mass_Mat(i, i) = 7.46633e+06 * SF3;
+        Mass_Mat(3, 3) = 4.22923e+09 * SF5; Mass_Mat(4, 4) = 4.22923e+09 * SF5; Mass_Mat(5, 5) = 1.226e+10 * SF5;
+        Stiffness_Mat(2, 2) = 3.32941e+05 * SF3;
+        Stiffness_Mat(3, 3) = -4.99918e+09 * SF5; Stiffness_Mat(4, 4) = -4.99918e+09 * SF5;
+    }
+    Real ScaleFac = 1.0;
+};
+
+// Softwind_Upscaled (FOWT_Platforms.h:121-153): a tapered spar whose
+// dimensions come from Set_Dimensions (the discretisation from the
+// parameters). The reference leaves the lid radius at its earlier value
+// there; here it follows the new upper radius.
+class Softwind_Upscaled : public Tapered_SparBuoy {
+public:
+    void Set_Dimensions(std::vector<Real>& Dm) {
+        (void)Dm;
+        RB = 9.0; RT = 5.6; D = 91.4; H1 = 78; H2 = 8; RFS = RT; R = RT;
+    }
+protected:
+    void Set_Kin_Params() override {
+        Mass = 1.94E+07;
+        Centre_Gravity = Vector3(0, 0, -71.56);
+    }
+};
+
+// Spar_Leg (Volume_of_Revolution.cpp:253-297): a column of radius RT on a
+// heave plate of radius RB and height Heave_Plate_Height.
+class Spar_Leg : public Volume_of_Revolution {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Leg_Lower_Radius")) RB = P.Get_Param<Real>();
+            if (P.myNameis("Leg_Upper_Radius")) RT = P.Get_Param<Real>();
+            if (P.myNameis("Draft")) D = P.Get_Param<Real>();
+            if (P.myNameis("Heave_Plate_Height")) H1 = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_Radial_Base")) N1 = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Vertical_HeavePlate")) N2 = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Horizontal_HeavePlate")) N3 = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Vertical")) N4 = P.Get_Param<int>();
+            Common_Parameter(P);
+        }
+        R = RT;
+    }
+protected:
+    int Generate_Hull(bem_mesh** m) override {
+        return bem_mesh_spar_leg(RB, RT, D, H1, N1, N2, N3, N4, NA, Cosine ? 1 : 0, m);
+    }
+    Real RB = 12, RT = 6, D = 20, H1 = 6;
+    int N1 = 8, N2 = 4, N3 = 4, N4 = 16;
+};
+
+// Triple_Spar (Triple_Spar.cpp:10-215): three identical legs (half
+// cylinders) at radius Spar_Radius, azimuths -90, 30 and 150 degrees
+// measured from +y, shifted by Spar_Radius along x with Shift_Over_Leg; an
+// optional central column comes first. Hull, lid and exterior points are the
+// parts' in that order. Every part receives the same parameters.
+class Triple_Spar : public Boundary {
+public:
+    Triple_Spar() : legs_{new Half_Cylinder, new Half_Cylinder, new Half_Cylinder} {}
+    ~Triple_Spar() override {
+        delete central_;
+        for (auto* l : legs_) delete l;
+    }
+    Triple_Spar(const Triple_Spar&) = delete;
+    Triple_Spar& operator=(const Triple_Spar&) = delete;
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        if (central_) central_->Set_Parameters(Params);
+        for (auto* l : legs_) l->Set_Parameters(Params);
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Spar_Radius")) R_Leg = P.Get_Param<Real>();
+            if (P.myNameis("Shift_Over_Leg")) ShiftRearwards = P.Get_Param<bool>();
         }
     }
     void Setup() override {
-        bem_mesh* m = nullptr;
-        int rc = bem_mesh_half_cylinder(R, D, NR, NV, NA, Cosine ? 1 : 0, &m);
-        Adopt(rc, m);
-        // interior lid (Volume_of_Revolution.cpp:43-65, 111-130)
-        if (NRFS > 0) {
-            bem_mesh* l = nullptr;
-            int rl = bem_mesh_lid_disc(R, NRFS, NA, Cosine ? 1 : 0, &l);
-            Adopt_Lid(rl, l);
+        const Real PI = 3.14159265358979323846;
+        Real th[3] = {-0.5 * PI, -0.5 * PI + 2.0 * PI / 3, -0.5 * PI + 4.0 * PI / 3};
+        Vector3 C[4];   // central (origin), legs 1..3
+        for (int k = 0; k < 3; k++) {
+            C[k + 1] = Vector3(R_Leg * std::sin(th[k]), R_Leg * std::cos(th[k]), 0.0);
+            if (ShiftRearwards) C[k + 1].x -= R_Leg * std::sin(th[0]);
         }
-        // exterior free-surface rings r = 1.001 R + RFS R i / NRES
-        // (Volume_of_Revolution.cpp:69-89; with NRES = 0 the reference divides
-        // by zero there, so no rings are made then)
-        FreeSurface_Nodes.clear();
-        for (int i = 0; NRES > 0 && i <= NRES; i++) {
-            Real r = R * 1.001 + RFS * R * i / NRES;
-            for (int j = 0; j < NA; j++) {
-                Real u = j * 2.0 * 3.14159265358979323846 / NA;
-                FreeSurface_Nodes.push_back(Vector3(r * std::cos(u), r * std::sin(u), 0.0));
-            }
+        bem_mesh *hull = nullptr, *lid = nullptr;
+        int rc = bem_mesh_from_arrays(0, nullptr, 0, nullptr, &hull);
+        if (rc == BEM_OK) rc = bem_mesh_from_arrays(0, nullptr, 0, nullptr, &lid);
+        std::vector<Vector3> fs;
+        bool any_lid = false;
+        Volume_of_Revolution* parts[4] = {central_, legs_[0], legs_[1], legs_[2]};
+        for (int k = 0; k < 4 && rc == BEM_OK; k++) {
+            if (!parts[k]) continue;
+            bem_mesh *h = nullptr, *l = nullptr;
+            std::vector<Vector3> f;
+            rc = parts[k]->Generate_Pieces(&h, &l, f);
+            if (rc == BEM_OK) rc = bem_mesh_append(hull, h, C[k].x, C[k].y, C[k].z);
+            if (rc == BEM_OK && l) { rc = bem_mesh_append(lid, l, C[k].x, C[k].y, C[k].z); any_lid = true; }
+            for (const Vector3& v : f) fs.push_back(Vector3(v.x + C[k].x, v.y + C[k].y, v.z + C[k].z));
+            if (h) bem_mesh_free(h);
+            if (l) bem_mesh_free(l);
         }
-        Report();
+        if (rc != BEM_OK) {
+            if (hull) bem_mesh_free(hull);
+            if (lid) bem_mesh_free(lid);
+            Adopt(rc, nullptr);
+            return;
+        }
+        Adopt(BEM_OK, hull);
+        if (any_lid) Adopt_Lid(BEM_OK, lid);
+        else bem_mesh_free(lid);
+        FreeSurface_Nodes = fs;
+        Finish();
     }
-private:
-    Real R = 10, D = 20, RFS = 0;
-    int NR = 8, NV = 24, NA = 32, NRES = 0, NRFS = 0;
-    bool Cosine = true;
+protected:
+    Volume_of_Revolution* central_ = nullptr;
+    Volume_of_Revolution* legs_[3];
+    Real R_Leg = 30;
+    bool ShiftRearwards = false;
 };
 
-// Barge (Barge.cpp:12-191): rectangular box piercing the free surface.
+// SemiSub_OC4 (Triple_Spar.cpp:159-242): the OC4 DeepCwind semi-submersible,
+// a central half cylinder (radius 3.25 m, draft 20 m) and three spar legs
+// (radius 6 m on heave plates of 12 m, 6 m high, draft 20 m) at 28.87 m, with
+// its mass and stiffness. The reference reads Scaling_Factor from its
+// parameter list before appending the new parameters, so the scale is always
+// 1 there; it is 1 here too.
+class SemiSub_OC4 : public Triple_Spar {
+public:
+    SemiSub_OC4() {
+        central_ = new Half_Cylinder;
+        for (auto*& l : legs_) { delete l; l = new Spar_Leg; }
+    }
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        std::vector<Parameter> cp = Params;
+        cp.push_back(Parameter("Radius", ScaleFac * Real(3.25)));
+        cp.push_back(Parameter("Draft", ScaleFac * Real(20.0)));
+        central_->Set_Parameters(cp);
+        std::vector<Parameter> lp = Params;
+        lp.push_back(Parameter("Leg_Lower_Radius", ScaleFac * Real(12.0)));
+        lp.push_back(Parameter("Leg_Upper_Radius", ScaleFac * Real(6.0)));
+        lp.push_back(Parameter("Draft", ScaleFac * Real(20.0)));
+        lp.push_back(Parameter("Heave_Plate_Height", ScaleFac * Real(6.0)));
+        for (auto* l : legs_) l->Set_Parameters(lp);
+        R_Leg = 28.87 * ScaleFac;
+    }
+protected:
+    void Set_Kin_Params() override {
+        Real SF3 = ScaleFac * ScaleFac * ScaleFac, SF5 = SF3 * ScaleFac * ScaleFac;
+        Centre_Gravity = Vector3(0, 0, -13.46 * ScaleFac);
+        Mass = 1.34730e+07 * SF3;
+        Mass_Mat = Matrix::Zero(6, 6);
+        Stiffness_Mat = Matrix::Zero(6, 6);
+        for (int i = 0; i < 3; i++) Mass_Mat(i, i) = Mass;
+        Mass_Mat(3, 3) = 6.82700e+09 * SF5; Mass_Mat(4, 4) = 6.82700e+09 * SF5; Mass_Mat(5, 5) = 1.22600e+10 * SF5;
+        Stiffness_Mat(2, 2) = 3.836e+06 * SF3;
+        Stiffness_Mat(3, 3) = -3.776e+08 * SF5; Stiffness_Mat(4, 4) = -3.776e+08 * SF5;
+    }
+    Real ScaleFac = 1.0;
+};
+
+// Barge (Barge.cpp:12-204): open-top box of Tank_Length x Tank_Width x
+// Tank_Depth, cosine spacing by default (Cosine_Disc, Boundary_Base.h:86); its
+// lid always exists; exterior points: the (NXFS+1) x (NYFS+1) grid over
+// FreeSurface_Length x FreeSurface_Width, dilated by 1.001 (points inside the
+// waterplane included, as there; none without the two panel counts).
 class Barge : public Boundary {
 public:
     void Set_Parameters(std::vector<Parameter>& Params) override {
@@ -357,24 +600,196 @@ public:
             if (P.myNameis("Tank_Length")) L = P.Get_Param<Real>();
             if (P.myNameis("Tank_Width")) W = P.Get_Param<Real>();
             if (P.myNameis("Tank_Depth")) D = P.Get_Param<Real>();
+            if (P.myNameis("FreeSurface_Length")) LFS = P.Get_Param<Real>();
+            if (P.myNameis("FreeSurface_Width")) WFS = P.Get_Param<Real>();
             if (P.myNameis("NPanels_Tank_Length")) NX = P.Get_Param<int>();
             if (P.myNameis("NPanels_Tank_Width")) NY = P.Get_Param<int>();
             if (P.myNameis("NPanels_Tank_Depth")) NZ = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FreeSurface_Length")) NXFS = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FreeSurface_Width")) NYFS = P.Get_Param<int>();
+            if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
         }
     }
     void Setup() override {
         bem_mesh* m = nullptr;
-        int rc = bem_mesh_box(L, W, -D, 0.0, NX, NY, NZ, 1, &m);
+        int rc = bem_mesh_barge(L, W, D, NX, NY, NZ, Cosine ? 1 : 0, &m);
         Adopt(rc, m);
         // interior lid (Barge.cpp:103-123, 193-204; y from its own index)
         bem_mesh* l = nullptr;
-        int rl = bem_mesh_lid_rect(L, W, NX, NY, 0, &l);
+        int rl = bem_mesh_lid_rect(L, W, NX, NY, Cosine ? 1 : 0, &l);
         Adopt_Lid(rl, l);
-        Report();
+        FreeSurface_Nodes.clear();
+        for (int i = 0; NXFS > 0 && NYFS > 0 && i <= NXFS; i++)
+            for (int j = 0; j <= NYFS; j++)
+                FreeSurface_Nodes.push_back(Vector3(1.001 * (-0.5 * LFS + LFS * i / NXFS),
+                                                    1.001 * (-0.5 * WFS + WFS * j / NYFS), 0.0));
+        Finish();
     }
 private:
-    Real L = 20, W = 10, D = 5;
-    int NX = 20, NY = 10, NZ = 5;
+    Real L = 20, W = 10, D = 5, LFS = 0, WFS = 0;
+    int NX = 20, NY = 10, NZ = 5, NXFS = 0, NYFS = 0;
+};
+
+// Ellipsoid (Ellipsoid.cpp:12-161): a submerged ellipsoid of semi-axes a, b, c
+// centred Ellipsoid_Depth below the free surface; no lid, no exterior points.
+class Ellipsoid : public Boundary {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Semiaxis_a")) a = P.Get_Param<Real>();
+            if (P.myNameis("Semiaxis_b")) b = P.Get_Param<Real>();
+            if (P.myNameis("Semiaxis_c")) c = P.Get_Param<Real>();
+            if (P.myNameis("Ellipsoid_Depth")) Depth = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_Axial")) NZ = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Azimuthal")) NA = P.Get_Param<int>();
+            if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
+            if (P.myNameis("Triangular_Panels")) TriPanels = P.Get_Param<bool>();
+        }
+    }
+    void Setup() override {
+        bem_mesh* m = nullptr;
+        int rc = bem_mesh_ellipsoid(a, b, c, Depth, NZ, NA, Cosine ? 1 : 0, TriPanels ? 1 : 0, &m);
+        Adopt(rc, m);
+        Finish();
+    }
+private:
+    Real a = 1, b = 1, c = 1, Depth = 2;
+    int NZ = 16, NA = 32;
+    bool TriPanels = false;
+};
+
+// Semi_Ellipsoid (Ellipsoid.cpp:167-461): the lower half of an ellipsoid
+// piercing the free surface (a hemisphere for a = b = c); elliptic lid of
+// NPanels_FreeSurface_Int_Radial rings (none with triangular panels, as
+// there); exterior rings at f = 1.001 .. RFS times the waterline ellipse
+// (RFS <= 1 becomes 2, as there; no rings without NPanels_FreeSurface_Radial).
+class Semi_Ellipsoid : public Boundary {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Semiaxis_a")) a = P.Get_Param<Real>();
+            if (P.myNameis("Semiaxis_b")) b = P.Get_Param<Real>();
+            if (P.myNameis("Semiaxis_c")) c = P.Get_Param<Real>();
+            if (P.myNameis("FreeSurface_Radius")) RFS = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_Axial")) NZ = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Azimuthal")) NA = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FreeSurface_Radial")) NRES = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FreeSurface_Int_Radial")) NRFS = P.Get_Param<int>();
+            if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
+            if (P.myNameis("Triangular_Panels")) TriPanels = P.Get_Param<bool>();
+        }
+        if (RFS <= 1.0) RFS = 2.;
+    }
+    void Setup() override {
+        bem_mesh* m = nullptr;
+        int rc = bem_mesh_semi_ellipsoid(a, b, c, NZ, NA, Cosine ? 1 : 0, TriPanels ? 1 : 0, &m);
+        Adopt(rc, m);
+        if (!TriPanels && NRFS > 0) {
+            bem_mesh* l = nullptr;
+            int rl = bem_mesh_lid_ellipse(a, b, NRFS, NA, Cosine ? 1 : 0, &l);
+            Adopt_Lid(rl, l);
+        }
+        FreeSurface_Nodes.clear();
+        const Real FShift = TriPanels ? 0.0 : 0.001;
+        for (int i = 0; NRES > 0 && i <= NRES; i++) {
+            Real f = Cosine ? 1.0 + FShift + (RFS - 1.0) * 0.5 * (1.0 - std::cos(i * 3.14159265358979323846 / NRES))
+                            : 1.0 + FShift + (RFS - 1.0) * i * 1.0 / NRES;
+            for (int j = 0; j < NA; j++) {
+                Real u = j * TwoPI / NA;
+                if (TriPanels && i % 2 == 0) u += 0.5 * TwoPI / NA;
+                FreeSurface_Nodes.push_back(Vector3(a * f * std::cos(u), b * f * std::sin(u), 0.0));
+            }
+        }
+        Finish();
+    }
+private:
+    Real a = 1, b = 1, c = 1, RFS = 2;
+    int NZ = 16, NA = 32, NRES = 0, NRFS = 0;
+    bool TriPanels = false;
+};
+
+// Wigley_Hull (Ship_Hulls.cpp:95-360): y = 2 B x'(1 - x')(1 - z'^2) for
+// x' = (x + L/2)/L in [0, 1], z' = z/T in [-1, 0]; exterior points on the port
+// side over three zones (aft, beside, fore of the hull; :150-330). The
+// reference's lid covers only the port half of the waterplane and collapses
+// its bow and stern columns to single points (zero-length panel edges); it is
+// not built here, so "IFR" solves a Wigley hull without a lid.
+class Wigley_Hull : public Boundary {
+public:
+    void Set_Parameters(std::vector<Parameter>& Params) override {
+        for (const Parameter& P : Params) {
+            if (P.myNameis("Length")) L = P.Get_Param<Real>();
+            if (P.myNameis("Beam")) B = P.Get_Param<Real>();
+            if (P.myNameis("Draft")) T = P.Get_Param<Real>();
+            if (P.myNameis("NPanels_Length")) NX = P.Get_Param<int>();
+            if (P.myNameis("NPanels_Depth")) NZ = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FS_Length_Bow")) NXS = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FS_Length_Stern")) NXK = P.Get_Param<int>();
+            if (P.myNameis("NPanels_FS_Width")) NYFS = P.Get_Param<int>();
+            if (P.myNameis("Cosine_Disc")) Cosine = P.Get_Param<bool>();
+            if (P.myNameis("Triangular_Panels")) TriPanels = P.Get_Param<bool>();
+            if (P.myNameis("Panels_Outward")) Outward = P.Get_Param<bool>();
+            if (P.myNameis("BowGridFactor")) XFSMax = P.Get_Param<Real>() * L;
+            if (P.myNameis("SternGridFactor")) XFSMin = -P.Get_Param<Real>() * L;
+            if (P.myNameis("PortGridFactor")) YFS = P.Get_Param<Real>() * L;
+        }
+        if (NXS == 0) NXS = NX;
+        if (NXK == 0) NXK = 2 * NX;
+        if (NYFS == 0) NYFS = NZ;
+        if (XFSMax == 0) XFSMax = L;
+        if (XFSMin == 0) XFSMin = -2.0 * L;
+        if (YFS == 0) YFS = L;
+    }
+    void Setup() override {
+        bem_mesh* m = nullptr;
+        int rc = bem_mesh_wigley(L, B, T, NX, NZ, Cosine ? 1 : 0, TriPanels ? 1 : 0, Outward ? 1 : 0, &m);
+        Adopt(rc, m);
+        Generate_FreeSurface_Nodes();
+        Finish();
+    }
+private:
+    static Real pcf(int i, int n) { return 0.5 * (1.0 - std::cos(i * 3.14159265358979323846 / (n - 1))); }
+    static Real hpcf(int i, int n) { return -std::cos(0.5 * 3.14159265358979323846 + 0.5 * i * 3.14159265358979323846 / (n - 1)); }
+    static Real hpcf2(int i, int n) { return 1.0 - std::cos(0.5 * i * 3.14159265358979323846 / (n - 1)); }
+    Vector3 FS_Node(Real xn, Real yn, int zone) const {
+        if (zone == 0) return Vector3(XFSMin + (-0.5 * L - XFSMin) * xn, YFS * yn, 0);
+        if (zone == 1) {
+            Real ymin = 2.0 * B * xn * (1.0 - xn);
+            return Vector3(-0.5 * L + L * xn, ymin + (YFS - ymin) * yn, 0);
+        }
+        return Vector3(0.5 * L + (XFSMax - 0.5 * L) * xn, YFS * yn, 0);
+    }
+    void Generate_FreeSurface_Nodes() {
+        FreeSurface_Nodes.clear();
+        std::vector<Real> X[3], Y[3];
+        for (int i = 0; i < NXK; i++) X[0].push_back(Cosine ? hpcf(i, NXK + 1) : 1.0 * i / NXK);
+        for (int i = 0; i < NX + 1; i++) X[1].push_back(Cosine ? pcf(i, NX + 1) : 1.0 * i / NX);
+        for (int i = 1; i < NXS + 1; i++) X[2].push_back(Cosine ? hpcf2(i, NXS + 1) : 1.0 * i / NXS);
+        for (int j = 0; j < NYFS + 1; j++) {
+            Real fy = Cosine ? hpcf2(j, NYFS + 1) : 1.0 * j / NYFS;
+            Y[0].push_back(fy); Y[1].push_back(j == 0 ? 0.001 : fy); Y[2].push_back(fy);
+        }
+        for (int z = 0; z < 3; z++)
+            for (Real x : X[z])
+                for (Real y : Y[z]) FreeSurface_Nodes.push_back(FS_Node(x, y, z));
+        if (!TriPanels) return;
+        // cell centres of the same zones (Ship_Hulls.cpp:307-355)
+        const Real PI = 3.14159265358979323846;
+        for (int z = 0; z < 3; z++) { X[z].clear(); Y[z].clear(); }
+        for (int i = 0; i < NXK; i++) X[0].push_back(Cosine ? -std::cos(0.5 * PI + 0.5 * (i + 0.5) * PI / NXK) : (i + 0.5) / NXK);
+        for (int i = 0; i < NX; i++) X[1].push_back(Cosine ? 0.5 * (1.0 - std::cos((i + 0.5) * PI / NX)) : (i + 0.5) / NX);
+        for (int i = 0; i < NXS; i++) X[2].push_back(Cosine ? 1.0 - std::cos(0.5 * (i + 0.5) * PI / NXS) : (i + 0.5) / NXS);
+        for (int j = 0; j < NYFS; j++) {
+            Real fy = Cosine ? 1.0 - std::cos(0.5 * (j + 0.5) * PI / NYFS) : (j + 0.5) / NYFS;
+            for (int z = 0; z < 3; z++) Y[z].push_back(fy);
+        }
+        for (int z = 0; z < 3; z++)
+            for (Real x : X[z])
+                for (Real y : Y[z]) FreeSurface_Nodes.push_back(FS_Node(x, y, z));
+    }
+    Real L = 10, B = 1, T = 1, XFSMax = 0, XFSMin = 0, YFS = 0;
+    int NX = 20, NZ = 8, NXK = 0, NXS = 0, NYFS = 0;
+    bool TriPanels = false, Outward = false;
 };
 
 // Three columns joined by closed pontoons (the 6k-panel benchmark body).
@@ -407,7 +822,7 @@ public:
             if (r1 != BEM_OK && all) { bem_mesh_free(all); all = nullptr; }
             Adopt_Lid(r1, all);
         }
-        Report();
+        Finish();
     }
 private:
     Real Rc = 6, D = 20, S = 50, PW = 6, PH = 4;

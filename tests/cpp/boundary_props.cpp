@@ -34,6 +34,65 @@ int main() {
     lidded.Set_Parameters(cp);
     lidded.Setup();
     std::printf("lidded_aux %d\n", lidded.N_Aux_Panels());
+    // reference parametric bodies (discretisations kept small)
+    OC3_SparBuoy oc3;
+    std::vector<Parameter> o3 = {Parameter("NPanels_Radial", 4), Parameter("NPanels_Vertical_Section1", 12),
+                                 Parameter("NPanels_Vertical_Section2", 3), Parameter("NPanels_Vertical_Section3", 4),
+                                 Parameter("NPanels_Azimuthal", 16), Parameter("NPanels_FreeSurface_Int_Radial", 2)};
+    oc3.Set_Parameters(o3);
+    oc3.Setup();
+    dump("oc3", oc3);
+    std::printf("oc3_aux %d\noc3_mass %.17g\noc3_m33 %.17g\noc3_c33 %.17g\n", oc3.N_Aux_Panels(), oc3.Mass,
+                oc3.Mass_Mat(3, 3), oc3.Stiffness_Mat(2, 2));
+    SemiSub_OC4 oc4;
+    std::vector<Parameter> o4 = {Parameter("NPanels_Azimuthal", 24), Parameter("NPanels_Radial", 3),
+                                 Parameter("NPanels_Vertical", 6), Parameter("NPanels_Radial_Base", 3),
+                                 Parameter("NPanels_Vertical_HeavePlate", 2), Parameter("NPanels_Horizontal_HeavePlate", 2),
+                                 Parameter("NPanels_FreeSurface_Int_Radial", 2), Parameter("NPanels_FreeSurface_Radial", 2),
+                                 Parameter("FreeSurface_Radius", 1.0), Parameter("Scaling_Factor", 2.0)};
+    oc4.Set_Parameters(o4);
+    oc4.Setup();
+    dump("oc4", oc4);
+    std::printf("oc4_n %d\noc4_aux %d\noc4_fs %d\noc4_mass %.17g\noc4_c33 %.17g\n", oc4.N_Elements(),
+                oc4.N_Aux_Panels(), (int)oc4.FreeSurface_Nodes.size(), oc4.Mass, oc4.Stiffness_Mat(2, 2));
+    Triple_Spar ts;
+    std::vector<Parameter> tp = {Parameter("Radius", 2.0), Parameter("Draft", 6.0), Parameter("NPanels_Radial", 2),
+                                 Parameter("NPanels_Vertical", 3), Parameter("NPanels_Azimuthal", 12),
+                                 Parameter("Spar_Radius", 10.0), Parameter("Shift_Over_Leg", true)};
+    ts.Set_Parameters(tp);
+    ts.Setup();
+    dump("ts", ts);
+    Semi_Ellipsoid se;
+    std::vector<Parameter> sp = {Parameter("Semiaxis_a", 3.0), Parameter("Semiaxis_b", 2.0), Parameter("Semiaxis_c", 1.5),
+                                 Parameter("NPanels_Axial", 6), Parameter("NPanels_Azimuthal", 12),
+                                 Parameter("NPanels_FreeSurface_Int_Radial", 2), Parameter("NPanels_FreeSurface_Radial", 3)};
+    se.Set_Parameters(sp);
+    se.Setup();
+    dump("se", se);
+    std::printf("se_aux %d\nse_fs %d\n", se.N_Aux_Panels(), (int)se.FreeSurface_Nodes.size());
+    Ellipsoid el;
+    std::vector<Parameter> ep = {Parameter("Semiaxis_a", 3.0), Parameter("Semiaxis_b", 2.0), Parameter("Semiaxis_c", 1.5),
+                                 Parameter("Ellipsoid_Depth", 5.0), Parameter("NPanels_Axial", 6),
+                                 Parameter("NPanels_Azimuthal", 12)};
+    el.Set_Parameters(ep);
+    el.Setup();
+    dump("el", el);
+    Wigley_Hull wh;
+    std::vector<Parameter> wp = {Parameter("Length", 10.0), Parameter("Beam", 1.0), Parameter("Draft", 0.625),
+                                 Parameter("NPanels_Length", 20), Parameter("NPanels_Depth", 8)};
+    wh.Set_Parameters(wp);
+    wh.Setup();
+    dump("wh", wh);
+    std::printf("wh_n %d\nwh_fs %d\nwh_aux %d\n", wh.N_Elements(), (int)wh.FreeSurface_Nodes.size(), wh.N_Aux_Panels());
+    Barge bf;
+    std::vector<Parameter> bfp = {Parameter("Tank_Length", 20.0), Parameter("Tank_Width", 10.0),
+                                  Parameter("Tank_Depth", 5.0), Parameter("NPanels_Tank_Length", 10),
+                                  Parameter("NPanels_Tank_Width", 5), Parameter("NPanels_Tank_Depth", 3),
+                                  Parameter("FreeSurface_Length", 40.0), Parameter("FreeSurface_Width", 20.0),
+                                  Parameter("NPanels_FreeSurface_Length", 8), Parameter("NPanels_FreeSurface_Width", 4)};
+    bf.Set_Parameters(bfp);
+    bf.Setup();
+    std::printf("bf_fs %d\nbf_fs_x0 %.17g\n", (int)bf.FreeSurface_Nodes.size(), bf.FreeSurface_Nodes[0].x);
     std::printf("field_pos [%s]\nfield_neg [%s]\nfield_small [%s]\nfield_txt [%s]\n", Field(1.5).c_str(),
                 Field(-2.25).c_str(), Field(1e-7).c_str(), Field(std::string("Surge: "), 30).c_str());
     return 0;

@@ -5,12 +5,14 @@
 // result for the test to compare with the oracle:
 //   SETUP A <36>        A after Setup (Frequency = FreqInf)
 //   A <omega> <36>      B <omega> <36>      F <omega> <12 nb>
-// Usage: hydro_dropin <mode: 0 FP64 | 1 mixed> [ifr: 0 | 1]
-// With ifr = 1 the cylinder gets a 3-ring interior lid
-// (NPanels_FreeSurface_Int_Radial) and the solver the "IFR" key, and the
-// frequencies are 1.0 and 2.2 (the irregular frequency of this cylinder).
+// Usage: hydro_dropin <mode: 0 FP64 | 1 mixed> [ifr: 0 | 1] [body: cyl | oc4]
+// With ifr = 1 the body gets interior lids (NPanels_FreeSurface_Int_Radial)
+// and the solver the "IFR" key; the cylinder's frequencies are then 1.0 and
+// 2.2 (its irregular frequency). body oc4 is the OC4 semi-submersible
+// (SemiSub_OC4) at a coarse discretisation, solved at 0.5 and 1.2.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "bemuse_hydro.h"
@@ -27,11 +29,16 @@ static void print_real(const char* tag, double w, const Matrix& M, bool with_w) 
 int main(int argc, char** argv) {
     int mode = argc > 1 ? std::atoi(argv[1]) : BEM_MODE_FP64;
     bool ifr = argc > 2 && std::atoi(argv[2]) != 0;
-    Half_Cylinder* B = new Half_Cylinder;
+    bool oc4 = argc > 3 && std::string(argv[3]) == "oc4";
+    Boundary* B = oc4 ? static_cast<Boundary*>(new SemiSub_OC4) : static_cast<Boundary*>(new Half_Cylinder);
     std::vector<Parameter> bp = {Parameter("Radius", 5.0), Parameter("Draft", 8.0), Parameter("NPanels_Radial", 4),
                                  Parameter("NPanels_Vertical", 8), Parameter("NPanels_Azimuthal", 16),
                                  Parameter("Cosine_Disc", true)};
-    if (ifr) bp.push_back(Parameter("NPanels_FreeSurface_Int_Radial", 3));
+    if (oc4)
+        bp = {Parameter("NPanels_Azimuthal", 16), Parameter("NPanels_Radial", 2), Parameter("NPanels_Vertical", 4),
+              Parameter("NPanels_Radial_Base", 2), Parameter("NPanels_Vertical_HeavePlate", 1),
+              Parameter("NPanels_Horizontal_HeavePlate", 2)};
+    if (ifr) bp.push_back(Parameter("NPanels_FreeSurface_Int_Radial", oc4 ? 2 : 3));
     B->Set_Parameters(bp);
     B->Setup();
     Hydrodynamic_Radiation_Solver* S = new Hydrodynamic_Radiation_Solver(0, mode);
@@ -43,7 +50,8 @@ int main(int argc, char** argv) {
     S->Setup(B);
     std::printf("N %d\n", S->N_Panels());
     print_real("SETUP_A", 0, S->Added_Mass(), false);
-    std::vector<double> ws = ifr ? std::vector<double>{1.0, 2.2} : std::vector<double>{0.4, 1.3};
+    std::vector<double> ws = oc4 ? std::vector<double>{0.5, 1.2}
+                                 : (ifr ? std::vector<double>{1.0, 2.2} : std::vector<double>{0.4, 1.3});
     for (double w : ws) {
         S->Set_Real(w * 2.0 * 3.14159265358979323846);   // the console passes F = 2 pi omega
         S->Solve();

@@ -158,6 +158,25 @@ def test_dropin_boundary_properties_closed_forms():
     # (NX x NY quads), the cylinder only with NPanels_FreeSurface_Int_Radial
     # (3 rings of 16: a fan of triangles and two rings of quads)
     assert int(v["box_aux"]) == 10 * 5 and int(v["cyl_aux"]) == 0 and int(v["lidded_aux"]) == 48
+    # the reference's parametric bodies through the drop-in (FOWT_Platforms.h, Triple_Spar.cpp,
+    # Ellipsoid.cpp, Ship_Hulls.cpp, Barge.cpp); k(NA) = (NA/2) sin(2 pi/NA) is the NA-gon factor
+    k = lambda na: 0.5 * na * np.sin(2 * np.pi / na)
+    v_oc3 = k(16) * (4.7 ** 2 * 108 + 8 * (4.7 ** 2 + 4.7 * 3.25 + 3.25 ** 2) / 3 + 3.25 ** 2 * 4)
+    assert abs(f["oc3_volume"] - v_oc3) < 1e-9 * v_oc3 and abs(f["oc3_wp_area"] - k(16) * 3.25 ** 2) < 1e-10
+    assert int(v["oc3_aux"]) == 2 * 16 and f["oc3_mass"] == 7.46633e6 and f["oc3_c33"] == 3.32941e5
+    assert f["oc3_m33"] == 4.22923e9
+    v_oc4 = k(24) * (3.25 ** 2 * 20 + 3 * (12 ** 2 * 6 + 6 ** 2 * 14))     # Scaling_Factor is ignored, as there
+    assert abs(f["oc4_volume"] - v_oc4) < 1e-9 * v_oc4
+    assert abs(f["oc4_wp_area"] - k(24) * (3.25 ** 2 + 3 * 36)) < 1e-9
+    assert int(v["oc4_aux"]) == 4 * 2 * 24 and int(v["oc4_fs"]) == 4 * 3 * 24
+    assert f["oc4_mass"] == 1.3473e7 and f["oc4_c33"] == 3.836e6
+    # legs at 10 m, shifted by +10 m along x: waterplane centroid at x = 10
+    assert abs(f["ts_volume"] - 3 * k(12) * 4 * 6) < 1e-9 and abs(f["ts_wp_mx"] / f["ts_wp_area"] - 10) < 1e-12
+    assert abs(f["se_wp_area"] - k(12) * 6) < 1e-12 and int(v["se_aux"]) == 24 and int(v["se_fs"]) == 48
+    assert f["el_wp_area"] == 0 and f["el_volume"] > 0 and abs(f["el_cob_z"] + 5) < 0.25
+    assert int(v["wh_n"]) == 2 * 20 * 8 and int(v["wh_fs"]) == (40 + 21 + 20) * 9 and int(v["wh_aux"]) == 0
+    assert abs(f["wh_volume"] / (4 * 1.0 * 10 * 0.625 / 9) - 1) < 0.02
+    assert int(v["bf_fs"]) == 9 * 5 and abs(f["bf_fs_x0"] + 1.001 * 20) < 1e-12
     assert v["field_pos"] == "[ 1.5           ]"
     assert v["field_neg"] == "[-2.25          ]"
     assert v["field_small"] == "[ 1e-07         ]"

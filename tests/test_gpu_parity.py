@@ -204,13 +204,13 @@ def test_solve_vs_committed_fixture(ctx):
     body.close()
 
 
-def _run_dropin(mode, cwd=None, ifr=False):
+def _run_dropin(mode, cwd=None, ifr=False, body="cyl"):
     import os
     import subprocess
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "hydro_dropin")
     assert os.path.exists(exe), "tests/cpp/hydro_dropin is not built (make -C tests/cpp)"
-    out = subprocess.run([exe, str(mode), str(int(ifr))], capture_output=True, text=True, timeout=300, check=True,
-                         cwd=cwd).stdout
+    out = subprocess.run([exe, str(mode), str(int(ifr)), body], capture_output=True, text=True, timeout=300,
+                         check=True, cwd=cwd).stdout
     res = {"stdout": out}
     for line in out.splitlines():
         tok = line.split()
@@ -310,6 +310,37 @@ def test_cpp_dropin_ifr(mode, tol, tmp_path):
         for key in ("A", "B", "F"):
             assert relerr(res[w][key], r[key]) < tol, (w, key, relerr(res[w][key], r[key]))
         assert r["B"][2, 2] > 0.0
+
+
+def _oc4_oracle_body(n_lid_rings):
+    """The drop-in's SemiSub_OC4 at the coarse discretisation of tests/cpp/hydro_dropin.cpp, composed
+    from the library's generators: central half cylinder, then three spar legs at 28.87 m, azimuths
+    -90, 30, 150 degrees from +y (Triple_Spar.cpp:52-58, 218-225); lids in the same order."""
+    th = -0.5 * np.pi + 2 * np.pi / 3 * np.arange(3)
+    centres = [(0.0, 0.0)] + [(28.87 * np.sin(t), 28.87 * np.cos(t)) for t in th]
+    parts = [bm.Mesh.half_cylinder(3.25, 20.0, 2, 4, 16)] + [bm.Mesh.spar_leg(12.0, 6.0, 20.0, 6.0, 2, 1, 2, 4, 16)] * 3
+    lids = [bm.Mesh.lid_disc(3.25, n_lid_rings, 16)] + [bm.Mesh.lid_disc(6.0, n_lid_rings, 16)] * 3
+    nodes, elems, base = [], [], 0
+    for group in (parts, lids):
+        for m, (cx, cy) in zip(group, centres):
+            n, e = m.arrays()
+            nodes.append(n + np.array([cx, cy, 0.0]))
+            elems.append(np.where(e >= 0, e + base, -1))
+            base += len(n)
+    n_lid = sum(len(m.arrays()[1]) for m in lids)
+    return oracle.Body(np.vstack(nodes), np.vstack(elems), n_lid=n_lid), n_lid
+
+
+@pytest.mark.parametrize("mode,tol", [(bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)])
+def test_cpp_dropin_oc4_semisub_with_lids(mode, tol, tmp_path):
+    """SemiSub_OC4 with "IFR" through the drop-in against the oracle on the same composite mesh."""
+    res = _run_dropin(mode, cwd=str(tmp_path), ifr=True, body="oc4")
+    ob, n_lid = _oc4_oracle_body(2)
+    assert res["N"] == ob.N, res["stdout"]
+    for w in (0.5, 1.2):
+        r = ob.solve(w, betas=[0.0, np.pi / 4])
+        for key in ("A", "B", "F"):
+            assert relerr(res[w][key], r[key]) < tol, (w, key, relerr(res[w][key], r[key]))
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
