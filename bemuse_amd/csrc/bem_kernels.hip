@@ -350,162 +350,6 @@ __device__ void diag_inverses(const T (*lur)[LU_NB + 1], const T (*lui)[LU_NB + 
     }
 }
 
-// Factor the NB x NB diagonal block at (k0, k0) without interchanges. Each of
-// the 256 threads keeps a 4 x 4 tile (rows 4tr.., columns 4tc..) of A and of
-// L^-1 in registers; step j broadcasts row j and column j through LDS (double
-// buffered, one barrier per step), every thread forms the multipliers of its
-// rows and applies the rank-1 update to its A columns right of j and to its
-// L^-1 tile, since L^-1 = E_63 ... E_0 with E_j the elementary elimination of
-// step j (row j of the partial L^-1 is zero right of j). U^-1 follows from a
-// backward sweep of the same shape (scale row k of the identity by 1/u_kk,
-// subtract u_mk times it from rows m < k). The updates are masked
-// arithmetically (no divergent per-element branches), and the LDS reads of a
-// step issue together. Saves the block,
-// writes L\U, L^-1 and U^-1, and sets flag[panel] bit 1 if a multiplier
-// exceeds 1 in modulus (1|2 on a zero pivot).
-template <typename T>
-__device__ inline T pick4(const T (&v)[4], int k) {
-    return k == 0 ? v[0] : k == 1 ? v[1] : k == 2 ? v[2] : v[3];
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
-                                                 T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
-    constexpr int LP = LU_NB + 4;                                // 16-byte aligned rows
-    __shared__ __align__(16) T lur[LU_NB][LP], lui[LU_NB][LP];   // final L\U [col][row]
-    __shared__ __align__(16) T rowr[2][2][LU_NB], rowi[2][2][LU_NB];   // [buf][A or X][col]
-    __shared__ __align__(16) T colr[2][LU_NB], coli[2][LU_NB];         // [buf][row]
-    __shared__ int bad;
-    int tid = threadIdx.x;
-    int tr = tid & 15, tc = tid >> 4;
-    int rb = 4 * tr, cb = 4 * tc;
-    if (tid == 0) bad = 0;
-    T ar[4][4], ai[4][4], xr[4][4], xi[4][4];
-#pragma unroll
-    for (int cc = 0; cc < 4; cc++)
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            size_t g = (size_t)(k0 + cb + cc) * Np + k0 + rb + rr;
-            ar[rr][cc] = Ar[g]; ai[rr][cc] = Ai[g];
-            save_r[(cb + cc) * LU_NB + rb + rr] = ar[rr][cc];
-            save_i[(cb + cc) * LU_NB + rb + rr] = ai[rr][cc];
-            xr[rr][cc] = (rb + rr == cb + cc) ? (T)1 : (T)0; xi[rr][cc] = 0;
-        }
-    int mybad = 0;
-    for (int j = 0; j < LU_NB; j++) {
-        int buf = j & 1, jt = j >> 2, jj = j & 3;
-        if (tr == jt) {            // row j of A and of the partial L^-1
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                T c0r[4] = {ar[0][cc], ar[1][cc], ar[2][cc], ar[3][cc]}, c0i[4] = {ai[0][cc], ai[1][cc], ai[2][cc], ai[3][cc]};
-                T c1r[4] = {xr[0][cc], xr[1][cc], xr[2][cc], xr[3][cc]}, c1i[4] = {xi[0][cc], xi[1][cc], xi[2][cc], xi[3][cc]};
-                rowr[buf][0][cb + cc] = pick4(c0r, jj); rowi[buf][0][cb + cc] = pick4(c0i, jj);
-                rowr[buf][1][cb + cc] = pick4(c1r, jj); rowi[buf][1][cb + cc] = pick4(c1i, jj);
-            }
-        }
-        if (tc == jt) {            // column j of A
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                colr[buf][rb + rr] = pick4(ar[rr], jj);
-                coli[buf][rb + rr] = pick4(ai[rr], jj);
-            }
-        }
-        __syncthreads();
-        T pr = colr[buf][j], pi = coli[buf][j];
-        T den = cabs2(pr, pi);
-        T inv = (T)1 / den;
-        T qr = pr * inv, qi = -pi * inv;
-        if (tid == 0 && den == (T)0) mybad |= 3;
-        T cr4[4], ci4[4], ur[4], ui[4], vr[4], vi[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            cr4[k] = colr[buf][rb + k]; ci4[k] = coli[buf][rb + k];
-            ur[k] = rowr[buf][0][cb + k]; ui[k] = rowi[buf][0][cb + k];
-            vr[k] = rowr[buf][1][cb + k]; vi[k] = rowi[buf][1][cb + k];
-        }
-        T lr[4], li[4];
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            bool below = rb + rr > j;
-            T l_r = cr4[rr] * qr - ci4[rr] * qi, l_i = cr4[rr] * qi + ci4[rr] * qr;
-            if (below && cabs2(l_r, l_i) > (T)1) mybad |= 1;
-            lr[rr] = below ? l_r : (T)0;
-            li[rr] = below ? l_i : (T)0;
-        }
-#pragma unroll
-        for (int cc = 0; cc < 4; cc++) {
-            bool right = cb + cc > j, here = cb + cc == j;
-            T u_r = right ? ur[cc] : (T)0, u_i = right ? ui[cc] : (T)0;
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                T nr_ = ar[rr][cc] - (lr[rr] * u_r - li[rr] * u_i);
-                T ni_ = ai[rr][cc] - (lr[rr] * u_i + li[rr] * u_r);
-                bool setl = here && (rb + rr > j);
-                ar[rr][cc] = setl ? lr[rr] : nr_;
-                ai[rr][cc] = setl ? li[rr] : ni_;
-                xr[rr][cc] -= lr[rr] * vr[cc] - li[rr] * vi[cc];
-                xi[rr][cc] -= lr[rr] * vi[cc] + li[rr] * vr[cc];
-            }
-        }
-    }
-    if (mybad) atomicOr(&bad, mybad);
-#pragma unroll
-    for (int cc = 0; cc < 4; cc++)
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            size_t g = (size_t)(k0 + cb + cc) * Np + k0 + rb + rr;
-            Ar[g] = ar[rr][cc]; Ai[g] = ai[rr][cc];
-            lur[cb + cc][rb + rr] = ar[rr][cc]; lui[cb + cc][rb + rr] = ai[rr][cc];
-            Li_r[(cb + cc) * LU_NB + rb + rr] = xr[rr][cc]; Li_i[(cb + cc) * LU_NB + rb + rr] = xi[rr][cc];
-            xr[rr][cc] = (rb + rr == cb + cc) ? (T)1 : (T)0; xi[rr][cc] = 0;
-        }
-    __syncthreads();
-    // U^-1: backward sweep on the identity
-    for (int k = LU_NB - 1; k >= 0; k--) {
-        int buf = k & 1, kt = k >> 2, kk = k & 3;
-        if (tr == kt) {
-            T dr = lur[k][k], di = lui[k][k];
-            T inv = (T)1 / cabs2(dr, di);
-            T qr = dr * inv, qi = -di * inv;
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    if (rr == kk) {
-                        T yr = xr[rr][cc] * qr - xi[rr][cc] * qi, yi = xr[rr][cc] * qi + xi[rr][cc] * qr;
-                        xr[rr][cc] = yr; xi[rr][cc] = yi;
-                        rowr[buf][0][cb + cc] = yr; rowi[buf][0][cb + cc] = yi;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        T yr[4], yi[4], u_r[4], u_i[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            yr[q] = rowr[buf][0][cb + q]; yi[q] = rowi[buf][0][cb + q];
-            bool above = rb + q < k;
-            u_r[q] = above ? lur[k][rb + q] : (T)0;
-            u_i[q] = above ? lui[k][rb + q] : (T)0;
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                xr[rr][cc] -= u_r[rr] * yr[cc] - u_i[rr] * yi[cc];
-                xi[rr][cc] -= u_r[rr] * yi[cc] + u_i[rr] * yr[cc];
-            }
-    }
-#pragma unroll
-    for (int cc = 0; cc < 4; cc++)
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            Ui_r[(cb + cc) * LU_NB + rb + rr] = xr[rr][cc]; Ui_i[(cb + cc) * LU_NB + rb + rr] = xi[rr][cc];
-        }
-    __syncthreads();
-    if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
-}
-
 // ---------------------------------------------------------------------------
 // MFMA 16x16x4 (lane l = i + 16 g): A supplies A[i][k(g)], B supplies
 // B[k(g)][j]. Accumulator register r of lane l holds D[row][j] with
@@ -545,6 +389,267 @@ __device__ inline void cmma16(const T* ar_, const T* ai_, size_t lda, const T* b
         ci = MT::mma(a_r, bi[t], ci);
         ci = MT::mma(a_i, br[t], ci);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Factor the NB x NB diagonal block at (k0, k0) without interchanges and form
+// the inverses of its unit-lower and upper triangles, blocked by DB = 16 so
+// that most of the work is 16 x 16 MFMA block products and the work-group
+// meets at 20 barriers (the unblocked form needs one per column, 128 in all).
+// The block lives in LDS, [col][row]. For sub-block column kb = 0..3:
+//   wave 0 factors the 16 x 16 diagonal block (wave-synchronous, one column
+//   per step); waves 0 and 1 form its triangle inverses Li_kk, Ui_kk;
+//   the waves share L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb),
+//   then the trailing blocks A_ij -= L_ik U_kj.
+// The inverses of the whole triangles follow block diagonal by block
+// diagonal: X_ij = -Li_ii sum_{k=j}^{i-1} L_ik X_kj (L^-1), and
+// Y_ij = -Ui_ii sum_{k=i+1}^{j} U_ik Y_kj (U^-1), the blocks of earlier
+// diagonals read back from the outputs. Saves the block, writes L\U, L^-1 and
+// U^-1 (NB x NB column-major), and sets flag[panel] bit 1 if a multiplier
+// exceeds 1 in modulus (1|2 on a zero pivot).
+// ---------------------------------------------------------------------------
+constexpr int DB = 16;
+constexpr int DLD = LU_NB + 4;       // LDS column stride of the block
+
+template <typename T>
+struct DiagLds {
+    T ar[LU_NB][DLD], ai[LU_NB][DLD];               // the block, [col][row]
+    T lr[4][DB][DB + 1], li[4][DB][DB + 1];         // Li_kk, [col][row]
+    T ur[4][DB][DB + 1], ui[4][DB][DB + 1];         // Ui_kk
+    T tr[4][DB][DB + 1], ti[4][DB][DB + 1];         // per-wave scratch block
+};
+
+// acc (cr, ci) += A B for 16 x 16 blocks, A (i, k) at a[k lda + i], B (k, j) at
+// b[j ldb + k]; lane l = i + 16 g; the result (row MT::row(g, r), column i) is
+// in register r.
+template <typename T>
+__device__ inline void blk_mma(const T* ar_, const T* ai_, int lda, const T* br_, const T* bi_, int ldb,
+                               typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci, int i, int g) {
+    typedef MfmaT<T> MT;
+    T br[4], bi[4], a_r[4], a_i[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        br[t] = br_[i * ldb + 4 * g + t]; bi[t] = bi_[i * ldb + 4 * g + t];
+        a_r[t] = ar_[(4 * g + t) * lda + i]; a_i[t] = ai_[(4 * g + t) * lda + i];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        cr = MT::mma(a_r[t], br[t], cr);
+        ci = MT::mma(a_r[t], bi[t], ci);
+        cr = MT::mma(-a_i[t], bi[t], cr);
+        ci = MT::mma(a_i[t], br[t], ci);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
+                                                 T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
+    __shared__ DiagLds<T> S;
+    __shared__ int bad;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, g = lane >> 4;       // MFMA lane coordinates; also (column, row group) of a 16 x 16 block
+    if (tid == 0) bad = 0;
+    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
+        int c = t / LU_NB, r = t % LU_NB;
+        size_t gi = (size_t)(k0 + c) * Np + k0 + r;
+        T vr = Ar[gi], vi = Ai[gi];
+        S.ar[c][r] = vr; S.ai[c][r] = vi;
+        save_r[t] = vr; save_i[t] = vi;
+    }
+    __syncthreads();
+    int mybad = 0;
+    for (int kb = 0; kb < 4; kb++) {
+        const int o = DB * kb;
+        if (wave == 0) {
+            // 16 x 16 diagonal block: lane (i, g) owns column i, rows 4g..4g+3
+            for (int j = 0; j < DB; j++) {
+                T pr = S.ar[o + j][o + j], pi = S.ai[o + j][o + j];
+                T den = cabs2(pr, pi);
+                if (den == (T)0) mybad |= 3;
+                T inv = (T)1 / den;
+                T qr = pr * inv, qi = -pi * inv;
+                if (i == j) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        int row = 4 * g + r;
+                        if (row > j) {
+                            T ar_ = S.ar[o + j][o + row], ai_ = S.ai[o + j][o + row];
+                            T mr = ar_ * qr - ai_ * qi, mi = ar_ * qi + ai_ * qr;
+                            if (cabs2(mr, mi) > (T)1) mybad |= 1;
+                            S.ar[o + j][o + row] = mr; S.ai[o + j][o + row] = mi;
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (i > j) {
+                    T u_r = S.ar[o + i][o + j], u_i = S.ai[o + i][o + j];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        int row = 4 * g + r;
+                        if (row > j) {
+                            T mr = S.ar[o + j][o + row], mi = S.ai[o + j][o + row];
+                            S.ar[o + i][o + row] -= mr * u_r - mi * u_i;
+                            S.ai[o + i][o + row] -= mr * u_i + mi * u_r;
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            // Li_kk: column i of the unit-lower inverse by forward substitution
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                S.lr[kb][i][4 * g + r] = (4 * g + r == i) ? (T)1 : (T)0;
+                S.li[kb][i][4 * g + r] = 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int k = 0; k < DB - 1; k++) {
+                T xr = S.lr[kb][i][k], xi = S.li[kb][i][k];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int row = 4 * g + r;
+                    if (row > k) {
+                        T l_r = S.ar[o + k][o + row], l_i = S.ai[o + k][o + row];
+                        S.lr[kb][i][row] -= l_r * xr - l_i * xi;
+                        S.li[kb][i][row] -= l_r * xi + l_i * xr;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else if (wave == 1) {
+            // Ui_kk: column i of the upper inverse by back substitution
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                S.ur[kb][i][4 * g + r] = (4 * g + r == i) ? (T)1 : (T)0;
+                S.ui[kb][i][4 * g + r] = 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int k = DB - 1; k >= 0; k--) {
+                T dr = S.ar[o + k][o + k], di = S.ai[o + k][o + k];
+                T inv = (T)1 / cabs2(dr, di);
+                T qr = dr * inv, qi = -di * inv;
+                T xr = S.ur[kb][i][k], xi = S.ui[kb][i][k];
+                T yr = xr * qr - xi * qi, yi = xr * qi + xi * qr;     // y_k = x_k / u_kk
+                __builtin_amdgcn_wave_barrier();
+                if (g == (k >> 2)) { S.ur[kb][i][k] = yr; S.ui[kb][i][k] = yi; }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int row = 4 * g + r;
+                    if (row < k) {
+                        T u_r = S.ar[o + k][o + row], u_i = S.ai[o + k][o + row];
+                        S.ur[kb][i][row] -= u_r * yr - u_i * yi;
+                        S.ui[kb][i][row] -= u_r * yi + u_i * yr;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+        // L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb), one block per wave in turn
+        const int nrest = 3 - kb;
+        for (int job = wave; job < 2 * nrest; job += 4) {
+            acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
+            int b = kb + 1 + (job % nrest);
+            if (job < nrest) {
+                blk_mma<T>(&S.ar[o][DB * b], &S.ai[o][DB * b], DLD, &S.ur[kb][0][0], &S.ui[kb][0][0], DB + 1, cr, ci,
+                           i, g);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int row = MT::row(g, r);
+                    if (cabs2(cr[r], ci[r]) > (T)1) mybad |= 1;
+                    S.ar[o + i][DB * b + row] = cr[r]; S.ai[o + i][DB * b + row] = ci[r];
+                }
+            } else {
+                blk_mma<T>(&S.lr[kb][0][0], &S.li[kb][0][0], DB + 1, &S.ar[DB * b][o], &S.ai[DB * b][o], DLD, cr, ci,
+                           i, g);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    int row = MT::row(g, r);
+                    S.ar[DB * b + i][o + row] = cr[r]; S.ai[DB * b + i][o + row] = ci[r];
+                }
+            }
+        }
+        __syncthreads();
+        // trailing blocks A_ij -= L_ik U_kj
+        for (int job = wave; job < nrest * nrest; job += 4) {
+            int bi_ = kb + 1 + job / nrest, bj = kb + 1 + job % nrest;
+            acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
+            blk_mma<T>(&S.ar[o][DB * bi_], &S.ai[o][DB * bi_], DLD, &S.ar[DB * bj][o], &S.ai[DB * bj][o], DLD, cr, ci,
+                       i, g);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int row = MT::row(g, r);
+                S.ar[DB * bj + i][DB * bi_ + row] -= cr[r];
+                S.ai[DB * bj + i][DB * bi_ + row] -= ci[r];
+            }
+        }
+        __syncthreads();
+    }
+    if (mybad) atomicOr(&bad, mybad);
+    // L\U out; diagonal blocks of L^-1 and U^-1 and their zero triangles
+    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
+        int c = t / LU_NB, r = t % LU_NB;
+        size_t gi = (size_t)(k0 + c) * Np + k0 + r;
+        Ar[gi] = S.ar[c][r]; Ai[gi] = S.ai[c][r];
+        int bc = c / DB, br = r / DB;
+        T lv_r = 0, lv_i = 0, uv_r = 0, uv_i = 0;
+        if (bc == br) {
+            lv_r = S.lr[bc][c % DB][r % DB]; lv_i = S.li[bc][c % DB][r % DB];
+            uv_r = S.ur[bc][c % DB][r % DB]; uv_i = S.ui[bc][c % DB][r % DB];
+        }
+        if (br >= bc) { Li_r[t] = lv_r; Li_i[t] = lv_i; }
+        if (br <= bc) { Ui_r[t] = uv_r; Ui_i[t] = uv_i; }
+        if (br < bc) { Li_r[t] = 0; Li_i[t] = 0; }
+        if (br > bc) { Ui_r[t] = 0; Ui_i[t] = 0; }
+    }
+    __syncthreads();
+    // off-diagonal blocks of the inverses, diagonal d = 1, 2, 3 (3, 2, 1 blocks
+    // of each); job < nd: X_{j+d, j}; job >= nd: Y_{j, j+d}
+    for (int d = 1; d < 4; d++) {
+        const int nd = 4 - d;
+        for (int job = wave; job < 2 * nd; job += 4) {
+            const bool lower = job < nd;
+            const int j = job % nd;
+            acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
+            if (lower) {
+                const int ib = j + d;   // X_{ib, j} = -Li_ib sum_{k=j}^{ib-1} L_{ib,k} X_{k,j}
+                for (int k = j; k < ib; k++)
+                    blk_mma<T>(&S.ar[DB * k][DB * ib], &S.ai[DB * k][DB * ib], DLD, Li_r + DB * j * LU_NB + DB * k,
+                               Li_i + DB * j * LU_NB + DB * k, LU_NB, cr, ci, i, g);
+            } else {
+                const int jb = j + d;   // Y_{j, jb} = -Ui_j sum_{k=j+1}^{jb} U_{j,k} Y_{k,jb}
+                for (int k = j + 1; k <= jb; k++)
+                    blk_mma<T>(&S.ar[DB * k][DB * j], &S.ai[DB * k][DB * j], DLD, Ui_r + DB * jb * LU_NB + DB * k,
+                               Ui_i + DB * jb * LU_NB + DB * k, LU_NB, cr, ci, i, g);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int row = MT::row(g, r);
+                S.tr[wave][i][row] = cr[r]; S.ti[wave][i][row] = ci[r];
+            }
+            __builtin_amdgcn_wave_barrier();
+            acc_t xr = acc_t{0, 0, 0, 0}, xi = acc_t{0, 0, 0, 0};
+            const int dblk = lower ? j + d : j;
+            const T* pr_ = lower ? &S.lr[dblk][0][0] : &S.ur[dblk][0][0];
+            const T* pi_ = lower ? &S.li[dblk][0][0] : &S.ui[dblk][0][0];
+            blk_mma<T>(pr_, pi_, DB + 1, &S.tr[wave][0][0], &S.ti[wave][0][0], DB + 1, xr, xi, i, g);
+            T* outr = lower ? Li_r + (size_t)(DB * j) * LU_NB + DB * (j + d) : Ui_r + (size_t)(DB * (j + d)) * LU_NB + DB * j;
+            T* outi = lower ? Li_i + (size_t)(DB * j) * LU_NB + DB * (j + d) : Ui_i + (size_t)(DB * (j + d)) * LU_NB + DB * j;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                int row = MT::row(g, r);
+                outr[i * LU_NB + row] = -xr[r];
+                outi[i * LU_NB + row] = -xi[r];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
 }
 
 // L21 = A21 Uinv_p in place for rows k0+NB .. Np-1. Each wave owns 16 rows
