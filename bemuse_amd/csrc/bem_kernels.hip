@@ -395,18 +395,18 @@ __device__ inline void cmma16(const T* ar_, const T* ai_, size_t lda, const T* b
 // Factor the NB x NB diagonal block at (k0, k0) without interchanges and form
 // the inverses of its unit-lower and upper triangles, blocked by DB = 16 so
 // that most of the work is 16 x 16 MFMA block products and the work-group
-// meets at 20 barriers (the unblocked form needs one per column, 128 in all).
+// meets at 14 barriers (the unblocked form needs one per column, 128 in all).
 // The block lives in LDS, [col][row]. For sub-block column kb = 0..3:
-//   wave 0 factors the 16 x 16 diagonal block (wave-synchronous, one column
-//   per step); waves 0 and 1 form its triangle inverses Li_kk, Ui_kk;
+//   wave 0 factors the 16 x 16 diagonal block in registers (lane = row, the
+//   pivot row broadcast with readlane) and forms its triangle inverses
+//   Li_kk, Ui_kk (lane = column);
 //   the waves share L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb),
 //   then the trailing blocks A_ij -= L_ik U_kj.
-// The inverses of the whole triangles follow block diagonal by block
-// diagonal: X_ij = -Li_ii sum_{k=j}^{i-1} L_ik X_kj (L^-1), and
-// Y_ij = -Ui_ii sum_{k=i+1}^{j} U_ik Y_kj (U^-1), the blocks of earlier
-// diagonals read back from the outputs. Saves the block, writes L\U, L^-1 and
-// U^-1 (NB x NB column-major), and sets flag[panel] bit 1 if a multiplier
-// exceeds 1 in modulus (1|2 on a zero pivot).
+// Then wave j forms block column j of both inverses, keeping its blocks in
+// MFMA accumulators: X_ij = -Li_ii sum_{k=j}^{i-1} L_ik X_kj (L^-1) and
+// Y_ij = -Ui_ii sum_{k=i+1}^{j} U_ik Y_kj (U^-1). Saves the block, writes L\U,
+// L^-1 and U^-1 (NB x NB column-major), and sets flag[panel] bit 1 if a
+// multiplier exceeds 1 in modulus (1|2 on a zero pivot).
 // ---------------------------------------------------------------------------
 constexpr int DB = 16;
 constexpr int DLD = LU_NB + 4;       // LDS column stride of the block
@@ -416,7 +416,6 @@ struct DiagLds {
     T ar[LU_NB][DLD], ai[LU_NB][DLD];               // the block, [col][row]
     T lr[4][DB][DB + 1], li[4][DB][DB + 1];         // Li_kk, [col][row]
     T ur[4][DB][DB + 1], ui[4][DB][DB + 1];         // Ui_kk
-    T tr[4][DB][DB + 1], ti[4][DB][DB + 1];         // per-wave scratch block
 };
 
 // acc (cr, ci) += A B for 16 x 16 blocks, A (i, k) at a[k lda + i], B (k, j) at
@@ -441,6 +440,154 @@ __device__ inline void blk_mma(const T* ar_, const T* ai_, int lda, const T* br_
     }
 }
 
+// acc (cr, ci) += A B with B a 16 x 16 block held in accumulator layout (lane
+// (i, g), register t: B(MT::row(g, t), i)). Step t pairs lane group g with
+// k = MT::row(g, t), so B is read straight from the registers.
+template <typename T>
+__device__ inline void blk_mma_rb(const T* ar_, const T* ai_, int lda, const typename MfmaT<T>::acc& Br,
+                                  const typename MfmaT<T>::acc& Bi, typename MfmaT<T>::acc& cr,
+                                  typename MfmaT<T>::acc& ci, int i, int g) {
+    typedef MfmaT<T> MT;
+    T a_r[4], a_i[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        int k = MT::row(g, t);
+        a_r[t] = ar_[k * lda + i]; a_i[t] = ai_[k * lda + i];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        cr = MT::mma(a_r[t], Br[t], cr);
+        ci = MT::mma(a_r[t], Bi[t], ci);
+        cr = MT::mma(-a_i[t], Bi[t], cr);
+        ci = MT::mma(a_i[t], Br[t], ci);
+    }
+}
+
+// Wave 0: the 16 x 16 diagonal block at (o, o) factored in registers (lane
+// rho & 15 holds row rho), then the columns of its triangle inverses
+// (lane & 15 = column) from the factor's registers; results to LDS.
+template <typename T>
+__device__ inline void diag16(DiagLds<T>& S, int o, int kb, int lane, int& mybad) {
+    const int rho = lane & 15;
+    const bool writer = lane < DB;
+    T ar[DB], ai[DB];
+#pragma unroll
+    for (int c = 0; c < DB; c++) { ar[c] = S.ar[o + c][o + rho]; ai[c] = S.ai[o + c][o + rho]; }
+#pragma unroll
+    for (int j = 0; j < DB; j++) {
+        T pr = bcast(ar[j], j), pi = bcast(ai[j], j);
+        T den = cabs2(pr, pi);
+        if (den == (T)0) mybad |= 3;
+        T inv = (T)1 / den;
+        T qr = pr * inv, qi = -pi * inv;
+        const bool below = rho > j;
+        T mr = ar[j] * qr - ai[j] * qi, mi = ar[j] * qi + ai[j] * qr;
+        if (below && cabs2(mr, mi) > (T)1) mybad |= 1;
+        mr = below ? mr : (T)0;
+        mi = below ? mi : (T)0;
+        ar[j] = below ? mr : ar[j];
+        ai[j] = below ? mi : ai[j];
+#pragma unroll
+        for (int c = j + 1; c < DB; c++) {
+            T ur = bcast(ar[c], j), ui = bcast(ai[c], j);
+            ar[c] -= mr * ur - mi * ui;
+            ai[c] -= mr * ui + mi * ur;
+        }
+    }
+    if (writer) {
+#pragma unroll
+        for (int c = 0; c < DB; c++) { S.ar[o + c][o + rho] = ar[c]; S.ai[o + c][o + rho] = ai[c]; }
+    }
+    const int col = lane & 15;
+    T xr[DB], xi[DB];
+    // column col of the unit-lower inverse: forward substitution, L(r, k) from lane r
+#pragma unroll
+    for (int r = 0; r < DB; r++) { xr[r] = (r == col) ? (T)1 : (T)0; xi[r] = 0; }
+#pragma unroll
+    for (int k = 0; k < DB - 1; k++) {
+#pragma unroll
+        for (int r = k + 1; r < DB; r++) {
+            T lr = bcast(ar[k], r), li = bcast(ai[k], r);
+            xr[r] -= lr * xr[k] - li * xi[k];
+            xi[r] -= lr * xi[k] + li * xr[k];
+        }
+    }
+    if (writer) {
+#pragma unroll
+        for (int r = 0; r < DB; r++) { S.lr[kb][col][r] = xr[r]; S.li[kb][col][r] = xi[r]; }
+    }
+    // column col of the upper inverse: back substitution, U(r, k) from lane r
+#pragma unroll
+    for (int r = 0; r < DB; r++) { xr[r] = (r == col) ? (T)1 : (T)0; xi[r] = 0; }
+#pragma unroll
+    for (int k = DB - 1; k >= 0; k--) {
+        T dr = bcast(ar[k], k), di = bcast(ai[k], k);
+        T inv = (T)1 / cabs2(dr, di);
+        T qr = dr * inv, qi = -di * inv;
+        T yr = xr[k] * qr - xi[k] * qi, yi = xr[k] * qi + xi[k] * qr;
+        xr[k] = yr; xi[k] = yi;
+#pragma unroll
+        for (int r = 0; r < k; r++) {
+            T u_r = bcast(ar[k], r), u_i = bcast(ai[k], r);
+            xr[r] -= u_r * yr - u_i * yi;
+            xi[r] -= u_r * yi + u_i * yr;
+        }
+    }
+    if (writer) {
+#pragma unroll
+        for (int r = 0; r < DB; r++) { S.ur[kb][col][r] = xr[r]; S.ui[kb][col][r] = xi[r]; }
+    }
+}
+
+template <typename T>
+__device__ inline void load_blk(const T (*br)[DB + 1], const T (*bi)[DB + 1], typename MfmaT<T>::acc& r_,
+                                typename MfmaT<T>::acc& i_, int i, int g) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) { r_[r] = br[i][MfmaT<T>::row(g, r)]; i_[r] = bi[i][MfmaT<T>::row(g, r)]; }
+}
+
+template <typename T>
+__device__ inline void store_blk(T* pr, T* pi, const typename MfmaT<T>::acc& r_, const typename MfmaT<T>::acc& i_,
+                                 int i, int g) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) { pr[i * LU_NB + MfmaT<T>::row(g, r)] = r_[r]; pi[i * LU_NB + MfmaT<T>::row(g, r)] = i_[r]; }
+}
+
+// Wave J: block column J of L^-1 (blocks J..3) and of U^-1 (blocks 0..J).
+template <typename T, int J>
+__device__ inline void inv_column(DiagLds<T>& S, T* Li_r, T* Li_i, T* Ui_r, T* Ui_i, int i, int g) {
+    typedef typename MfmaT<T>::acc acc_t;
+    const acc_t z = acc_t{0, 0, 0, 0};
+    acc_t Xr[4], Xi[4];
+    load_blk<T>(S.lr[J], S.li[J], Xr[J], Xi[J], i, g);
+#pragma unroll
+    for (int ib = J + 1; ib < 4; ib++) {
+        acc_t tr = z, ti = z;
+#pragma unroll
+        for (int k = J; k < ib; k++) blk_mma_rb<T>(&S.ar[DB * k][DB * ib], &S.ai[DB * k][DB * ib], DLD, Xr[k], Xi[k], tr, ti, i, g);
+        acc_t xr = z, xi = z;
+        blk_mma_rb<T>(&S.lr[ib][0][0], &S.li[ib][0][0], DB + 1, tr, ti, xr, xi, i, g);
+        Xr[ib] = -xr; Xi[ib] = -xi;
+    }
+#pragma unroll
+    for (int ib = J; ib < 4; ib++)
+        store_blk<T>(Li_r + DB * J * LU_NB + DB * ib, Li_i + DB * J * LU_NB + DB * ib, Xr[ib], Xi[ib], i, g);
+    acc_t Yr[4], Yi[4];
+    load_blk<T>(S.ur[J], S.ui[J], Yr[J], Yi[J], i, g);
+#pragma unroll
+    for (int ib = J - 1; ib >= 0; ib--) {
+        acc_t tr = z, ti = z;
+#pragma unroll
+        for (int k = ib + 1; k <= J; k++) blk_mma_rb<T>(&S.ar[DB * k][DB * ib], &S.ai[DB * k][DB * ib], DLD, Yr[k], Yi[k], tr, ti, i, g);
+        acc_t yr = z, yi = z;
+        blk_mma_rb<T>(&S.ur[ib][0][0], &S.ui[ib][0][0], DB + 1, tr, ti, yr, yi, i, g);
+        Yr[ib] = -yr; Yi[ib] = -yi;
+    }
+#pragma unroll
+    for (int ib = 0; ib <= J; ib++)
+        store_blk<T>(Ui_r + DB * J * LU_NB + DB * ib, Ui_i + DB * J * LU_NB + DB * ib, Yr[ib], Yi[ib], i, g);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
                                                  T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
@@ -449,7 +596,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     __shared__ DiagLds<T> S;
     __shared__ int bad;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i = lane & 15, g = lane >> 4;       // MFMA lane coordinates; also (column, row group) of a 16 x 16 block
+    const int i = lane & 15, g = lane >> 4;       // MFMA lane coordinates
     if (tid == 0) bad = 0;
     for (int t = tid; t < LU_NB * LU_NB; t += 256) {
         int c = t / LU_NB, r = t % LU_NB;
@@ -462,92 +609,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     int mybad = 0;
     for (int kb = 0; kb < 4; kb++) {
         const int o = DB * kb;
-        if (wave == 0) {
-            // 16 x 16 diagonal block: lane (i, g) owns column i, rows 4g..4g+3
-            for (int j = 0; j < DB; j++) {
-                T pr = S.ar[o + j][o + j], pi = S.ai[o + j][o + j];
-                T den = cabs2(pr, pi);
-                if (den == (T)0) mybad |= 3;
-                T inv = (T)1 / den;
-                T qr = pr * inv, qi = -pi * inv;
-                if (i == j) {
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        int row = 4 * g + r;
-                        if (row > j) {
-                            T ar_ = S.ar[o + j][o + row], ai_ = S.ai[o + j][o + row];
-                            T mr = ar_ * qr - ai_ * qi, mi = ar_ * qi + ai_ * qr;
-                            if (cabs2(mr, mi) > (T)1) mybad |= 1;
-                            S.ar[o + j][o + row] = mr; S.ai[o + j][o + row] = mi;
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (i > j) {
-                    T u_r = S.ar[o + i][o + j], u_i = S.ai[o + i][o + j];
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        int row = 4 * g + r;
-                        if (row > j) {
-                            T mr = S.ar[o + j][o + row], mi = S.ai[o + j][o + row];
-                            S.ar[o + i][o + row] -= mr * u_r - mi * u_i;
-                            S.ai[o + i][o + row] -= mr * u_i + mi * u_r;
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
-            // Li_kk: column i of the unit-lower inverse by forward substitution
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                S.lr[kb][i][4 * g + r] = (4 * g + r == i) ? (T)1 : (T)0;
-                S.li[kb][i][4 * g + r] = 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int k = 0; k < DB - 1; k++) {
-                T xr = S.lr[kb][i][k], xi = S.li[kb][i][k];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    int row = 4 * g + r;
-                    if (row > k) {
-                        T l_r = S.ar[o + k][o + row], l_i = S.ai[o + k][o + row];
-                        S.lr[kb][i][row] -= l_r * xr - l_i * xi;
-                        S.li[kb][i][row] -= l_r * xi + l_i * xr;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        } else if (wave == 1) {
-            // Ui_kk: column i of the upper inverse by back substitution
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                S.ur[kb][i][4 * g + r] = (4 * g + r == i) ? (T)1 : (T)0;
-                S.ui[kb][i][4 * g + r] = 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int k = DB - 1; k >= 0; k--) {
-                T dr = S.ar[o + k][o + k], di = S.ai[o + k][o + k];
-                T inv = (T)1 / cabs2(dr, di);
-                T qr = dr * inv, qi = -di * inv;
-                T xr = S.ur[kb][i][k], xi = S.ui[kb][i][k];
-                T yr = xr * qr - xi * qi, yi = xr * qi + xi * qr;     // y_k = x_k / u_kk
-                __builtin_amdgcn_wave_barrier();
-                if (g == (k >> 2)) { S.ur[kb][i][k] = yr; S.ui[kb][i][k] = yi; }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    int row = 4 * g + r;
-                    if (row < k) {
-                        T u_r = S.ar[o + k][o + row], u_i = S.ai[o + k][o + row];
-                        S.ur[kb][i][row] -= u_r * yr - u_i * yi;
-                        S.ui[kb][i][row] -= u_r * yi + u_i * yr;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
+        if (wave == 0) diag16<T>(S, o, kb, lane, mybad);
         __syncthreads();
         // L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb), one block per wave in turn
         const int nrest = 3 - kb;
@@ -590,65 +652,23 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
         __syncthreads();
     }
     if (mybad) atomicOr(&bad, mybad);
-    // L\U out; diagonal blocks of L^-1 and U^-1 and their zero triangles
+    // L\U out, and the zero triangles of the inverses (strictly upper blocks
+    // of L^-1, strictly lower of U^-1); the other blocks come from inv_column
     for (int t = tid; t < LU_NB * LU_NB; t += 256) {
         int c = t / LU_NB, r = t % LU_NB;
         size_t gi = (size_t)(k0 + c) * Np + k0 + r;
         Ar[gi] = S.ar[c][r]; Ai[gi] = S.ai[c][r];
         int bc = c / DB, br = r / DB;
-        T lv_r = 0, lv_i = 0, uv_r = 0, uv_i = 0;
-        if (bc == br) {
-            lv_r = S.lr[bc][c % DB][r % DB]; lv_i = S.li[bc][c % DB][r % DB];
-            uv_r = S.ur[bc][c % DB][r % DB]; uv_i = S.ui[bc][c % DB][r % DB];
-        }
-        if (br >= bc) { Li_r[t] = lv_r; Li_i[t] = lv_i; }
-        if (br <= bc) { Ui_r[t] = uv_r; Ui_i[t] = uv_i; }
         if (br < bc) { Li_r[t] = 0; Li_i[t] = 0; }
         if (br > bc) { Ui_r[t] = 0; Ui_i[t] = 0; }
     }
-    __syncthreads();
-    // off-diagonal blocks of the inverses, diagonal d = 1, 2, 3 (3, 2, 1 blocks
-    // of each); job < nd: X_{j+d, j}; job >= nd: Y_{j, j+d}
-    for (int d = 1; d < 4; d++) {
-        const int nd = 4 - d;
-        for (int job = wave; job < 2 * nd; job += 4) {
-            const bool lower = job < nd;
-            const int j = job % nd;
-            acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
-            if (lower) {
-                const int ib = j + d;   // X_{ib, j} = -Li_ib sum_{k=j}^{ib-1} L_{ib,k} X_{k,j}
-                for (int k = j; k < ib; k++)
-                    blk_mma<T>(&S.ar[DB * k][DB * ib], &S.ai[DB * k][DB * ib], DLD, Li_r + DB * j * LU_NB + DB * k,
-                               Li_i + DB * j * LU_NB + DB * k, LU_NB, cr, ci, i, g);
-            } else {
-                const int jb = j + d;   // Y_{j, jb} = -Ui_j sum_{k=j+1}^{jb} U_{j,k} Y_{k,jb}
-                for (int k = j + 1; k <= jb; k++)
-                    blk_mma<T>(&S.ar[DB * k][DB * j], &S.ai[DB * k][DB * j], DLD, Ui_r + DB * jb * LU_NB + DB * k,
-                               Ui_i + DB * jb * LU_NB + DB * k, LU_NB, cr, ci, i, g);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int row = MT::row(g, r);
-                S.tr[wave][i][row] = cr[r]; S.ti[wave][i][row] = ci[r];
-            }
-            __builtin_amdgcn_wave_barrier();
-            acc_t xr = acc_t{0, 0, 0, 0}, xi = acc_t{0, 0, 0, 0};
-            const int dblk = lower ? j + d : j;
-            const T* pr_ = lower ? &S.lr[dblk][0][0] : &S.ur[dblk][0][0];
-            const T* pi_ = lower ? &S.li[dblk][0][0] : &S.ui[dblk][0][0];
-            blk_mma<T>(pr_, pi_, DB + 1, &S.tr[wave][0][0], &S.ti[wave][0][0], DB + 1, xr, xi, i, g);
-            T* outr = lower ? Li_r + (size_t)(DB * j) * LU_NB + DB * (j + d) : Ui_r + (size_t)(DB * (j + d)) * LU_NB + DB * j;
-            T* outi = lower ? Li_i + (size_t)(DB * j) * LU_NB + DB * (j + d) : Ui_i + (size_t)(DB * (j + d)) * LU_NB + DB * j;
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                int row = MT::row(g, r);
-                outr[i * LU_NB + row] = -xr[r];
-                outi[i * LU_NB + row] = -xi[r];
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-        __syncthreads();
+    switch (wave) {
+        case 0: inv_column<T, 0>(S, Li_r, Li_i, Ui_r, Ui_i, i, g); break;
+        case 1: inv_column<T, 1>(S, Li_r, Li_i, Ui_r, Ui_i, i, g); break;
+        case 2: inv_column<T, 2>(S, Li_r, Li_i, Ui_r, Ui_i, i, g); break;
+        default: inv_column<T, 3>(S, Li_r, Li_i, Ui_r, Ui_i, i, g); break;
     }
+    __syncthreads();
     if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
 }
 
