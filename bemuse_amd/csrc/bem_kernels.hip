@@ -1428,37 +1428,57 @@ __global__ void k_refine_reset(RefineCtrl* ctrl) {
     }
 }
 
-// After a residual R = B - A X: rel = max_j max|r_j| / (|A| max|x_j|) over
-// rows < N. Converged when rel <= tol; stops (not converged) on stagnation
-// (rel > 0.5 prev) or after max_iter corrections. One work-group.
-__global__ void k_refine_check(int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
-                               const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
+// After a residual R = B - A X, per right-hand side j over rows < N:
+//   rel_j = max( max|r_j| / (|A| max|x_j|),
+//                max|Im r_j| / (|A| max(max|Im x_j|, 0.01 max|x_j|)) ).
+// The second term asks the imaginary part for the same relative accuracy: at
+// low frequency it is orders of magnitude below the real part (the wave term
+// is O(K)), the damping comes from it, and a residual small against |x| alone
+// can leave it inaccurate. The 0.01 floor keeps its roundoff floor (about
+// 1e2 u |A| |x|) under the tolerance. Converged when max_j rel_j <= tol;
+// stops (not converged) on stagnation (rel > 0.5 prev) or after max_iter
+// corrections. One work-group of 512; per-wave maxima by shuffles.
+constexpr int RC_THREADS = 512;
+
+__global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int nr, const double* Rr,
+                                                             const double* Ri, const double* Xr, const double* Xi,
+                                                             const double* anorm, RefineCtrl* ctrl, double tol,
+                                                             int max_iter) {
     if (ctrl->done) return;
-    __shared__ double sm[2][MV_MAXR][256];
+    constexpr int NW = RC_THREADS / 64;
+    __shared__ double sm[NW][4][MV_MAXR];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int j = 0; j < nr; j++) {
-        double a = 0, b = 0;
-        for (int r = threadIdx.x; r < N; r += 256) {
+        double mr = 0, mx = 0, mri = 0, mxi = 0;   // max |r|, |x|, |Im r|, |Im x|
+        for (int r = threadIdx.x; r < N; r += RC_THREADS) {
             size_t o = (size_t)j * Np + r;
-            a = fmax(a, sqrt(Rr[o] * Rr[o] + Ri[o] * Ri[o]));
-            b = fmax(b, sqrt(Xr[o] * Xr[o] + Xi[o] * Xi[o]));
+            double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
+            mr = fmax(mr, sqrt(rr * rr + ri * ri));
+            mx = fmax(mx, sqrt(xr * xr + xi * xi));
+            mri = fmax(mri, fabs(ri));
+            mxi = fmax(mxi, fabs(xi));
         }
-        sm[0][j][threadIdx.x] = a; sm[1][j][threadIdx.x] = b;
+        for (int off = 32; off > 0; off >>= 1) {
+            mr = fmax(mr, __shfl_xor(mr, off));
+            mx = fmax(mx, __shfl_xor(mx, off));
+            mri = fmax(mri, __shfl_xor(mri, off));
+            mxi = fmax(mxi, __shfl_xor(mxi, off));
+        }
+        if (lane == 0) { sm[wave][0][j] = mr; sm[wave][1][j] = mx; sm[wave][2][j] = mri; sm[wave][3][j] = mxi; }
     }
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (threadIdx.x < h)
-            for (int j = 0; j < nr; j++) {
-                sm[0][j][threadIdx.x] = fmax(sm[0][j][threadIdx.x], sm[0][j][threadIdx.x + h]);
-                sm[1][j][threadIdx.x] = fmax(sm[1][j][threadIdx.x], sm[1][j][threadIdx.x + h]);
-            }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
         double an = anorm[0], rel = 0.0;
         for (int j = 0; j < nr; j++) {
-            double den = an * sm[1][j][0];
-            double v = den > 0 ? sm[0][j][0] / den : (sm[0][j][0] > 0 ? 1e300 : 0.0);
-            rel = fmax(rel, v);
+            double a = 0, b = 0, c = 0, d = 0;
+            for (int w = 0; w < NW; w++) {
+                a = fmax(a, sm[w][0][j]); b = fmax(b, sm[w][1][j]);
+                c = fmax(c, sm[w][2][j]); d = fmax(d, sm[w][3][j]);
+            }
+            double den = an * b, deni = an * fmax(d, 0.01 * b);
+            double v = den > 0 ? a / den : (a > 0 ? 1e300 : 0.0);
+            double vi = deni > 0 ? c / deni : (c > 0 ? 1e300 : 0.0);
+            rel = fmax(rel, fmax(v, vi));
         }
         if (!(rel == rel)) rel = 1e300;
         ctrl->prev = ctrl->rel;
@@ -1857,7 +1877,7 @@ void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl) {
 }
 void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
                          const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
-    hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(256), 0, s, N, Np, nr, Rr, Ri, Xr, Xi, anorm, ctrl, tol, max_iter);
+    hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(RC_THREADS), 0, s, N, Np, nr, Rr, Ri, Xr, Xi, anorm, ctrl, tol, max_iter);
 }
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb) {
     hipLaunchKernelGGL(k_copy2, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, ca, cb);
