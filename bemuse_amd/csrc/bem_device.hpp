@@ -12,8 +12,43 @@ constexpr double kPI = 3.14159265358979323846;
 constexpr double kFourPIinv = 0.25 / 3.14159265358979323846;
 constexpr double kEUL = 0.5772156649;   // truncated Euler constant of the reference (BEMUse_Math_Types.h:99)
 
-// Struve H0 / H1 approximations (Kernels.cpp:35-116); for x > 3 they add the
-// Bessel Y0 / Y1 (ROCm device libm) to a rational function of (3/x)^2.
+// 1/x: the hardware reciprocal refined by two Newton steps (error within
+// about 1 ulp; the arguments here are finite, positive and far from the
+// exponent limits). Replaces IEEE divisions, whose scaling and fix-up steps
+// cost several times more in the Havelock evaluation's inner loop.
+__host__ __device__ inline double frcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// Struve H0 and H1 together (Kernels.cpp:35-116): below 3 polynomials in
+// (x/3)^2; above, the Bessel Y0 / Y1 plus rational functions of (3/x)^2,
+// whose two denominators share one reciprocal.
+__host__ __device__ inline void struve_h01(double x, double y0, double y1, double& h0, double& h1) {
+    if (x <= 3.0) {
+        double x3 = x * (1.0 / 3.0), yy = x3 * x3;
+        h0 = (1.909859164 + (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 - 0.000876918 * yy) * yy) * yy) * yy) * yy) * x3;
+        h1 = (1.909859286 + (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 - 0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
+        return;
+    }
+    double u = 3.0 * frcp(x), yy = u * u;
+    double c1a = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
+    double c2a = kPI * x * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
+    double c1b = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
+    double c2b = kPI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
+    double rab = frcp(c2a * c2b);
+    h0 = c1a * c2b * rab + y0;
+    h1 = c1b * c2a * rab + y1;
+}
+
+// The two separately (Kernels.cpp:35-116), for tests of the approximation.
 __host__ __device__ inline double struve_h0(double x, double y0) {
     if (x <= 3.0) {
         double yy = (x / 3.0) * (x / 3.0);
@@ -63,7 +98,7 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
         J0 = j0(x); J1 = j1(x); Y0 = y0(x); Y1 = y1(x);
         return;
     }
-    const double u = 3.0 / x, t = 2.0 * u - 1.0, t2 = 2.0 * t;
+    const double u = 3.0 * frcp(x), t = 2.0 * u - 1.0, t2 = 2.0 * t;
     // Clenshaw for the four series at once
     double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, r1 = 0.0, r2 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
@@ -131,25 +166,28 @@ constexpr GFCoeffs kGF = {
 // DGFX = GFh X/h, DGFY = GFh Y/h (0 when h <= 1e-6), DGFZ = GF.
 struct GFVal { double gr, gi, hr, hi, h; };
 
+// Quotients are products with frcp reciprocals (1/d shared by alpha and beta,
+// rho = d/(1+d) and 1 - rho from one reciprocal).
 __host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     double hh = sqrt(X * X + Y * Y);
     double dd = sqrt(hh * hh + vv * vv);
-    double al = -vv / dd;
-    double be = hh / dd;
-    double rho = dd / (1.0 + dd);
-    double omr = 1.0 - rho;
+    double rdd = frcp(dd);
+    double al = -vv * rdd;
+    double be = hh * rdd;
+    double omr = frcp(1.0 + dd);
+    double rho = dd * omr;
     double rw = rho * omr * omr * omr;
     double dd3 = dd * dd * dd;
     double ev = exp(vv);
-    double inv1 = 1.0 / (1.0 + dd3);
+    double inv1 = frcp(1.0 + dd3);
     // local flow L0 (GF_Func_L0, :258-282)
     double PP = (log(0.5 * (dd - vv)) + kEUL - 2.0 * dd * dd) * ev + dd * dd - vv;
     double A = horner(kGF.A, 10, rho), B = horner(kGF.B, 10, rho);
     double C = horner(kGF.C, 8, rho), D = horner(kGF.D, 10, rho);
-    double RR = (1.0 - be) * A - be * B - al * C / (1.0 + 6.0 * al * rho * omr) + be * (1.0 - be) * D;
+    double RR = (1.0 - be) * A - be * B - al * C * frcp(1.0 + 6.0 * al * rho * omr) + be * (1.0 - be) * D;
     double L0 = 2.0 * PP * inv1 + 2.0 * rw * RR;
     // derivative local flow Ls (GF_Func_Ls, :297-325)
-    double PS = (be + hh) / (dd - vv) - 2.0 * be + 2.0 * dd * ev - hh;
+    double PS = (be + hh) * frcp(dd - vv) - 2.0 * be + 2.0 * dd * ev - hh;
     double QS = exp(-dd) * (1.0 - be) * (1.0 + dd * inv1);
     double dA = horner(kGF.dA, 10, rho), dB = horner(kGF.dB, 10, rho), dC = horner(kGF.dC, 6, rho);
     double R2 = be * dA - (1.0 - al) * dB + be * (1.0 - be) * rho * (1.0 - 2.0 * rho) * dC;
@@ -157,11 +195,13 @@ __host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     // wave parts W, Wh (GF_Func_W / GF_Func_Wh, :284-295, :327-338)
     double j0v, j1v, y0v, y1v;
     bessel01(hh, j0v, j1v, y0v, y1v);
+    double h0v, h1v;
+    struve_h01(hh, y0v, y1v, h0v, h1v);
     double E = 2.0 * kPI * ev;
     GFVal o;
-    o.gr = L0 + E * struve_h0(hh, y0v);
+    o.gr = L0 + E * h0v;
     o.gi = -E * j0v;
-    o.hr = Ls + E * (2.0 / kPI - struve_h1(hh, y1v));
+    o.hr = Ls + E * (2.0 / kPI - h1v);
     o.hi = E * j1v;
     o.h = hh;
     return o;
@@ -184,7 +224,7 @@ __host__ __device__ inline void wave_panel(const double* q, double xr, double yr
         sr += w * (g.gr * s1);
         si += w * (g.gi * s1);
         double cx = 0.0, cy = 0.0;
-        if (g.h > 1.0e-6) { cx = X / g.h; cy = Y / g.h; }
+        if (g.h > 1.0e-6) { double rh = frcp(g.h); cx = X * rh; cy = Y * rh; }
         double tr = (g.hr * cx) * p[3] + (g.hr * cy) * p[4] + g.gr * p[5];
         double ti = (g.hi * cx) * p[3] + (g.hi * cy) * p[4] + g.gi * p[5];
         dr += w * (tr * s2);

@@ -122,7 +122,7 @@ __global__ void k_havelock_eval(int n, const double* X, const double* Y, const d
     if (i >= n) return;
     GFVal g = havelock_gf(X[i], Y[i], V[i]);
     double cx = 0.0, cy = 0.0;
-    if (g.h > 1.0e-6) { cx = X[i] / g.h; cy = Y[i] / g.h; }
+    if (g.h > 1.0e-6) { double rh = frcp(g.h); cx = X[i] * rh; cy = Y[i] * rh; }
     double* o = out + 8 * (size_t)i;
     o[0] = g.gr; o[1] = g.gi;
     o[2] = g.hr * cx; o[3] = g.hi * cx;
