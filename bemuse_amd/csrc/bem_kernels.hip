@@ -294,6 +294,12 @@ __global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm, con
 // ---------------------------------------------------------------------------
 constexpr int LU_NB = 64;
 constexpr int NB2 = LU_NB * LU_NB;
+// Panels per deferred trailing update (K = group NB). Four halves the passes
+// over C but moves twice the flops into the thin updates of the group's own
+// panels; measured on MI355X, that pays in FP64 and for large orders, while
+// FP32 at N ~ 6000 is faster with pairs (74.9 vs 71.4 frequency-solves/s).
+template <typename T>
+inline int lu_group(int Np) { return (sizeof(T) == 8 || Np >= 12288) ? 4 : 2; }
 
 template <typename T>
 __device__ inline T cabs2(T r, T i) { return r * r + i * i; }
@@ -1603,7 +1609,7 @@ template <typename T> constexpr TileCfg cfg_of(int i) { return kCfg[i]; }
 
 constexpr int kMaxDev = 16, kClasses = 11;
 struct TuneTable {
-    signed char pick[2][kClasses][kClasses][2];   // [prec][M class][N class][K == 128]
+    signed char pick[2][kClasses][kClasses][3];   // [prec][M class][N class][K class: 64, 128, 192-256]
     bool tuned_np[2][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128)
     TuneTable() {
         std::memset(pick, -1, sizeof(pick));
@@ -1612,6 +1618,8 @@ struct TuneTable {
 };
 TuneTable g_tune[kMaxDev];
 std::mutex g_tune_mu;
+
+inline int k_class(int K) { return K <= 64 ? 0 : (K <= 128 ? 1 : 2); }
 
 inline int size_class(int x) {
     int c = 0;
@@ -1652,21 +1660,26 @@ int default_cfg(int M, int N) {
 
 template <typename T>
 int pick_cfg(int dev, int M, int N, int K) {
-    int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].pick[sizeof(T) == 8][size_class(M)][size_class(N)][K >= 128] : -1;
+    int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].pick[sizeof(T) == 8][size_class(M)][size_class(N)][k_class(K)] : -1;
     return v >= 0 ? v : default_cfg<T>(M, N);
 }
 
-// The (M, N, K) of every trailing-update launch of launch_lu at padded order Np.
-std::vector<std::array<int, 3>> lu_gemm_shapes(int Np) {
+// The (M, N, K) of every update launch of launch_lu at padded order Np, in
+// launch order (the same loop structure as launch_lu).
+std::vector<std::array<int, 3>> lu_gemm_shapes(int Np, int group) {
     std::vector<std::array<int, 3>> v;
     int npanel = Np / LU_NB;
-    for (int p = 0; p < npanel; p += 2) {
-        int rest = Np - p * LU_NB - LU_NB, rest2 = rest - LU_NB;
-        v.push_back({rest, LU_NB, LU_NB});
-        if (rest2 <= 0) break;
-        v.push_back({LU_NB, rest2, LU_NB});
-        v.push_back({rest2, LU_NB, 2 * LU_NB});
-        if (rest2 > LU_NB) v.push_back({rest2, rest2 - LU_NB, 2 * LU_NB});
+    for (int p = 0; p < npanel; p += group) {
+        int g = std::min(group, npanel - p), k0 = p * LU_NB;
+        for (int j = 1; j < g; j++) {
+            int kj = k0 + j * LU_NB;
+            v.push_back({Np - kj, LU_NB, j * LU_NB});
+            if (Np - kj - LU_NB > 0) v.push_back({LU_NB, Np - kj - LU_NB, j * LU_NB});
+        }
+        int kg = k0 + g * LU_NB, restg = Np - kg;
+        if (restg <= 0) break;
+        v.push_back({restg, LU_NB, g * LU_NB});
+        if (restg > LU_NB) v.push_back({restg, restg - LU_NB, g * LU_NB});
     }
     return v;
 }
@@ -1685,8 +1698,8 @@ int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai) {
         if (g_tune[dev].tuned_np[prec][Np / 128]) return 0;
     }
     std::map<std::array<int, 3>, std::array<int, 3>> reps;   // (mc, nc, kb) -> largest (M, N, K)
-    for (auto& sh : lu_gemm_shapes(Np)) {
-        std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), sh[2] >= 128};
+    for (auto& sh : lu_gemm_shapes(Np, lu_group<T>(Np))) {
+        std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), k_class(sh[2])};
         auto it = reps.find(key);
         if (it == reps.end() || (double)sh[0] * sh[1] > (double)it->second[0] * it->second[1]) reps[key] = sh;
     }
@@ -1759,20 +1772,25 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
     }
 }
 
-// Right-looking blocked LU over pairs of panels, with lookahead. For the pair
-// (p, p+1) at k0 = p NB:
-//   main stream:      fix p (pivot check / fallback + row swaps), U12_p, then
-//                     the K = 64 update of panel p+1's columns;
-//   panel stream s2:  factor panel p+1 (diagonal block, L21);
-//   main stream:      fix p+1, the K = 64 update of panel p+1's rows right of
-//                     it (L21_p rows x U12_p), U12_{p+1}, the K = 128 update of
-//                     panel p+2's columns with [L21_p L21_{p+1}] [U12_p; U12_{p+1}];
-//   panel stream s2:  factor panel p+2, while the main stream applies the same
-//                     K = 128 update to all columns right of panel p+2.
-// Deferring panel p's trailing update to the K = 128 product doubles the flops
-// per byte of C moved. Row interchanges of panel p+1 (fallback pivoting) hit
-// the not yet updated trailing rows and L21_p alike, which commutes with the
-// update. Events ev (2 per panel) order the two streams.
+// Right-looking blocked LU over groups of lu_group<T>(Np) panels, with lookahead.
+// For the group p .. p+g-1 at k0 = p NB (panel p already factored on the
+// panel stream s2, every column right of it up to date with the groups
+// before):
+//   for j = 0 .. g-1, panel q = p + j at kj = k0 + j NB:
+//     (j > 0) main stream: the K = j NB update of panel q's columns by the
+//             group's panels before it, [L21_p .. L21_{q-1}] [U12 ...];
+//             panel stream: factor panel q (diagonal block, L21);
+//     main stream: fix q (pivot check / fallback + row swaps elsewhere);
+//     (j > 0) the K = j NB update of panel q's rows right of it;
+//     U12_q = Linv_q A12;
+//   main stream: the K = g NB update of the next group's first panel, which
+//   the panel stream then factors while the main stream applies the same
+//   K = g NB update to every column right of it.
+// Deferring the group's trailing update to one K = g NB product divides the
+// passes over C by g. Row interchanges of panel q (fallback
+// pivoting) hit the trailing rows not yet updated by the group's earlier
+// panels and those panels' L21 alike, which commutes with the update. Events
+// ev (2 per panel) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
                int* ipiv, LuProfile* prof) {
@@ -1804,37 +1822,37 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 64), dim3(256), 0, s, Np, k0, Ar, Ai,
                                w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2);
     };
-    const int NB = LU_NB;
+    const int NB = LU_NB, group = lu_group<T>(Np);
     panel(s, 0);
-    for (int p = 0; p < npanel; p += 2) {
-        int k0 = p * NB, q = p / 2;
-        hipEvent_t evA = ev[4 * q], evB = ev[4 * q + 1], evC = ev[4 * q + 2], evD = ev[4 * q + 3];
-        if (p > 0) (void)hipStreamWaitEvent(s, ev[4 * q - 1], 0);   // panel p factored on s2
-        fix(p);
-        u12(p);
-        int rest = Np - k0 - NB;                                    // >= NB: panel p+1 exists
-        gemm_sub<T>(s, dev, Np, rest, NB, NB, Ar, Ai, k0 + NB, k0, k0, k0 + NB, k0 + NB, k0 + NB, prof);
-        (void)hipEventRecord(evA, s);
-        (void)hipStreamWaitEvent(s2, evA, 0);
-        panel(s2, p + 1);
-        (void)hipEventRecord(evB, s2);
-        (void)hipStreamWaitEvent(s, evB, 0);
-        fix(p + 1);
-        int rest2 = rest - NB;
-        if (rest2 <= 0) break;
-        // rows of panel p+1, right of it: A -= L21_p(those rows) U12_p
-        gemm_sub<T>(s, dev, Np, NB, rest2, NB, Ar, Ai, k0 + NB, k0, k0, k0 + 2 * NB, k0 + NB, k0 + 2 * NB, prof);
-        u12(p + 1);
-        // panel p+2's columns, then hand it to the panel stream
-        gemm_sub<T>(s, dev, Np, rest2, NB, 2 * NB, Ar, Ai, k0 + 2 * NB, k0, k0, k0 + 2 * NB, k0 + 2 * NB, k0 + 2 * NB,
-                    prof);
-        (void)hipEventRecord(evC, s);
-        (void)hipStreamWaitEvent(s2, evC, 0);
-        panel(s2, p + 2);
-        (void)hipEventRecord(evD, s2);
-        if (rest2 > NB)
-            gemm_sub<T>(s, dev, Np, rest2, rest2 - NB, 2 * NB, Ar, Ai, k0 + 2 * NB, k0, k0, k0 + 3 * NB, k0 + 2 * NB,
-                        k0 + 3 * NB, prof);
+    for (int p = 0; p < npanel; p += group) {
+        const int g = std::min(group, npanel - p), k0 = p * NB;
+        if (p > 0) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);   // panel p factored on s2
+        for (int j = 0; j < g; j++) {
+            const int q = p + j, kj = k0 + j * NB;
+            if (j > 0) {
+                // panel q's columns: A(kj.., q) -= L21(kj.., p..q-1) U12(p..q-1, q)
+                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof);
+                (void)hipEventRecord(ev[2 * q], s);
+                (void)hipStreamWaitEvent(s2, ev[2 * q], 0);
+                panel(s2, q);
+                (void)hipEventRecord(ev[2 * q + 1], s2);
+                (void)hipStreamWaitEvent(s, ev[2 * q + 1], 0);
+            }
+            fix(q);
+            if (j > 0 && Np - kj - NB > 0)   // panel q's rows right of it
+                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof);
+            u12(q);
+        }
+        const int kg = k0 + g * NB, restg = Np - kg;
+        if (restg <= 0) break;
+        // the next group's first panel, then hand it to the panel stream
+        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof);
+        (void)hipEventRecord(ev[2 * (p + g)], s);
+        (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
+        panel(s2, p + g);
+        (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
+        if (restg > NB)   // trailing update of everything right of it
+            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,

@@ -139,7 +139,7 @@ def main():
     prof_64 = body.profile_lu(kap, bm.MODE_FP64)
     achieved = prof_mx["flops_per_launch"] / (prof_mx["ms_gemm_per_launch"] * 1e-3)
     roofline = {
-        "kernel": "k_cgemm_sub<float, WM, WN> (LU trailing updates, FP32 MFMA 16x16x4, K = 64/128, autotuned tiles)",
+        "kernel": "k_cgemm_sub<float, WM, WN> (LU updates, FP32 MFMA 16x16x4, three real products per complex product, K = 64..256, autotuned tiles)",
         "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32, "traffic": measured_traffic(),
         "flops_per_launch": prof_mx["flops_per_launch"], "launches": int(prof_mx["gemm_launches"]),

@@ -91,8 +91,10 @@ def test_dense_solve_diag_dominant(ctx, n, precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("n", [64, 200])
+@pytest.mark.parametrize("n", [64, 200, 600])
 def test_dense_solve_needs_pivoting(ctx, n, precision):
+    # n = 600 (ten panels): groups of four panels, four and a final two, with
+    # interchanges in every panel
     rng = np.random.default_rng(100 + n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
