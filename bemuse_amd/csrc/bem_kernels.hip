@@ -173,72 +173,12 @@ __global__ void k_incident(int N, int Np, int nb, const double* __restrict__ bod
 }
 
 // ---------------------------------------------------------------------------
-// Tall-skinny products Y = op(M X) for an Np x Np complex matrix M and
-// nr <= 16 columns X, split over K slabs (partials reduced in a fixed order).
-// A work-group covers 256 rows (one per lane) of one slab; chunks of 64 rows
-// of X are staged in LDS and each lane keeps 8 columns of M in flight.
-// Optionally accumulates the row sums of |re| + |im| of M.
+// Tall-skinny products Y = M X for an Np x Np complex matrix M and nr <= 16
+// columns X, split over K slabs (partials reduced in a fixed order by
+// k_matvec_reduce); the kernel, k_matvec_mfma, is defined after the MFMA
+// helpers below.
 // ---------------------------------------------------------------------------
-constexpr int MV_ROWS = 256, MV_MAXR = 16, MV_KC = 64;
-
-template <typename TM, bool ROWABS>
-__global__ __launch_bounds__(256) void k_matvec_partial(int Np, int nr, int kslab, const TM* __restrict__ Mr,
-                                                        const TM* __restrict__ Mi, const double* __restrict__ Xr,
-                                                        const double* __restrict__ Xi, double* Pr, double* Pi,
-                                                        double* Pabs, const int* skip) {
-    if (skip && *skip) return;
-    __shared__ double xs[MV_KC][MV_MAXR][2];
-    int row = blockIdx.x * MV_ROWS + threadIdx.x, ks = blockIdx.y;
-    bool live = row < Np;
-    if (!live) row = Np - 1;           // keeps the loads in bounds; results discarded
-    double ar[MV_MAXR], ai[MV_MAXR];
-#pragma unroll
-    for (int j = 0; j < MV_MAXR; j++) { ar[j] = 0; ai[j] = 0; }
-    double sabs = 0.0;
-    int kb = ks * kslab, ke = min(Np, kb + kslab);
-    const TM* pr = Mr + row;
-    const TM* pi = Mi + row;
-    for (int kc = kb; kc < ke; kc += MV_KC) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < MV_KC * MV_MAXR; t += 256) {
-            int k = t % MV_KC, j = t / MV_KC;
-            bool ok = j < nr && kc + k < ke;
-            xs[k][j][0] = ok ? Xr[(size_t)j * Np + kc + k] : 0.0;
-            xs[k][j][1] = ok ? Xi[(size_t)j * Np + kc + k] : 0.0;
-        }
-        __syncthreads();
-        int kn = min(MV_KC, ke - kc);
-        for (int k8 = 0; k8 < kn; k8 += 8) {
-            double mr[8], mi[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                size_t o = (size_t)(kc + k8 + u) * Np;
-                mr[u] = (double)pr[o]; mi[u] = (double)pi[o];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (ROWABS) sabs += fabs(mr[u]) + fabs(mi[u]);
-#pragma unroll
-                for (int j = 0; j < MV_MAXR; j++) {
-                    if (j < nr) {
-                        double xr = xs[k8 + u][j][0], xi = xs[k8 + u][j][1];
-                        ar[j] += mr[u] * xr - mi[u] * xi;
-                        ai[j] += mr[u] * xi + mi[u] * xr;
-                    }
-                }
-            }
-        }
-    }
-    if (!live) return;
-#pragma unroll
-    for (int j = 0; j < MV_MAXR; j++) {
-        if (j < nr) {
-            size_t o = ((size_t)ks * MV_MAXR + j) * Np + row;
-            Pr[o] = ar[j]; Pi[o] = ai[j];
-        }
-    }
-    if (ROWABS) Pabs[(size_t)ks * Np + row] = sabs;
-}
+constexpr int MV_ROWS = 64, MV_MAXR = 16, MV_KC = 64;
 
 // Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1); with Pabs, also the max
 // row sum of |re| + |im| into norm[0] (one work-group) -- see k_rownorm.
@@ -1569,6 +1509,67 @@ void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, c
                        Xr, Xi, ldx, PhiIr, PhiIi, DAr, DAi);
 }
 
+// Y_partial(ks) = M(:, slab ks) X(slab ks, :) on FP64 MFMA 16x16x4: wave w of
+// a work-group owns rows 16w..16w+15 of the group's 64, all 16 right-hand
+// sides in the accumulators (lane column = right-hand side, columns >= nr are
+// zero). Chunks of 64 k of X are staged in LDS; each lane loads its 16 M
+// operands of a chunk before the MFMAs, so a wave has 16 KB of M in flight.
+// ROWABS: also the row sums of |re| + |im| of M (for |A|_inf), per slab.
+template <typename TM, bool ROWABS>
+__global__ __launch_bounds__(256) void k_matvec_mfma(int Np, int nr, int kslab, const TM* __restrict__ Mr,
+                                                     const TM* __restrict__ Mi, const double* __restrict__ Xr,
+                                                     const double* __restrict__ Xi, double* Pr, double* Pi,
+                                                     double* Pabs, const int* skip) {
+    typedef MfmaT<double> MT;
+    typedef MT::acc acc_t;
+    if (skip && *skip) return;
+    __shared__ double xs[MV_KC][MV_MAXR][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, g = lane >> 4;
+    const int row = blockIdx.x * MV_ROWS + wave * 16 + i, ks = blockIdx.y;
+    const int kb = ks * kslab, ke = min(Np, kb + kslab);
+    acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
+    double sabs = 0.0;
+    for (int kc = kb; kc < ke; kc += MV_KC) {
+        double mr[16], mi[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            size_t o = (size_t)(kc + 4 * t + g) * Np + row;
+            mr[t] = (double)Mr[o]; mi[t] = (double)Mi[o];
+        }
+        __syncthreads();                 // the previous chunk's operands are read
+        for (int t = tid; t < MV_KC * MV_MAXR; t += 256) {
+            int k = t % MV_KC, j = t / MV_KC;
+            bool ok = j < nr;
+            xs[k][j][0] = ok ? Xr[(size_t)j * Np + kc + k] : 0.0;
+            xs[k][j][1] = ok ? Xi[(size_t)j * Np + kc + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            double xr = xs[4 * t + g][i][0], xi = xs[4 * t + g][i][1];
+            cr = MT::mma(mr[t], xr, cr);
+            cr = MT::mma(-mi[t], xi, cr);
+            ci = MT::mma(mr[t], xi, ci);
+            ci = MT::mma(mi[t], xr, ci);
+            if (ROWABS) sabs += fabs(mr[t]) + fabs(mi[t]);
+        }
+    }
+    if (ROWABS) {
+        sabs += __shfl_xor(sabs, 16);
+        sabs += __shfl_xor(sabs, 32);
+        if (g == 0) Pabs[(size_t)ks * Np + row] = sabs;
+    }
+    if (i < nr) {
+        const int rb = blockIdx.x * MV_ROWS + wave * 16;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            size_t o = ((size_t)ks * MV_MAXR + i) * Np + rb + MT::row(g, r);
+            Pr[o] = cr[r]; Pi[o] = ci[r];
+        }
+    }
+}
+
 int matvec_kslabs(int Np) { return Np >= 4096 ? 32 : (Np >= 1024 ? 16 : 4); }
 
 template <typename TM>
@@ -1578,12 +1579,12 @@ void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, co
     int nks = matvec_kslabs(Np);
     int kslab = (Np / nks + 63) / 64 * 64;
     nks = (Np + kslab - 1) / kslab;
-    dim3 grid(cdiv(Np, MV_ROWS), nks);
+    dim3 grid(Np / MV_ROWS, nks);
     if (Pabs)
-        hipLaunchKernelGGL((k_matvec_partial<TM, true>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi,
+        hipLaunchKernelGGL((k_matvec_mfma<TM, true>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi,
                            Pabs, skip);
     else
-        hipLaunchKernelGGL((k_matvec_partial<TM, false>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr,
+        hipLaunchKernelGGL((k_matvec_mfma<TM, false>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr,
                            Pi, Pabs, skip);
     hipLaunchKernelGGL(k_matvec_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, Np, nr, nks, Pr, Pi, Br, Bi,
                        mode, Yr, Yi, skip);
