@@ -147,6 +147,7 @@ _SIGS = {
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
+    "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
     "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
                                  _PD]),
@@ -404,6 +405,12 @@ class Context:
         out = np.zeros((len(X), 8))
         _check(lib().bem_eval_havelock(self._h, 64, len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
         return out
+
+    def measure_peaks(self):
+        """Measured FP32 / FP64 MFMA TFLOP/s and streaming-copy GB/s of the device."""
+        out = np.zeros(3)
+        _check(lib().bem_measure_peaks(self._h, _dp(out)))
+        return dict(fp32_mfma_tflops=float(out[0]), fp64_mfma_tflops=float(out[1]), copy_gbs=float(out[2]))
 
     def set_graphs(self, enable):
         """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""

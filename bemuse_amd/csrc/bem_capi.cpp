@@ -1267,6 +1267,13 @@ int bem_surrogate_default_config(bem_surrogate_config* cfg) {
 
 int bem_surrogate_param_count(void) { return kSurParams; }
 
+int bem_measure_peaks(bem_ctx* c, double* out) {
+    if (!c || !out) return fail(BEM_ERR_ARG, "bem_measure_peaks: null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (measure_peaks(c->stream, out) != 0) return fail(BEM_ERR_HIP, "bem_measure_peaks: a HIP call failed");
+    return BEM_OK;
+}
+
 int bem_set_graphs(bem_ctx* c, int enable) {
     if (!c) return fail(BEM_ERR_ARG, "bem_set_graphs: null context");
     c->graphs = enable != 0;

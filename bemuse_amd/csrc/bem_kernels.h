@@ -19,6 +19,8 @@ void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int c
                          const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
                          float* D32i);
 void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out);
+// FP32 and FP64 MFMA TFLOP/s and streaming-copy GB/s measured on the current device; 0 on success.
+int measure_peaks(hipStream_t s, double* out);
 void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,
                      const double* freq, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
                      double* DAr, double* DAi);

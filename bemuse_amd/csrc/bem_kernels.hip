@@ -1570,6 +1570,86 @@ __global__ __launch_bounds__(256) void k_matvec_mfma(int Np, int nr, int kslab, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Machine-peak microbenchmarks (reported beside the published peaks): MFMA
+// 16x16x4 chains on registers (8 independent accumulators per wave, operands
+// from the thread index so nothing folds), and a streaming copy.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_peak_mfma(int iters, T* sink) {
+    typedef MfmaT<T> MT;
+    typedef typename MT::acc acc_t;
+    T a = (T)(threadIdx.x & 7) * (T)0.125, b = (T)(threadIdx.x >> 3) * (T)0.0625;
+    acc_t c[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[q] = acc_t{(T)q, 0, 0, 0};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) c[q] = MT::mma(a, b, c[q]);
+    }
+    T v = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) v += c[q][0] + c[q][1] + c[q][2] + c[q][3];
+    if (v == (T)-1.2345) sink[blockIdx.x] = v;    // never true; keeps the chains live
+}
+
+__global__ __launch_bounds__(256) void k_peak_copy(size_t n4, const float4* __restrict__ src, float4* dst) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride) dst[t] = src[t];
+}
+
+int measure_peaks(hipStream_t s, double* out) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 1; }
+    double* sink = nullptr;
+    float4 *a = nullptr, *b = nullptr;
+    const size_t bytes = (size_t)1 << 30;
+    int rc = 0;
+    if (hipMalloc(&sink, 4096 * sizeof(double)) != hipSuccess || hipMalloc(&a, bytes) != hipSuccess ||
+        hipMalloc(&b, bytes) != hipSuccess) {
+        rc = 1;
+    } else {
+        const int wgs = ncu * 8, iters = 8192;
+        auto time_launch = [&](auto launch) {
+            launch();   // warm-up
+            (void)hipEventRecord(e0, s);
+            launch();
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return (double)ms * 1e-3;
+        };
+        const double mfma_flops = (double)wgs * 4 * iters * 8 * (16.0 * 16 * 4 * 2);
+        double t32 = time_launch([&] {
+            hipLaunchKernelGGL(k_peak_mfma<float>, dim3(wgs), dim3(256), 0, s, iters, (float*)sink);
+        });
+        double t64 = time_launch([&] {
+            hipLaunchKernelGGL(k_peak_mfma<double>, dim3(wgs), dim3(256), 0, s, iters, sink);
+        });
+        (void)hipMemsetAsync(a, 0, bytes, s);
+        const size_t n4 = bytes / sizeof(float4);
+        double tc = time_launch([&] {
+            hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, n4, (const float4*)a, b);
+        });
+        out[0] = mfma_flops / t32 / 1e12;
+        out[1] = mfma_flops / t64 / 1e12;
+        out[2] = 2.0 * (double)bytes / tc / 1e9;
+        if (hipStreamSynchronize(s) != hipSuccess) rc = 1;
+    }
+    if (sink) (void)hipFree(sink);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
 int matvec_kslabs(int Np) { return Np >= 4096 ? 32 : (Np >= 1024 ? 16 : 4); }
 
 template <typename TM>

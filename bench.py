@@ -52,6 +52,16 @@ def measured_traffic():
     return None
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def rel(a, b):
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
@@ -150,6 +160,9 @@ def main():
         "lu_ms": {"mixed": prof_mx["ms_lu"], "fp64": prof_64["ms_lu"],
                   "gemm_share_mixed": prof_mx["ms_gemm"] / prof_mx["ms_lu"],
                   "gemm_share_fp64": prof_64["ms_gemm"] / prof_64["ms_lu"]},
+        # the device's peaks measured in this run (MFMA chains on registers, a
+        # 1 GiB streaming copy), beside the published ones used for "frac"
+        "measured_peaks": ctx.measure_peaks(),
     }
 
     surrogate = None
@@ -176,6 +189,7 @@ def main():
         err_vs_oracle = {"mixed": max(abf_err(a, b) for a, b in zip(gm, ref)),
                          "fp64": max(abf_err(a, b) for a, b in zip(g64, ref))}
         cpu = {"value": 1.0 / t_cpu, "unit": "frequency-solves/s", "cores": nthreads, "kind": "port",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
                "sample": f"{len(sample)} frequency (omega={sample[0]:.4f} rad/s) of the same {N}-panel workload, "
                          f"FP64 oracle (OpenMP assembly + blocked partial-pivot LU); Rankine assembly "
                          f"{t_rk:.1f} s once per body, excluded like on the GPU",

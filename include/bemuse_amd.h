@@ -169,6 +169,13 @@ void bem_destroy(bem_ctx* ctx);
  * solves are host loops). */
 int bem_set_graphs(bem_ctx* ctx, int enable);
 
+/* Measured machine peaks of the context's device, for reporting beside the
+ * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
+ * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +
+ * 1 GiB written). Allocates 2 GiB for the copy while it runs. No counterpart
+ * in the reference (a measurement aid). */
+int bem_measure_peaks(bem_ctx* ctx, double* out);
+
 /* Uploads the geometry of a mesh (Hydrodynamic_Radiation_Solver::Setup,
  * Hydrodynamic_Solver.cpp:269-279, geometry part). */
 int bem_body_create(bem_ctx* ctx, const bem_mesh* m, bem_body** out);
