@@ -813,11 +813,11 @@ __global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int
 // system's imaginary part (the wave term, O(K)) is orders of magnitude
 // below its real part, which is where the damping comes from. FP64, the
 // unrefined factorisation, therefore keeps the four-product form.
-// A is staged negated and the accumulators start from C, loaded at the top
-// of the kernel so that its latency overlaps the main loop:
-//   three products: P = -T1,  Q = Cr + T2,  R = (Ci - Cr) - T3, and
-//                   Cr - Re(A B) = Q + P,  Ci - Im(A B) = R - P + Q;
-//   four products:  Q = Cr - Re(A B),  R = Ci - Im(A B).
+// A is staged negated and the accumulators start from zero; C is read into
+// registers during the first chunk and added at the end:
+//   three products: P = -T1,  Q = T2,  R = -T3, and
+//                   Cr - Re(A B) = Cr + (Q + P),  Ci - Im(A B) = Ci + ((R - P) + Q);
+//   four products:  Q = -Re(A B),  R = -Im(A B),  C - A B = (Cr + Q, Ci + R).
 // ---------------------------------------------------------------------------
 constexpr int GK = 16;           // k per LDS chunk
 constexpr int GKP = GK + 4;      // padded row length in LDS (bank spread)
@@ -912,6 +912,11 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
     bool active = (row0 < mrows) && (col0 < ncols);
     constexpr bool kGauss = sizeof(T) == 4;
     acc_t p1[WM][WN], q2[WM][WN], r3[WM][WN];
+    // C is added after the last product, so its loads (issued behind those
+    // of the second chunk) overlap the first chunks instead of holding up
+    // the first MFMA; the work-groups of one dispatch run in step, and C
+    // read up front made every CU wait on the same burst.
+    acc_t c_re[WM][WN], c_im[WM][WN];
     int nchunk = K / GK;
     gload(0);
 #pragma unroll
@@ -919,24 +924,27 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 #pragma unroll
         for (int b = 0; b < WN; b++) {
             p1[a][b] = acc_t{0, 0, 0, 0};
-            if (active) {
-                size_t off = (size_t)(cc + colt + col0 + 16 * b + i) * ldl + rc + rowt + row0 + 16 * a;
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    T c_r = Pr[off + MT::row(g, r)], c_i = Pi[off + MT::row(g, r)];
-                    q2[a][b][r] = c_r;
-                    r3[a][b][r] = kGauss ? c_i - c_r : c_i;
-                }
-            } else {
-                q2[a][b] = acc_t{0, 0, 0, 0};
-                r3[a][b] = acc_t{0, 0, 0, 0};
-            }
+            q2[a][b] = acc_t{0, 0, 0, 0};
+            r3[a][b] = acc_t{0, 0, 0, 0};
         }
     sstore(0);
     __syncthreads();
     for (int c = 0; c < nchunk; c++) {
         int buf = c & 1;
         if (c + 1 < nchunk) gload((c + 1) * GK);
+        if (c == 0 && active) {
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) {
+                    size_t off = (size_t)(cc + colt + col0 + 16 * b + i) * ldl + rc + rowt + row0 + 16 * a;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        c_re[a][b][r] = Pr[off + MT::row(g, r)];
+                        c_im[a][b][r] = Pi[off + MT::row(g, r)];
+                    }
+                }
+        }
         if (active) {
             T ar[WM][4], ai[WM][4], br[WN][4], bi[WN][4];
 #pragma unroll
@@ -1011,11 +1019,11 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
             for (int r = 0; r < 4; r++) {
                 int rr = MT::row(g, r);
                 if (kGauss) {
-                    Pr[off + rr] = q2[a][b][r] + p1[a][b][r];
-                    Pi[off + rr] = (r3[a][b][r] - p1[a][b][r]) + q2[a][b][r];
+                    Pr[off + rr] = c_re[a][b][r] + (q2[a][b][r] + p1[a][b][r]);
+                    Pi[off + rr] = c_im[a][b][r] + ((r3[a][b][r] - p1[a][b][r]) + q2[a][b][r]);
                 } else {
-                    Pr[off + rr] = q2[a][b][r];
-                    Pi[off + rr] = r3[a][b][r];
+                    Pr[off + rr] = c_re[a][b][r] + q2[a][b][r];
+                    Pi[off + rr] = c_im[a][b][r] + r3[a][b][r];
                 }
             }
         }
