@@ -224,8 +224,8 @@ __global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm, con
 // pivoting (max |a_ik|, first maximum) chooses exactly these pivots iff every
 // multiplier satisfies |l_ik| <= 1, so the kernels check that bound; a panel
 // that violates it is restored from its saved copy and refactored with row
-// interchanges (k_lu_panel_gepp), and the interchanges are applied to the
-// other columns (k_laswp). Results therefore follow partial pivoting.
+// interchanges (k_lu_panel_gepp), which also applies the interchanges to the
+// other columns. Results therefore follow partial pivoting.
 //
 // Per panel p the factorisation also keeps the inverses of the unit-lower and
 // upper triangles of its diagonal block (Linv_p, Uinv_p, NB x NB column-major
@@ -537,6 +537,8 @@ __device__ inline void inv_column(DiagLds<T>& S, T* Li_r, T* Li_i, T* Ui_r, T* U
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
                                                  T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
+    // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
+    __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     __shared__ DiagLds<T> S;
@@ -625,6 +627,8 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
                                                 T* save_r, T* save_i, int* flag) {
+    // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
+    __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -665,6 +669,8 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
 // columns (all NB rows).
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i) {
+    // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
+    __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -691,12 +697,15 @@ __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, co
 
 // Fallback for a rejected panel: restore it and factor it with partial
 // pivoting in one work-group, streaming through global memory; then rebuild
-// the triangle inverses of its diagonal block. ipiv receives absolute row
-// indices. Runs only if flag[panel] is set.
+// the triangle inverses of its diagonal block and apply the interchanges to
+// all other columns. ipiv receives absolute row indices. Runs only if
+// flag[panel] is set (rare), so one launch per panel serves both steps.
 template <typename T>
 __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T* Ai, const T* sd_r, const T* sd_i,
                                                        const T* s21_r, const T* s21_i, T* Li_r, T* Li_i, T* Ui_r,
                                                        T* Ui_i, int* flag, int* ipiv) {
+    // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
+    __builtin_amdgcn_s_setprio(3);
     int f = flag[k0 / LU_NB];
     if (!(f & 1)) return;
     int tid = threadIdx.x;
@@ -771,22 +780,18 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
     }
     __syncthreads();
     diag_inverses<T>(lur, lui, rdr, rdi, Li_r, Li_i, Ui_r, Ui_i);
-}
-
-// Apply the panel's interchanges to all columns outside the panel
-// (sequentially in pivot order, parallel over columns). Only if flagged.
-template <typename T>
-__global__ void k_laswp(int Np, int k0, T* Ar, T* Ai, const int* flag, const int* ipiv) {
-    if (!(flag[k0 / LU_NB] & 1)) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Np || (c >= k0 && c < k0 + LU_NB)) return;
-    size_t a = (size_t)c * Np;
-    for (int j = k0; j < k0 + LU_NB; j++) {
-        int p = ipiv[j];
-        if (p != j) {
-            T tr = Ar[a + j], ti = Ai[a + j];
-            Ar[a + j] = Ar[a + p]; Ai[a + j] = Ai[a + p];
-            Ar[a + p] = tr; Ai[a + p] = ti;
+    // the panel's interchanges on every column outside it, in pivot order
+    // (one column per thread; ipiv was written before the barriers above)
+    for (int c = tid; c < Np; c += 1024) {
+        if (c >= k0 && c < k0 + LU_NB) continue;
+        size_t a = (size_t)c * Np;
+        for (int j = k0; j < k0 + LU_NB; j++) {
+            int p = ipiv[j];
+            if (p != j) {
+                T tr = Ar[a + j], ti = Ai[a + j];
+                Ar[a + j] = Ar[a + p]; Ai[a + j] = Ai[a + p];
+                Ar[a + p] = tr; Ai[a + p] = ti;
+            }
         }
     }
 }
@@ -831,7 +836,11 @@ template <> struct Vec4<double> { typedef double4 type; };
 // the same order for any (WM, WN), so all tile shapes give identical results.
 template <typename T, int WM, int WN>
 __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __restrict__ Pr, T* __restrict__ Pi,
-                                                      int ld, int ra, int ca, int rb, int cb, int rc, int cc) {
+                                                      int ld, int ra, int ca, int rb, int cb, int rc, int cc,
+                                                      int critical) {
+    // the factorisation's thin updates (panel columns and rows) gate the
+    // next panel: their waves issue ahead of the trailing updates on a CU
+    if (critical) __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     typedef typename Vec4<T>::type v4;
@@ -1718,21 +1727,21 @@ inline int size_class(int x) {
 
 template <typename T, int WM, int WN>
 void launch_tile(hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int ra, int ca, int rb, int cb, int rc,
-                 int cc) {
+                 int cc, int critical = 0) {
     constexpr int TM = 32 * WM, TN = 32 * WN;
     int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
     int per = (tiles + 7) / 8;
     hipLaunchKernelGGL((k_cgemm_sub<T, WM, WN>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb,
-                       rc, cc);
+                       rc, cc, critical);
 }
 
 template <typename T>
 void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int ld, int ra, int ca, int rb, int cb,
-                int rc, int cc) {
+                int rc, int cc, int critical = 0) {
     switch (cfg) {
-    case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-    case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
-    default: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc); return;
+    case 1: launch_tile<T, 4, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical); return;
+    case 2: launch_tile<T, 2, 4>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical); return;
+    default: launch_tile<T, 2, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical); return;
     }
 }
 
@@ -1841,7 +1850,7 @@ template int gemm_tile_choice<double>(int, int, int);
 // around the launch and the flops of the unpadded part).
 template <typename T>
 static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar, T* Ai, int ra, int ca, int rb,
-                     int cb, int rc, int cc, LuProfile* prof) {
+                     int cb, int rc, int cc, LuProfile* prof, int critical) {
     bool timed = prof && prof->n < LuProfile::kMax;
     if (timed) {
         if (!prof->ev[2 * prof->n]) {
@@ -1850,7 +1859,7 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
         }
         (void)hipEventRecord(prof->ev[2 * prof->n], s);
     }
-    launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc);
+    launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
     if (timed) {
         (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
         double mu = (double)std::max(0, std::min(M, prof->N - rc));
@@ -1903,7 +1912,6 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
         hipLaunchKernelGGL(k_lu_panel_gepp<T>, dim3(1), dim3(1024), 0, s, Np, k0, Ar, Ai, w.sd_r, w.sd_i, w.s21_r,
                            w.s21_i, Lir, Lii, Uir, Uii, flag, ipiv);
-        hipLaunchKernelGGL(k_laswp<T>, dim3(cdiv(Np, 256)), dim3(256), 0, s, Np, k0, Ar, Ai, flag, ipiv);
     };
     auto u12 = [&](int p) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
@@ -1920,7 +1928,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             const int q = p + j, kj = k0 + j * NB;
             if (j > 0) {
                 // panel q's columns: A(kj.., q) -= L21(kj.., p..q-1) U12(p..q-1, q)
-                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof);
+                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof, 1);
                 (void)hipEventRecord(ev[2 * q], s);
                 (void)hipStreamWaitEvent(s2, ev[2 * q], 0);
                 panel(s2, q);
@@ -1929,19 +1937,19 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             }
             fix(q);
             if (j > 0 && Np - kj - NB > 0)   // panel q's rows right of it
-                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof);
+                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof, 1);
             u12(q);
         }
         const int kg = k0 + g * NB, restg = Np - kg;
         if (restg <= 0) break;
         // the next group's first panel, then hand it to the panel stream
-        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof);
+        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1);
         (void)hipEventRecord(ev[2 * (p + g)], s);
         (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
         panel(s2, p + g);
         (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
         if (restg > NB)   // trailing update of everything right of it
-            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof);
+            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,

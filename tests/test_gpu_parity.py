@@ -133,6 +133,35 @@ def test_solve_barge_both_modes(ctx):
     body.close()
 
 
+PARAMETRIC = {
+    "tapered_spar": lambda: bm.Mesh.tapered_spar(4.7, 3.25, 20.0, 12.0, 4.0, 3, 3, 2, 2, 12),
+    "semi_ellipsoid_tri": lambda: bm.Mesh.semi_ellipsoid(3.0, 2.0, 1.5, 5, 12, triangles=True),
+    "submerged_ellipsoid": lambda: bm.Mesh.ellipsoid(3.0, 2.0, 1.5, 5.0, 6, 12),
+    "wigley_tri": lambda: bm.Mesh.wigley(10.0, 1.0, 0.625, 12, 5, triangles=True),
+    "barge_cosine": lambda: bm.Mesh.barge(20.0, 10.0, 5.0, 8, 4, 3, cosine=True),
+}
+
+
+@pytest.mark.parametrize("name", sorted(PARAMETRIC))
+def test_parametric_bodies_both_modes(ctx, name):
+    """Triangle-only meshes, a fully submerged body, cosine-clustered grids."""
+    mesh = PARAMETRIC[name]()
+    nodes, elems = mesh.arrays()
+    body = bm.Body(ctx, mesh)
+    print(f"\n[gpu] {name} N={body.N}", flush=True)
+    body.assemble_rankine()
+    ob = oracle.Body(nodes, elems)
+    betas = [0.0, 0.5]
+    r = ob.solve(0.8, betas=betas)
+    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+        g = body.solve(0.8, betas=betas, mode=mode)
+        for key in ("A", "B", "F"):
+            assert relerr(g[key], r[key]) < tol, (name, mode, key, relerr(g[key], r[key]))
+        if mode == bm.MODE_MIXED:
+            assert g["info"]["fell_back_fp64"] == 0
+    body.close()
+
+
 def test_compose_cases(cyl):
     """Omega == 0 (Rankine sum) and the infinite-frequency composition."""
     mesh, body, ob = cyl
