@@ -546,12 +546,24 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = lane & 15, g = lane >> 4;       // MFMA lane coordinates
     if (tid == 0) bad = 0;
-    for (int t = tid; t < LU_NB * LU_NB; t += 256) {
-        int c = t / LU_NB, r = t % LU_NB;
-        size_t gi = (size_t)(k0 + c) * Np + k0 + r;
-        T vr = Ar[gi], vi = Ai[gi];
-        S.ar[c][r] = vr; S.ai[c][r] = vi;
-        save_r[t] = vr; save_i[t] = vi;
+    {
+        // all loads first: the save stores below may alias them as far as
+        // the compiler knows, so interleaving would make every load wait for
+        // the previous store (one memory round trip per element pair)
+        constexpr int PER = LU_NB * LU_NB / 256;
+        T vr[PER], vi[PER];
+#pragma unroll
+        for (int n = 0; n < PER; n++) {
+            int t = tid + 256 * n, c = t / LU_NB, r = t % LU_NB;
+            size_t gi = (size_t)(k0 + c) * Np + k0 + r;
+            vr[n] = Ar[gi]; vi[n] = Ai[gi];
+        }
+#pragma unroll
+        for (int n = 0; n < PER; n++) {
+            int t = tid + 256 * n, c = t / LU_NB, r = t % LU_NB;
+            S.ar[c][r] = vr[n]; S.ai[c][r] = vi[n];
+            save_r[t] = vr[n]; save_i[t] = vi[n];
+        }
     }
     __syncthreads();
     int mybad = 0;
@@ -634,11 +646,21 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int rowl = blockIdx.x * 64 + wave * 16;             // relative to k0 + NB
     int row0 = k0 + LU_NB + rowl;
-    for (int t = lane; t < 16 * LU_NB; t += 64) {
-        int r = t % 16, c = t / 16;
-        size_t g = (size_t)(k0 + c) * Np + row0 + r;
-        save_r[(size_t)c * Np + rowl + r] = Ar[g];
-        save_i[(size_t)c * Np + rowl + r] = Ai[g];
+    {
+        constexpr int PER = 16 * LU_NB / 64;      // loads first (see k_lu_diag)
+        T vr[PER], vi[PER];
+#pragma unroll
+        for (int n = 0; n < PER; n++) {
+            int t = lane + 64 * n, r = t % 16, c = t / 16;
+            size_t g = (size_t)(k0 + c) * Np + row0 + r;
+            vr[n] = Ar[g]; vi[n] = Ai[g];
+        }
+#pragma unroll
+        for (int n = 0; n < PER; n++) {
+            int t = lane + 64 * n, r = t % 16, c = t / 16;
+            save_r[(size_t)c * Np + rowl + r] = vr[n];
+            save_i[(size_t)c * Np + rowl + r] = vi[n];
+        }
     }
     int i = lane & 15, g = lane >> 4;
     acc_t cr[4], ci[4];
