@@ -341,11 +341,11 @@ __device__ inline void cmma16(const T* ar_, const T* ai_, size_t lda, const T* b
 // Factor the NB x NB diagonal block at (k0, k0) without interchanges and form
 // the inverses of its unit-lower and upper triangles, blocked by DB = 16 so
 // that most of the work is 16 x 16 MFMA block products and the work-group
-// meets at 14 barriers (the unblocked form needs one per column, 128 in all).
+// meets at 18 barriers (the unblocked form needs one per column, 128 in all).
 // The block lives in LDS, [col][row]. For sub-block column kb = 0..3:
 //   wave 0 factors the 16 x 16 diagonal block in registers (lane = row, the
-//   pivot row broadcast with readlane) and forms its triangle inverses
-//   Li_kk, Ui_kk (lane = column);
+//   pivot row broadcast with readlane); then waves 0 and 1 form Li_kk and
+//   Ui_kk side by side from the factor in LDS (lane = column of the inverse);
 //   the waves share L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb),
 //   then the trailing blocks A_ij -= L_ik U_kj.
 // Then wave j forms block column j of both inverses, keeping its blocks in
@@ -410,10 +410,10 @@ __device__ inline void blk_mma_rb(const T* ar_, const T* ai_, int lda, const typ
 }
 
 // Wave 0: the 16 x 16 diagonal block at (o, o) factored in registers (lane
-// rho & 15 holds row rho), then the columns of its triangle inverses
-// (lane & 15 = column) from the factor's registers; results to LDS.
+// rho & 15 holds row rho, the pivot row broadcast with readlane), written
+// back to LDS.
 template <typename T>
-__device__ inline void diag16(DiagLds<T>& S, int o, int kb, int lane, int& mybad) {
+__device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad) {
     const int rho = lane & 15;
     const bool writer = lane < DB;
     T ar[DB], ai[DB];
@@ -444,42 +444,56 @@ __device__ inline void diag16(DiagLds<T>& S, int o, int kb, int lane, int& mybad
 #pragma unroll
         for (int c = 0; c < DB; c++) { S.ar[o + c][o + rho] = ar[c]; S.ai[o + c][o + rho] = ai[c]; }
     }
+}
+
+// Column (lane & 15) of the unit-lower inverse of the factored diagonal
+// block by forward substitution, L(r, k) read from LDS as broadcasts (the
+// factor is final here, so register broadcasts would all be hoisted and
+// spill).
+template <typename T>
+__device__ inline void linv16(DiagLds<T>& S, int o, int kb, int lane) {
     const int col = lane & 15;
     T xr[DB], xi[DB];
-    // column col of the unit-lower inverse: forward substitution, L(r, k) from lane r
 #pragma unroll
     for (int r = 0; r < DB; r++) { xr[r] = (r == col) ? (T)1 : (T)0; xi[r] = 0; }
 #pragma unroll
     for (int k = 0; k < DB - 1; k++) {
 #pragma unroll
         for (int r = k + 1; r < DB; r++) {
-            T lr = bcast(ar[k], r), li = bcast(ai[k], r);
+            T lr = S.ar[o + k][o + r], li = S.ai[o + k][o + r];
             xr[r] -= lr * xr[k] - li * xi[k];
             xi[r] -= lr * xi[k] + li * xr[k];
         }
     }
-    if (writer) {
+    if (lane < DB) {
 #pragma unroll
         for (int r = 0; r < DB; r++) { S.lr[kb][col][r] = xr[r]; S.li[kb][col][r] = xi[r]; }
     }
-    // column col of the upper inverse: back substitution, U(r, k) from lane r
+}
+
+// Column (lane & 15) of the upper inverse by back substitution, U(r, k) read
+// from LDS as broadcasts; runs on a second wave beside linv16.
+template <typename T>
+__device__ inline void uinv16(DiagLds<T>& S, int o, int kb, int lane) {
+    const int col = lane & 15;
+    T xr[DB], xi[DB];
 #pragma unroll
     for (int r = 0; r < DB; r++) { xr[r] = (r == col) ? (T)1 : (T)0; xi[r] = 0; }
 #pragma unroll
     for (int k = DB - 1; k >= 0; k--) {
-        T dr = bcast(ar[k], k), di = bcast(ai[k], k);
+        T dr = S.ar[o + k][o + k], di = S.ai[o + k][o + k];
         T inv = (T)1 / cabs2(dr, di);
         T qr = dr * inv, qi = -di * inv;
         T yr = xr[k] * qr - xi[k] * qi, yi = xr[k] * qi + xi[k] * qr;
         xr[k] = yr; xi[k] = yi;
 #pragma unroll
         for (int r = 0; r < k; r++) {
-            T u_r = bcast(ar[k], r), u_i = bcast(ai[k], r);
+            T u_r = S.ar[o + k][o + r], u_i = S.ai[o + k][o + r];
             xr[r] -= u_r * yr - u_i * yi;
             xi[r] -= u_r * yi + u_i * yr;
         }
     }
-    if (writer) {
+    if (lane < DB) {
 #pragma unroll
         for (int r = 0; r < DB; r++) { S.ur[kb][col][r] = xr[r]; S.ui[kb][col][r] = xi[r]; }
     }
@@ -569,7 +583,10 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     int mybad = 0;
     for (int kb = 0; kb < 4; kb++) {
         const int o = DB * kb;
-        if (wave == 0) diag16<T>(S, o, kb, lane, mybad);
+        if (wave == 0) factor16<T>(S, o, lane, mybad);
+        __syncthreads();
+        if (wave == 0) linv16<T>(S, o, kb, lane);
+        else if (wave == 1) uinv16<T>(S, o, kb, lane);
         __syncthreads();
         // L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb), one block per wave in turn
         const int nrest = 3 - kb;
