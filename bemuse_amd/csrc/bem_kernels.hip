@@ -649,10 +649,12 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     if (tid == 0 && bad) atomicOr(flag + k0 / LU_NB, bad);
 }
 
-// L21 = A21 Uinv_p in place for rows k0+NB .. Np-1. Each wave owns 16 rows
-// (all NB columns), so reading and overwriting them cannot race. Saves A21
-// (rows relative to k0+NB, ld Np) for the pivoting fallback and flags
-// multipliers with |l| > 1.
+// L21 = A21 Uinv_p in place for rows k0+NB .. Np-1. A work-group owns 16
+// rows and wave b the 16 columns 16 b .. 16 b + 15 of them (a quarter of the
+// MFMA chain each, four times the work-groups of one wave per 16 rows); all
+// waves read all NB columns of the rows, so they meet before overwriting.
+// Saves A21 (rows relative to k0+NB, ld Np) for the pivoting fallback and
+// flags multipliers with |l| > 1.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
                                                 T* save_r, T* save_i, int* flag) {
@@ -660,52 +662,48 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int rowl = blockIdx.x * 64 + wave * 16;             // relative to k0 + NB
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int rowl = blockIdx.x * 16;                          // relative to k0 + NB
     int row0 = k0 + LU_NB + rowl;
     {
-        constexpr int PER = 16 * LU_NB / 64;      // loads first (see k_lu_diag)
+        constexpr int PER = 16 * LU_NB / 256;            // loads first (see k_lu_diag)
         T vr[PER], vi[PER];
 #pragma unroll
         for (int n = 0; n < PER; n++) {
-            int t = lane + 64 * n, r = t % 16, c = t / 16;
+            int t = tid + 256 * n, r = t % 16, c = t / 16;
             size_t g = (size_t)(k0 + c) * Np + row0 + r;
             vr[n] = Ar[g]; vi[n] = Ai[g];
         }
 #pragma unroll
         for (int n = 0; n < PER; n++) {
-            int t = lane + 64 * n, r = t % 16, c = t / 16;
+            int t = tid + 256 * n, r = t % 16, c = t / 16;
             save_r[(size_t)c * Np + rowl + r] = vr[n];
             save_i[(size_t)c * Np + rowl + r] = vi[n];
         }
     }
-    int i = lane & 15, g = lane >> 4;
-    acc_t cr[4], ci[4];
-#pragma unroll
-    for (int b = 0; b < 4; b++) { cr[b] = acc_t{0, 0, 0, 0}; ci[b] = acc_t{0, 0, 0, 0}; }
+    int i = lane & 15, g = lane >> 4, b = wave;
+    acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
     const T* A0r = Ar + (size_t)k0 * Np + row0;
     const T* A0i = Ai + (size_t)k0 * Np + row0;
-    for (int kk = 0; kk < LU_NB; kk += 16)
 #pragma unroll
-        for (int b = 0; b < 4; b++)
-            cmma16<T>(A0r + (size_t)kk * Np, A0i + (size_t)kk * Np, Np, Ui_r + (16 * b) * LU_NB + kk,
-                      Ui_i + (16 * b) * LU_NB + kk, LU_NB, cr[b], ci[b], i, g);
+    for (int kk = 0; kk < LU_NB; kk += 16)
+        cmma16<T>(A0r + (size_t)kk * Np, A0i + (size_t)kk * Np, Np, Ui_r + (16 * b) * LU_NB + kk,
+                  Ui_i + (16 * b) * LU_NB + kk, LU_NB, cr, ci, i, g);
+    __syncthreads();                                     // every wave has read the rows
     int mybad = 0;
 #pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            int rr = MT::row(g, r);
-            size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + rr;
-            T lr = cr[b][r], li = ci[b][r];
-            Ar[o] = lr; Ai[o] = li;
-            if (cabs2(lr, li) > (T)1) mybad = 1;
-        }
+    for (int r = 0; r < 4; r++) {
+        size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + MT::row(g, r);
+        T lr = cr[r], li = ci[r];
+        Ar[o] = lr; Ai[o] = li;
+        if (cabs2(lr, li) > (T)1) mybad = 1;
+    }
     if (mybad) atomicOr(flag + k0 / LU_NB, 1);
 }
 
-// U12 = Linv_p A12 in place for columns k0+NB .. Np-1. Each wave owns 16
-// columns (all NB rows).
+// U12 = Linv_p A12 in place for columns k0+NB .. Np-1. A work-group owns 16
+// columns and wave a rows 16 a .. 16 a + 15 of them; all waves read all NB
+// rows of the columns, so they meet before overwriting.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
@@ -713,25 +711,21 @@ __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, co
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int col0 = k0 + LU_NB + blockIdx.x * 64 + wave * 16;
-    int i = lane & 15, g = lane >> 4;
-    acc_t cr[4], ci[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) { cr[a] = acc_t{0, 0, 0, 0}; ci[a] = acc_t{0, 0, 0, 0}; }
+    int col0 = k0 + LU_NB + blockIdx.x * 16;
+    int i = lane & 15, g = lane >> 4, a = wave;
+    acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
     const T* B0r = Ar + (size_t)col0 * Np + k0;
     const T* B0i = Ai + (size_t)col0 * Np + k0;
+#pragma unroll
     for (int kk = 0; kk < LU_NB; kk += 16)
+        cmma16<T>(Li_r + (size_t)kk * LU_NB + 16 * a, Li_i + (size_t)kk * LU_NB + 16 * a, LU_NB, B0r + kk,
+                  B0i + kk, Np, cr, ci, i, g);
+    __syncthreads();                                     // every wave has read the columns
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-            cmma16<T>(Li_r + (size_t)kk * LU_NB + 16 * a, Li_i + (size_t)kk * LU_NB + 16 * a, LU_NB, B0r + kk,
-                      B0i + kk, Np, cr[a], ci[a], i, g);
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            size_t o = (size_t)(col0 + i) * Np + k0 + 16 * a + MT::row(g, r);
-            Ar[o] = cr[a][r]; Ai[o] = ci[a][r];
-        }
+    for (int r = 0; r < 4; r++) {
+        size_t o = (size_t)(col0 + i) * Np + k0 + 16 * a + MT::row(g, r);
+        Ar[o] = cr[r]; Ai[o] = ci[r];
+    }
 }
 
 // Fallback for a rejected panel: restore it and factor it with partial
@@ -1942,7 +1936,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, w.sd_r, w.sd_i, Lir, Lii, Uir,
                            Uii, flag);
         if (rest > 0)
-            hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / 64), dim3(256), 0, st, Np, k0, Ar, Ai, Uir, Uii, w.s21_r,
+            hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / 16), dim3(256), 0, st, Np, k0, Ar, Ai, Uir, Uii, w.s21_r,
                                w.s21_i, flag);
     };
     auto fix = [&](int p) {      // pivot check and fallback of panel p, row swaps elsewhere
@@ -1955,7 +1949,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
     auto u12 = [&](int p) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
         if (rest > 0)
-            hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 64), dim3(256), 0, s, Np, k0, Ar, Ai,
+            hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 16), dim3(256), 0, s, Np, k0, Ar, Ai,
                                w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2);
     };
     const int NB = LU_NB, group = lu_group<T>(Np);
