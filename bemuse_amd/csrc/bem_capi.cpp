@@ -845,6 +845,11 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
     return BEM_OK;
 }
 
+// Frequencies: omega may be negative (the reference's set-up solve runs at
+// Frequency = -1), but must be finite; kappa = omega^2 / g must be finite
+// and >= 0. Anything else would run the whole path on NaN.
+bool freq_ok(double omega, double kappa) { return std::isfinite(omega) && std::isfinite(kappa) && kappa >= 0.0; }
+
 int check_solve_args(bem_ctx* c, bem_body* b, int mode, int compose, int nb, const double* betas) {
     if (!c || !b || b->ctx != c) return fail(BEM_ERR_ARG, "bad context/body");
     if (nb < 0 || nb > kMaxRhs - 6 || (nb > 0 && !betas)) return fail(BEM_ERR_ARG, "nb must be in [0, 10]");
@@ -921,6 +926,7 @@ int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double 
     int r = check_solve_args(c, b, mode, compose, nb, betas);
     if (r) return r;
     if (!A36 || !B36) return fail(BEM_ERR_ARG, "bem_solve_frequency: A36/B36 required");
+    if (!freq_ok(omega, kappa)) return fail(BEM_ERR_ARG, "bem_solve_frequency: omega and kappa must be finite, kappa >= 0");
     HIP_TRY(hipSetDevice(c->device));
     if ((r = ensure_lanes(b, 1, mode != BEM_MODE_FP64))) return r;
     Lane& L = *b->lanes[0];
@@ -953,7 +959,11 @@ int bem_solve_sweep(bem_ctx* c, bem_body* b, int mode, int nf, const double* ome
     for (int l = 0; l < lanes; l++)
         if ((r = upload_betas(*b->lanes[l], nb, betas))) return r;
     std::vector<Task> tasks(nf);
-    for (int f = 0; f < nf; f++) tasks[f] = {b, f, omegas[f], kappas ? kappas[f] : omegas[f] * omegas[f] / 9.81};
+    for (int f = 0; f < nf; f++) {
+        tasks[f] = {b, f, omegas[f], kappas ? kappas[f] : omegas[f] * omegas[f] / 9.81};
+        if (!freq_ok(tasks[f].omega, tasks[f].kappa))
+            return fail(BEM_ERR_ARG, "bem_solve_sweep: omega and kappa must be finite, kappa >= 0");
+    }
     std::vector<Lane*> ls(b->lanes.begin(), b->lanes.begin() + lanes);
     return run_tasks(ls, tasks, mode, compose, nb, A36, B36, F, FK, SC, infos);
 }
@@ -970,6 +980,9 @@ int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* n
         capNp = std::max(capNp, bodies[i]->Np);
         total += nfreq[i];
     }
+    for (int t = 0; t < total; t++)
+        if (!freq_ok(omegas[t], omegas[t] * omegas[t] / 9.81))
+            return fail(BEM_ERR_ARG, "bem_solve_batch: omegas must be finite");
     HIP_TRY(hipSetDevice(c->device));
     bool need32 = mode != BEM_MODE_FP64;
     // lane pool of the context, sized for the largest body of the batch
@@ -1030,6 +1043,7 @@ int bem_get_rankine(bem_ctx* c, bem_body* b, double* Ssrc, double* Srefl, double
 int bem_get_system(bem_ctx* c, bem_body* b, int mode, double kappa, int compose, double* SMat, double* DMat) {
     if (!c || !b || !b->have_rankine) return fail(BEM_ERR_STATE, "bem_get_system: Rankine matrices not assembled");
     if (compose < 0 || compose > 2) return fail(BEM_ERR_ARG, "bem_get_system: bad compose case");
+    if (!freq_ok(0.0, kappa)) return fail(BEM_ERR_ARG, "bem_get_system: kappa must be finite and >= 0");
     if (b->ctx != c || mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bem_get_system: bad context or mode");
     if (mode == BEM_MODE_MIXED_SURROGATE && (!c->surrogate || !c->surrogate->params))
         return fail(BEM_ERR_STATE, "bem_get_system: BEM_MODE_MIXED_SURROGATE needs a surrogate on the context");
@@ -1059,6 +1073,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     if (b->ctx != c || mode < BEM_MODE_FP64 || mode > BEM_MODE_MIXED_SURROGATE) return fail(BEM_ERR_ARG, "bem_profile_lu: bad context or mode");
     if (mode == BEM_MODE_MIXED_SURROGATE && (!c->surrogate || !c->surrogate->params))
         return fail(BEM_ERR_STATE, "bem_profile_lu: BEM_MODE_MIXED_SURROGATE needs a surrogate on the context");
+    if (!freq_ok(0.0, kappa)) return fail(BEM_ERR_ARG, "bem_profile_lu: kappa must be finite and >= 0");
     HIP_TRY(hipSetDevice(c->device));
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
@@ -1135,6 +1150,7 @@ int bem_kochin(bem_ctx* c, bem_body* b, double omega, double kappa, int nkoch, i
         (nb > 0 && (!betas || !phiS)))
         return fail(BEM_ERR_ARG, "bem_kochin: bad arguments (nkoch in [1, 65536], nb in [0, 10])");
     if (nb > 0 && omega == 0.0) return fail(BEM_ERR_ARG, "bem_kochin: diffraction needs omega != 0");
+    if (!freq_ok(omega, kappa)) return fail(BEM_ERR_ARG, "bem_kochin: omega and kappa must be finite, kappa >= 0");
     HIP_TRY(hipSetDevice(c->device));
     SurBuffers B;
     double *phr = nullptr, *phi = nullptr, *dbetas = nullptr, *dout = nullptr;
@@ -1157,6 +1173,7 @@ int bem_fs_elevation(bem_ctx* c, bem_body* b, double omega, double kappa, int np
         (nb > 0 && (!betas || !phiS)))
         return fail(BEM_ERR_ARG, "bem_fs_elevation: bad arguments (npts >= 1, nb in [0, 10])");
     if (nb > 0 && omega == 0.0) return fail(BEM_ERR_ARG, "bem_fs_elevation: diffraction needs omega != 0");
+    if (!freq_ok(omega, kappa)) return fail(BEM_ERR_ARG, "bem_fs_elevation: omega and kappa must be finite, kappa >= 0");
     HIP_TRY(hipSetDevice(c->device));
     SurBuffers B;
     double *phr = nullptr, *phi = nullptr, *dbetas = nullptr, *dpts = nullptr, *dout = nullptr;

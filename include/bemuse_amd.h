@@ -204,7 +204,11 @@ int bem_body_assemble_rankine(bem_ctx* ctx, bem_body* b);
  *   compose BEM_COMPOSE_*; rad_only = 1 stops after A/B (:469)
  *   betas   nb wave headings [rad]
  * Outputs: A36, B36 row-major 6x6; F, FK, SC row-major 6 x nb complex (may be
- * NULL); phiJ N x 6 and phiS N x nb complex column-major (may be NULL). */
+ * NULL); phiJ N x 6 and phiS N x nb complex column-major (may be NULL).
+ * BEM_ERR_ARG for a non-finite omega or kappa, or kappa < 0 (omega itself may be
+ * negative: the reference's set-up solve runs at Frequency = -1); the same
+ * check applies to every frequency of the sweep, batch and post-processing
+ * entry points. */
 int bem_solve_frequency(bem_ctx* ctx, bem_body* b, int mode, double omega, double kappa,
                         int compose, int rad_only, int nb, const double* betas,
                         double* A36, double* B36, double* F, double* FK, double* SC,

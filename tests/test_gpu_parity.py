@@ -162,6 +162,25 @@ def test_parametric_bodies_both_modes(ctx, name):
     body.close()
 
 
+def test_non_finite_frequencies_rejected(ctx, cyl):
+    """NaN or infinite frequencies fail with BemError before any GPU work; the
+    body stays usable, and a negative omega (the set-up solve's Frequency = -1)
+    is still accepted."""
+    mesh, body, ob = cyl
+    for bad in (float("nan"), float("inf"), -float("inf")):
+        with pytest.raises(bm.BemError):
+            body.solve(bad, betas=[0.0])
+        with pytest.raises(bm.BemError):
+            body.solve_sweep(np.array([0.5, bad]), betas=[0.0], mode=bm.MODE_FP64, lanes=2)
+        with pytest.raises(bm.BemError):
+            ctx.solve_batch([body], [np.array([bad])], betas=[0.0], mode=bm.MODE_FP64, lanes=1)
+    g = body.solve(0.7, betas=[0.0], mode=bm.MODE_FP64)
+    r = ob.solve(0.7, betas=[0.0])
+    assert relerr(g["A"], r["A"]) < 1e-8
+    neg = body.solve(-0.7, betas=[0.0], mode=bm.MODE_FP64)
+    assert np.all(np.isfinite(neg["A"])) and relerr(neg["A"], g["A"]) < 1e-12   # A depends on omega^2 only
+
+
 def test_compose_cases(cyl):
     """Omega == 0 (Rankine sum) and the infinite-frequency composition."""
     mesh, body, ob = cyl
