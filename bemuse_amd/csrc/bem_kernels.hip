@@ -890,12 +890,18 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
     int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
     int ntiles = tiles_m * tiles_n;
     // XCD-aware order: work-groups are dealt round-robin to the 8 XCDs, so
-    // give each XCD a contiguous run of tiles (row-major), sharing A rows in L2.
+    // each XCD gets a contiguous run of a grouped tile order: blocks of
+    // GROUP_M tile rows, walked column by column, so that an XCD's resident
+    // tiles share GROUP_M A slices and each B column slice in its L2 (in
+    // row-major order every tile row re-read the whole B panel).
+    constexpr int GROUP_M = 8;
     int bid = blockIdx.x;
     int per = (ntiles + 7) / 8;
     int tile = (bid % 8) * per + bid / 8;
     if (tile >= ntiles) return;
-    int tm = tile / tiles_n, tn = tile % tiles_n;
+    int group = tile / (GROUP_M * tiles_n), first_m = group * GROUP_M;
+    int gsize = min(GROUP_M, tiles_m - first_m), in_group = tile - group * GROUP_M * tiles_n;
+    int tm = first_m + in_group % gsize, tn = in_group / gsize;
     int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int wm = wave >> 1, wn = wave & 1;
     int rowt = tm * TM, colt = tn * TN;
