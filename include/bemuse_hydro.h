@@ -169,6 +169,26 @@ public:
     const bem_mesh* Get_Device_Mesh() const { return mesh_; }
     // auxiliary (interior free-surface lid) panels for IFR, if the boundary has them
     const bem_mesh* Get_Device_Lid() const { return lid_; }
+    // Array forms of the reference's element accessors (Boundary_Base.h:173-179),
+    // same layout as Get_Mesh: the lid (Get_Aux_Nodes/Elements) and the images
+    // about z = 0 that Generate_Symmetry_Nodes/Elements and their Aux variants
+    // build (Boundary_Base.cpp:51-112: z negated, each element's node order
+    // reversed so that its normal flips). The solver forms these images itself.
+    void Get_Aux_Mesh(std::vector<Real>& nodes, std::vector<int>& elems) const {
+        int n = 0, e = 0;
+        if (lid_) bem_mesh_size(lid_, &n, &e);
+        nodes.assign(3 * (size_t)n, 0.0);
+        elems.assign(4 * (size_t)e, 0);
+        if (lid_) bem_mesh_export(lid_, nodes.data(), elems.data());
+    }
+    void Get_Symm_Mesh(std::vector<Real>& nodes, std::vector<int>& elems) const {
+        Get_Mesh(nodes, elems);
+        Mirror_Z(nodes, elems);
+    }
+    void Get_Aux_Symm_Mesh(std::vector<Real>& nodes, std::vector<int>& elems) const {
+        Get_Aux_Mesh(nodes, elems);
+        Mirror_Z(nodes, elems);
+    }
 
     //--- geometric and kinematic properties (Boundary_Base.h:132-156)
     Real Volume = 0, Surface_Area = 0, Max_Dim = 0, Mass = 0;
@@ -186,6 +206,13 @@ public:
     int N_Aux_Panels() const { int n = 0, e = 0; if (lid_) bem_mesh_size(lid_, &n, &e); return e; }
 
 protected:
+    static void Mirror_Z(std::vector<Real>& nodes, std::vector<int>& elems) {
+        for (size_t i = 2; i < nodes.size(); i += 3) nodes[i] = -nodes[i];
+        for (size_t e = 0; e + 3 < elems.size(); e += 4) {
+            if (elems[e + 3] < 0) std::swap(elems[e], elems[e + 2]);               // triangle: 2 1 0
+            else { std::swap(elems[e], elems[e + 3]); std::swap(elems[e + 1], elems[e + 2]); }   // quad: 3 2 1 0
+        }
+    }
     // Boundary::Set_Kin_Params (Boundary_Base.h:163): bodies with known mass
     // properties set them here
     virtual void Set_Kin_Params() {}
@@ -843,6 +870,10 @@ public:
     virtual void Post_Processing(Boundary* B) { (void)B; }
     virtual void Generate_Output_File(Boundary* B) { (void)B; }
     virtual void Update_Output_File() {}
+    // Solver_Base.h:145; the hydrodynamic solver returns its VisMat, which the
+    // reference never assigns (Hydrodynamic_Solver.cpp:434 is commented out),
+    // so the result is an empty matrix in both
+    virtual CMatrix Get_VisMatrix() { return VisMat; }
     std::vector<CMatrix> RadSolArray;     // radiation potential per Solve (N x 6)
     std::vector<CMatrix> DiffSolArray;    // scattering potential per Solve (N x NBeta)
     std::vector<CMatrix> FS_Rad_SolArray;
@@ -850,6 +881,7 @@ public:
 protected:
     std::string OutputDirectory = "Output";
     std::string OutputPath = "Output/BEMUse_Outputs";
+    CMatrix VisMat;
 };
 
 class Hydrodynamic_Radiation_Solver : public Solver {

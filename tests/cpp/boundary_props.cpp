@@ -14,6 +14,40 @@ static void dump(const char* tag, Boundary& B) {
                 tag, B.WP_Area, tag, B.WP_Mx, tag, B.WP_My, tag, B.WP_Mxx, tag, B.WP_Mxy, tag, B.WP_Myy);
 }
 
+
+// Mirror checks of Get_Symm_Mesh / Get_Aux_Symm_Mesh: node map exact (z
+// negated, element order reversed), and the image, rebuilt as a mesh, encloses
+// the same volume with its normals still pointing into it (positive volume).
+static void mirror(const char* tag, const std::vector<Real>& n0, const std::vector<int>& e0,
+                   const std::vector<Real>& n1, const std::vector<int>& e1) {
+    int bad = (n0.size() != n1.size() || e0.size() != e1.size()) ? 1 : 0;
+    for (size_t i = 0; !bad && i < n0.size(); i++) bad |= (i % 3 == 2 ? n1[i] != -n0[i] : n1[i] != n0[i]);
+    for (size_t e = 0; !bad && e + 3 < e0.size(); e += 4) {
+        if (e0[e + 3] < 0) bad |= !(e1[e] == e0[e + 2] && e1[e + 1] == e0[e + 1] && e1[e + 2] == e0[e] && e1[e + 3] < 0);
+        else bad |= !(e1[e] == e0[e + 3] && e1[e + 1] == e0[e + 2] && e1[e + 2] == e0[e + 1] && e1[e + 3] == e0[e]);
+    }
+    double vol = 0, vol0 = 0;
+    for (int k = 0; k < 2; k++) {
+        const std::vector<Real>& nn = k ? n1 : n0;
+        const std::vector<int>& ee = k ? e1 : e0;
+        bem_mesh* m = nullptr;
+        int ne = (int)(ee.size() / 4);
+        if (ne == 0 || bem_mesh_from_arrays((int)(nn.size() / 3), nn.data(), ne, ee.data(), &m) != BEM_OK) continue;
+        std::vector<double> body(32 * (size_t)ne), gen(8 * (size_t)ne);
+        if (bem_geometry(m, body.data(), nullptr, nullptr, nullptr, gen.data()) == BEM_OK) {
+            double v = 0;
+            for (int i = 0; i < ne; i++) {
+                const double* c = &body[32 * (size_t)i];
+                const double* g = &gen[8 * (size_t)i];
+                v -= (c[0] * g[0] + c[1] * g[1] + c[2] * g[2]) * g[6] / 3.0;   // normals point into the body
+            }
+            (k ? vol : vol0) = v;
+        }
+        bem_mesh_free(m);
+    }
+    std::printf("%s_mirror_bad %d\n%s_mirror_vol %.17g\n%s_vol %.17g\n", tag, bad, tag, vol, tag, vol0);
+}
+
 int main() {
     Barge box;
     std::vector<Parameter> bp = {Parameter("Tank_Length", 20.0), Parameter("Tank_Width", 10.0),
@@ -34,6 +68,18 @@ int main() {
     lidded.Set_Parameters(cp);
     lidded.Setup();
     std::printf("lidded_aux %d\n", lidded.N_Aux_Panels());
+    {
+        std::vector<Real> n0, n1;
+        std::vector<int> e0, e1;
+        lidded.Get_Mesh(n0, e0);
+        lidded.Get_Symm_Mesh(n1, e1);
+        mirror("hull", n0, e0, n1, e1);
+        lidded.Get_Aux_Mesh(n0, e0);
+        lidded.Get_Aux_Symm_Mesh(n1, e1);
+        mirror("lid", n0, e0, n1, e1);
+        Hydrodynamic_Radiation_Solver S;
+        std::printf("vismat %d %d\n", (int)S.Get_VisMatrix().rows(), (int)S.Get_VisMatrix().cols());
+    }
     // reference parametric bodies (discretisations kept small)
     OC3_SparBuoy oc3;
     std::vector<Parameter> o3 = {Parameter("NPanels_Radial", 4), Parameter("NPanels_Vertical_Section1", 12),

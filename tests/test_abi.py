@@ -158,6 +158,13 @@ def test_dropin_boundary_properties_closed_forms():
     # (NX x NY quads), the cylinder only with NPanels_FreeSurface_Int_Radial
     # (3 rings of 16: a fan of triangles and two rings of quads)
     assert int(v["box_aux"]) == 10 * 5 and int(v["cyl_aux"]) == 0 and int(v["lidded_aux"]) == 48
+    # images about z = 0 (Generate_Symmetry_*, Boundary_Base.cpp:51-112): exact node map, and the
+    # image encloses the hull's volume with inward normals (a wrong node order flips the sign)
+    assert v["hull_mirror_bad"] == "0" and v["lid_mirror_bad"] == "0"
+    hv, hm = float(v["hull_vol"]), float(v["hull_mirror_vol"])
+    assert hv > 0 and abs(hm - hv) < 1e-12 * hv
+    assert abs(float(v["lid_vol"]) + float(v["lid_mirror_vol"])) < 1e-9      # a flat lid at z = 0 encloses nothing
+    assert v["vismat"] == "0 0"                                               # VisMat is never assigned
     # the reference's parametric bodies through the drop-in (FOWT_Platforms.h, Triple_Spar.cpp,
     # Ellipsoid.cpp, Ship_Hulls.cpp, Barge.cpp); k(NA) = (NA/2) sin(2 pi/NA) is the NA-gon factor
     k = lambda na: 0.5 * na * np.sin(2 * np.pi / na)
