@@ -78,7 +78,7 @@ int main() {
         lidded.Get_Aux_Symm_Mesh(n1, e1);
         mirror("lid", n0, e0, n1, e1);
         Hydrodynamic_Radiation_Solver S;
-        std::printf("vismat %d %d\n", (int)S.Get_VisMatrix().rows(), (int)S.Get_VisMatrix().cols());
+        std::printf("vismat_rows %d\nvismat_cols %d\n", (int)S.Get_VisMatrix().rows(), (int)S.Get_VisMatrix().cols());
     }
     // reference parametric bodies (discretisations kept small)
     OC3_SparBuoy oc3;

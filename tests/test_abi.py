@@ -164,7 +164,7 @@ def test_dropin_boundary_properties_closed_forms():
     hv, hm = float(v["hull_vol"]), float(v["hull_mirror_vol"])
     assert hv > 0 and abs(hm - hv) < 1e-12 * hv
     assert abs(float(v["lid_vol"]) + float(v["lid_mirror_vol"])) < 1e-9      # a flat lid at z = 0 encloses nothing
-    assert v["vismat"] == "0 0"                                               # VisMat is never assigned
+    assert v["vismat_rows"] == "0" and v["vismat_cols"] == "0"               # VisMat is never assigned
     # the reference's parametric bodies through the drop-in (FOWT_Platforms.h, Triple_Spar.cpp,
     # Ellipsoid.cpp, Ship_Hulls.cpp, Barge.cpp); k(NA) = (NA/2) sin(2 pi/NA) is the NA-gon factor
     k = lambda na: 0.5 * na * np.sin(2 * np.pi / na)
