@@ -1,8 +1,8 @@
 """FP64 CPU oracle of the BEMUse per-frequency hot path (ctypes over liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker. The product (bemuse_amd) never
-imports it.
+cpu_baseline leg of bench.py, to check and time results against. The product
+(bemuse_amd) never imports it.
 
 Each function restates the reference arithmetic; see bem_oracle.cpp for the
 file:line each one follows. Parity status of every function (which checks pin
