@@ -1,6 +1,6 @@
 // FP64 CPU oracle of the BEMUse per-frequency hydrodynamic hot path.
 //
-// TEST INFRASTRUCTURE ONLY. This file is the checker: it may be linked or
+// TEST INFRASTRUCTURE ONLY. Results are compared against it: it may be linked or
 // called only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
 // bench.py. The product (libbemuse_amd.so) never links, loads or calls it.
 //

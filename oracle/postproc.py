@@ -2,7 +2,7 @@
 functions and free-surface elevation at exterior points (numpy over the
 oracle's C primitives).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference for
 bem_kochin / bem_fs_elevation. The product never imports it.
 
 Follows Hydrodynamic_Solver.cpp:

@@ -1,6 +1,6 @@
 """FP64 CPU oracle of the wave-term surrogate (numpy).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the device
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference for the device
 sampler, network forward pass and training gradients. The product
 (bemuse_amd) never imports it.
 

@@ -60,8 +60,8 @@ def test_context_without_gpu_fails_cleanly():
 
 
 def test_product_does_not_use_the_oracle():
-    """The library and its Python binding never load the oracle (it is the
-    checker only)."""
+    """The library and its Python binding never load the oracle (it serves the
+    tests only)."""
     pkg = os.path.join(ROOT, "bemuse_amd")
     for dirpath, _, files in os.walk(pkg):
         for f in files:
