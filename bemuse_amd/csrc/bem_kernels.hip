@@ -199,18 +199,30 @@ __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restric
 }
 
 // norm[0] = max over rows of sum_ks Pabs (the infinity norm with |re|+|im|).
-__global__ void k_rownorm(int Np, int nks, const double* Pabs, double* norm, const int* skip) {
+// One work-group of RN_THREADS; four independent partial sums per row keep
+// several loads in flight per thread (a single running sum made every load
+// wait for the previous add).
+constexpr int RN_THREADS = 1024;
+__global__ __launch_bounds__(RN_THREADS) void k_rownorm(int Np, int nks, const double* Pabs, double* norm,
+                                                        const int* skip) {
     if (skip && *skip) return;
-    __shared__ double s[256];
+    __shared__ double s[RN_THREADS];
     double m = 0.0;
-    for (int r = threadIdx.x; r < Np; r += 256) {
-        double v = 0.0;
-        for (int ks = 0; ks < nks; ks++) v += Pabs[(size_t)ks * Np + r];
-        m = fmax(m, v);
+    for (int r = threadIdx.x; r < Np; r += RN_THREADS) {
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        int ks = 0;
+        for (; ks + 4 <= nks; ks += 4) {
+            v0 += Pabs[(size_t)ks * Np + r];
+            v1 += Pabs[(size_t)(ks + 1) * Np + r];
+            v2 += Pabs[(size_t)(ks + 2) * Np + r];
+            v3 += Pabs[(size_t)(ks + 3) * Np + r];
+        }
+        for (; ks < nks; ks++) v0 += Pabs[(size_t)ks * Np + r];
+        m = fmax(m, (v0 + v1) + (v2 + v3));
     }
     s[threadIdx.x] = m;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = RN_THREADS / 2; h > 0; h >>= 1) {
         if (threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
         __syncthreads();
     }
@@ -1724,7 +1736,7 @@ void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, co
                            Pi, Pabs, skip);
     hipLaunchKernelGGL(k_matvec_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, Np, nr, nks, Pr, Pi, Br, Bi,
                        mode, Yr, Yi, skip);
-    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(256), 0, s, Np, nks, Pabs, norm, skip);
+    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(RN_THREADS), 0, s, Np, nks, Pabs, norm, skip);
 }
 template void launch_matvec<double>(hipStream_t, int, int, const double*, const double*, const double*, const double*,
                                     double*, double*, const double*, const double*, int, double*, double*, double*,
