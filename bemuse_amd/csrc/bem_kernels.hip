@@ -1451,8 +1451,9 @@ __global__ void k_refine_reset(RefineCtrl* ctrl) {
 // can leave it inaccurate. The 0.01 floor keeps its roundoff floor (about
 // 1e2 u |A| |x|) under the tolerance. Converged when max_j rel_j <= tol;
 // stops (not converged) on stagnation (rel > 0.5 prev) or after max_iter
-// corrections. One work-group of 512; per-wave maxima by shuffles.
-constexpr int RC_THREADS = 512;
+// corrections. One work-group of 1024 with two rows in flight per thread;
+// per-wave maxima by shuffles.
+constexpr int RC_THREADS = 1024;
 
 __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int nr, const double* Rr,
                                                              const double* Ri, const double* Xr, const double* Xi,
@@ -1464,7 +1465,22 @@ __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int j = 0; j < nr; j++) {
         double mr = 0, mx = 0, mri = 0, mxi = 0;   // max |r|, |x|, |Im r|, |Im x|
-        for (int r = threadIdx.x; r < N; r += RC_THREADS) {
+        double mr2 = 0, mx2 = 0, mri2 = 0, mxi2 = 0;
+        int r = threadIdx.x;
+        for (; r + RC_THREADS < N; r += 2 * RC_THREADS) {
+            size_t o = (size_t)j * Np + r, o2 = o + RC_THREADS;
+            double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
+            double rr2 = Rr[o2], ri2 = Ri[o2], xr2 = Xr[o2], xi2 = Xi[o2];
+            mr = fmax(mr, sqrt(rr * rr + ri * ri));
+            mx = fmax(mx, sqrt(xr * xr + xi * xi));
+            mri = fmax(mri, fabs(ri));
+            mxi = fmax(mxi, fabs(xi));
+            mr2 = fmax(mr2, sqrt(rr2 * rr2 + ri2 * ri2));
+            mx2 = fmax(mx2, sqrt(xr2 * xr2 + xi2 * xi2));
+            mri2 = fmax(mri2, fabs(ri2));
+            mxi2 = fmax(mxi2, fabs(xi2));
+        }
+        if (r < N) {
             size_t o = (size_t)j * Np + r;
             double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
             mr = fmax(mr, sqrt(rr * rr + ri * ri));
@@ -1472,6 +1488,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int 
             mri = fmax(mri, fabs(ri));
             mxi = fmax(mxi, fabs(xi));
         }
+        mr = fmax(mr, mr2); mx = fmax(mx, mx2); mri = fmax(mri, mri2); mxi = fmax(mxi, mxi2);
         for (int off = 32; off > 0; off >>= 1) {
             mr = fmax(mr, __shfl_xor(mr, off));
             mx = fmax(mx, __shfl_xor(mx, off));
