@@ -1,25 +1,33 @@
 """Benchmark: frequency-solves/s of the per-frequency BEM path at ~6k panels.
 
 Workload (the 6k-panel headline configuration): a three-column semi-submersible
-(columns r = 6 m, draft 20 m, at the vertices of a 50 m triangle, joined by
-closed 6 m x 4 m pontoons), 44 x (24 + 8) panels per column and 598 per
-pontoon (N = 6018 panels), 40 wave frequencies uniform in 0.05-2.5 rad/s,
-6 radiation DOF + 5 headings (0, 45, 90, 135, 180 deg). Synthetic geometry,
-no input data.
+(columns r = 6 m, draft 20 m, at the vertices of a 50 m triangle) with three
+closed 6 m x 4 m pontoons along the triangle's sides at the column bottoms.
+The pontoons end 0.5 m short of the column walls (separate closed bodies, not
+fused to the columns: the mesh has no intersection curves). 44 x (24 + 8)
+panels per column and 598 per pontoon (N = 6018 panels), 40 wave frequencies
+uniform in 0.05-2.5 rad/s, 6 radiation DOF + 5 headings (0, 45, 90, 135,
+180 deg). Synthetic geometry, no input data.
 
 A step is one frequency solve (wave assembly + compose, LU, radiation and
-diffraction solves, refinement, A/B/F reductions); steps cycle through the 40
-frequencies and are pipelined over --lanes concurrent solve lanes (one C-ABI
-sweep call covers the timed steps; GPU_MAX_HW_QUEUES is raised to 32 so that
-the lanes' streams do not share hardware queues). Rankine assembly (once per body) is
-outside the timed region and reported separately. The headline mode is MIXED (FP32-MFMA LU + FP64
-refinement); the FP64 mode is timed in the same run for the ratio, and every
-MIXED result is compared with the FP64 GPU result at the same frequency and
-with the FP64 CPU oracle at the sampled frequency. The MIXED_SURROGATE mode
-(wave term from the trained MLP) is trained and timed after the headline
-runs, on every 5th frequency, and reported in the "surrogate" object: it
-does not reach the 1e-6 bar and is slower than the exact kernel, so it is
-not the headline value.
+diffraction solves, refinement, A/B/F reductions). The timed steps are spread
+evenly over the 40 frequencies (step_frequencies: with K <= 40 steps, step i
+solves frequency floor(i * 40 / K); with more, they cycle), and are pipelined
+over --lanes concurrent solve lanes (one C-ABI sweep call covers the timed
+steps; GPU_MAX_HW_QUEUES is raised to 32 so that the lanes' streams do not
+share hardware queues). Rankine assembly (once per body) is outside the timed
+region and reported separately. The headline mode is MIXED (FP32-MFMA LU +
+FP64 refinement); the FP64 mode is timed in the same run on the same steps
+for the ratio, and every MIXED result is compared with the FP64 GPU result at
+the same frequency (A, B and F separately) and with the FP64 CPU oracle at the
+sampled frequency.
+
+Order: the headline legs, the roofline of the dominant kernel, the CPU
+baseline, then the optional MIXED_SURROGATE leg (wave term from the trained
+MLP, every 5th frequency, reported in the "surrogate" object: it does not
+reach the 1e-6 bar and is slower than the exact kernel, so it is not the
+headline value). A failing optional leg is reported in its object's "error"
+field; the line is still printed.
 
 Prints one JSON line (rank 0, N=1 only).
 """
@@ -67,8 +75,20 @@ def rel(a, b):
     return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
 
 
-def abf_err(g, r):
-    return max(rel(g["A"], r["A"]), rel(g["B"], r["B"]), rel(g["F"], r["F"]))
+def abf_errs(gs, rs):
+    """Max relative error of A, B and F separately over paired results; the
+    denominator is the max-norm of each reference matrix."""
+    return {k: max(rel(g[k], r[k]) for g, r in zip(gs, rs)) for k in ("A", "B", "F")}
+
+
+def step_frequencies(steps, nf):
+    """Frequency index solved by each timed step: with steps <= nf the steps
+    are spread evenly over the nf frequencies (step i solves floor(i nf /
+    steps)); with more steps they cycle through all nf in order."""
+    i = np.arange(steps)
+    if steps <= nf:
+        return (i * nf) // steps
+    return i % nf
 
 
 def main():
@@ -86,6 +106,9 @@ def main():
                     help="semisub6k: the headline line; batch64: 64 seeded uneven bodies x 16 frequencies; "
                          "spar20k: 20736-panel cone spar x 24 frequencies (extra lines, not the headline)")
     args = ap.parse_args()
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
     # Each lane drives two HIP streams. HIP maps streams onto GPU_MAX_HW_QUEUES
     # hardware queues (default 4); streams sharing a queue run in order, so a
     # lane's latency-bound panel kernels would wait behind another lane's GEMM.
@@ -109,9 +132,11 @@ def main():
         workload = "floating cylinder r=10 m draft 20 m, 1024 panels"
     else:
         mesh = bm.Mesh.semisub3()
-        workload = "three-column semi-sub (r=6 m, draft 20 m, 50 m triangle, 6x4 m pontoons)"
+        workload = ("three-column semi-sub: columns r=6 m, draft 20 m, at a 50 m triangle; three closed "
+                    "6x4 m pontoons along the sides ending 0.5 m short of the column walls (not fused)")
     omegas = np.linspace(0.05, 2.5, 40)
     betas = np.deg2rad([0.0, 45.0, 90.0, 135.0, 180.0])
+    fidx = step_frequencies(args.steps, len(omegas))
 
     ctx = bm.Context(0)
     body = bm.Body(ctx, mesh)
@@ -121,21 +146,24 @@ def main():
     t_rankine = time.perf_counter() - t0
     print(f"[bench] N={N} Rankine assembly {t_rankine * 1e3:.1f} ms", file=sys.stderr, flush=True)
 
-    def run(mode, steps, warmup):
-        if warmup:
-            body.solve_sweep(omegas[np.arange(warmup) % len(omegas)], betas=betas, mode=mode, lanes=args.lanes)
-        sel = omegas[np.arange(steps) % len(omegas)]
+    def run(mode):
+        if args.warmup:
+            body.solve_sweep(omegas[np.arange(args.warmup) % len(omegas)], betas=betas, mode=mode,
+                             lanes=args.lanes)
+        sel = omegas[fidx]
         t0 = time.perf_counter()
         sw = body.solve_sweep(sel, betas=betas, mode=mode, lanes=args.lanes)
         dt = time.perf_counter() - t0
-        res = [dict(A=sw["A"][i], B=sw["B"][i], F=sw["F"][i], info=sw["info"][i]) for i in range(steps)]
+        res = [dict(A=sw["A"][i], B=sw["B"][i], F=sw["F"][i], info=sw["info"][i]) for i in range(args.steps)]
         return dt, res
 
-    dt64, r64 = run(bm.MODE_FP64, args.steps, args.warmup)
+    dt64, r64 = run(bm.MODE_FP64)
     print(f"[bench] FP64 {args.steps} solves in {dt64:.3f} s ({args.steps / dt64:.2f}/s)", file=sys.stderr, flush=True)
-    dtmx, rmx = run(bm.MODE_MIXED, args.steps, args.warmup)
+    dtmx, rmx = run(bm.MODE_MIXED)
     print(f"[bench] MIXED {args.steps} solves in {dtmx:.3f} s ({args.steps / dtmx:.2f}/s)", file=sys.stderr, flush=True)
-    err_mx_vs_64 = max(abf_err(a, b) for a, b in zip(rmx, r64))
+    err_mx_vs_64 = abf_errs(rmx, r64)
+    # FP64 GPU results by frequency index, for the optional legs
+    fp64_by_freq = {int(j): r for j, r in zip(fidx, r64)}
     fallbacks = sum(r["info"]["fell_back_fp64"] for r in rmx)
     iters = [r["info"]["refine_iters"] for r in rmx]
 
@@ -165,36 +193,22 @@ def main():
         "measured_peaks": ctx.measure_peaks(),
     }
 
-    surrogate = None
-    if args.surrogate:
-        surrogate = surrogate_leg(bm, ctx, body, omegas, betas, r64, args.lanes)
-
     cpu = None
     err_vs_oracle = None
     if args.cpu_sample > 0:
-        import oracle
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        oracle.set_threads(nthreads)
-        nodes, elems = mesh.arrays()
-        ob = oracle.Body(nodes, elems)
-        t0 = time.perf_counter()
-        ob.rankine()
-        t_rk = time.perf_counter() - t0
-        sample = [omegas[len(omegas) // 2]][: args.cpu_sample]
-        t0 = time.perf_counter()
-        ref = [ob.solve(w, betas=betas) for w in sample]
-        t_cpu = (time.perf_counter() - t0) / len(sample)
-        gm = [body.solve(w, betas=betas, mode=bm.MODE_MIXED) for w in sample]
-        g64 = [body.solve(w, betas=betas, mode=bm.MODE_FP64) for w in sample]
-        err_vs_oracle = {"mixed": max(abf_err(a, b) for a, b in zip(gm, ref)),
-                         "fp64": max(abf_err(a, b) for a, b in zip(g64, ref))}
-        cpu = {"value": 1.0 / t_cpu, "unit": "frequency-solves/s", "cores": nthreads, "kind": "port",
-               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-               "sample": f"{len(sample)} frequency (omega={sample[0]:.4f} rad/s) of the same {N}-panel workload, "
-                         f"FP64 oracle (OpenMP assembly + blocked partial-pivot LU); Rankine assembly "
-                         f"{t_rk:.1f} s once per body, excluded like on the GPU",
-               "stages_s": ref[0]["timings"]}
-        print(f"[bench] CPU oracle {t_cpu:.2f} s/frequency on {nthreads} threads", file=sys.stderr, flush=True)
+        try:
+            cpu, err_vs_oracle = cpu_leg(bm, mesh, body, omegas, betas, args)
+        except Exception as e:          # reported, the line is still printed
+            cpu = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] CPU baseline failed: {cpu['error']}", file=sys.stderr, flush=True)
+
+    surrogate = None
+    if args.surrogate:
+        try:
+            surrogate = surrogate_leg(bm, ctx, body, omegas, betas, fp64_by_freq, args.lanes)
+        except Exception as e:
+            surrogate = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] surrogate leg failed: {surrogate['error']}", file=sys.stderr, flush=True)
 
     value = args.steps / dtmx
     line = {
@@ -212,6 +226,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "panels": N, "frequencies": len(omegas), "headings": len(betas),
                    "dof": 6, "lanes": args.lanes,
+                   "timed_frequencies": f"{len(np.unique(fidx))} distinct of {len(omegas)} "
+                                        f"(omega = {omegas[fidx[0]]:.4f} .. {omegas[fidx[-1]]:.4f} rad/s, "
+                                        + ("spread evenly)" if args.steps <= len(omegas) else "cycled)"),
                    "mode": "mixed (FP64 assembly, FP32-MFMA LU, FP64 iterative refinement)"},
         "fp64_value": args.steps / dt64,
         "speedup_vs_fp64": dt64 / dtmx,
@@ -228,28 +245,68 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def surrogate_leg(bm, ctx, body, omegas, betas, r64, lanes):
+def cpu_leg(bm, mesh, body, omegas, betas, args):
+    """The FP64 CPU oracle on the host cores for a bounded sample of the same
+    workload (cpu_sample frequencies from the middle of the range; its
+    once-per-body Rankine assembly is excluded as on the GPU), and the GPU's
+    A, B, F errors against it at those frequencies."""
+    import oracle
+    nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    oracle.set_threads(nthreads)
+    nodes, elems = mesh.arrays()
+    ob = oracle.Body(nodes, elems)
+    t0 = time.perf_counter()
+    ob.rankine()
+    t_rk = time.perf_counter() - t0
+    mid = len(omegas) // 2
+    sample = [omegas[(mid + k) % len(omegas)] for k in range(args.cpu_sample)]
+    t0 = time.perf_counter()
+    ref = [ob.solve(w, betas=betas) for w in sample]
+    t_cpu = (time.perf_counter() - t0) / len(sample)
+    gm = [body.solve(w, betas=betas, mode=bm.MODE_MIXED) for w in sample]
+    g64 = [body.solve(w, betas=betas, mode=bm.MODE_FP64) for w in sample]
+    err = {"mixed": abf_errs(gm, ref), "fp64": abf_errs(g64, ref)}
+    cpu = {"value": 1.0 / t_cpu, "unit": "frequency-solves/s", "cores": nthreads, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": f"{len(sample)} frequency of the same {body.N}-panel workload "
+                     f"(omega = {', '.join(f'{w:.4f}' for w in sample)} rad/s, 5 headings), FP64 oracle "
+                     f"(OpenMP assembly + blocked partial-pivot LU); its Rankine assembly "
+                     f"({t_rk:.1f} s once per body) is excluded like on the GPU",
+           "stages_s": ref[0]["timings"]}
+    print(f"[bench] CPU oracle {t_cpu:.2f} s/frequency on {nthreads} threads", file=sys.stderr, flush=True)
+    return cpu, err
+
+
+def surrogate_leg(bm, ctx, body, omegas, betas, fp64_by_freq, lanes):
     """MIXED_SURROGATE: trains the wave-term network on the device (default
     configuration), then solves every 5th frequency of the workload with the
     network in the assembly; errors against the FP64 GPU results of the same
-    frequencies. Reported beside the headline, which stays the exact-kernel
-    mixed path: the surrogate does not meet the 1e-6 bar."""
+    frequencies (taken from the timed FP64 leg where it solved them, solved
+    here otherwise). Reported beside the headline, which stays the
+    exact-kernel mixed path: the surrogate does not meet the 1e-6 bar."""
     t0 = time.perf_counter()
     rep = ctx.surrogate_train()
     t_train = time.perf_counter() - t0
     print(f"[bench] surrogate trained in {t_train:.2f} s: validation rel err rms {rep['rel_rms']:.2e} "
           f"p99 {rep['rel_p99']:.2e}", file=sys.stderr, flush=True)
     idx = np.arange(0, len(omegas), 5)
+    missing = [int(j) for j in idx if int(j) not in fp64_by_freq]
+    if missing:
+        sw64 = body.solve_sweep(omegas[missing], betas=betas, mode=bm.MODE_FP64, lanes=lanes)
+        for i, j in enumerate(missing):
+            fp64_by_freq[j] = dict(A=sw64["A"][i], B=sw64["B"][i], F=sw64["F"][i])
     body.solve_sweep(omegas[idx[:1]], betas=betas, mode=bm.MODE_MIXED_SURROGATE, lanes=lanes)
     t0 = time.perf_counter()
     sw = body.solve_sweep(omegas[idx], betas=betas, mode=bm.MODE_MIXED_SURROGATE, lanes=lanes)
     dt = time.perf_counter() - t0
-    errs = [max(rel(sw[k][i], r64[j][k]) for k in ("A", "B", "F")) for i, j in enumerate(idx)]
+    per = [{k: rel(sw[k][i], fp64_by_freq[int(j)][k]) for k in ("A", "B", "F")} for i, j in enumerate(idx)]
+    errs = [max(p.values()) for p in per]
     infos = sw["info"]
     print(f"[bench] surrogate {len(idx)} solves in {dt:.2f} s, max A/B/F rel err {max(errs):.2e}",
           file=sys.stderr, flush=True)
     return {"value": len(idx) / dt, "unit": "frequency-solves/s", "frequencies": [float(w) for w in omegas[idx]],
-            "max_rel_err_vs_fp64_gpu": max(errs), "rel_err_per_frequency": errs,
+            "max_rel_err_vs_fp64_gpu": {k: max(p[k] for p in per) for k in ("A", "B", "F")},
+            "rel_err_per_frequency": errs,
             "meets_1e-6": bool(max(errs) <= 1e-6),
             "ms_assemble": float(np.mean([i["ms_assemble"] for i in infos])),
             "pairs_surrogate": int(infos[0]["surrogate_points"]), "pairs_exact": int(infos[0]["exact_points"]),
@@ -284,8 +341,8 @@ def other_workload(args, bm):
             out = ctx.solve_batch(bodies, ws, betas=betas, mode=mode, lanes=args.lanes)
             res[mode] = (time.perf_counter() - t0, out)
             print(f"[bench] mode {mode}: {16 * 64} solves in {res[mode][0]:.2f} s", file=sys.stderr, flush=True)
-        err = max(rel(a, b) for om, o64 in zip(res[bm.MODE_MIXED][1], res[bm.MODE_FP64][1])
-                  for a, b in zip(om["A"], o64["A"]))
+        err = {k: max(rel(a, b) for om, o64 in zip(res[bm.MODE_MIXED][1], res[bm.MODE_FP64][1])
+                      for a, b in zip(om[k], o64[k])) for k in ("A", "B", "F")}
         ntask = 16 * 64
         cfg = {"workload": "64 seeded uneven bodies (cylinders/barges 5-40 m, drafts 5-25 m, seed 20261203)",
                "panels_min": int(min(ns)), "panels_max": int(max(ns)), "panels_mean": float(np.mean(ns)),
@@ -308,8 +365,8 @@ def other_workload(args, bm):
             sw = body.solve_sweep(omegas, betas=betas, mode=mode, lanes=lanes)
             res[mode] = (time.perf_counter() - t0, sw)
             print(f"[bench] mode {mode}: 24 solves in {res[mode][0]:.2f} s", file=sys.stderr, flush=True)
-        err = max(max(rel(res[bm.MODE_MIXED][1][k][i], res[bm.MODE_FP64][1][k][i]) for k in ("A", "B", "F"))
-                  for i in range(24))
+        err = {k: max(rel(res[bm.MODE_MIXED][1][k][i], res[bm.MODE_FP64][1][k][i]) for i in range(24))
+               for k in ("A", "B", "F")}
         ntask = 24
         cfg = {"workload": "truncated-cone spar with heave plate", "panels": body.N, "frequencies": 24,
                "headings": 3, "lanes": lanes}
