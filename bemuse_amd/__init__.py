@@ -140,6 +140,7 @@ _SIGS = {
     "bem_get_system": (c_int, [c_void_p, c_void_p, c_int, c_double, c_int, _PD, _PD]),
     "bem_eval_havelock": (c_int, [c_void_p, c_int, c_int, _PD, _PD, _PD, _PD]),
     "bem_gf_coefficients": (c_int, [_PD, c_int]),
+    "bem_struve_coefficients": (c_int, [_PD, c_int]),
     "bem_bessel01": (c_int, [c_int, _PD, _PD]),
     "bem_mesh_read": (c_int, [ctypes.c_char_p, c_int, ctypes.POINTER(c_void_p)]),
     "bem_solve_batch": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), _PI, _PD, c_int, c_int, _PD, c_int, _PD,
@@ -233,6 +234,13 @@ def gf_coefficients():
     """The HavelockGF polynomial tables and EUL the device code is built with (65 values)."""
     out = np.zeros(65)
     _check(lib().bem_gf_coefficients(_dp(out), 65))
+    return out
+
+
+def struve_coefficients():
+    """The Struve approximation literals the device code is built with (26 values)."""
+    out = np.zeros(26)
+    _check(lib().bem_struve_coefficients(_dp(out), 26))
     return out
 
 

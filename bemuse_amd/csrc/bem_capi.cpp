@@ -1215,6 +1215,18 @@ int bem_gf_coefficients(double* out, int n) {
     return BEM_OK;
 }
 
+int bem_struve_coefficients(double* out, int n) {
+    if (!out || n < 26) return fail(BEM_ERR_ARG, "bem_struve_coefficients: need room for 26 values");
+    int k = 0;
+    for (double v : dev::kStruve.h0p) out[k++] = v;
+    for (double v : dev::kStruve.h0a) out[k++] = v;
+    for (double v : dev::kStruve.h0b) out[k++] = v;
+    for (double v : dev::kStruve.h1p) out[k++] = v;
+    for (double v : dev::kStruve.h1a) out[k++] = v;
+    for (double v : dev::kStruve.h1b) out[k++] = v;
+    return BEM_OK;
+}
+
 int bem_eval_havelock(bem_ctx* c, int precision, int n, const double* X, const double* Y, const double* V,
                       double* out) {
     if (!c || n <= 0 || !X || !Y || !V || !out) return fail(BEM_ERR_ARG, "bem_eval_havelock: bad arguments");

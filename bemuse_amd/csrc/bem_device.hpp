@@ -28,48 +28,47 @@ __host__ __device__ inline double frcp(double x) {
 #endif
 }
 
-// Struve H0 and H1 together (Kernels.cpp:35-116): below 3 polynomials in
-// (x/3)^2; above, the Bessel Y0 / Y1 plus rational functions of (3/x)^2,
-// whose two denominators share one reciprocal.
+// Coefficients of the Struve approximations (Kernels.cpp:35-116): for x <= 3
+// polynomials in (x/3)^2 (p), above rational functions of (3/x)^2 with
+// numerator a and denominator 1 + b. H0: p0..p5, a0..a3, b1..b3; H1: p1..p6,
+// a0..a3, b1..b3.
+struct StruveCoeffs {
+    double h0p[6], h0a[4], h0b[3];
+    double h1p[6], h1a[4], h1b[3];
+};
+constexpr StruveCoeffs kStruve = {
+    {1.909859164, -1.909855001, 0.687514637, -0.126164557, 0.013828813, -0.000876918},
+    {0.99999906, 4.77228920, 3.85542044, 0.32303607},
+    {4.88331068, 4.28957333, 0.52120508},
+    {1.909859286, -1.145914713, 0.294656958, -0.042070508, 0.003785727, -0.000207183},
+    {1.00000004, 3.92205313, 2.64893033, 0.27450895},
+    {3.81095112, 2.26216956, 0.10885141},
+};
+
+// Struve H0 and H1 together: below 3 the polynomials; above, the Bessel
+// Y0 / Y1 plus the rational functions, whose two denominators share one
+// reciprocal.
 __host__ __device__ inline void struve_h01(double x, double y0, double y1, double& h0, double& h1) {
+    constexpr const double* P = kStruve.h0p;
+    constexpr const double* Q = kStruve.h1p;
     if (x <= 3.0) {
         double x3 = x * (1.0 / 3.0), yy = x3 * x3;
-        h0 = (1.909859164 + (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 - 0.000876918 * yy) * yy) * yy) * yy) * yy) * x3;
-        h1 = (1.909859286 + (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 - 0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
+        h0 = (P[0] + (P[1] + (P[2] + (P[3] + (P[4] + P[5] * yy) * yy) * yy) * yy) * yy) * x3;
+        h1 = (Q[0] + (Q[1] + (Q[2] + (Q[3] + (Q[4] + Q[5] * yy) * yy) * yy) * yy) * yy) * yy;
         return;
     }
+    constexpr const double* a = kStruve.h0a;
+    constexpr const double* b = kStruve.h0b;
+    constexpr const double* c = kStruve.h1a;
+    constexpr const double* d = kStruve.h1b;
     double u = 3.0 * frcp(x), yy = u * u;
-    double c1a = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
-    double c2a = kPI * x * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
-    double c1b = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
-    double c2b = kPI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
+    double c1a = 2.0 * (a[0] + (a[1] + (a[2] + a[3] * yy) * yy) * yy);
+    double c2a = kPI * x * (1.0 + (b[0] + (b[1] + b[2] * yy) * yy) * yy);
+    double c1b = 2.0 * (c[0] + (c[1] + (c[2] + c[3] * yy) * yy) * yy);
+    double c2b = kPI * (1.0 + (d[0] + (d[1] + d[2] * yy) * yy) * yy);
     double rab = frcp(c2a * c2b);
     h0 = c1a * c2b * rab + y0;
     h1 = c1b * c2a * rab + y1;
-}
-
-// The two separately (Kernels.cpp:35-116), for tests of the approximation.
-__host__ __device__ inline double struve_h0(double x, double y0) {
-    if (x <= 3.0) {
-        double yy = (x / 3.0) * (x / 3.0);
-        double s = 1.909859164 + (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 - 0.000876918 * yy) * yy) * yy) * yy) * yy;
-        return s * (x / 3.0);
-    }
-    double yy = (3.0 / x) * (3.0 / x);
-    double c1 = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
-    double c2 = kPI * x * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
-    return c1 / c2 + y0;
-}
-
-__host__ __device__ inline double struve_h1(double x, double y1) {
-    if (x <= 3.0) {
-        double yy = (x / 3.0) * (x / 3.0);
-        return (1.909859286 + (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 - 0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
-    }
-    double yy = (3.0 / x) * (3.0 / x);
-    double c1 = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
-    double c2 = kPI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
-    return c1 / c2 + y1;
 }
 
 // Bessel J0(x), J1(x) for x >= 0 and, for x > 3, Y0(x), Y1(x) (0 below):

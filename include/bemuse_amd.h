@@ -253,6 +253,11 @@ int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const
  * GF_FuncA..D and GF_dFuncA..C (Kernels.cpp:120-256) in that order (64
  * values), then EUL (BEMUse_Math_Types.h:99). n must be >= 65. */
 int bem_gf_coefficients(double* out, int n);
+/* The Struve approximation literals the device HavelockGF is compiled with
+ * (StruveH0 / StruveH1, Kernels.cpp:35-116), for checking against the
+ * reference text without a GPU: P0..P5, a0..a3, b1..b3 of StruveH0, then
+ * P1..P6, a0..a3, b1..b3 of StruveH1 (26 values). n must be >= 26. */
+int bem_struve_coefficients(double* out, int n);
 /* The Bessel evaluator of the device HavelockGF (the reference calls
  * std::tr1::cyl_bessel_j / cyl_neumann, Kernels.cpp:31-34), run on the host
  * for checking without a GPU: out[4 i .. 4 i + 3] = J0, J1, Y0, Y1 at x[i] >= 0,

@@ -54,8 +54,10 @@ def lib():
         L.orc_struve.argtypes = [c_int, _PD, _PD, _PD]
         L.orc_havelock.argtypes = [c_int, _PD, _PD, _PD, _PD]
         L.orc_poly_coeffs.argtypes = [_PD]
+        L.orc_struve_coeffs.argtypes = [_PD]
         L.orc_num_threads.restype = c_int
         L.orc_set_threads.argtypes = [c_int]
+        L.orc_set_diffraction_area.argtypes = [c_int]
         _lib = L
     return _lib
 
@@ -100,6 +102,13 @@ def poly_coeffs():
     """The GF polynomial tables cA, cB, cC, cD, cdA, cdB, cdC (64 values) then EUL."""
     out = np.zeros(65)
     lib().orc_poly_coeffs(_dp(out))
+    return out
+
+
+def struve_coeffs():
+    """The Struve approximation literals (26): H0 P0..P5, a0..a3, b1..b3, then H1 P1..P6, a0..a3, b1..b3."""
+    out = np.zeros(26)
+    lib().orc_struve_coeffs(_dp(out))
     return out
 
 
@@ -185,7 +194,10 @@ class Body:
         lib().orc_wave(self._h, K, S.ctypes.data_as(_PD), D.ctypes.data_as(_PD))
         return S, D
 
-    def solve(self, omega, kappa=None, betas=(), compose=0, rad_only=False, want_phi=False):
+    def solve(self, omega, kappa=None, betas=(), compose=0, rad_only=False, want_phi=False, diff_area=True):
+        """diff_area: the reference's area factor on the diffraction right-hand
+        side (Hydrodynamic_Solver.cpp:445); False solves the physical
+        scattering problem D Phi_S = S (-dPhi_I/dn) instead (a check only)."""
         if kappa is None:
             kappa = omega * omega / 9.81
         betas = np.ascontiguousarray(betas, dtype=np.float64)
@@ -196,9 +208,13 @@ class Body:
         phiJ = np.zeros((self.N, 6), dtype=np.complex128, order="F")
         phiS = np.zeros((self.N, max(nb, 1)), dtype=np.complex128, order="F")
         t = np.zeros(3)
-        lib().orc_solve(self._h, omega, kappa, compose, int(rad_only), nb, _dp(betas) if nb else None, _dp(A), _dp(B),
-                        F.ctypes.data_as(_PD), FK.ctypes.data_as(_PD), SC.ctypes.data_as(_PD),
-                        phiJ.ctypes.data_as(_PD), phiS.ctypes.data_as(_PD), _dp(t))
+        lib().orc_set_diffraction_area(int(bool(diff_area)))
+        try:
+            lib().orc_solve(self._h, omega, kappa, compose, int(rad_only), nb, _dp(betas) if nb else None, _dp(A),
+                            _dp(B), F.ctypes.data_as(_PD), FK.ctypes.data_as(_PD), SC.ctypes.data_as(_PD),
+                            phiJ.ctypes.data_as(_PD), phiS.ctypes.data_as(_PD), _dp(t))
+        finally:
+            lib().orc_set_diffraction_area(1)
         res = dict(A=A.reshape(6, 6), B=B.reshape(6, 6), F=F[:, :nb], FK=FK[:, :nb], SC=SC[:, :nb],
                    timings=dict(assemble=t[0], factor=t[1], rest=t[2]))
         if want_phi:

@@ -237,38 +237,45 @@ inline double BesselJ1(double x) { return std::tr1::cyl_bessel_j(1.0, x); }
 inline double BesselY0(double x) { return std::tr1::cyl_neumann(0.0, x); }
 inline double BesselY1(double x) { return std::tr1::cyl_neumann(1.0, x); }
 
+// Struve approximation literals (Kernels.cpp:44-49,56-63 for H0; :84-89,96-103
+// for H1), in the reference's order: P0..P5, a0..a3, b1..b3 of StruveH0, then
+// P1..P6, a0..a3, b1..b3 of StruveH1.
+const double kStruveTab[26] = {
+    +1.909859164, -1.909855001, +0.687514637, -0.126164557, +0.013828813, -0.000876918,
+    0.99999906, 4.77228920, 3.85542044, 0.32303607,
+    4.88331068, 4.28957333, 0.52120508,
+    +1.909859286, -1.145914713, +0.294656958, -0.042070508, +0.003785727, -0.000207183,
+    1.00000004, 3.92205313, 2.64893033, 0.27450895,
+    3.81095112, 2.26216956, 0.10885141};
+
 double StruveH0(double xx) {
+    const double* P = kStruveTab;
+    const double *a = kStruveTab + 6, *b = kStruveTab + 10;
     double SH0;
     if (xx <= 3.0) {
         double yy = (xx / 3.0) * (xx / 3.0);
-        const double P0 = +1.909859164, P1 = -1.909855001, P2 = +0.687514637,
-                     P3 = -0.126164557, P4 = +0.013828813, P5 = -0.000876918;
-        SH0 = P0 + (P1 + (P2 + (P3 + (P4 + P5 * yy) * yy) * yy) * yy) * yy;
+        SH0 = P[0] + (P[1] + (P[2] + (P[3] + (P[4] + P[5] * yy) * yy) * yy) * yy) * yy;
         SH0 *= xx / 3.0;
     } else {
         double yy = (3.0 / xx) * (3.0 / xx);
-        const double a0 = 0.99999906, a1 = 4.77228920, a2 = 3.85542044, a3 = 0.32303607;
-        const double b1 = 4.88331068, b2 = 4.28957333, b3 = 0.52120508;
-        double c1 = 2.0 * (a0 + (a1 + (a2 + a3 * yy) * yy) * yy);
-        double c2 = PI * xx * (1.0 + (b1 + (b2 + b3 * yy) * yy) * yy);
+        double c1 = 2.0 * (a[0] + (a[1] + (a[2] + a[3] * yy) * yy) * yy);
+        double c2 = PI * xx * (1.0 + (b[0] + (b[1] + b[2] * yy) * yy) * yy);
         SH0 = c1 / c2 + BesselY0(xx);
     }
     return SH0;
 }
 
 double StruveH1(double xx) {
+    const double* P = kStruveTab + 13;
+    const double *a = kStruveTab + 19, *b = kStruveTab + 23;
     double SH1;
     if (xx <= 3.0) {
         double yy = (xx / 3.0) * (xx / 3.0);
-        const double P1 = +1.909859286, P2 = -1.145914713, P3 = +0.294656958,
-                     P4 = -0.042070508, P5 = +0.003785727, P6 = -0.000207183;
-        SH1 = (P1 + (P2 + (P3 + (P4 + (P5 + P6 * yy) * yy) * yy) * yy) * yy) * yy;
+        SH1 = (P[0] + (P[1] + (P[2] + (P[3] + (P[4] + P[5] * yy) * yy) * yy) * yy) * yy) * yy;
     } else {
         double yy = (3.0 / xx) * (3.0 / xx);
-        const double a0 = 1.00000004, a1 = 3.92205313, a2 = 2.64893033, a3 = 0.27450895;
-        const double b1 = 3.81095112, b2 = 2.26216956, b3 = 0.10885141;
-        double c1 = 2.0 * (a0 + (a1 + (a2 + a3 * yy) * yy) * yy);
-        double c2 = PI * (1.0 + (b1 + (b2 + b3 * yy) * yy) * yy);
+        double c1 = 2.0 * (a[0] + (a[1] + (a[2] + a[3] * yy) * yy) * yy);
+        double c2 = PI * (1.0 + (b[0] + (b[1] + b[2] * yy) * yy) * yy);
         SH1 = c1 / c2 + BesselY1(xx);
     }
     return SH1;
@@ -472,6 +479,8 @@ void orc_havelock(int n, const double* X, const double* Y, const double* V, doub
     }
 }
 
+void orc_struve_coeffs(double* out /*26*/) { std::memcpy(out, kStruveTab, sizeof(kStruveTab)); }
+
 void orc_poly_coeffs(double* out /*65: 64 coefficients, then EUL*/) {
     int k = 0;
     for (double c : cA) out[k++] = c;
@@ -650,6 +659,15 @@ void orc_lu_solve(int n, const double* LU, const int* piv, int nrhs, double* B) 
 // after A/B (:469). Outputs: A36, B36 (6x6 row-major), F, FK, SC (6 x nb
 // complex, row-major, interleaved), optional PhiJ (N x 6) / PhiS (N x nb),
 // column-major interleaved. timings (optional, 3 doubles): assemble, factor, rest [s].
+// Area factor of the diffraction right-hand side (Hydrodynamic_Solver.cpp:445:
+// -DPhi_I_DN o PanAreaLong; the radiation right-hand side at :437 has none).
+// 1 (default) restates the reference; 0 solves the physical scattering problem
+// D Phi_S = S (-dPhi_I/dn), which the tests use to pin the diffraction solve
+// by the Haskind relation (direct pressure integration of Phi_I + Phi_S equals
+// the excitation from Phi_J).
+static int g_diff_area = 1;
+void orc_set_diffraction_area(int on) { g_diff_area = on ? 1 : 0; }
+
 int orc_solve(void* h, double omega, double kappa, int compose, int rad_only,
               int nb, const double* betas, double* A36, double* B36,
               double* F, double* FK, double* SC, double* PhiJ, double* PhiS, double* timings) {
@@ -707,7 +725,7 @@ int orc_solve(void* h, double omega, double kappa, int compose, int rad_only,
             PhiI[(size_t)bb * N + i] = pi;
             cd dn = n.x * dpx + n.y * dpy + n.z * dpz;
             DPhiI[(size_t)bb * N + i] = dn;
-            Xin[(size_t)(6 + bb) * N + i] = -(dn * dS);        // -DPhi_I_DN o PanAreaLong (:445)
+            Xin[(size_t)(6 + bb) * N + i] = g_diff_area ? -(dn * dS) : -dn;   // -DPhi_I_DN o PanAreaLong (:445)
         }
     }
     // RHS = SMat * Xin (:437, :445)

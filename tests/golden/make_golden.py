@@ -4,10 +4,12 @@ at test time).
 
 gf_poly_coeffs.json
     The numeric literals of the seven polynomials of the Havelock global
-    approximation (GF_FuncA..D, GF_dFuncA..C, src/Kernels.cpp:120-256) and
-    the Euler constant literal EUL (src/BEMUse_Math_Types.h:99), parsed from
-    the reference text by a tokeniser (the running-power statements
-    `S = c0` / `S += c*t;`; commented-out lines are skipped). The oracle's
+    approximation (GF_FuncA..D, GF_dFuncA..C, src/Kernels.cpp:120-256), of
+    the Struve approximations (StruveH0/H1, src/Kernels.cpp:35-116: the
+    `Real P0 = ...;` declarations) and the Euler constant literal EUL
+    (src/BEMUse_Math_Types.h:99), parsed from the reference text by a
+    tokeniser (the running-power statements `S = c0` / `S += c*t;`;
+    commented-out lines are skipped). The oracle's and the device's
     hand-typed tables are compared with these.
 
 cylinder_abf.npz
@@ -54,15 +56,37 @@ def parse_poly(text, name):
     return coeffs
 
 
+STRUVE_NAMES = {"StruveH0": ["P0", "P1", "P2", "P3", "P4", "P5", "a0", "a1", "a2", "a3", "b1", "b2", "b3"],
+                "StruveH1": ["P1", "P2", "P3", "P4", "P5", "P6", "a0", "a1", "a2", "a3", "b1", "b2", "b3"]}
+
+
+def parse_struve(text, name):
+    """The `Real NAME = literal;` declarations in the body of StruveH0/H1,
+    in the order listed in STRUVE_NAMES."""
+    m = re.search(r"inline\s+Real\s+" + name + r"\s*\(const Real &xx\)\s*\{(.*?)\n\}", text, re.S)
+    if not m:
+        raise RuntimeError(f"{name} not found")
+    vals = {}
+    for line in m.group(1).splitlines():
+        line = line.split("//", 1)[0].strip()
+        d = re.match(r"Real\s+(\w+)\s*=\s*(" + NUM + r")\s*;", line)
+        if d:
+            vals[d.group(1)] = float(d.group(2).replace(" ", ""))
+    return [vals[k] for k in STRUVE_NAMES[name]]
+
+
 def poly_fixture(ref):
     text = open(os.path.join(ref, "src", "Kernels.cpp")).read()
     out = {}
     for nm in ("GF_FuncA", "GF_FuncB", "GF_FuncC", "GF_FuncD", "GF_dFuncA", "GF_dFuncB", "GF_dFuncC"):
         out[nm] = parse_poly(text, nm)
+    out["StruveH0"] = parse_struve(text, "StruveH0")
+    out["StruveH1"] = parse_struve(text, "StruveH1")
     mt = open(os.path.join(ref, "src", "BEMUse_Math_Types.h")).read()
     m = re.search(r"EUL\s*=\s*(" + NUM + ")", mt)
     out["EUL"] = float(m.group(1))
-    out["_source"] = "src/Kernels.cpp:120-256 (GF_Func*/GF_dFunc* literals), src/BEMUse_Math_Types.h:99 (EUL)"
+    out["_source"] = ("src/Kernels.cpp:120-256 (GF_Func*/GF_dFunc* literals), src/Kernels.cpp:35-116 "
+                      "(StruveH0/H1: P, a, b literals), src/BEMUse_Math_Types.h:99 (EUL)")
     with open(os.path.join(HERE, "gf_poly_coeffs.json"), "w") as f:
         json.dump(out, f, indent=1)
     print("wrote gf_poly_coeffs.json:", {k: len(v) for k, v in out.items() if isinstance(v, list)})
@@ -89,6 +113,9 @@ def cylinder_fixture():
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", choices=["poly", "cylinder"], default=None)
     args = ap.parse_args()
-    poly_fixture(args.reference)
-    cylinder_fixture()
+    if args.only in (None, "poly"):
+        poly_fixture(args.reference)
+    if args.only in (None, "cylinder"):
+        cylinder_fixture()

@@ -12,8 +12,9 @@ index, transposed operand) fails:
                                                                 (tests/golden/gf_poly_coeffs.json); the device
                                                                 table too
   Bessel J0 J1 Y0 Y1            Kernels.cpp:31-34 (TR1)         scipy.special, 1e-12
-  StruveH0/H1                   Kernels.cpp:35-116              scipy.special.struve, abs 5e-8 (the published
-                                                                accuracy of the approximation)
+  StruveH0/H1                   Kernels.cpp:35-116              literals parsed from the reference text (the
+                                                                device table too); scipy.special.struve, abs
+                                                                5e-8 (the approximation's published accuracy)
   havelock (imaginary parts)    Kernels.cpp:284-295,327-338     closed forms -2 pi e^v J0(h), 2 pi e^v J1(h)
   havelock (real parts)         Kernels.cpp:258-282,297-325     exact PV integrals (scipy quad) within the
                                                                 global approximation's accuracy; dGF/dV =
@@ -33,6 +34,12 @@ index, transposed operand) fails:
   orc_solve (A, B, F)           Hydrodynamic_Solver.cpp:400-477 hemisphere closed forms A11(0) = A33(inf) =
                                                                 V/2; Newman's relation B_jj vs |F_j|^2;
                                                                 reciprocity; axisymmetry
+  orc_solve (Phi_S)             Hydrodynamic_Solver.cpp:444-451 Haskind: with the area factor of :445 switched
+                                                                off, direct pressure integration -i w M Phi_S
+                                                                equals the scattering force SC from Phi_J
+                                                                (:472-477) to the discretisation error (3e-4 on
+                                                                the hemisphere); on a mesh of equal panel areas
+                                                                a, Phi_S with the factor = a x Phi_S without
 
 Parity unpinned (below every stated tolerance; would need a reference build):
 Eigen PartialPivLU blocking/summation order, Eigen FromTwoVectors' choice
@@ -153,6 +160,18 @@ def test_gf_tables_match_reference_literals():
     np.testing.assert_array_equal(oracle.poly_coeffs(), want)
     import bemuse_amd
     np.testing.assert_array_equal(bemuse_amd.gf_coefficients(), want)
+
+
+def test_struve_tables_match_reference_literals():
+    """The 26 literals of StruveH0/H1 (Kernels.cpp:35-116), parsed from the
+    reference text, against the oracle's and the device's tables (both
+    evaluators read their literals from these tables)."""
+    g = json.load(open(os.path.join(HERE, "golden", "gf_poly_coeffs.json")))
+    want = np.array(g["StruveH0"] + g["StruveH1"])
+    assert len(want) == 26
+    np.testing.assert_array_equal(oracle.struve_coeffs(), want)
+    import bemuse_amd
+    np.testing.assert_array_equal(bemuse_amd.struve_coefficients(), want)
 
 
 def test_bessel_vs_scipy():
@@ -441,6 +460,73 @@ def test_newman_relation_and_symmetries(hemi, omega):
     assert abs(F[0, 0] - F[1, 1]) < 1e-12 * abs(F[0, 0])     # surge at 0 deg = sway at 90 deg
     assert abs(F[2, 0] - F[2, 1]) < 1e-12 * abs(F[2, 0])     # heave independent of heading
     np.testing.assert_allclose(F, np.conj(r["FK"] + r["SC"]), rtol=0, atol=1e-15 * np.abs(F).max())
+
+
+def box_mesh(L, W, D, nx, ny, nz):
+    """Open-top box on a uniform grid (every panel L/nx x W/ny etc.), inward normals."""
+    nodes, el = [], []
+
+    def face(p0, du, dv, nu, nv):
+        base = len(nodes)
+        for j in range(nv + 1):
+            for i in range(nu + 1):
+                nodes.append(p0 + du * i / nu + dv * j / nv)
+        for j in range(nv):
+            for i in range(nu):
+                a = base + j * (nu + 1) + i
+                el.append((a, a + 1, a + nu + 2, a + nu + 1))
+
+    ex, ey, ez = np.array([L, 0.0, 0.0]), np.array([0.0, W, 0.0]), np.array([0.0, 0.0, D])
+    o = np.array([-L / 2, -W / 2, -D])
+    face(o, ex, ey, nx, ny)
+    face(o, ex, ez, nx, nz)
+    face(o + ey, ex, ez, nx, nz)
+    face(o, ey, ez, ny, nz)
+    face(o + ex, ey, ez, ny, nz)
+    nodes, el = np.array(nodes), np.array(el, np.int32)
+    return nodes, orient_inward(nodes, el, np.array([0.0, 0.0, -D / 2]))
+
+
+def _moment_rows(ob):
+    """M (6 x N): [n dS; (x x n) dS] at the centroids (Hydrodynamic_Solver.cpp:140-175)."""
+    g = ob.geometry()
+    C, n, A = g["centroid"], g["normal"], g["area"]
+    return np.vstack([n.T * A, np.cross(C, n).T * A])
+
+
+@pytest.mark.parametrize("omega", [0.5, 1.5, 2.5])
+def test_diffraction_solve_haskind_relation(hemi, omega):
+    """The scattered potential, solved without the reference's area factor on
+    its right-hand side, gives by direct pressure integration the scattering
+    force the reference takes from Phi_J (Haskind): -i w M Phi_S = SC, so
+    F = conj(FK + SC) = conj(-i w M (Phi_I + Phi_S)). Green's theorem makes the
+    two equal up to the discretisation; a sign, a transposed operand or a
+    wrong incident normal derivative breaks it by O(1)."""
+    nodes, el, ob = hemi
+    betas = [0.0, np.pi / 3]
+    r = ob.solve(omega, betas=betas, want_phi=True, diff_area=False)
+    direct = -1j * omega * (_moment_rows(ob) @ r["phiS"])
+    scale = np.abs(r["SC"]).max()
+    assert np.abs(direct - r["SC"]).max() < 1e-3 * scale
+    assert np.abs(-direct - r["SC"]).max() > 1.9 * scale
+    # radiation results do not depend on the diffraction right-hand side
+    ra = ob.solve(omega, betas=betas, want_phi=True)
+    assert np.array_equal(ra["phiJ"], r["phiJ"]) and np.array_equal(ra["F"], r["F"])
+
+
+def test_diffraction_area_factor_on_equal_panels():
+    """On a mesh whose panels all have area a = 4, the reference's diffraction
+    right-hand side S (-dPhi_I/dn o Area) (Hydrodynamic_Solver.cpp:445) makes
+    Phi_S exactly a times the physical scattered potential."""
+    nodes, el = box_mesh(12.0, 6.0, 4.0, 6, 3, 2)
+    ob = oracle.Body(nodes, el)
+    assert np.allclose(ob.geometry()["area"], 4.0, rtol=1e-14)
+    for w in (0.6, 1.4):
+        r = ob.solve(w, betas=[0.0, 0.7], want_phi=True, diff_area=False)
+        ra = ob.solve(w, betas=[0.0, 0.7], want_phi=True)
+        assert np.abs(ra["phiS"] - 4.0 * r["phiS"]).max() < 1e-13 * np.abs(ra["phiS"]).max()
+        direct = -1j * w * (_moment_rows(ob) @ r["phiS"])
+        assert np.abs(direct - r["SC"]).max() < 3e-2 * np.abs(r["SC"]).max()   # coarse 54-panel box
 
 
 def test_oracle_matches_committed_fixture():
