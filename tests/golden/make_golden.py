@@ -19,7 +19,14 @@ cylinder_abf.npz
     oracle live in tests/test_oracle.py); the GPU parity tests compare with
     it as well as with the live oracle.
 
-Usage: python tests/golden/make_golden.py [--reference /root/reference]
+semisub6018_abf.npz
+    A, B, F, FK, SC of the oracle for the 6018-panel three-column semi-sub
+    (the 6k-panel workload) at omega = 0.05, 1.3064 and 2.5 rad/s, headings
+    0, 45, 90, 135, 180 deg, and Phi_J, Phi_S at 1.3064 rad/s. About a
+    minute per frequency on 8 threads, which is why the GPU tests read it
+    instead of running the oracle live.
+
+Usage: python tests/golden/make_golden.py [--reference /root/reference] [--only poly|cylinder|semisub]
 """
 import argparse
 import json
@@ -110,12 +117,43 @@ def cylinder_fixture():
     print("wrote cylinder_abf.npz N =", len(elems))
 
 
+def semisub_fixture():
+    """The 6018-panel three-column semi-sub workload (bemuse_amd.Mesh.semisub3
+    defaults): oracle A, B, F, FK, SC at three of its 40 frequencies (the
+    lowest, a middle one, the highest) for its 5 headings, and the radiation
+    and diffraction potentials at the middle one."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    import bemuse_amd as bm
+    oracle.set_threads(os.cpu_count() or 1)
+    mesh = bm.Mesh.semisub3()
+    nodes, elems = mesh.arrays()
+    ob = oracle.Body(nodes, elems)
+    all_w = np.linspace(0.05, 2.5, 40)
+    idx = np.array([0, 20, 39])
+    betas = np.deg2rad([0.0, 45.0, 90.0, 135.0, 180.0])
+    out = {k: [] for k in ("A", "B", "F", "FK", "SC")}
+    for i in idx:
+        r = ob.solve(all_w[i], betas=betas, want_phi=(i == 20))
+        for k in out:
+            out[k].append(r[k])
+        if i == 20:
+            phiJ, phiS = r["phiJ"], r["phiS"]
+        print(f"semisub omega={all_w[i]:.4f} done", flush=True)
+    np.savez_compressed(os.path.join(HERE, "semisub6018_abf.npz"), nodes=nodes, elems=elems, omegas=all_w[idx],
+                        betas=betas, phi_omega=all_w[20], phiJ=phiJ, phiS=phiS,
+                        **{k: np.array(v) for k, v in out.items()})
+    print("wrote semisub6018_abf.npz N =", len(elems))
+
+
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["poly", "cylinder"], default=None)
+    ap.add_argument("--only", choices=["poly", "cylinder", "semisub"], default=None)
     args = ap.parse_args()
     if args.only in (None, "poly"):
         poly_fixture(args.reference)
     if args.only in (None, "cylinder"):
         cylinder_fixture()
+    if args.only in (None, "semisub"):
+        semisub_fixture()

@@ -1,8 +1,13 @@
 """GPU parity: the HIP path (through the C-ABI) against the FP64 CPU oracle.
 
-Tolerances: FP64 kernels per-matrix relative max-norm <= 1e-11; A, B, F
-<= 1e-8 relative (max-norm of the oracle matrix as denominator) in FP64 mode
-and <= 1e-6 in the mixed FP32-LU + FP64-refinement mode.
+Tolerances (relative, max-norm of the oracle matrix as denominator):
+  - FP64 kernels per matrix <= 1e-11;
+  - A, B, F: the required bars, 1e-8 in FP64 mode and 1e-6 in the mixed
+    FP32-LU + FP64-refinement mode, asserted on their own, and the tighter
+    levels the path achieves, TIGHT = 1e-11 (FP64) and 1e-10 (mixed; the
+    refinement stops at a 1e-13 relative residual), so that a regression that
+    loses digits fails long before the bars do;
+  - potentials Phi_J and Phi_S: 10x the A, B, F levels.
 """
 import numpy as np
 import pytest
@@ -11,6 +16,18 @@ import bemuse_amd as bm
 import oracle
 
 pytestmark = pytest.mark.gpu
+
+
+BAR = {bm.MODE_FP64: 1e-8, bm.MODE_MIXED: 1e-6}
+TIGHT = {bm.MODE_FP64: 1e-11, bm.MODE_MIXED: 1e-10}
+
+
+def assert_abf(g, r, mode, keys=("A", "B", "F"), what=""):
+    """A, B, F (or the given keys) within the bar and the tight level of the mode."""
+    for key in keys:
+        e = relerr(g[key], r[key])
+        assert e < BAR[mode], (what, key, e, "bar", BAR[mode])
+        assert e < TIGHT[mode], (what, key, e, "tight", TIGHT[mode])
 
 
 def relerr(a, b):
@@ -113,7 +130,11 @@ def test_solve_frequency_cylinder(cyl, mode, tol, omega):
     r = ob.solve(omega, betas=betas, want_phi=True)
     for key in ("A", "B", "F", "FK", "SC"):
         assert relerr(g[key], r[key]) < tol, (key, relerr(g[key], r[key]), g["info"])
-    assert relerr(g["phiJ"], r["phiJ"]) < tol * 10
+    assert_abf(g, r, mode, ("A", "B", "F", "FK", "SC"), "cylinder")
+    # radiation and diffraction potentials (Phi_S with the area factor of
+    # Hydrodynamic_Solver.cpp:445, as the reference returns it)
+    assert relerr(g["phiJ"], r["phiJ"]) < TIGHT[mode] * 10
+    assert relerr(g["phiS"], r["phiS"]) < TIGHT[mode] * 10
     if mode == bm.MODE_MIXED:
         assert g["info"]["fell_back_fp64"] == 0
         assert g["info"]["residual"] < 1e-13
@@ -126,10 +147,9 @@ def test_solve_barge_both_modes(ctx):
     body.assemble_rankine()
     ob = oracle.Body(nodes, elems)
     r = ob.solve(0.8, betas=[0.0, 0.5])
-    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+    for mode in (bm.MODE_FP64, bm.MODE_MIXED):
         g = body.solve(0.8, betas=[0.0, 0.5], mode=mode)
-        for key in ("A", "B", "F"):
-            assert relerr(g[key], r[key]) < tol, (mode, key)
+        assert_abf(g, r, mode, what="barge")
     body.close()
 
 
@@ -153,10 +173,9 @@ def test_parametric_bodies_both_modes(ctx, name):
     ob = oracle.Body(nodes, elems)
     betas = [0.0, 0.5]
     r = ob.solve(0.8, betas=betas)
-    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+    for mode in (bm.MODE_FP64, bm.MODE_MIXED):
         g = body.solve(0.8, betas=betas, mode=mode)
-        for key in ("A", "B", "F"):
-            assert relerr(g[key], r[key]) < tol, (name, mode, key, relerr(g[key], r[key]))
+        assert_abf(g, r, mode, what=name)
         if mode == bm.MODE_MIXED:
             assert g["info"]["fell_back_fp64"] == 0
     body.close()
@@ -206,8 +225,7 @@ def test_sweep_matches_single_solves(cyl, mode):
         assert relerr(sw["F"][i], g["F"]) < tol
         assert sw["info"][i]["fell_back_fp64"] == 0
     r = ob.solve(omegas[3], betas=betas)
-    for key in ("A", "B", "F"):
-        assert relerr(sw[key][3], r[key]) < (1e-8 if mode == bm.MODE_FP64 else 1e-6)
+    assert_abf({k: sw[k][3] for k in ("A", "B", "F")}, r, mode, what="sweep")
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
@@ -246,11 +264,11 @@ def test_solve_vs_committed_fixture(ctx):
     fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cylinder_abf.npz"))
     body = bm.Body(ctx, bm.Mesh.from_arrays(fx["nodes"], fx["elems"]))
     body.assemble_rankine()
-    for mode, tol in ((bm.MODE_FP64, 1e-8), (bm.MODE_MIXED, 1e-6)):
+    for mode in (bm.MODE_FP64, bm.MODE_MIXED):
         sw = body.solve_sweep(fx["omegas"], betas=fx["betas"], mode=mode, lanes=2)
-        for k in ("A", "B", "F"):
-            for i in range(len(fx["omegas"])):
-                assert relerr(sw[k][i], fx[k][i]) < tol, (mode, k, i)
+        for i in range(len(fx["omegas"])):
+            assert_abf({k: sw[k][i] for k in ("A", "B", "F")}, {k: fx[k][i] for k in ("A", "B", "F")}, mode,
+                       what=f"fixture {i}")
     body.close()
 
 
@@ -291,6 +309,7 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
     om = -1.0 / (2 * np.pi)
     r = ob.solve(om, kappa=om * om / 9.81, compose=0, rad_only=True)
     assert relerr(res["setup_A"], r["A"]) < tol
+    assert relerr(res["setup_A"], r["A"]) < TIGHT[mode]
     refs = {}
     for w in (0.4, 1.3):
         g = res[w]
@@ -298,6 +317,7 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
         refs[w] = r
         for key in ("A", "B", "F"):
             assert relerr(g[key], r[key]) < tol, (w, key, relerr(g[key], r[key]))
+        assert_abf(g, r, mode, what=f"drop-in {w}")
     # WAMIT-style files: .1 rows "F i j A B" (F = 2 pi omega), FreqInf rows "-1 i j A"
     out = tmp_path / "Output"
     rows1 = [l.split() for l in (out / "dropin_test.1").read_text().splitlines() if l.strip()]
@@ -359,6 +379,7 @@ def test_cpp_dropin_ifr(mode, tol, tmp_path):
         r = ob.solve(w, betas=[0.0, np.pi / 4])
         for key in ("A", "B", "F"):
             assert relerr(res[w][key], r[key]) < tol, (w, key, relerr(res[w][key], r[key]))
+        assert_abf(res[w], r, mode, what=f"drop-in IFR {w}")
         assert r["B"][2, 2] > 0.0
 
 
@@ -391,6 +412,7 @@ def test_cpp_dropin_oc4_semisub_with_lids(mode, tol, tmp_path):
         r = ob.solve(w, betas=[0.0, np.pi / 4])
         for key in ("A", "B", "F"):
             assert relerr(res[w][key], r[key]) < tol, (w, key, relerr(res[w][key], r[key]))
+        assert_abf(res[w], r, mode, what=f"drop-in OC4 {w}")
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
@@ -413,9 +435,7 @@ def test_batch_matches_per_body_sweeps(ctx, mode):
             assert relerr(g["F"][k], ref["F"][k]) < 1e-12
     nodes, elems = meshes[1].arrays()
     r = oracle.Body(nodes, elems).solve(2.0, betas=betas)
-    tol = 1e-8 if mode == bm.MODE_FP64 else 1e-6
-    for key in ("A", "B", "F"):
-        assert relerr(got[1][key][1], r[key]) < tol
+    assert_abf({k: got[1][k][1] for k in ("A", "B", "F")}, r, mode, what="batch")
     for b in bodies:
         b.close()
 
