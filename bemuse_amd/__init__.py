@@ -40,7 +40,14 @@ atexit.register(_mark_exit)
 
 
 class BemError(RuntimeError):
-    pass
+    """A failed C-ABI call; .code is the BEM_ERR_* status (ERR_* below)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
 
 
 class SolveInfo(ctypes.Structure):
@@ -148,6 +155,7 @@ _SIGS = {
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
+    "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
     "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
@@ -189,7 +197,7 @@ def lib():
 def _check(rc):
     if rc != 0:
         msg = lib().bem_last_error().decode(errors="replace")
-        raise BemError(f"bemuse_amd error {rc}: {msg}")
+        raise BemError(f"bemuse_amd error {rc}: {msg}", rc)
 
 
 def _dp(a):
@@ -419,6 +427,11 @@ class Context:
         out = np.zeros(3)
         _check(lib().bem_measure_peaks(self._h, _dp(out)))
         return dict(fp32_mfma_tflops=float(out[0]), fp64_mfma_tflops=float(out[1]), copy_gbs=float(out[2]))
+
+    def set_refine_limit(self, max_steps):
+        """Correction steps allowed to the mixed modes' refinement (0..4, default 4);
+        0 sends every mixed frequency to the FP64 re-solve."""
+        _check(lib().bem_set_refine_limit(self._h, int(max_steps)))
 
     def set_graphs(self, enable):
         """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""

@@ -83,8 +83,16 @@ struct SurState {
     }
 };
 
+// Iterative refinement of the mixed modes: at most kRefineMaxIter correction
+// steps (bem_set_refine_limit lowers it), stopping at a max-norm relative
+// residual of kRefineTol: with the system's condition numbers this keeps A, B,
+// F orders of magnitude inside the 1e-6 bar while saving correction sweeps.
+constexpr int kRefineMaxIter = 4;
+constexpr double kRefineTol = 1e-13;
+
 struct bem_ctx {
     int device = 0;
+    int refine_max = kRefineMaxIter;   // correction steps allowed (bem_set_refine_limit)
     SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
     bool graphs = false;               // replay captured frequency graphs on the lanes (bem_set_graphs)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
@@ -167,10 +175,10 @@ struct LinSys {
 // surrogate use.
 struct GraphKey {
     unsigned long long body = 0;
-    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1;
+    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1;
     bool operator==(const GraphKey& o) const {
         return body == o.body && Np == o.Np && mode == o.mode && nb == o.nb && rad_only == o.rad_only &&
-               compose == o.compose && sur == o.sur;
+               compose == o.compose && sur == o.sur && refine == o.refine;
     }
 };
 
@@ -204,6 +212,7 @@ struct Lane {
     // bookkeeping of the frequency in flight
     int pending = -1;
     int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
+    int refine_max = kRefineMaxIter;
     bool sur_used = false;
     double omega = 0, kappa = 0;
     // captured frequency graph and the configuration it was captured for
@@ -292,15 +301,10 @@ void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) {
         launch_lu<float>(L.s1, L.s2, L.lu_events.data(), s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof);
 }
 
-constexpr int kRefineMaxIter = 4;
-// Refinement stops at a max-norm relative residual of 1e-13: with the
-// system's condition numbers this keeps A, B, F orders of magnitude inside
-// the 1e-6 bar while saving correction sweeps.
-constexpr double kRefineTol = 1e-13;
-
 // Solve D X = B for nr columns (B in the workspace, X out). FP32 mode:
 // FP32 LU solve followed by FP64 iterative refinement against the FP64
-// matrix s.Dr/s.Di, controlled on the device (no host round trip).
+// matrix s.Dr/s.Di, controlled on the device (no host round trip), with at
+// most L.refine_max correction steps.
 void solve_system(Lane& L, int precision, int nr) {
     LinSys& s = L.sys;
     size_t n = (size_t)s.Np * nr;
@@ -317,13 +321,13 @@ void solve_system(Lane& L, int precision, int nr) {
     launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, L.counter,
                                   L.pub);
     launch_from_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i);
-    for (int it = 0; it <= kRefineMaxIter; it++) {
+    const int max_iter = std::max(0, std::min(L.refine_max, kRefineMaxIter));
+    for (int it = 0; it <= max_iter; it++) {
         // r = b - A x in FP64; the first residual also yields |A|_inf
         launch_matvec<double>(L.s1, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
                               it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
-        launch_refine_check(L.s1, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, L.ctrl, kRefineTol,
-                            kRefineMaxIter);
-        if (it == kRefineMaxIter) break;
+        launch_refine_check(L.s1, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, L.ctrl, kRefineTol, max_iter);
+        if (it == max_iter) break;
         // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
         launch_to_f32(L.s1, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
         launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
@@ -748,10 +752,11 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     L.pending = index;
     L.mode = mode; L.prec = (mode == BEM_MODE_FP64) ? 64 : 32;
     L.omega = omega; L.kappa = kappa; L.compose = compose; L.rad_only = rad_only; L.nb = nb;
+    L.refine_max = b->ctx->refine_max;
     L.hout->freq[0] = omega;
     L.hout->freq[1] = kappa;
     const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
-    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0};
+    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max};
     const bool use_graph = b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
     L.last_key = key;
     L.graphed = use_graph;
@@ -783,7 +788,10 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
 }
 
 // Waits for the lane's frequency and writes its results (FP64 re-solve if
-// the refinement did not converge).
+// the refinement did not converge). BEM_ERR_NUMERIC, with the results still
+// written, when the final factorisation met a zero or non-finite pivot or a
+// result is not finite (the reference's Eigen LU would return the same NaN
+// silently).
 int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* info) {
     HIP_TRY(hipEventSynchronize(L.done));
     bem_solve_info loc{};
@@ -842,6 +850,16 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
     }
     if (info) *info = loc;
     L.pending = -1;
+    const bool singular = (L.hout->flags[np] & 2) != 0;
+    bool finite = true;
+    const int ncoef = 2 * (36 + ((L.rad_only || nb == 0) ? 0 : 12 * nb));
+    for (int q = 0; q < ncoef; q++) finite = finite && std::isfinite(out[q]);
+    if (singular || !finite) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "%s at omega = %.17g", singular ?
+                      "numerically singular system (zero or non-finite pivot)" : "non-finite A, B or F", omega);
+        return fail(BEM_ERR_NUMERIC, msg);
+    }
     return BEM_OK;
 }
 
@@ -879,6 +897,15 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
     std::vector<unsigned long long> since(nl, 0);
     unsigned long long seq = 0;
     size_t next = 0;
+    int numeric = BEM_OK;
+    std::string numeric_msg;
+    // on any other error no lane may be left with a frequency in flight
+    auto drain = [&](int r) {
+        std::string msg = g_err;
+        for (int l = 0; l < nl; l++)
+            if (busy[l] >= 0) { (void)hipEventSynchronize(lanes[l]->done); lanes[l]->pending = -1; }
+        return fail(r, msg);
+    };
     for (;;) {
         int pick = -1;
         if (next < tasks.size())
@@ -896,18 +923,26 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
             size_t f = (size_t)t.out;
             r = harvest(t.b, *lanes[pick], A36 + 36 * f, B36 + 36 * f, F ? F + fs * f : nullptr,
                         FK ? FK + fs * f : nullptr, SC ? SC + fs * f : nullptr, infos ? infos + f : nullptr);
-            if (r) return r;
             busy[pick] = -1;
+            if (r == BEM_ERR_NUMERIC) {   // that frequency's results are NaN; the others go on
+                if (numeric == BEM_OK) { numeric = r; numeric_msg = g_err; }
+            } else if (r) {
+                return drain(r);
+            }
         }
         if (next < tasks.size()) {
             const Task& t = tasks[next];
-            if ((r = issue_frequency(t.b, *lanes[pick], (int)next, mode, t.omega, t.kappa, compose, 0, nb))) return r;
+            if ((r = issue_frequency(t.b, *lanes[pick], (int)next, mode, t.omega, t.kappa, compose, 0, nb))) {
+                (void)hipEventSynchronize(lanes[pick]->done);
+                lanes[pick]->pending = -1;
+                return drain(r);
+            }
             busy[pick] = (int)next;
             since[pick] = seq++;
             next++;
         }
     }
-    return BEM_OK;
+    return numeric ? fail(numeric, numeric_msg) : BEM_OK;
 }
 
 int upload_betas(Lane& L, int nb, const double* betas) {
@@ -919,6 +954,13 @@ int upload_betas(Lane& L, int nb, const double* betas) {
 }
 
 }  // namespace
+
+int bem_set_refine_limit(bem_ctx* c, int max_steps) {
+    if (!c || max_steps < 0 || max_steps > kRefineMaxIter)
+        return fail(BEM_ERR_ARG, "bem_set_refine_limit: max_steps must be in [0, 4]");
+    c->refine_max = max_steps;
+    return BEM_OK;
+}
 
 int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double kappa, int compose, int rad_only,
                         int nb, const double* betas, double* A36, double* B36, double* F, double* FK, double* SC,
@@ -1533,10 +1575,15 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     if (!c || n <= 0 || !A || !B || !X || nrhs < 1 || nrhs > kMaxRhs || (precision != 64 && precision != 32))
         return fail(BEM_ERR_ARG, "bem_dense_solve: bad arguments (nrhs in [1,16], precision 64 or 32)");
     if (round_up(n, kPad) / kLuNb > kMaxPanels) return fail(BEM_ERR_ARG, "bem_dense_solve: matrix too large");
+    for (size_t q = 0; q < 2 * (size_t)n * n; q++)
+        if (!std::isfinite(A[q])) return fail(BEM_ERR_ARG, "bem_dense_solve: A has a non-finite entry");
+    for (size_t q = 0; q < 2 * (size_t)n * nrhs; q++)
+        if (!std::isfinite(B[q])) return fail(BEM_ERR_ARG, "bem_dense_solve: B has a non-finite entry");
     HIP_TRY(hipSetDevice(c->device));
     Lane L;
     int r = L.init(n);
     if (r) return r;
+    L.refine_max = c->refine_max;
     LinSys& s = L.sys;
     int Np = s.Np;
     {
@@ -1569,12 +1616,23 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     HIP_TRY(hipStreamSynchronize(L.s1));
     RefineCtrl ctrl{};
     if (precision == 32) HIP_TRY(hipMemcpy(&ctrl, L.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    int used = precision;
+    if (precision == 32 && !ctrl.converged) {
+        // refinement stagnated: factor and solve the FP64 matrix (still intact:
+        // mixed mode factors the FP32 copy)
+        factor_system(L, 64);
+        solve_system(L, 64, nrhs);
+        HIP_TRY(hipEventRecord(L.t[2], L.s1));
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(L.s1));
+        used = 64;
+    }
     std::vector<int> flags(Np / kLuNb + 1);
-    HIP_TRY(hipMemcpy(flags.data(), precision == 64 ? s.flag : s.flag32, flags.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(flags.data(), used == 64 ? s.flag : s.flag32, flags.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < Np / kLuNb; i++) loc.pivot_fallback += (flags[i] & 1) ? 1 : 0;
     loc.refine_iters = ctrl.iters;
     loc.residual = precision == 32 ? ctrl.rel : 0.0;
-    loc.fell_back_fp64 = (precision == 32 && !ctrl.converged) ? 1 : 0;
+    loc.fell_back_fp64 = (precision == 32 && used == 64) ? 1 : 0;
     float t1 = 0, t2 = 0;
     (void)hipEventElapsedTime(&t1, L.t[0], L.t[1]);
     (void)hipEventElapsedTime(&t2, L.t[1], L.t[2]);
@@ -1584,6 +1642,9 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     HIP_TRY(hipMemcpy(xi.data(), s.Xi, xi.size() * 8, hipMemcpyDeviceToHost));
     planar_to_interleaved(xr, xi, n, Np, nrhs, X);
     if (info) *info = loc;
+    if (flags[Np / kLuNb] & 2) return fail(BEM_ERR_NUMERIC, "bem_dense_solve: numerically singular matrix (zero or non-finite pivot)");
+    for (size_t q = 0; q < 2 * (size_t)n * nrhs; q++)
+        if (!std::isfinite(X[q])) return fail(BEM_ERR_NUMERIC, "bem_dense_solve: non-finite solution");
     return BEM_OK;
 }
 

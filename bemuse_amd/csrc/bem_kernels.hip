@@ -424,6 +424,10 @@ __device__ inline void blk_mma_rb(const T* ar_, const T* ai_, int lda, const typ
 // Wave 0: the 16 x 16 diagonal block at (o, o) factored in registers (lane
 // rho & 15 holds row rho, the pivot row broadcast with readlane), written
 // back to LDS.
+// |pivot|^2 usable: positive and finite (zero, NaN and inf are not).
+template <typename T>
+__device__ inline bool pivot_ok(T den) { return den > (T)0 && den < (T)INFINITY; }
+
 template <typename T>
 __device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad) {
     const int rho = lane & 15;
@@ -435,12 +439,14 @@ __device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad) {
     for (int j = 0; j < DB; j++) {
         T pr = bcast(ar[j], j), pi = bcast(ai[j], j);
         T den = cabs2(pr, pi);
-        if (den == (T)0) mybad |= 3;
+        // a zero or non-finite pivot: the pivoting fallback decides (it flags
+        // the panel numerically singular if no usable pivot exists)
+        if (!pivot_ok(den)) mybad |= 1;
         T inv = (T)1 / den;
         T qr = pr * inv, qi = -pi * inv;
         const bool below = rho > j;
         T mr = ar[j] * qr - ai[j] * qi, mi = ar[j] * qi + ai[j] * qr;
-        if (below && cabs2(mr, mi) > (T)1) mybad |= 1;
+        if (below && !(cabs2(mr, mi) <= (T)1)) mybad |= 1;   // also NaN
         mr = below ? mr : (T)0;
         mi = below ? mi : (T)0;
         ar[j] = below ? mr : ar[j];
@@ -611,7 +617,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int row = MT::row(g, r);
-                    if (cabs2(cr[r], ci[r]) > (T)1) mybad |= 1;
+                    if (!(cabs2(cr[r], ci[r]) <= (T)1)) mybad |= 1;
                     S.ar[o + i][DB * b + row] = cr[r]; S.ai[o + i][DB * b + row] = ci[r];
                 }
             } else {
@@ -708,7 +714,7 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
         size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + MT::row(g, r);
         T lr = cr[r], li = ci[r];
         Ar[o] = lr; Ai[o] = li;
-        if (cabs2(lr, li) > (T)1) mybad = 1;
+        if (!(cabs2(lr, li) <= (T)1)) mybad = 1;
     }
     if (mybad) atomicOr(flag + k0 / LU_NB, 1);
 }
@@ -789,7 +795,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
             }
             __syncthreads();
         }
-        if (tid == 0) { piv_s = bi[0]; ipiv[col] = bi[0]; }
+        // no finite candidate (every |a| NaN): keep the diagonal, flagged below
+        if (tid == 0) { int p0 = bi[0] < Np ? bi[0] : col; piv_s = p0; ipiv[col] = p0; }
         __syncthreads();
         int p = piv_s;
         if (p != col && tid < LU_NB) {
@@ -803,7 +810,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel_gepp(int Np, int k0, T* Ar, T
         __syncthreads();
         T pr = prow_r[j], pi = prow_i[j];
         T den = cabs2(pr, pi);
-        if (den == (T)0) { if (tid == 0) flag[k0 / LU_NB] |= 2; continue; }
+        if (!pivot_ok(den)) { if (tid == 0) flag[k0 / LU_NB] |= 2; continue; }   // numerically singular
         for (int r = col + 1 + tid; r < Np; r += 1024) {
             size_t g = (size_t)col * Np + r;
             T ar = Ar[g], ai = Ai[g];
@@ -1451,9 +1458,14 @@ __global__ void k_refine_reset(RefineCtrl* ctrl) {
 // can leave it inaccurate. The 0.01 floor keeps its roundoff floor (about
 // 1e2 u |A| |x|) under the tolerance. Converged when max_j rel_j <= tol;
 // stops (not converged) on stagnation (rel > 0.5 prev) or after max_iter
-// corrections. One work-group of 1024 with two rows in flight per thread;
-// per-wave maxima by shuffles.
+// corrections; any NaN in r or x makes rel NaN, read as not converged. One
+// work-group of 1024 with two rows in flight per thread; per-wave maxima by
+// shuffles.
 constexpr int RC_THREADS = 1024;
+
+// max that keeps NaN (fmax drops it): a NaN residual or solution must read as
+// not converged, never as a zero residual
+__device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
 __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int nr, const double* Rr,
                                                              const double* Ri, const double* Xr, const double* Xi,
@@ -1471,29 +1483,29 @@ __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int 
             size_t o = (size_t)j * Np + r, o2 = o + RC_THREADS;
             double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
             double rr2 = Rr[o2], ri2 = Ri[o2], xr2 = Xr[o2], xi2 = Xi[o2];
-            mr = fmax(mr, sqrt(rr * rr + ri * ri));
-            mx = fmax(mx, sqrt(xr * xr + xi * xi));
-            mri = fmax(mri, fabs(ri));
-            mxi = fmax(mxi, fabs(xi));
-            mr2 = fmax(mr2, sqrt(rr2 * rr2 + ri2 * ri2));
-            mx2 = fmax(mx2, sqrt(xr2 * xr2 + xi2 * xi2));
-            mri2 = fmax(mri2, fabs(ri2));
-            mxi2 = fmax(mxi2, fabs(xi2));
+            mr = nmax(mr, sqrt(rr * rr + ri * ri));
+            mx = nmax(mx, sqrt(xr * xr + xi * xi));
+            mri = nmax(mri, fabs(ri));
+            mxi = nmax(mxi, fabs(xi));
+            mr2 = nmax(mr2, sqrt(rr2 * rr2 + ri2 * ri2));
+            mx2 = nmax(mx2, sqrt(xr2 * xr2 + xi2 * xi2));
+            mri2 = nmax(mri2, fabs(ri2));
+            mxi2 = nmax(mxi2, fabs(xi2));
         }
         if (r < N) {
             size_t o = (size_t)j * Np + r;
             double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
-            mr = fmax(mr, sqrt(rr * rr + ri * ri));
-            mx = fmax(mx, sqrt(xr * xr + xi * xi));
-            mri = fmax(mri, fabs(ri));
-            mxi = fmax(mxi, fabs(xi));
+            mr = nmax(mr, sqrt(rr * rr + ri * ri));
+            mx = nmax(mx, sqrt(xr * xr + xi * xi));
+            mri = nmax(mri, fabs(ri));
+            mxi = nmax(mxi, fabs(xi));
         }
-        mr = fmax(mr, mr2); mx = fmax(mx, mx2); mri = fmax(mri, mri2); mxi = fmax(mxi, mxi2);
+        mr = nmax(mr, mr2); mx = nmax(mx, mx2); mri = nmax(mri, mri2); mxi = nmax(mxi, mxi2);
         for (int off = 32; off > 0; off >>= 1) {
-            mr = fmax(mr, __shfl_xor(mr, off));
-            mx = fmax(mx, __shfl_xor(mx, off));
-            mri = fmax(mri, __shfl_xor(mri, off));
-            mxi = fmax(mxi, __shfl_xor(mxi, off));
+            mr = nmax(mr, __shfl_xor(mr, off));
+            mx = nmax(mx, __shfl_xor(mx, off));
+            mri = nmax(mri, __shfl_xor(mri, off));
+            mxi = nmax(mxi, __shfl_xor(mxi, off));
         }
         if (lane == 0) { sm[wave][0][j] = mr; sm[wave][1][j] = mx; sm[wave][2][j] = mri; sm[wave][3][j] = mxi; }
     }
@@ -1503,13 +1515,14 @@ __global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int 
         for (int j = 0; j < nr; j++) {
             double a = 0, b = 0, c = 0, d = 0;
             for (int w = 0; w < NW; w++) {
-                a = fmax(a, sm[w][0][j]); b = fmax(b, sm[w][1][j]);
-                c = fmax(c, sm[w][2][j]); d = fmax(d, sm[w][3][j]);
+                a = nmax(a, sm[w][0][j]); b = nmax(b, sm[w][1][j]);
+                c = nmax(c, sm[w][2][j]); d = nmax(d, sm[w][3][j]);
             }
             double den = an * b, deni = an * fmax(d, 0.01 * b);
             double v = den > 0 ? a / den : (a > 0 ? 1e300 : 0.0);
             double vi = deni > 0 ? c / deni : (c > 0 ? 1e300 : 0.0);
-            rel = fmax(rel, fmax(v, vi));
+            const bool finite = a == a && b == b && c == c && d == d && an == an;
+            rel = finite ? nmax(rel, nmax(v, vi)) : 1e300;
         }
         if (!(rel == rel)) rel = 1e300;
         ctrl->prev = ctrl->rel;

@@ -31,7 +31,9 @@ enum {
     BEM_ERR_ARG = 1,      /* invalid argument */
     BEM_ERR_HIP = 2,      /* HIP runtime error (no device, launch failure, ...) */
     BEM_ERR_NOMEM = 3,    /* device allocation failed */
-    BEM_ERR_NUMERIC = 4,  /* zero pivot or non-finite result */
+    BEM_ERR_NUMERIC = 4,  /* zero or non-finite pivot, or a non-finite result: the outputs are
+                             still written (NaN where the system is singular), and a sweep or
+                             batch still solves its other frequencies */
     BEM_ERR_STATE = 5     /* call out of order (e.g. solve before Rankine assembly) */
 };
 
@@ -169,6 +171,14 @@ void bem_destroy(bem_ctx* ctx);
  * solves are host loops). */
 int bem_set_graphs(bem_ctx* ctx, int enable);
 
+/* Iterative refinement of the mixed modes: at most max_steps (0..4, default
+ * 4) correction steps per frequency. A frequency whose refinement has not
+ * reached the residual target by then (or stagnates earlier) is re-solved
+ * with the FP64 LU and reports fell_back_fp64 = 1; max_steps = 0 therefore
+ * sends every mixed frequency to the FP64 path (which is how the tests reach
+ * that path). No counterpart in the reference (it has no refinement). */
+int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
+
 /* Measured machine peaks of the context's device, for reporting beside the
  * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
  * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +
@@ -205,6 +215,8 @@ int bem_body_assemble_rankine(bem_ctx* ctx, bem_body* b);
  *   betas   nb wave headings [rad]
  * Outputs: A36, B36 row-major 6x6; F, FK, SC row-major 6 x nb complex (may be
  * NULL); phiJ N x 6 and phiS N x nb complex column-major (may be NULL).
+ * BEM_ERR_NUMERIC when the system is numerically singular or a result is not
+ * finite (A, B, F are written; the potentials are not).
  * BEM_ERR_ARG for a non-finite omega or kappa, or kappa < 0 (omega itself may be
  * negative: the reference's set-up solve runs at Frequency = -1); the same
  * check applies to every frequency of the sweep, batch and post-processing
@@ -264,8 +276,10 @@ int bem_struve_coefficients(double* out, int n);
  * with Y0 = Y1 = 0 for x <= 3 (HavelockGF uses them only above 3). */
 int bem_bessel01(int n, const double* x, double* out);
 /* Device LU + solve of a caller matrix (complex n x n, column-major):
- * precision 64 (FP64 LU) or 32 (FP32-MFMA LU + FP64 refinement); X receives
- * the solution of A X = B for nrhs columns. */
+ * precision 64 (FP64 LU) or 32 (FP32-MFMA LU + FP64 refinement, FP64 LU if
+ * the refinement stagnates); X receives the solution of A X = B for nrhs
+ * columns. BEM_ERR_ARG for non-finite entries of A or B; BEM_ERR_NUMERIC (X
+ * written) for a numerically singular A or a non-finite solution. */
 int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
                     double* X, bem_solve_info* info);
 
