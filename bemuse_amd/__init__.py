@@ -48,6 +48,7 @@ class BemError(RuntimeError):
 
 
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
+NOALLOC_MAX_N = 512     # BEM_NOALLOC_MAX_N
 
 
 class SolveInfo(ctypes.Structure):
@@ -153,6 +154,7 @@ _SIGS = {
     "bem_solve_batch": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p), _PI, _PD, c_int, c_int, _PD, c_int, _PD,
                                 _PD, _PD, c_void_p]),
     "bem_dense_solve": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
+    "bem_dense_solve_noalloc": (c_int, [c_void_p, c_int, c_int, _PD, c_int, _PD, _PD, POINTER(SolveInfo)]),
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
     "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
@@ -491,7 +493,10 @@ class Context:
         _check(lib().bem_surrogate_eval(self._h, len(h), _dp(h), _dp(v), _dp(out), _ip(used)))
         return out, used
 
-    def dense_solve(self, A, B, precision=64):
+    def dense_solve(self, A, B, precision=64, noalloc=False):
+        """X with A X = B on the device (precision 64: FP64 LU; 32: FP32-MFMA
+        LU + FP64 refinement). noalloc: bem_dense_solve_noalloc (no device
+        allocation, n <= NOALLOC_MAX_N)."""
         A = np.asfortranarray(A, dtype=np.complex128)
         B = np.asfortranarray(B, dtype=np.complex128)
         if B.ndim == 1:
@@ -499,8 +504,9 @@ class Context:
         n, nrhs = B.shape
         X = np.zeros((n, nrhs), dtype=np.complex128, order="F")
         info = SolveInfo()
-        _check(lib().bem_dense_solve(self._h, precision, n, A.ctypes.data_as(_PD), nrhs, B.ctypes.data_as(_PD),
-                                     X.ctypes.data_as(_PD), ctypes.byref(info)))
+        fn = lib().bem_dense_solve_noalloc if noalloc else lib().bem_dense_solve
+        _check(fn(self._h, precision, n, A.ctypes.data_as(_PD), nrhs, B.ctypes.data_as(_PD), X.ctypes.data_as(_PD),
+                  ctypes.byref(info)))
         return X, info.as_dict()
 
     def solve_batch(self, bodies, omegas, betas=(), mode=1, lanes=4):

@@ -17,6 +17,7 @@
 #include <cctype>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,9 +91,19 @@ struct SurState {
 constexpr int kRefineMaxIter = 4;
 constexpr double kRefineTol = 1e-13;
 
+// Largest order of bem_dense_solve_noalloc and its padded order.
+constexpr int kNaMaxN = BEM_NOALLOC_MAX_N;
+constexpr int kNaNp = (kNaMaxN + 127) / 128 * 128;
+constexpr int kNaEvents = 2 * (kNaNp / 64) + 2;    // LU lookahead events at that order
+
 struct bem_ctx {
     int device = 0;
     int refine_max = kRefineMaxIter;   // correction steps allowed (bem_set_refine_limit)
+    // streams and events of bem_dense_solve_noalloc, made here so that the
+    // entry point itself creates nothing
+    hipStream_t na_s2 = nullptr;
+    hipEvent_t na_ev[kNaEvents] = {};
+    hipEvent_t na_t[3] = {};
     SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
     bool graphs = false;               // replay captured frequency graphs on the lanes (bem_set_graphs)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
@@ -292,49 +303,74 @@ struct bem_body {
 
 namespace {
 
-// Device LU in the given precision (FP64 in place, or FP32 on the FP32 copy).
-void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) {
-    LinSys& s = L.sys;
-    if (precision == 64)
-        launch_lu<double>(L.s1, L.s2, L.lu_events.data(), s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof);
-    else
-        launch_lu<float>(L.s1, L.s2, L.lu_events.data(), s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof);
+// What a factorisation and solve run on: the streams (main, LU lookahead),
+// the lookahead events (2 per panel + 2), the system workspace, the
+// triangular-solve ticket counter and publication words, the refinement
+// state and its step limit. A solve lane provides one (Lane::view); the
+// no-allocation dense solve builds one over its fixed device array.
+struct SolveView {
+    hipStream_t s1 = nullptr, s2 = nullptr;
+    hipEvent_t* ev = nullptr;
+    LinSys* sys = nullptr;
+    unsigned long long* counter = nullptr;
+    unsigned long long* pub = nullptr;
+    RefineCtrl* ctrl = nullptr;
+    int refine_max = kRefineMaxIter;
+};
+
+SolveView lane_view(Lane& L) {
+    SolveView v;
+    v.s1 = L.s1; v.s2 = L.s2; v.ev = L.lu_events.data(); v.sys = &L.sys;
+    v.counter = L.counter; v.pub = L.pub; v.ctrl = L.ctrl; v.refine_max = L.refine_max;
+    return v;
 }
+
+// Device LU in the given precision (FP64 in place, or FP32 on the FP32 copy).
+void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr) {
+    LinSys& s = *V.sys;
+    if (precision == 64)
+        launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof);
+    else
+        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof);
+}
+void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
 
 // Solve D X = B for nr columns (B in the workspace, X out). FP32 mode:
 // FP32 LU solve followed by FP64 iterative refinement against the FP64
 // matrix s.Dr/s.Di, controlled on the device (no host round trip), with at
-// most L.refine_max correction steps.
-void solve_system(Lane& L, int precision, int nr) {
-    LinSys& s = L.sys;
+// most V.refine_max correction steps.
+void solve_system(const SolveView& V, int precision, int nr) {
+    LinSys& s = *V.sys;
+    hipStream_t st = V.s1;
     size_t n = (size_t)s.Np * nr;
     if (precision == 64) {
-        launch_copy2(L.s1, n, s.Br, s.Bi, s.Xr, s.Xi);
-        launch_lu_solve<double, double>(L.s1, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
-                                        L.counter, L.pub);
+        launch_copy2(st, n, s.Br, s.Bi, s.Xr, s.Xi);
+        launch_lu_solve<double, double>(st, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
+                                        V.counter, V.pub);
         return;
     }
     const int* fany = s.flag32 + s.Np / kLuNb;
-    const int* skip = &L.ctrl->done;
-    launch_refine_reset(L.s1, L.ctrl);
-    launch_to_f32(L.s1, n, s.Br, s.Bi, s.X32r, s.X32i);
-    launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, L.counter,
-                                  L.pub);
-    launch_from_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i);
-    const int max_iter = std::max(0, std::min(L.refine_max, kRefineMaxIter));
+    const int* skip = &V.ctrl->done;
+    launch_refine_reset(st, V.ctrl);
+    launch_to_f32(st, n, s.Br, s.Bi, s.X32r, s.X32i);
+    launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, V.counter,
+                                  V.pub);
+    launch_from_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i);
+    const int max_iter = std::max(0, std::min(V.refine_max, kRefineMaxIter));
     for (int it = 0; it <= max_iter; it++) {
         // r = b - A x in FP64; the first residual also yields |A|_inf
-        launch_matvec<double>(L.s1, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
+        launch_matvec<double>(st, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
                               it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
-        launch_refine_check(L.s1, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, L.ctrl, kRefineTol, max_iter);
+        launch_refine_check(st, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, V.ctrl, kRefineTol, max_iter);
         if (it == max_iter) break;
         // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
-        launch_to_f32(L.s1, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
-        launch_lu_solve<float, float>(L.s1, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                      L.counter, L.pub, skip);
-        launch_add_f32(L.s1, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
+        launch_to_f32(st, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
+        launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
+                                      V.counter, V.pub, skip);
+        launch_add_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
     }
 }
+void solve_system(Lane& L, int precision, int nr) { solve_system(lane_view(L), precision, nr); }
 
 void planar_to_interleaved(const std::vector<double>& re, const std::vector<double>& im, int N, int Np, int ncol,
                            double* out) {
@@ -546,6 +582,101 @@ int bem_geometry(const bem_mesh* m, double* body, double* refl, double* quad, do
 }
 
 // ---- context -----------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// No-allocation dense solve: every device buffer of bem_dense_solve_noalloc
+// lives in one fixed array per GPU, `scratch`. Host code never uses the
+// array's name as an address: at set-up (bem_create, once per device, with
+// that device current) hipGetSymbolAddress yields the device's own address of
+// it, kept per device in g_na_ptr, and every kernel argument and copy goes
+// through that pointer.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNaRhs = kMaxRhs;
+constexpr int kNaKslabs = 4;                       // matvec_kslabs(Np) for Np < 1024
+constexpr size_t kNaAlign = 256;
+
+// Byte offsets of the buffers in the array, laid out for the largest order
+// (smaller systems use the same places with their own leading dimension).
+struct NaLayout {
+    size_t Dr, Di, D32r, D32i, ipiv, flag, ipiv32, flag32;
+    size_t w64[12], w32[12];
+    size_t Br, Bi, Xr, Xi, Rr, Ri, X32r, X32i, Pr, Pi, Pabs, anorm;
+    size_t counter, pub, ctrl, stageA, stageB;
+    size_t total;
+};
+
+constexpr NaLayout na_layout() {
+    NaLayout L{};
+    size_t o = 0;
+    auto take = [&o](size_t bytes) {
+        size_t at = o;
+        o = (o + bytes + kNaAlign - 1) / kNaAlign * kNaAlign;
+        return at;
+    };
+    const size_t NN = (size_t)kNaNp * kNaNp, NR = (size_t)kNaNp * kNaRhs, pan = (size_t)kNaNp * kLuNb;
+    const size_t nflag = kNaNp / kLuNb + 1;
+    L.Dr = take(NN * 8); L.Di = take(NN * 8);
+    L.D32r = take(NN * 4); L.D32i = take(NN * 4);
+    L.ipiv = take(kNaNp * 4); L.flag = take(nflag * 4);
+    L.ipiv32 = take(kNaNp * 4); L.flag32 = take(nflag * 4);
+    // LuWork: sd_r, sd_i (NB x NB), then s21, Linv, Uinv, PL, PU (Np x NB) re and im
+    L.w64[0] = take(kLuNb * kLuNb * 8); L.w64[1] = take(kLuNb * kLuNb * 8);
+    for (int k = 2; k < 12; k++) L.w64[k] = take(pan * 8);
+    L.w32[0] = take(kLuNb * kLuNb * 4); L.w32[1] = take(kLuNb * kLuNb * 4);
+    for (int k = 2; k < 12; k++) L.w32[k] = take(pan * 4);
+    L.Br = take(NR * 8); L.Bi = take(NR * 8); L.Xr = take(NR * 8); L.Xi = take(NR * 8);
+    L.Rr = take(NR * 8); L.Ri = take(NR * 8);
+    L.X32r = take(NR * 4); L.X32i = take(NR * 4);
+    L.Pr = take(kNaKslabs * NR * 8); L.Pi = take(kNaKslabs * NR * 8); L.Pabs = take((size_t)kNaKslabs * kNaNp * 8);
+    L.anorm = take(8);
+    L.counter = take(8);
+    L.pub = take((size_t)(kNaNp / kLuNb) * kNaRhs * kLuNb * 2 * 2 * 8);   // lu_solve_pub_words(kNaNp) words
+    L.ctrl = take(sizeof(RefineCtrl));
+    L.stageA = take((size_t)2 * kNaMaxN * kNaMaxN * 8);                    // interleaved A as the caller passes it
+    L.stageB = take((size_t)2 * kNaMaxN * kNaRhs * 8);                     // interleaved B in, X out
+    L.total = o;
+    return L;
+}
+
+constexpr size_t kNaBytes = BEM_NOALLOC_SCRATCH_BYTES;
+static_assert(na_layout().total <= kNaBytes, "the no-allocation layout must fit its array");
+
+constexpr int kNaMaxDev = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDev] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+// Once per device, with the device current: the array's address on this
+// device, and the triangular-solve state a new solve lane starts from.
+int noalloc_setup(bem_ctx* c) {
+    if (c->device < 0 || c->device >= kNaMaxDev) return fail(BEM_ERR_ARG, "bem_create: device index too large");
+    HIP_TRY(hipStreamCreateWithFlags(&c->na_s2, hipStreamNonBlocking));
+    for (auto& e : c->na_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& e : c->na_t) HIP_TRY(hipEventCreate(&e));
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (g_na_ptr[c->device]) return BEM_OK;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(BEM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    constexpr NaLayout L = na_layout();
+    char* base = static_cast<char*>(p);
+    launch_solve_state_init(c->stream, reinterpret_cast<unsigned long long*>(base + L.counter),
+                            reinterpret_cast<unsigned long long*>(base + L.pub), lu_solve_pub_words(kNaNp));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    g_na_ptr[c->device] = p;
+    return BEM_OK;
+}
+
+}  // namespace
+
 int bem_create(int device, bem_ctx** out) {
     if (!out) return fail(BEM_ERR_ARG, "null output");
     int n = 0;
@@ -559,6 +690,8 @@ int bem_create(int device, bem_ctx** out) {
         return fail(BEM_ERR_HIP, "hipStreamCreate failed");
     }
     for (auto& e : c->ev) (void)hipEventCreate(&e);
+    int r = noalloc_setup(c);
+    if (r) { bem_destroy(c); return r; }
     *out = c;
     return BEM_OK;
 }
@@ -571,8 +704,14 @@ void bem_destroy(bem_ctx* c) {
     delete c->surrogate;
     c->surrogate = nullptr;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->na_s2) (void)hipStreamSynchronize(c->na_s2);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->na_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->na_t)
+        if (e) (void)hipEventDestroy(e);
+    if (c->na_s2) (void)hipStreamDestroy(c->na_s2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1645,6 +1784,97 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     if (flags[Np / kLuNb] & 2) return fail(BEM_ERR_NUMERIC, "bem_dense_solve: numerically singular matrix (zero or non-finite pivot)");
     for (size_t q = 0; q < 2 * (size_t)n * nrhs; q++)
         if (!std::isfinite(X[q])) return fail(BEM_ERR_NUMERIC, "bem_dense_solve: non-finite solution");
+    return BEM_OK;
+}
+
+int bem_dense_solve_noalloc(bem_ctx* c, int precision, int n, const double* A, int nrhs, const double* B,
+                            double* X, bem_solve_info* info) {
+    if (!c || n <= 0 || !A || !B || !X || nrhs < 1 || nrhs > kMaxRhs || (precision != 64 && precision != 32))
+        return fail(BEM_ERR_ARG, "bem_dense_solve_noalloc: bad arguments (nrhs in [1,16], precision 64 or 32)");
+    if (n > kNaMaxN)
+        return fail(BEM_ERR_ARG, "bem_dense_solve_noalloc: n = " + std::to_string(n) + " exceeds the maximum " +
+                                     std::to_string(kNaMaxN) + " of its fixed device array (bem_dense_solve has no limit)");
+    for (size_t q = 0; q < 2 * (size_t)n * n; q++)
+        if (!std::isfinite(A[q])) return fail(BEM_ERR_ARG, "bem_dense_solve_noalloc: A has a non-finite entry");
+    for (size_t q = 0; q < 2 * (size_t)n * nrhs; q++)
+        if (!std::isfinite(B[q])) return fail(BEM_ERR_ARG, "bem_dense_solve_noalloc: B has a non-finite entry");
+    HIP_TRY(hipSetDevice(c->device));
+    char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_na_mu);
+        base = static_cast<char*>(g_na_ptr[c->device]);
+    }
+    if (!base) return fail(BEM_ERR_STATE, "bem_dense_solve_noalloc: device array not set up");
+    const int Np = round_up(n, kPad);
+    if (matvec_kslabs(Np) > kNaKslabs) return fail(BEM_ERR_ARG, "bem_dense_solve_noalloc: order too large");
+    constexpr NaLayout L = na_layout();
+    auto at = [base](size_t off) { return static_cast<void*>(base + off); };
+    LinSys s;    // a view: owns nothing, frees nothing
+    s.N = n; s.Np = Np; s.capNp = kNaNp; s.have32 = true;
+    s.Dr = (double*)at(L.Dr); s.Di = (double*)at(L.Di);
+    s.D32r = (float*)at(L.D32r); s.D32i = (float*)at(L.D32i);
+    s.ipiv = (int*)at(L.ipiv); s.flag = (int*)at(L.flag); s.ipiv32 = (int*)at(L.ipiv32); s.flag32 = (int*)at(L.flag32);
+    double** w64[12] = {&s.w64.sd_r, &s.w64.sd_i, &s.w64.s21_r, &s.w64.s21_i, &s.w64.Linv_r, &s.w64.Linv_i,
+                        &s.w64.Uinv_r, &s.w64.Uinv_i, &s.w64.PL_r, &s.w64.PL_i, &s.w64.PU_r, &s.w64.PU_i};
+    float** w32[12] = {&s.w32.sd_r, &s.w32.sd_i, &s.w32.s21_r, &s.w32.s21_i, &s.w32.Linv_r, &s.w32.Linv_i,
+                       &s.w32.Uinv_r, &s.w32.Uinv_i, &s.w32.PL_r, &s.w32.PL_i, &s.w32.PU_r, &s.w32.PU_i};
+    for (int k = 0; k < 12; k++) { *w64[k] = (double*)at(L.w64[k]); *w32[k] = (float*)at(L.w32[k]); }
+    s.Br = (double*)at(L.Br); s.Bi = (double*)at(L.Bi); s.Xr = (double*)at(L.Xr); s.Xi = (double*)at(L.Xi);
+    s.Rr = (double*)at(L.Rr); s.Ri = (double*)at(L.Ri);
+    s.X32r = (float*)at(L.X32r); s.X32i = (float*)at(L.X32i);
+    s.Pr = (double*)at(L.Pr); s.Pi = (double*)at(L.Pi); s.Pabs = (double*)at(L.Pabs); s.anorm = (double*)at(L.anorm);
+    double* stA = (double*)at(L.stageA);
+    double* stB = (double*)at(L.stageB);
+    SolveView V;
+    V.s1 = c->stream; V.s2 = c->na_s2; V.ev = c->na_ev; V.sys = &s;
+    V.counter = (unsigned long long*)at(L.counter); V.pub = (unsigned long long*)at(L.pub);
+    V.ctrl = (RefineCtrl*)at(L.ctrl); V.refine_max = c->refine_max;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(stA, A, 16 * (size_t)n * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(stB, B, 16 * (size_t)n * nrhs, hipMemcpyHostToDevice, st));
+    // every element the solve reads is rewritten here: the padded matrix
+    // (identity on the padding diagonal) and the padded right-hand sides
+    launch_pack_planar(st, n, n, Np, Np, stA, s.Dr, s.Di, 1);
+    launch_pack_planar(st, n, nrhs, Np, nrhs, stB, s.Br, s.Bi, 0);
+    if (precision == 32) launch_to_f32(st, (size_t)Np * Np, s.Dr, s.Di, s.D32r, s.D32i);
+    HIP_TRY(hipEventRecord(c->na_t[0], st));
+    factor_system(V, precision);
+    HIP_TRY(hipEventRecord(c->na_t[1], st));
+    solve_system(V, precision, nrhs);
+    HIP_TRY(hipEventRecord(c->na_t[2], st));
+    HIP_TRY(hipGetLastError());
+    RefineCtrl ctrl{};
+    if (precision == 32) HIP_TRY(hipMemcpyAsync(&ctrl, V.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int used = precision;
+    if (precision == 32 && !ctrl.converged) {   // the FP64 matrix is intact: mixed mode factors the copy
+        factor_system(V, 64);
+        solve_system(V, 64, nrhs);
+        HIP_TRY(hipEventRecord(c->na_t[2], st));
+        HIP_TRY(hipGetLastError());
+        used = 64;
+    }
+    int flags[kNaNp / kLuNb + 1] = {};
+    HIP_TRY(hipMemcpyAsync(flags, used == 64 ? s.flag : s.flag32, (Np / kLuNb + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, st));
+    launch_unpack_planar(st, n, nrhs, Np, s.Xr, s.Xi, stB);
+    HIP_TRY(hipMemcpyAsync(X, stB, 16 * (size_t)n * nrhs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    bem_solve_info loc{};
+    for (int i = 0; i < Np / kLuNb; i++) loc.pivot_fallback += (flags[i] & 1) ? 1 : 0;
+    loc.refine_iters = ctrl.iters;
+    loc.residual = precision == 32 ? ctrl.rel : 0.0;
+    loc.fell_back_fp64 = (precision == 32 && used == 64) ? 1 : 0;
+    float t1 = 0, t2 = 0;
+    (void)hipEventElapsedTime(&t1, c->na_t[0], c->na_t[1]);
+    (void)hipEventElapsedTime(&t2, c->na_t[1], c->na_t[2]);
+    loc.ms_factor = t1; loc.ms_solve = t2;
+    if (info) *info = loc;
+    if (flags[Np / kLuNb] & 2)
+        return fail(BEM_ERR_NUMERIC, "bem_dense_solve_noalloc: numerically singular matrix (zero or non-finite pivot)");
+    for (size_t q = 0; q < 2 * (size_t)n * nrhs; q++)
+        if (!std::isfinite(X[q])) return fail(BEM_ERR_NUMERIC, "bem_dense_solve_noalloc: non-finite solution");
     return BEM_OK;
 }
 

@@ -72,6 +72,16 @@ void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, co
 inline size_t lu_solve_pub_words(int Np) { return (size_t)(Np / kLuNb) * kMaxRhs * kLuNb * 2 * 2; }
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb,
                    const int* skip = nullptr);
+// Interleaved complex n_rows x ncol (ld n_rows) -> planar Np x ncolp (ld Np),
+// zero beyond the source except a unit diagonal when ident = 1.
+void launch_pack_planar(hipStream_t s, int n_rows, int ncol, int Np, int ncolp, const double* src, double* re,
+                        double* im, int ident);
+// Planar (ld Np) -> interleaved complex n_rows x ncol (ld n_rows).
+void launch_unpack_planar(hipStream_t s, int n_rows, int ncol, int Np, const double* re, const double* im,
+                          double* dst);
+// Triangular-solve state as a new lane starts it: counter at epoch 1,
+// nwords publication words at epoch 0.
+void launch_solve_state_init(hipStream_t s, unsigned long long* counter, unsigned long long* pub, size_t nwords);
 void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb,
                     const int* skip = nullptr);
 

@@ -1414,6 +1414,42 @@ __global__ void k_from_f32(size_t n, double* a, double* b, const float* fa, cons
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { a[i] = (double)fa[i]; b[i] = (double)fb[i]; }
 }
+// Interleaved complex n_rows x ncol (column-major, ld n_rows) to planar
+// Np x ncolp (ld Np): rows and columns beyond the source are zero, except the
+// diagonal when ident = 1 (the identity padding of a system matrix).
+__global__ void k_pack_planar(int n_rows, int ncol, int Np, int ncolp, const double* __restrict__ src,
+                              double* __restrict__ re, double* __restrict__ im, int ident) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)Np * ncolp) return;
+    int i = (int)(t % Np), j = (int)(t / Np);
+    double vr = 0.0, vi = 0.0;
+    if (i < n_rows && j < ncol) {
+        size_t o = 2 * ((size_t)j * n_rows + i);
+        vr = src[o]; vi = src[o + 1];
+    } else if (ident && i == j) {
+        vr = 1.0;
+    }
+    re[t] = vr; im[t] = vi;
+}
+
+// Planar Np-row block (ld Np) to interleaved n_rows x ncol (ld n_rows).
+__global__ void k_unpack_planar(int n_rows, int ncol, int Np, const double* __restrict__ re,
+                                const double* __restrict__ im, double* __restrict__ dst) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_rows * ncol) return;
+    int i = (int)(t % n_rows), j = (int)(t / n_rows);
+    size_t o = (size_t)j * Np + i;
+    dst[2 * t] = re[o]; dst[2 * t + 1] = im[o];
+}
+
+// Fresh triangular-solve state: ticket counter at epoch 1, publication words
+// at epoch 0 (nwords of them).
+__global__ void k_solve_state_init(unsigned long long* counter, unsigned long long* pub, size_t nwords) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nwords) pub[t] = 0ull;
+    if (t == 0) *counter = 1ull << 32;
+}
+
 __global__ void k_copy2(size_t n, const double* a, const double* b, double* ca, double* cb) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { ca[i] = a[i]; cb[i] = b[i]; }
@@ -2074,6 +2110,22 @@ void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl) {
 void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
                          const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
     hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(RC_THREADS), 0, s, N, Np, nr, Rr, Ri, Xr, Xi, anorm, ctrl, tol, max_iter);
+}
+void launch_pack_planar(hipStream_t s, int n_rows, int ncol, int Np, int ncolp, const double* src, double* re,
+                        double* im, int ident) {
+    size_t n = (size_t)Np * ncolp;
+    hipLaunchKernelGGL(k_pack_planar, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_rows, ncol, Np, ncolp, src,
+                       re, im, ident);
+}
+void launch_unpack_planar(hipStream_t s, int n_rows, int ncol, int Np, const double* re, const double* im,
+                          double* dst) {
+    size_t n = (size_t)n_rows * ncol;
+    hipLaunchKernelGGL(k_unpack_planar, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_rows, ncol, Np, re, im,
+                       dst);
+}
+void launch_solve_state_init(hipStream_t s, unsigned long long* counter, unsigned long long* pub, size_t nwords) {
+    size_t n = nwords > 0 ? nwords : 1;
+    hipLaunchKernelGGL(k_solve_state_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, counter, pub, nwords);
 }
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb) {
     hipLaunchKernelGGL(k_copy2, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, ca, cb);

@@ -283,6 +283,24 @@ int bem_bessel01(int n, const double* x, double* out);
 int bem_dense_solve(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
                     double* X, bem_solve_info* info);
 
+/* As bem_dense_solve, allocating nothing on the device. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every device buffer it needs (the padded FP64 matrix and its FP32 copy, the
+ * LU workspace, right-hand sides and solution, refinement and
+ * triangular-solve state, staging of A, B and X) lives in one fixed device
+ * array of BEM_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from
+ * the first bem_create on that GPU until the process ends; its streams and
+ * events are made by bem_create. Limits: n <= BEM_NOALLOC_MAX_N (BEM_ERR_ARG
+ * beyond), nrhs <= 16. Results, errors and info are those of bem_dense_solve.
+ * Every call on one GPU shares that GPU's array (it keeps the last call's
+ * contents; each call rewrites what it reads), so calls that overlap on one
+ * GPU, through the same or different contexts, race: serialise them. */
+#define BEM_NOALLOC_MAX_N 512
+#define BEM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int bem_dense_solve_noalloc(bem_ctx* ctx, int precision, int n, const double* A, int nrhs, const double* B,
+                            double* X, bem_solve_info* info);
+
 /* ---- post-processing of a solved frequency -------------------------------------
  * Both take the potentials bem_solve_frequency returns (phiJ N x 6, phiS
  * N x nb, complex column-major) at the same omega, kappa and headings. */
