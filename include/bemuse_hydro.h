@@ -938,6 +938,7 @@ public:
         if (Omega == 0 && IFR) Omega = 1.0e-3;
         Period = 1.0 / Frequency;
         Kappa = Omega * Omega / Gravity;
+        ExtRadMat = CMatrix();   // this frequency's exterior elevation, if any
         if (!body_) { std::cout << "BEMUse: Solve before Setup" << std::endl; set_nan(); return; }
         int compose = (Omega == 0.0) ? BEM_COMPOSE_ZERO : (Omega == FreqInf ? BEM_COMPOSE_INF : BEM_COMPOSE_WAVE);
         int rad_only = (Frequency == FreqInf) ? 1 : 0;
@@ -985,7 +986,8 @@ public:
             if (check(bem_fs_elevation(ctx_, body_, Omega, Kappa, nw, pts.data(), NBeta,
                                        NBeta ? BetaArray.data() : nullptr, phiJ.data(),
                                        NBeta ? phiS.data() : nullptr, er.data(), NBeta ? ed.data() : nullptr))) {
-                FS_Rad_SolArray.push_back(unpack_rows(er, nw, 6));
+                ExtRadMat = unpack_rows(er, nw, 6);
+                FS_Rad_SolArray.push_back(ExtRadMat);
                 FS_Scat_SolArray.push_back(unpack_rows(ed, nw, NBeta));
             }
         }
@@ -1219,6 +1221,14 @@ private:
                 for (int j = 0; j < 6; j++)
                     f << "     " << i + 1 << "     " << j + 1 << "   " << Field(M_ij(i, j) / Rho) << std::endl;
         }
+        // exterior wave-elevation points (:486-497): index, x, y. As there, the
+        // list of files cleared above has ".6", not the ".6r" that
+        // Update_Output_File appends to.
+        if (!Wave_Nodes.empty()) {
+            std::ofstream f(OutputPath + "_WaveElPos.bem");
+            for (size_t i = 0; i < Wave_Nodes.size(); i++)
+                f << i << "     " << Field(Wave_Nodes[i].x) << Field(Wave_Nodes[i].y) << std::endl;
+        }
     }
 
     // Hydrodynamic_Solver_IO.cpp:502-581 (radiation and Haskind force files)
@@ -1271,6 +1281,17 @@ private:
                       << Field(std::real(KochinDiff(i, d))) << Field(std::imag(KochinDiff(i, d))) << std::endl;
                 }
         }
+        // radiated wave elevation at the exterior points (:624-643): frequency,
+        // point index, |eta_j| and arg eta_j for the six modes
+        if (!Wave_Nodes.empty() && Omega != 0.0 && Frequency != FreqInf &&
+            ExtRadMat.rows() == (int)Wave_Nodes.size()) {
+            std::ofstream f(OutputPath + ".6r", std::ios_base::app);
+            for (int i = 0; i < ExtRadMat.rows(); i++) {
+                f << Field(Frequency) << i << "     ";
+                for (int d = 0; d < 6; d++) f << Field(std::abs(ExtRadMat(i, d))) << Field(std::arg(ExtRadMat(i, d)));
+                f << std::endl;
+            }
+        }
     }
 
     bool check(int rc) {
@@ -1312,6 +1333,7 @@ private:
     CMatrix F_i, FK_i, SC_i;
     CMatrix KochinRad, KochinDiff;
     std::vector<Vector3> Wave_Nodes;
+    CMatrix ExtRadMat;                  // radiated elevation at Wave_Nodes of the last Solve (npts x 6)
     bem_solve_info Info = bem_solve_info();
 };
 

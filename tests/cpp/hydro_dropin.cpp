@@ -6,6 +6,7 @@
 //   SETUP A <36>        A after Setup (Frequency = FreqInf)
 //   A <omega> <36>      B <omega> <36>      F <omega> <12 nb>
 // Usage: hydro_dropin <mode: 0 FP64 | 1 mixed> [ifr: 0 | 1] [body: cyl | oc4]
+// The plain cylinder (ifr = 0) has five exterior wave-elevation points.
 // With ifr = 1 the body gets interior lids (NPanels_FreeSurface_Int_Radial)
 // and the solver the "IFR" key; the cylinder's frequencies are then 1.0 and
 // 2.2 (its irregular frequency). body oc4 is the OC4 semi-submersible
@@ -41,6 +42,12 @@ int main(int argc, char** argv) {
     if (ifr) bp.push_back(Parameter("NPanels_FreeSurface_Int_Radial", oc4 ? 2 : 3));
     B->Set_Parameters(bp);
     B->Setup();
+    // the plain cylinder also gets exterior wave-elevation points (the
+    // reference takes them from numerical-wave-tank boundaries): .6r and
+    // _WaveElPos.bem are written for them
+    if (!oc4 && !ifr)
+        B->FreeSurface_Nodes = {Vector3(8.0, 0.0, 0.0), Vector3(0.0, 12.0, 0.0), Vector3(-15.0, 3.0, 0.0),
+                                Vector3(10.0, -10.0, 0.0), Vector3(30.0, 20.0, 0.0)};
     Hydrodynamic_Radiation_Solver* S = new Hydrodynamic_Radiation_Solver(0, mode);
     std::vector<Parameter> sp = {Parameter("WaveAngle", 0.0), Parameter("WaveAngle", 0.7853981633974483)};
     if (ifr) sp.push_back(Parameter("IFR", true));

@@ -14,6 +14,7 @@ import pytest
 
 import bemuse_amd as bm
 import oracle
+import oracle.postproc as opp
 
 pytestmark = pytest.mark.gpu
 
@@ -341,13 +342,57 @@ def test_cpp_dropin_console_flow(cyl, mode, tol, tmp_path):
             Fref = refs[w]["F"][i, b]
             mag = abs(Fref) / 9.81
             assert abs(float(r[3]) - mag) <= 1e-5 * np.abs(refs[w]["F"]).max() / 9.81 + 1e-12
-    for ext in (".bem", ".2fk", ".2sc", ".hst", ".mmx"):
-        assert (out / ("dropin_test" + ext)).exists()
+    for ext in (".bem", ".2fk", ".2sc", ".hst", ".mmx", ".6r", "_WaveElPos.bem"):
+        assert (out / ("dropin_test" + ext)).exists(), ext
+    # .2fk / .2sc (Hydrodynamic_Solver_IO.cpp:527-570): "F beta dof |X|/g s arg X Re X/g s Im X/g" with
+    # s = +1 for Froude-Krylov and -1 for scattering, against the oracle's FK and SC
+    for ext, key, sg in ((".2fk", "FK", 1.0), (".2sc", "SC", -1.0)):
+        rows = [l.split() for l in (out / ("dropin_test" + ext)).read_text().splitlines() if l.strip()]
+        assert len(rows) == 2 * 2 * 6, ext
+        for k, w in enumerate((0.4, 1.3)):
+            ref = refs[w][key]
+            scale = np.abs(ref).max() / 9.81
+            for r in rows[12 * k: 12 * (k + 1)]:
+                assert abs(float(r[0]) - 2 * np.pi * w) < 1e-5 * 2 * np.pi * w
+                b = int(np.argmin(np.abs(np.array([0.0, 45.0]) - float(r[1]))))
+                X = ref[int(r[2]) - 1, b] / 9.81
+                assert abs(float(r[3]) - abs(X)) < 1e-5 * scale, (ext, r)
+                assert abs(float(r[5]) - X.real) < 1e-5 * scale and abs(float(r[6]) - sg * X.imag) < 1e-5 * scale
+    # .hst (C / rho / g) and .mmx (M / rho) of the 16-gon prism r = 5 m, draft 8 m: waterplane area
+    # 8 sin(pi/8) 25 m^2, volume area x draft, mass rho V on the diagonal, no waterplane first moments
+    awp = 8 * np.sin(np.pi / 8) * 25.0
+    hst = {(int(r[0]), int(r[1])): float(r[2]) for r in
+           (l.split() for l in (out / "dropin_test.hst").read_text().splitlines() if l.strip())}
+    assert len(hst) == 36 and abs(hst[3, 3] / awp - 1) < 1e-5
+    assert abs(hst[3, 4]) < 1e-9 * awp and abs(hst[3, 5]) < 1e-9 * awp and hst[6, 6] == 0.0
+    mmx = (out / "dropin_test.mmx").read_text().splitlines()
+    vol = float(next(l for l in mmx if l.startswith("Volume")).split()[1])
+    assert abs(vol / (awp * 8.0) - 1) < 1e-5
+    mrows = {(int(t[0]), int(t[1])): float(t[2]) for t in (l.split() for l in mmx) if len(t) == 3 and t[0].isdigit()}
+    assert len(mrows) == 36 and all(abs(mrows[i, i] / vol - 1) < 1e-5 for i in (1, 2, 3))
+    # exterior wave-elevation points (_WaveElPos.bem, :486-497) and the radiated elevation there
+    # (.6r, :624-643: "F index |eta_1| arg eta_1 ... |eta_6| arg eta_6") against the oracle
+    pts = np.array([[8.0, 0, 0], [0, 12.0, 0], [-15.0, 3.0, 0], [10.0, -10.0, 0], [30.0, 20.0, 0]])
+    pos = [l.split() for l in (out / "dropin_test_WaveElPos.bem").read_text().splitlines() if l.strip()]
+    assert [int(r[0]) for r in pos] == list(range(5))
+    assert np.abs(np.array([[float(r[1]), float(r[2])] for r in pos]) - pts[:, :2]).max() < 1e-9
+    rows6 = [l.split() for l in (out / "dropin_test.6r").read_text().splitlines() if l.strip()]
+    assert len(rows6) == 2 * 5
+    for k, w in enumerate((0.4, 1.3)):
+        K = w * w / 9.81
+        r = ob.solve(w, kappa=K, betas=[0.0, np.pi / 4], want_phi=True)
+        eta, _ = opp.fs_elevation(ob, w, K, pts, [0.0, np.pi / 4], r["phiJ"], r["phiS"])
+        scale = np.abs(eta).max()
+        for row in rows6[5 * k: 5 * (k + 1)]:
+            assert abs(float(row[0]) - 2 * np.pi * w) < 1e-5 * 2 * np.pi * w
+            i = int(row[1])
+            v = np.array(row[2:], float).reshape(6, 2)
+            got = v[:, 0] * np.exp(1j * v[:, 1])
+            assert np.abs(got - eta[i]).max() < 2e-5 * scale, (w, i)
     bem = (out / "dropin_test.bem").read_text()
     assert bem.count("Frequency:") == 3 and "Excitation Forces:" in bem
     # Kochin files (Hydrodynamic_Solver_IO.cpp:589-621), NKochin = 36 by default:
     # "F dof theta |H| arg Re Im" and "F beta theta |H| arg Re Im"
-    import oracle.postproc as opp
     rowsk = [l.split() for l in (out / "dropin_test_KochinRad.bem").read_text().splitlines() if l.strip()]
     rowsd = [l.split() for l in (out / "dropin_test_KochinDiff.bem").read_text().splitlines() if l.strip()]
     assert len(rowsk) == 2 * 36 * 6 and len(rowsd) == 2 * 2 * 36
