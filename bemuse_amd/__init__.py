@@ -49,6 +49,7 @@ class BemError(RuntimeError):
 
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
 NOALLOC_MAX_N = 512     # BEM_NOALLOC_MAX_N
+LU_FP32, LU_SPLIT_F16 = 0, 1
 
 
 class SolveInfo(ctypes.Structure):
@@ -158,6 +159,7 @@ _SIGS = {
     "bem_profile_lu": (c_int, [c_void_p, c_void_p, c_int, c_double, _PD]),
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
     "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
+    "bem_set_lu_arith": (c_int, [c_void_p, c_int]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
     "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
@@ -434,6 +436,10 @@ class Context:
         """Correction steps allowed to the mixed modes' refinement (0..4, default 4);
         0 sends every mixed frequency to the FP64 re-solve."""
         _check(lib().bem_set_refine_limit(self._h, int(max_steps)))
+
+    def set_lu_arith(self, arith):
+        """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default) or LU_FP32."""
+        _check(lib().bem_set_lu_arith(self._h, int(arith)))
 
     def set_graphs(self, enable):
         """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""

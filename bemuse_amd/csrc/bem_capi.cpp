@@ -99,6 +99,7 @@ constexpr int kNaEvents = 2 * (kNaNp / 64) + 2;    // LU lookahead events at tha
 struct bem_ctx {
     int device = 0;
     int refine_max = kRefineMaxIter;   // correction steps allowed (bem_set_refine_limit)
+    int lu_arith = kLuArithSplitF16;   // arithmetic of the FP32 factorisation's updates (bem_set_lu_arith)
     // streams and events of bem_dense_solve_noalloc, made here so that the
     // entry point itself creates nothing
     hipStream_t na_s2 = nullptr;
@@ -151,6 +152,7 @@ struct LinSys {
             (r = get(&w.Uinv_r, pan)) || (r = get(&w.Uinv_i, pan)) || (r = get(&w.PL_r, pan)) ||
             (r = get(&w.PL_i, pan)) || (r = get(&w.PU_r, pan)) || (r = get(&w.PU_i, pan)))
             return r;
+        if (sizeof(T) == 4 && (r = get(&w.umax, (size_t)capNp / kLuNb))) return r;
         return BEM_OK;
     }
     int init(int n) {
@@ -186,10 +188,10 @@ struct LinSys {
 // surrogate use.
 struct GraphKey {
     unsigned long long body = 0;
-    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1;
+    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1, arith = -1;
     bool operator==(const GraphKey& o) const {
         return body == o.body && Np == o.Np && mode == o.mode && nb == o.nb && rad_only == o.rad_only &&
-               compose == o.compose && sur == o.sur && refine == o.refine;
+               compose == o.compose && sur == o.sur && refine == o.refine && arith == o.arith;
     }
 };
 
@@ -224,6 +226,7 @@ struct Lane {
     int pending = -1;
     int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
     int refine_max = kRefineMaxIter;
+    int lu_arith = kLuArithSplitF16;
     bool sur_used = false;
     double omega = 0, kappa = 0;
     // captured frequency graph and the configuration it was captured for
@@ -316,12 +319,13 @@ struct SolveView {
     unsigned long long* pub = nullptr;
     RefineCtrl* ctrl = nullptr;
     int refine_max = kRefineMaxIter;
+    int lu_arith = kLuArithSplitF16;
 };
 
 SolveView lane_view(Lane& L) {
     SolveView v;
     v.s1 = L.s1; v.s2 = L.s2; v.ev = L.lu_events.data(); v.sys = &L.sys;
-    v.counter = L.counter; v.pub = L.pub; v.ctrl = L.ctrl; v.refine_max = L.refine_max;
+    v.counter = L.counter; v.pub = L.pub; v.ctrl = L.ctrl; v.refine_max = L.refine_max; v.lu_arith = L.lu_arith;
     return v;
 }
 
@@ -331,7 +335,7 @@ void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr)
     if (precision == 64)
         launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof);
     else
-        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof);
+        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith);
 }
 void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
 
@@ -600,7 +604,7 @@ constexpr size_t kNaAlign = 256;
 // (smaller systems use the same places with their own leading dimension).
 struct NaLayout {
     size_t Dr, Di, D32r, D32i, ipiv, flag, ipiv32, flag32;
-    size_t w64[12], w32[12];
+    size_t w64[12], w32[12], umax;
     size_t Br, Bi, Xr, Xi, Rr, Ri, X32r, X32i, Pr, Pi, Pabs, anorm;
     size_t counter, pub, ctrl, stageA, stageB;
     size_t total;
@@ -625,6 +629,7 @@ constexpr NaLayout na_layout() {
     for (int k = 2; k < 12; k++) L.w64[k] = take(pan * 8);
     L.w32[0] = take(kLuNb * kLuNb * 4); L.w32[1] = take(kLuNb * kLuNb * 4);
     for (int k = 2; k < 12; k++) L.w32[k] = take(pan * 4);
+    L.umax = take((size_t)(kNaNp / kLuNb) * 4);
     L.Br = take(NR * 8); L.Bi = take(NR * 8); L.Xr = take(NR * 8); L.Xi = take(NR * 8);
     L.Rr = take(NR * 8); L.Ri = take(NR * 8);
     L.X32r = take(NR * 4); L.X32i = take(NR * 4);
@@ -811,7 +816,7 @@ int ensure_lanes(bem_body* b, int n, bool need32) {
     Lane& L0 = *b->lanes[0];
     if (L0.pending < 0) {
         tune_lu_gemms<double>(L0.s1, b->Np, L0.sys.Dr, L0.sys.Di);
-        if (need32) tune_lu_gemms<float>(L0.s1, b->Np, L0.sys.D32r, L0.sys.D32i);
+        if (need32) tune_lu_gemms<float>(L0.s1, b->Np, L0.sys.D32r, L0.sys.D32i, L0.sys.w32.umax);
         HIP_TRY(hipStreamSynchronize(L0.s1));
     }
     return BEM_OK;
@@ -892,10 +897,11 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     L.mode = mode; L.prec = (mode == BEM_MODE_FP64) ? 64 : 32;
     L.omega = omega; L.kappa = kappa; L.compose = compose; L.rad_only = rad_only; L.nb = nb;
     L.refine_max = b->ctx->refine_max;
+    L.lu_arith = b->ctx->lu_arith;
     L.hout->freq[0] = omega;
     L.hout->freq[1] = kappa;
     const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
-    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max};
+    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max, L.lu_arith};
     const bool use_graph = b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
     L.last_key = key;
     L.graphed = use_graph;
@@ -1101,6 +1107,13 @@ int bem_set_refine_limit(bem_ctx* c, int max_steps) {
     return BEM_OK;
 }
 
+int bem_set_lu_arith(bem_ctx* c, int arith) {
+    if (!c || (arith != BEM_LU_FP32 && arith != BEM_LU_SPLIT_F16))
+        return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32 or BEM_LU_SPLIT_F16");
+    c->lu_arith = arith == BEM_LU_FP32 ? kLuArithF32 : kLuArithSplitF16;
+    return BEM_OK;
+}
+
 int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double kappa, int compose, int rad_only,
                         int nb, const double* betas, double* A36, double* B36, double* F, double* FK, double* SC,
                         double* phiJ, double* phiS, bem_solve_info* info) {
@@ -1188,7 +1201,7 @@ int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* n
     nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
     for (int np : nps) {
         tune_lu_gemms<double>(ls[0]->s1, np, ls[0]->sys.Dr, ls[0]->sys.Di);
-        if (need32) tune_lu_gemms<float>(ls[0]->s1, np, ls[0]->sys.D32r, ls[0]->sys.D32i);
+        if (need32) tune_lu_gemms<float>(ls[0]->s1, np, ls[0]->sys.D32r, ls[0]->sys.D32i, ls[0]->sys.w32.umax);
     }
     HIP_TRY(hipStreamSynchronize(ls[0]->s1));
     // tasks, largest body first (the long factorisations start early)
@@ -1259,6 +1272,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     int r = ensure_lanes(b, 1, mode != BEM_MODE_FP64);
     if (r) return r;
     Lane& L = *b->lanes[0];
+    L.lu_arith = c->lu_arith;
     set_frequency(L, 0.0, kappa);
     assemble(b, L, mode, kappa, BEM_COMPOSE_WAVE);
     LuProfile* prof = new LuProfile;
@@ -1723,6 +1737,7 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     int r = L.init(n);
     if (r) return r;
     L.refine_max = c->refine_max;
+    L.lu_arith = c->lu_arith;
     LinSys& s = L.sys;
     int Np = s.Np;
     {
@@ -1819,6 +1834,7 @@ int bem_dense_solve_noalloc(bem_ctx* c, int precision, int n, const double* A, i
     float** w32[12] = {&s.w32.sd_r, &s.w32.sd_i, &s.w32.s21_r, &s.w32.s21_i, &s.w32.Linv_r, &s.w32.Linv_i,
                        &s.w32.Uinv_r, &s.w32.Uinv_i, &s.w32.PL_r, &s.w32.PL_i, &s.w32.PU_r, &s.w32.PU_i};
     for (int k = 0; k < 12; k++) { *w64[k] = (double*)at(L.w64[k]); *w32[k] = (float*)at(L.w32[k]); }
+    s.w32.umax = (float*)at(L.umax);
     s.Br = (double*)at(L.Br); s.Bi = (double*)at(L.Bi); s.Xr = (double*)at(L.Xr); s.Xi = (double*)at(L.Xi);
     s.Rr = (double*)at(L.Rr); s.Ri = (double*)at(L.Ri);
     s.X32r = (float*)at(L.X32r); s.X32i = (float*)at(L.X32i);
@@ -1828,7 +1844,7 @@ int bem_dense_solve_noalloc(bem_ctx* c, int precision, int n, const double* A, i
     SolveView V;
     V.s1 = c->stream; V.s2 = c->na_s2; V.ev = c->na_ev; V.sys = &s;
     V.counter = (unsigned long long*)at(L.counter); V.pub = (unsigned long long*)at(L.pub);
-    V.ctrl = (RefineCtrl*)at(L.ctrl); V.refine_max = c->refine_max;
+    V.ctrl = (RefineCtrl*)at(L.ctrl); V.refine_max = c->refine_max; V.lu_arith = c->lu_arith;
     hipStream_t st = c->stream;
     HIP_TRY(hipMemcpyAsync(stA, A, 16 * (size_t)n * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(stB, B, 16 * (size_t)n * nrhs, hipMemcpyHostToDevice, st));

@@ -50,17 +50,25 @@ struct LuWork {
     T *sd_r = nullptr, *sd_i = nullptr, *s21_r = nullptr, *s21_i = nullptr;
     T *Linv_r = nullptr, *Linv_i = nullptr, *Uinv_r = nullptr, *Uinv_i = nullptr;
     T *PL_r = nullptr, *PL_i = nullptr, *PU_r = nullptr, *PU_i = nullptr;   // block-solve coefficients
+    float* umax = nullptr;   // per panel: max |Re|, |Im| of its U12 rows (FP32 split-arithmetic updates)
 };
 
+// Arithmetic of the FP32 factorisation's trailing updates.
+enum { kLuArithF32 = 0, kLuArithSplitF16 = 1 };
+
 // s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
+// arith (FP32 only): kLuArithF32 (FP32 MFMA) or kLuArithSplitF16 (fp16 MFMA on
+// split operands, w.umax required).
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof = nullptr);
+               int* ipiv, LuProfile* prof = nullptr, int arith = kLuArithF32);
 // Times the trailing-update tile shapes on the GEMM shapes of a factorisation
 // of padded order Np (once per device, precision and Np) using the Np x Np
 // scratch matrix Ar/Ai; returns the number of shape classes tuned.
+// umax (FP32: Np / 64 floats, overwritten): the panel maxima the split-
+// arithmetic tiles read; with it the split tiles are tuned too.
 template <typename T>
-int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai);
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax = nullptr);
 // Tile chosen for an M x N x K update on the current device: 1000 * rows + cols.
 template <typename T>
 int gemm_tile_choice(int M, int N, int K);

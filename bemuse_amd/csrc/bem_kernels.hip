@@ -723,7 +723,8 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
 // columns and wave a rows 16 a .. 16 a + 15 of them; all waves read all NB
 // rows of the columns, so they meet before overwriting.
 template <typename T>
-__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i) {
+__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i,
+                                                float* umax) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
@@ -739,10 +740,16 @@ __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, co
         cmma16<T>(Li_r + (size_t)kk * LU_NB + 16 * a, Li_i + (size_t)kk * LU_NB + 16 * a, LU_NB, B0r + kk,
                   B0i + kk, Np, cr, ci, i, g);
     __syncthreads();                                     // every wave has read the columns
+    float m = 0.0f;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         size_t o = (size_t)(col0 + i) * Np + k0 + 16 * a + MT::row(g, r);
         Ar[o] = cr[r]; Ai[o] = ci[r];
+        m = fmaxf(m, fmaxf(fabsf((float)cr[r]), fabsf((float)ci[r])));
+    }
+    if (umax) {   // the panel's max |Re|, |Im| of U12 (non-negative floats order as their bits)
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (lane == 0) atomicMax(reinterpret_cast<unsigned*>(umax + k0 / LU_NB), __float_as_uint(m));
     }
 }
 
@@ -1097,6 +1104,244 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub(int M, int N, int K, T* __
 }
 
 // ---------------------------------------------------------------------------
+// The same update for FP32 operands on fp16 MFMA (v_mfma_f32_32x32x16_f16,
+// FP32 accumulation): every FP32 operand x is scaled by a power of two and
+// split into two fp16 terms, x s = h + l with h = fp16(x s) and
+// l = fp16(x s - h), and each real product takes three MFMAs,
+//   a b s_a s_b ~ h_a h_b + h_a l_b + l_a h_b,
+// dropping l_a l_b. The pair carries 22 significant bits, so a product is
+// good to about 2^-22 of |a| |b| (FP32 FMA: 2^-24); the accumulation is FP32
+// as in the FP32 kernel. This is the split-operand emulation of single
+// precision on half-precision matrix units (Markidis et al., IPDPSW 2018;
+// Ootomo and Yokota, IJHPCA 2022), here with both scales chosen so that no
+// term overflows fp16 and small entries stay well above its subnormal range
+// relative to the largest: A holds LU multipliers (|l| <= 1 under partial
+// pivoting), scaled by 2^12; B holds rows of U, scaled by 2^(10 - e) with
+// 2^e > max |Re|, |Im| over the panels the launch reads (recorded by the U12
+// kernel in umax). The refinement against the FP64 matrix that follows the
+// factorisation absorbs the 2^-22 products; a non-finite term (growth beyond
+// the margin) makes the refinement fail and the frequency fall back to FP64.
+//
+// Four real products per complex product (Re = Ar Br - Ai Bi, Im = Ar Bi +
+// Ai Br), twelve MFMAs per 32 x 32 block and k chunk; the kernel is bound by
+// its C traffic, not by these. A is staged negated, so C + (-A) B; the
+// result is C + acc / (s_a s_b), an exact power-of-two scaling.
+//
+// Tiles (64 WM) x (64 WN), four waves in a 2 x 2 grid, each with WM x WN
+// blocks of 32 x 32. The chunk of 16 k is staged in LDS as the four fp16
+// parts (Re h, Re l, Im h, Im l) of A [row][k] and of B [col][k], so an MFMA
+// operand (8 consecutive k of one row or column) is one 16-byte LDS read.
+// Every element accumulates its products in the same order for any (WM, WN).
+// ---------------------------------------------------------------------------
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 h4_t __attribute__((ext_vector_type(4)));
+typedef float f16x_t __attribute__((ext_vector_type(16)));
+constexpr int HK = 16;          // k per LDS chunk: one MFMA depth
+constexpr int HKP = HK + 8;     // padded row length (halves) in LDS
+constexpr int kSplitExpA = 12;  // A scale 2^12
+constexpr int kSplitExpB = 10;  // max |B| s_b < 2^10
+
+__device__ inline h8_t neg8(h8_t v) { return -v; }
+
+__device__ inline f16x_t mfma32(h8_t a, h8_t b, f16x_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+template <int WM, int WN>
+__global__ __launch_bounds__(256, 2) void k_cgemm_sub_h2(int M, int N, int K, float* __restrict__ Pr,
+                                                         float* __restrict__ Pi, int ld, int ra, int ca, int rb,
+                                                         int cb, int rc, int cc, int critical,
+                                                         const float* __restrict__ umax) {
+    if (critical) __builtin_amdgcn_s_setprio(3);
+    constexpr int TM = 64 * WM, TN = 64 * WN;
+    constexpr int NQA = TM * HK / 4 / 256;        // quads of 4 k per thread per chunk (A)
+    constexpr int NQB = TN * HK / 4 / 256;
+    __shared__ __align__(16) _Float16 As[2][4][TM][HKP];   // [buf][Re h, Re l, Im h, Im l][row][k]
+    __shared__ __align__(16) _Float16 Bs[2][4][TN][HKP];   // [buf][part][col][k]
+    int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
+    int ntiles = tiles_m * tiles_n;
+    constexpr int GROUP_M = 8;                     // XCD-aware grouped order, as in k_cgemm_sub
+    int bid = blockIdx.x;
+    int per = (ntiles + 7) / 8;
+    int tile = (bid % 8) * per + bid / 8;
+    if (tile >= ntiles) return;
+    int group = tile / (GROUP_M * tiles_n), first_m = group * GROUP_M;
+    int gsize = min(GROUP_M, tiles_m - first_m), in_group = tile - group * GROUP_M * tiles_n;
+    int tm = first_m + in_group % gsize, tn = in_group / gsize;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int rowt = tm * TM, colt = tn * TN;
+    const int mrows = min(TM, M - rowt), ncols = min(TN, N - colt);
+    const size_t ldl = (size_t)ld;
+    // operand scales
+    float um = 0.0f;
+    for (int q = 0; q < K / LU_NB; q++) um = fmaxf(um, umax[rb / LU_NB + q]);
+    int eb = 0;
+    if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);   // um < 2^eb
+    const float sa = -(float)(1 << kSplitExpA);              // negated A
+    const float sb = ldexpf(1.0f, kSplitExpB - eb);
+    const float inv = ldexpf(1.0f, eb - kSplitExpB - kSplitExpA);
+
+    float ga[2][NQA][4];
+    float4 gb[2][NQB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int l = 0; l < NQA; l++) {
+            int q = tid + 256 * l, row = min(q % TM, mrows - 1), kq = q / TM;
+            size_t o = (size_t)(ca + k0 + 4 * kq) * ldl + ra + rowt + row;
+#pragma unroll
+            for (int t = 0; t < 4; t++) { ga[0][l][t] = Pr[o + t * ldl]; ga[1][l][t] = Pi[o + t * ldl]; }
+        }
+#pragma unroll
+        for (int l = 0; l < NQB; l++) {
+            int q = tid + 256 * l, col = min(q / 4, ncols - 1), kq = q % 4;
+            size_t o = (size_t)(cb + colt + col) * ldl + rb + k0 + 4 * kq;
+            gb[0][l] = *reinterpret_cast<const float4*>(Pr + o);
+            gb[1][l] = *reinterpret_cast<const float4*>(Pi + o);
+        }
+    };
+    auto split4 = [](const float* x, float s, h4_t& hi, h4_t& lo) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float v = x[t] * s;
+            _Float16 h = (_Float16)v;
+            hi[t] = h;
+            lo[t] = (_Float16)(v - (float)h);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int l = 0; l < NQA; l++) {
+            int q = tid + 256 * l, row = q % TM, kq = q / TM;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                h4_t hi, lo;
+                split4(ga[p][l], sa, hi, lo);
+                *reinterpret_cast<h4_t*>(&As[buf][2 * p][row][4 * kq]) = hi;
+                *reinterpret_cast<h4_t*>(&As[buf][2 * p + 1][row][4 * kq]) = lo;
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < NQB; l++) {
+            int q = tid + 256 * l, col = q / 4, kq = q % 4;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                float x[4] = {gb[p][l].x, gb[p][l].y, gb[p][l].z, gb[p][l].w};
+                h4_t hi, lo;
+                split4(x, sb, hi, lo);
+                *reinterpret_cast<h4_t*>(&Bs[buf][2 * p][col][4 * kq]) = hi;
+                *reinterpret_cast<h4_t*>(&Bs[buf][2 * p + 1][col][4 * kq]) = lo;
+            }
+        }
+    };
+
+    const int row0 = wm * 32 * WM, col0 = wn * 32 * WN;     // the wave's blocks within the tile
+    const bool active = row0 < mrows && col0 < ncols;
+    const int li = lane & 31, lk = 8 * (lane >> 5);          // operand row / column and k offset
+    f16x_t accR[WM][WN], accI[WM][WN];
+    float4 cR[WM][WN][4], cI[WM][WN][4];
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            accR[a][b] = f16x_t{};
+            accI[a][b] = f16x_t{};
+        }
+    const int nchunk = K / HK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nchunk) gload((c + 1) * HK);
+        if (c == 0 && active) {   // C, added at the end: its loads overlap the first chunk
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) {
+                    size_t off = (size_t)(cc + colt + col0 + 32 * b + li) * ldl + rc + rowt + row0 + 32 * a +
+                                 4 * (lane >> 5);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        cR[a][b][q] = *reinterpret_cast<const float4*>(Pr + off + 8 * q);
+                        cI[a][b][q] = *reinterpret_cast<const float4*>(Pi + off + 8 * q);
+                    }
+                }
+        }
+        if (active) {
+            h8_t arh[WM], arl[WM], aih[WM], ail[WM], nih[WM], nil[WM];
+            h8_t brh[WN], brl[WN], bih[WN], bil[WN];
+#pragma unroll
+            for (int a = 0; a < WM; a++) {
+                int r = row0 + 32 * a + li;
+                arh[a] = *reinterpret_cast<const h8_t*>(&As[buf][0][r][lk]);
+                arl[a] = *reinterpret_cast<const h8_t*>(&As[buf][1][r][lk]);
+                aih[a] = *reinterpret_cast<const h8_t*>(&As[buf][2][r][lk]);
+                ail[a] = *reinterpret_cast<const h8_t*>(&As[buf][3][r][lk]);
+                nih[a] = neg8(aih[a]);
+                nil[a] = neg8(ail[a]);
+            }
+#pragma unroll
+            for (int b = 0; b < WN; b++) {
+                int cl = col0 + 32 * b + li;
+                brh[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][0][cl][lk]);
+                brl[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][1][cl][lk]);
+                bih[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][2][cl][lk]);
+                bil[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][3][cl][lk]);
+            }
+            // Re += Ar Br - Ai Bi,  Im += Ar Bi + Ai Br  (three split products each),
+            // alternating the two accumulators
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) {
+                    f16x_t R = accR[a][b], I = accI[a][b];
+                    R = mfma32(arh[a], brh[b], R);
+                    I = mfma32(arh[a], bih[b], I);
+                    R = mfma32(arh[a], brl[b], R);
+                    I = mfma32(arh[a], bil[b], I);
+                    R = mfma32(arl[a], brh[b], R);
+                    I = mfma32(arl[a], bih[b], I);
+                    R = mfma32(nih[a], bih[b], R);
+                    I = mfma32(aih[a], brh[b], I);
+                    R = mfma32(nih[a], bil[b], R);
+                    I = mfma32(aih[a], brl[b], I);
+                    R = mfma32(nil[a], bih[b], R);
+                    I = mfma32(ail[a], brh[b], I);
+                    accR[a][b] = R;
+                    accI[a][b] = I;
+                }
+        }
+        if (c + 1 < nchunk) {
+            sstore(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    if (!active) return;
+    // accumulator element e of a 32 x 32 block: row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            size_t off = (size_t)(cc + colt + col0 + 32 * b + li) * ldl + rc + rowt + row0 + 32 * a + 4 * (lane >> 5);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float4 r = cR[a][b][q], m = cI[a][b][q];
+                r.x = fmaf(accR[a][b][4 * q + 0], inv, r.x);
+                r.y = fmaf(accR[a][b][4 * q + 1], inv, r.y);
+                r.z = fmaf(accR[a][b][4 * q + 2], inv, r.z);
+                r.w = fmaf(accR[a][b][4 * q + 3], inv, r.w);
+                m.x = fmaf(accI[a][b][4 * q + 0], inv, m.x);
+                m.y = fmaf(accI[a][b][4 * q + 1], inv, m.y);
+                m.z = fmaf(accI[a][b][4 * q + 2], inv, m.z);
+                m.w = fmaf(accI[a][b][4 * q + 3], inv, m.w);
+                *reinterpret_cast<float4*>(Pr + off + 8 * q) = r;
+                *reinterpret_cast<float4*>(Pi + off + 8 * q) = m;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with NB-row blocks, one persistent launch per solve.
 //
 // Block form of the lower solve (the upper one mirrors it with c > b):
@@ -1391,10 +1636,11 @@ __global__ void k_flags_any(int npanel, int* flag) {
     if (threadIdx.x == 0) flag[npanel] = s;
 }
 
-__global__ void k_init_lu(int Np, int npanel, int* flag, int* ipiv) {
+__global__ void k_init_lu(int Np, int npanel, int* flag, int* ipiv, float* umax) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < Np) ipiv[i] = i;
     if (i <= npanel) flag[i] = 0;
+    if (umax && i < npanel) umax[i] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1825,9 +2071,11 @@ template <typename T> constexpr TileCfg cfg_of(int i) { return kCfg[i]; }
 constexpr int kMaxDev = 16, kClasses = 11;
 struct TuneTable {
     signed char pick[2][kClasses][kClasses][3];   // [prec][M class][N class][K class: 64, 128, 192-256]
-    bool tuned_np[2][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128)
+    signed char split[kClasses][kClasses][3];     // split-arithmetic FP32 updates
+    bool tuned_np[3][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128): FP32, FP64, split
     TuneTable() {
         std::memset(pick, -1, sizeof(pick));
+        std::memset(split, -1, sizeof(split));
         std::memset(tuned_np, 0, sizeof(tuned_np));
     }
 };
@@ -1862,6 +2110,27 @@ void launch_cfg(int cfg, hipStream_t s, int M, int N, int K, T* Ar, T* Ai, int l
     }
 }
 
+// Split-arithmetic tiles: 64 x 64, 128 x 64, 64 x 128 (WM, WN blocks of 64).
+template <int WM, int WN>
+void launch_split_tile(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int ra, int ca, int rb,
+                       int cb, int rc, int cc, int critical, const float* umax) {
+    constexpr int TM = 64 * WM, TN = 64 * WN;
+    int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
+    int per = (tiles + 7) / 8;
+    hipLaunchKernelGGL((k_cgemm_sub_h2<WM, WN>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb,
+                       rc, cc, critical, umax);
+}
+constexpr int kSplitCfgs = 3;
+void launch_split(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int ra, int ca, int rb,
+                  int cb, int rc, int cc, int critical, const float* umax) {
+    switch (cfg) {
+    case 1: launch_split_tile<2, 1>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical, umax); return;
+    case 2: launch_split_tile<1, 2>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical, umax); return;
+    default: launch_split_tile<1, 1>(s, M, N, K, Ar, Ai, ld, ra, ca, rb, cb, rc, cc, critical, umax); return;
+    }
+}
+int default_split(int M, int N) { return cdiv(M, 128) * cdiv(N, 64) >= 512 ? 1 : 0; }
+
 // Size rule for untuned shapes: the larger FP32 tile where it still gives 2
 // work-groups per CU, else 64 x 64.
 template <typename T>
@@ -1877,6 +2146,11 @@ template <typename T>
 int pick_cfg(int dev, int M, int N, int K) {
     int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].pick[sizeof(T) == 8][size_class(M)][size_class(N)][k_class(K)] : -1;
     return v >= 0 ? v : default_cfg<T>(M, N);
+}
+
+int pick_split(int dev, int M, int N, int K) {
+    int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].split[size_class(M)][size_class(N)][k_class(K)] : -1;
+    return v >= 0 ? v : default_split(M, N);
 }
 
 // The (M, N, K) of every update launch of launch_lu at padded order Np, in
@@ -1900,17 +2174,68 @@ std::vector<std::array<int, 3>> lu_gemm_shapes(int Np, int group) {
 }
 }  // namespace
 
+namespace {
+// The split-arithmetic tiles timed like the FP32 ones (tune_lu_gemms), with
+// the panel maxima cleared (scale 1: the timing does not depend on it).
+int tune_split(hipStream_t s, int Np, float* Ar, float* Ai, float* umax) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_tune[dev].tuned_np[2][Np / 128]) return 0;
+    }
+    if (hipMemsetAsync(umax, 0, sizeof(float) * (Np / LU_NB), s) != hipSuccess) return 0;
+    std::map<std::array<int, 3>, std::array<int, 3>> reps;
+    for (auto& sh : lu_gemm_shapes(Np, lu_group<float>(Np))) {
+        std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), k_class(sh[2])};
+        auto it = reps.find(key);
+        if (it == reps.end() || (double)sh[0] * sh[1] > (double)it->second[0] * it->second[1]) reps[key] = sh;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 0;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 0; }
+    std::vector<std::pair<std::array<int, 3>, int>> picks;
+    for (auto& kv : reps) {
+        int M = std::min(kv.second[0], Np - 128), N = std::min(kv.second[1], Np - 128), K = kv.second[2];
+        if (M < 64 || N < 64) continue;
+        float best = 1e30f;
+        int bi = default_split(M, N);
+        for (int c = 0; c < kSplitCfgs; c++) {
+            launch_split(c, s, M, N, K, Ar, Ai, Np, 128, 0, 0, 128, 128, 128, 0, umax);   // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3; r++) launch_split(c, s, M, N, K, Ar, Ai, Np, 128, 0, 0, 128, 128, 128, 0, umax);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; bi = c; }
+        }
+        picks.push_back({kv.first, bi});
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& pk : picks) g_tune[dev].split[pk.first[0]][pk.first[1]][pk.first[2]] = (signed char)pk.second;
+    g_tune[dev].tuned_np[2][Np / 128] = true;
+    return (int)picks.size();
+}
+}  // namespace
+
 // Times every tile shape on the shapes a factorisation of order Np launches
 // (one representative per class, the largest of the class) and records the
 // fastest. Ar, Ai: an Np x Np scratch matrix (its contents are overwritten).
 template <typename T>
-int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai) {
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 0;
     int prec = sizeof(T) == 8;
+    int tuned = 0;
+    if constexpr (sizeof(T) == 4) {
+        if (umax) tuned += tune_split(s, Np, Ar, Ai, umax);
+    }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        if (g_tune[dev].tuned_np[prec][Np / 128]) return 0;
+        if (g_tune[dev].tuned_np[prec][Np / 128]) return tuned;
     }
     std::map<std::array<int, 3>, std::array<int, 3>> reps;   // (mc, nc, kb) -> largest (M, N, K)
     for (auto& sh : lu_gemm_shapes(Np, lu_group<T>(Np))) {
@@ -1949,8 +2274,8 @@ int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai) {
     g_tune[dev].tuned_np[prec][Np / 128] = true;
     return (int)picks.size();
 }
-template int tune_lu_gemms<float>(hipStream_t, int, float*, float*);
-template int tune_lu_gemms<double>(hipStream_t, int, double*, double*);
+template int tune_lu_gemms<float>(hipStream_t, int, float*, float*, float*);
+template int tune_lu_gemms<double>(hipStream_t, int, double*, double*, float*);
 
 template <typename T>
 int gemm_tile_choice(int M, int N, int K) {
@@ -1967,7 +2292,7 @@ template int gemm_tile_choice<double>(int, int, int);
 // around the launch and the flops of the unpadded part).
 template <typename T>
 static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar, T* Ai, int ra, int ca, int rb,
-                     int cb, int rc, int cc, LuProfile* prof, int critical) {
+                     int cb, int rc, int cc, LuProfile* prof, int critical, const float* umax) {
     bool timed = prof && prof->n < LuProfile::kMax;
     if (timed) {
         if (!prof->ev[2 * prof->n]) {
@@ -1976,7 +2301,12 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
         }
         (void)hipEventRecord(prof->ev[2 * prof->n], s);
     }
-    launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
+    if constexpr (sizeof(T) == 4) {
+        if (umax) launch_split(pick_split(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
+        else launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
+    } else {
+        launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
+    }
     if (timed) {
         (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
         double mu = (double)std::max(0, std::min(M, prof->N - rc));
@@ -2008,11 +2338,13 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
 // ev (2 per panel) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof) {
+               int* ipiv, LuProfile* prof, int arith) {
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
-    hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv);
+    const bool split = sizeof(T) == 4 && arith == kLuArithSplitF16 && w.umax;
+    float* umax = split ? w.umax : nullptr;
+    hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv, umax);
     auto panel = [&](hipStream_t st, int p) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
         T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
@@ -2034,7 +2366,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
         if (rest > 0)
             hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 16), dim3(256), 0, s, Np, k0, Ar, Ai,
-                               w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2);
+                               w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2, umax);
     };
     const int NB = LU_NB, group = lu_group<T>(Np);
     panel(s, 0);
@@ -2045,7 +2377,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             const int q = p + j, kj = k0 + j * NB;
             if (j > 0) {
                 // panel q's columns: A(kj.., q) -= L21(kj.., p..q-1) U12(p..q-1, q)
-                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof, 1);
+                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof, 1, umax);
                 (void)hipEventRecord(ev[2 * q], s);
                 (void)hipStreamWaitEvent(s2, ev[2 * q], 0);
                 panel(s2, q);
@@ -2054,28 +2386,28 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             }
             fix(q);
             if (j > 0 && Np - kj - NB > 0)   // panel q's rows right of it
-                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof, 1);
+                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof, 1, umax);
             u12(q);
         }
         const int kg = k0 + g * NB, restg = Np - kg;
         if (restg <= 0) break;
         // the next group's first panel, then hand it to the panel stream
-        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1);
+        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax);
         (void)hipEventRecord(ev[2 * (p + g)], s);
         (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
         panel(s2, p + g);
         (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
         if (restg > NB)   // trailing update of everything right of it
-            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0);
+            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,
                        w.Uinv_i, w.PL_r, w.PL_i, w.PU_r, w.PU_i);
 }
 template void launch_lu<double>(hipStream_t, hipStream_t, hipEvent_t*, int, double*, double*, const LuWork<double>&,
-                                 int*, int*, LuProfile*);
+                                 int*, int*, LuProfile*, int);
 template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float*, float*, const LuWork<float>&, int*,
-                                int*, LuProfile*);
+                                int*, LuProfile*, int);
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,

@@ -179,6 +179,18 @@ int bem_set_graphs(bem_ctx* ctx, int enable);
  * that path). No counterpart in the reference (it has no refinement). */
 int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
 
+/* Arithmetic of the trailing updates of the FP32 factorisation in the mixed
+ * modes (the FP64 refinement that follows makes both give the same results
+ * to the refinement tolerance):
+ *   BEM_LU_SPLIT_F16 (default): fp16 MFMA on operands split into two scaled
+ *     fp16 terms each, three products per real product, FP32 accumulation
+ *     (products good to about 2^-22, about 4x the FP32 rounding); the
+ *     updates are then bound by their FP32 C traffic;
+ *   BEM_LU_FP32: FP32 MFMA (16x16x4).
+ * No counterpart in the reference (its LU is Eigen's). */
+enum { BEM_LU_FP32 = 0, BEM_LU_SPLIT_F16 = 1 };
+int bem_set_lu_arith(bem_ctx* ctx, int arith);
+
 /* Measured machine peaks of the context's device, for reporting beside the
  * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
  * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +

@@ -510,3 +510,31 @@ def test_spar_20k_panels(ctx):
     K = w * w / 9.81
     assert abs(B[2, 2] / (K * abs(F[2, 0]) ** 2 / (2 * 9.81 ** 2)) - 1) < 1e-2
     body.close()
+
+
+@pytest.mark.parametrize("n", [130, 600])
+def test_lu_arithmetics_agree(ctx, cyl, n):
+    """The FP32 factorisation's updates on fp16 MFMA with split operands (the
+    default) and on FP32 MFMA give the same refined results: dense solves with
+    and without interchanges, and a frequency solve against the oracle."""
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal((n, 5)) + 1j * rng.standard_normal((n, 5))
+    mesh, body, ob = cyl
+    r = ob.solve(0.7, betas=[0.0, 1.1])
+    out = {}
+    try:
+        for arith in (bm.LU_FP32, bm.LU_SPLIT_F16):
+            ctx.set_lu_arith(arith)
+            X, info = ctx.dense_solve(A, B, 32)
+            assert info["fell_back_fp64"] == 0 and info["pivot_fallback"] > 0, (arith, info)
+            g = body.solve(0.7, betas=[0.0, 1.1], mode=bm.MODE_MIXED)
+            assert g["info"]["fell_back_fp64"] == 0 and g["info"]["residual"] < 1e-13
+            assert_abf(g, r, bm.MODE_MIXED, what=f"arith {arith}")
+            out[arith] = (X, g)
+    finally:
+        ctx.set_lu_arith(bm.LU_SPLIT_F16)
+    Xr = np.linalg.solve(A, B)
+    for arith, (X, g) in out.items():
+        assert relerr(X, Xr) < 1e-9, (arith, relerr(X, Xr))
+    assert relerr(out[0][0], out[1][0]) < 1e-9
