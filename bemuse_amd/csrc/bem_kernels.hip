@@ -251,7 +251,7 @@ constexpr int NB2 = LU_NB * LU_NB;
 // panels; measured on MI355X, that pays in FP64 and for large orders, while
 // FP32 at N ~ 6000 is faster with pairs (74.9 vs 71.4 frequency-solves/s).
 template <typename T>
-inline int lu_group(int Np) { return (sizeof(T) == 8 || Np >= 12288) ? 4 : 2; }
+inline int lu_group(int Np, bool split = false) { return (sizeof(T) == 8 || split || Np >= 12288) ? 4 : 2; }
 
 template <typename T>
 __device__ inline T cabs2(T r, T i) { return r * r + i * i; }
@@ -2186,7 +2186,7 @@ int tune_split(hipStream_t s, int Np, float* Ar, float* Ai, float* umax) {
     }
     if (hipMemsetAsync(umax, 0, sizeof(float) * (Np / LU_NB), s) != hipSuccess) return 0;
     std::map<std::array<int, 3>, std::array<int, 3>> reps;
-    for (auto& sh : lu_gemm_shapes(Np, lu_group<float>(Np))) {
+    for (auto& sh : lu_gemm_shapes(Np, lu_group<float>(Np, true))) {
         std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), k_class(sh[2])};
         auto it = reps.find(key);
         if (it == reps.end() || (double)sh[0] * sh[1] > (double)it->second[0] * it->second[1]) reps[key] = sh;
@@ -2368,7 +2368,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 16), dim3(256), 0, s, Np, k0, Ar, Ai,
                                w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2, umax);
     };
-    const int NB = LU_NB, group = lu_group<T>(Np);
+    const int NB = LU_NB, group = lu_group<T>(Np, split);
     panel(s, 0);
     for (int p = 0; p < npanel; p += group) {
         const int g = std::min(group, npanel - p), k0 = p * NB;
