@@ -585,10 +585,11 @@ class Body:
 
     def profile_lu(self, kappa, mode=MODE_FP64):
         """Instrumented factorisation: GEMM launch timings of one LU (see bem_profile_lu)."""
-        out = np.zeros(6)
+        out = np.zeros(9)
         _check(lib().bem_profile_lu(self.ctx._h, self._h, mode, kappa, _dp(out)))
         return dict(ms_lu=out[0], ms_gemm=out[1], gemm_flops=out[2], gemm_launches=int(out[3]),
-                    ms_gemm_per_launch=out[4], flops_per_launch=out[5])
+                    ms_gemm_per_launch=out[4], flops_per_launch=out[5], gemm_bytes=out[6],
+                    bytes_per_launch=out[7], ms_gemm_per_launch_replay=out[8])
 
     def solve_sweep(self, omegas, betas=(), mode=MODE_FP64, lanes=4, kappas=None, compose=COMPOSE_WAVE):
         """Frequencies pipelined over `lanes` concurrent solve lanes; returns

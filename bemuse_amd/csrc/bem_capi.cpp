@@ -1295,6 +1295,13 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     out[3] = prof->n;
     out[4] = prof->n ? sum / prof->n : 0.0;
     out[5] = prof->n ? prof->useful_flops / prof->n : 0.0;
+    out[6] = prof->useful_bytes;
+    out[7] = prof->n ? prof->useful_bytes / prof->n : 0.0;
+    out[8] = 0.0;
+    if (e == hipSuccess && prof->n) {
+        out[8] = mode == BEM_MODE_FP64 ? replay_lu_gemms<double>(*prof, L.s1) : replay_lu_gemms<float>(*prof, L.s1);
+        e = hipStreamSynchronize(L.s1);
+    }
     for (auto& ev : prof->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete prof;

@@ -40,7 +40,23 @@ struct LuProfile {
     int n = 0;
     int N = 0;
     double useful_flops = 0.0;
+    double useful_bytes = 0.0;   // C read and written, A and B read once, over the unpadded part
+    // every timed launch, as issued (operand pointers, offsets, tile choice),
+    // for replay_lu_gemms
+    struct Call {
+        int dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, split;
+        void *Ar, *Ai;
+        const float* umax;
+    };
+    Call call[kMax] = {};
 };
+
+// Launches every GEMM the profile recorded again, back to back on stream s
+// with one pair of HIP events around the run, and returns the mean ms per
+// launch: the kernels' own durations without the event packets between them.
+// The launches overwrite the factors (a measurement, not a solve).
+template <typename T>
+double replay_lu_gemms(const LuProfile& prof, hipStream_t s);
 
 // Work buffers of one factorisation in precision T: saved diagonal block
 // (NB x NB), saved A21 (Np x NB, ld Np) and the per-panel inverses of the

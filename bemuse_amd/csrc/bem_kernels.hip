@@ -2301,21 +2301,56 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
         }
         (void)hipEventRecord(prof->ev[2 * prof->n], s);
     }
+    bool split = sizeof(T) == 4 && umax;
+    int cfg = split ? pick_split(dev, M, N, K) : pick_cfg<T>(dev, M, N, K);
     if constexpr (sizeof(T) == 4) {
-        if (umax) launch_split(pick_split(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
-        else launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
+        if (split) launch_split(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
+        else launch_cfg<T>(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
     } else {
-        launch_cfg<T>(pick_cfg<T>(dev, M, N, K), s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
+        launch_cfg<T>(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
     }
     if (timed) {
         (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
+        prof->call[prof->n] = {dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, split ? 1 : 0, Ar, Ai, umax};
         double mu = (double)std::max(0, std::min(M, prof->N - rc));
         double nu = (double)std::max(0, std::min(N, prof->N - cc));
         double ku = (double)std::max(0, std::min(K, prof->N - ca));
         prof->useful_flops += 8.0 * mu * nu * ku;
+        prof->useful_bytes += 2.0 * sizeof(T) * (2.0 * mu * nu + mu * ku + ku * nu);
         prof->n++;
     }
 }
+
+template <typename T>
+double replay_lu_gemms(const LuProfile& prof, hipStream_t s) {
+    if (prof.n == 0) return 0.0;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < prof.n; i++) {
+        const LuProfile::Call& c = prof.call[i];
+        T* Ar = static_cast<T*>(c.Ar);
+        T* Ai = static_cast<T*>(c.Ai);
+        if constexpr (sizeof(T) == 4) {
+            if (c.split)
+                launch_split(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.ra, c.ca, c.rb, c.cb, c.rc, c.cc, c.critical, c.umax);
+            else
+                launch_cfg<T>(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.ra, c.ca, c.rb, c.cb, c.rc, c.cc, c.critical);
+        } else {
+            launch_cfg<T>(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.ra, c.ca, c.rb, c.cb, c.rc, c.cc, c.critical);
+        }
+    }
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / prof.n;
+}
+template double replay_lu_gemms<float>(const LuProfile&, hipStream_t);
+template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 
 // Right-looking blocked LU over groups of lu_group<T>(Np) panels, with lookahead.
 // For the group p .. p+g-1 at k0 = p NB (panel p already factored on the

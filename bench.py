@@ -16,8 +16,9 @@ solves frequency floor(i * 40 / K); with more, they cycle), and are pipelined
 over --lanes concurrent solve lanes (one C-ABI sweep call covers the timed
 steps; GPU_MAX_HW_QUEUES is raised to 32 so that the lanes' streams do not
 share hardware queues). Rankine assembly (once per body) is outside the timed
-region and reported separately. The headline mode is MIXED (FP32-MFMA LU +
-FP64 refinement); the FP64 mode is timed in the same run on the same steps
+region and reported separately. The headline mode is MIXED (FP32 LU whose
+trailing updates run on fp16 MFMA with split operands, + FP64 refinement);
+the FP64 mode is timed in the same run on the same steps
 for the ratio, and every MIXED result is compared with the FP64 GPU result at
 the same frequency (A, B and F separately) and with the FP64 CPU oracle at the
 sampled frequency.
@@ -44,13 +45,15 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12      # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_FP32 = 157.3e12     # MI355X dense FP32 matrix, flop/s
+PEAK_FP16 = 2.5e15       # MI355X dense FP16 matrix, flop/s
+PEAK_HBM = 8.0e12        # MI355X HBM3E, bytes/s
 METRIC = "frequency-solves/sec at 6k panels (mixed+surrogate vs FP64) at <=1e-6 rel err"
 
 
 def measured_traffic():
     """HBM bytes per launch of the dominant kernel, from the rocprof PMC summary
     committed with the code (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
-    path = os.path.join(ROOT, "profiles", "cgemm_f32_semisub6018_roofline.md")
+    path = os.path.join(ROOT, "profiles", "cgemm_split_f16_semisub6018_roofline.md")
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -170,21 +173,36 @@ def main():
     def stage_means(res):
         return {k: float(np.mean([r["info"][k] for r in res])) for k in ("ms_assemble", "ms_factor", "ms_solve")}
 
-    # roofline of the dominant kernel: the MFMA trailing-update GEMM of the LU,
-    # timed with HIP events around each launch of one isolated factorisation
+    # roofline of the dominant kernel: the trailing-update GEMM of the FP32 LU
+    # (split-operand fp16 MFMA, bound by its C traffic). One isolated
+    # factorisation records its GEMM launches (HIP events around each); the
+    # same launches are then issued again back to back inside one event pair on
+    # the factorisation's stream, which gives the kernels' own mean duration
+    # (the per-launch event packets add a few microseconds to each pair).
     kap = 1.0 ** 2 / 9.81
     prof_mx = body.profile_lu(kap, bm.MODE_MIXED)
     prof_64 = body.profile_lu(kap, bm.MODE_FP64)
-    achieved = prof_mx["flops_per_launch"] / (prof_mx["ms_gemm_per_launch"] * 1e-3)
+    s_launch = prof_mx["ms_gemm_per_launch_replay"] * 1e-3
+    achieved_bw = prof_mx["bytes_per_launch"] / s_launch
     roofline = {
-        "kernel": "k_cgemm_sub<float, WM, WN> (LU updates, FP32 MFMA 16x16x4, three real products per complex product, K = 64..256, autotuned tiles)",
-        "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32, "traffic": measured_traffic(),
-        "flops_per_launch": prof_mx["flops_per_launch"], "launches": int(prof_mx["gemm_launches"]),
-        "us_per_launch": prof_mx["ms_gemm_per_launch"] * 1e3,
-        "fp64_gemm": {"achieved": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / 1e12,
-                      "peak": PEAK_FP64 / 1e12,
-                      "frac": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch"] * 1e-3) / PEAK_FP64},
+        "kernel": "k_cgemm_sub_h2<WM, WN> (LU trailing updates C -= A B on FP32 planar complex; operands split into "
+                  "two scaled fp16 terms, v_mfma_f32_32x32x16_f16, FP32 accumulation; K = 64..256, autotuned tiles)",
+        "bound": "hbm", "achieved": achieved_bw / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+        "frac": achieved_bw / PEAK_HBM, "traffic": measured_traffic(),
+        "bytes_per_launch": prof_mx["bytes_per_launch"], "flops_per_launch": prof_mx["flops_per_launch"],
+        "launches": int(prof_mx["gemm_launches"]), "us_per_launch": s_launch * 1e6,
+        "us_per_launch_event_pairs": prof_mx["ms_gemm_per_launch"] * 1e3,
+        # the same launches against the matrix units: algorithmic complex flops
+        # (8 m n k) and the fp16 MFMA flops executed (three split products of
+        # four real products: 24 m n k)
+        "mfma": {"algorithmic_tflops": prof_mx["flops_per_launch"] / s_launch / 1e12,
+                 "executed_fp16_tflops": 3.0 * prof_mx["flops_per_launch"] / s_launch / 1e12,
+                 "peak_fp16_tflops": PEAK_FP16 / 1e12,
+                 "frac_fp16": 3.0 * prof_mx["flops_per_launch"] / s_launch / PEAK_FP16},
+        "fp64_gemm": {"achieved": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch_replay"] * 1e-3) / 1e12,
+                      "unit": "TFLOP/s", "peak": PEAK_FP64 / 1e12,
+                      "frac": prof_64["flops_per_launch"] / (prof_64["ms_gemm_per_launch_replay"] * 1e-3) / PEAK_FP64,
+                      "us_per_launch": prof_64["ms_gemm_per_launch_replay"] * 1e3},
         "lu_ms": {"mixed": prof_mx["ms_lu"], "fp64": prof_64["ms_lu"],
                   "gemm_share_mixed": prof_mx["ms_gemm"] / prof_mx["ms_lu"],
                   "gemm_share_fp64": prof_64["ms_gemm"] / prof_64["ms_lu"]},
@@ -222,14 +240,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-lu+fp64-refine",
+        "dtype": "fp32-lu(split-fp16-mfma)+fp64-refine",
         "data": "synthetic",
         "config": {"workload": workload, "panels": N, "frequencies": len(omegas), "headings": len(betas),
                    "dof": 6, "lanes": args.lanes,
                    "timed_frequencies": f"{len(np.unique(fidx))} distinct of {len(omegas)} "
                                         f"(omega = {omegas[fidx[0]]:.4f} .. {omegas[fidx[-1]]:.4f} rad/s, "
                                         + ("spread evenly)" if args.steps <= len(omegas) else "cycled)"),
-                   "mode": "mixed (FP64 assembly, FP32-MFMA LU, FP64 iterative refinement)"},
+                   "mode": "mixed (FP64 assembly; FP32 LU with split-operand fp16-MFMA updates; FP64 iterative "
+                           "refinement)"},
         "fp64_value": args.steps / dt64,
         "speedup_vs_fp64": dt64 / dtmx,
         "max_rel_err_mixed_vs_fp64_gpu": err_mx_vs_64,
@@ -373,7 +392,7 @@ def other_workload(args, bm):
         t_mx, t_64 = res[bm.MODE_MIXED][0], res[bm.MODE_FP64][0]
     line = {"metric": f"frequency-solves/sec ({args.workload})", "value": ntask / t_mx, "unit": "frequency-solves/s",
             "n_gpus": 1, "steps": ntask, "warmup": 1, "ms_per_step": t_mx / ntask * 1e3, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu+fp64-refine", "data": "synthetic",
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32-lu(split-fp16-mfma)+fp64-refine", "data": "synthetic",
             "config": cfg, "fp64_value": ntask / t_64, "speedup_vs_fp64": t_64 / t_mx,
             "max_rel_err_mixed_vs_fp64_gpu": err, "rankine_s": t_rk}
     print(json.dumps(line), flush=True)

@@ -404,10 +404,16 @@ int bem_surrogate_grad(bem_ctx* ctx, const float* params, int n, const float* X,
 int bem_surrogate_eval(bem_ctx* ctx, int n, const double* h, const double* v, double* out, int* used);
 
 /* Instrumented factorisation for measurement: assembles the system at kappa
- * (compose WAVE) and factors it in the mode's precision with HIP events
- * around every trailing-update GEMM launch. out (6 doubles): total LU ms,
- * summed GEMM ms, useful GEMM flops (unpadded order N), GEMM launches,
- * mean GEMM ms per launch, mean useful flops per launch. */
+ * (compose WAVE) and factors it in the mode's precision (and the context's
+ * LU arithmetic) with HIP events around every trailing-update GEMM launch.
+ * out (9 doubles): total LU ms, summed GEMM ms (event pair per launch),
+ * useful GEMM flops (unpadded order N), GEMM launches, mean GEMM ms per
+ * launch (event pairs), mean useful flops per launch, useful GEMM bytes (C
+ * read and written, A and B read once, unpadded), mean useful bytes per
+ * launch, mean GEMM ms per launch when the same launches are issued again
+ * back to back inside one event pair (kernel time without the per-launch
+ * event packets; this replay overwrites the factors, which the call
+ * discards). */
 int bem_profile_lu(bem_ctx* ctx, bem_body* b, int mode, double kappa, double* out);
 
 #ifdef __cplusplus
