@@ -2356,11 +2356,12 @@ template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 // For the group p .. p+g-1 at k0 = p NB (panel p already factored on the
 // panel stream s2, every column right of it up to date with the groups
 // before):
-//   for j = 0 .. g-1, panel q = p + j at kj = k0 + j NB:
-//     (j > 0) main stream: the K = j NB update of panel q's columns by the
-//             group's panels before it, [L21_p .. L21_{q-1}] [U12 ...];
-//             panel stream: factor panel q (diagonal block, L21);
-//     main stream: fix q (pivot check / fallback + row swaps elsewhere);
+//   for j = 0 .. g-1, panel q = p + j at kj = k0 + j NB, on the main stream:
+//     (j > 0) the K = j NB update of panel q's columns by the group's panels
+//             before it, [L21_p .. L21_{q-1}] [U12 ...], then factor panel q
+//             (diagonal block, L21): nothing else could run beside them, and
+//             a hand-off between streams costs about 10 us each way;
+//     fix q (pivot check / fallback + row swaps elsewhere);
 //     (j > 0) the K = j NB update of panel q's rows right of it;
 //     U12_q = Linv_q A12;
 //   main stream: the K = g NB update of the next group's first panel, which
@@ -2370,7 +2371,7 @@ template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 // passes over C by g. Row interchanges of panel q (fallback
 // pivoting) hit the trailing rows not yet updated by the group's earlier
 // panels and those panels' L21 alike, which commutes with the update. Events
-// ev (2 per panel) order the two streams.
+// ev[2 q], ev[2 q + 1] (q the first panel of a group) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
                int* ipiv, LuProfile* prof, int arith) {
@@ -2413,11 +2414,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             if (j > 0) {
                 // panel q's columns: A(kj.., q) -= L21(kj.., p..q-1) U12(p..q-1, q)
                 gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof, 1, umax);
-                (void)hipEventRecord(ev[2 * q], s);
-                (void)hipStreamWaitEvent(s2, ev[2 * q], 0);
-                panel(s2, q);
-                (void)hipEventRecord(ev[2 * q + 1], s2);
-                (void)hipStreamWaitEvent(s, ev[2 * q + 1], 0);
+                panel(s, q);   // nothing to overlap: same stream, no cross-stream hand-off
             }
             fix(q);
             if (j > 0 && Np - kj - NB > 0)   // panel q's rows right of it
