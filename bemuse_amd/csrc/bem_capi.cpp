@@ -227,6 +227,7 @@ struct Lane {
     int mode = 0, prec = 64, nb = 0, rad_only = 0, compose = 0;
     int refine_max = kRefineMaxIter;
     int lu_arith = kLuArithSplitF16;
+    int lu_latency = 1;    // set per run: 1 when this lane is the only one in flight
     bool sur_used = false;
     double omega = 0, kappa = 0;
     // captured frequency graph and the configuration it was captured for
@@ -320,12 +321,14 @@ struct SolveView {
     RefineCtrl* ctrl = nullptr;
     int refine_max = kRefineMaxIter;
     int lu_arith = kLuArithSplitF16;
+    int lu_latency = 1;    // one frequency in flight (launch_lu's latency)
 };
 
 SolveView lane_view(Lane& L) {
     SolveView v;
     v.s1 = L.s1; v.s2 = L.s2; v.ev = L.lu_events.data(); v.sys = &L.sys;
     v.counter = L.counter; v.pub = L.pub; v.ctrl = L.ctrl; v.refine_max = L.refine_max; v.lu_arith = L.lu_arith;
+    v.lu_latency = L.lu_latency;
     return v;
 }
 
@@ -333,9 +336,10 @@ SolveView lane_view(Lane& L) {
 void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr) {
     LinSys& s = *V.sys;
     if (precision == 64)
-        launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof);
+        launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof, kLuArithF32, V.lu_latency);
     else
-        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith);
+        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith,
+                         V.lu_latency);
 }
 void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
 
@@ -1038,6 +1042,8 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
               double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* infos) {
     size_t fs = (size_t)6 * nb * 2;
     int nl = (int)lanes.size();
+    // one lane: its factorisations have the device to themselves
+    for (Lane* L : lanes) L->lu_latency = nl == 1;
     std::vector<int> busy(nl, -1);
     std::vector<unsigned long long> since(nl, 0);
     unsigned long long seq = 0;
@@ -1124,6 +1130,7 @@ int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double 
     HIP_TRY(hipSetDevice(c->device));
     if ((r = ensure_lanes(b, 1, mode != BEM_MODE_FP64))) return r;
     Lane& L = *b->lanes[0];
+    L.lu_latency = 1;   // the one frequency in flight
     if ((r = upload_betas(L, nb, betas))) return r;
     if ((r = issue_frequency(b, L, 0, mode, omega, kappa, compose, rad_only, nb))) return r;
     if ((r = harvest(b, L, A36, B36, F, FK, SC, info))) return r;
@@ -1273,6 +1280,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     if (r) return r;
     Lane& L = *b->lanes[0];
     L.lu_arith = c->lu_arith;
+    L.lu_latency = 1;   // one factorisation alone on the device
     set_frequency(L, 0.0, kappa);
     assemble(b, L, mode, kappa, BEM_COMPOSE_WAVE);
     LuProfile* prof = new LuProfile;

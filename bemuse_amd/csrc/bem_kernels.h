@@ -74,10 +74,13 @@ enum { kLuArithF32 = 0, kLuArithSplitF16 = 1 };
 
 // s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
 // arith (FP32 only): kLuArithF32 (FP32 MFMA) or kLuArithSplitF16 (fp16 MFMA on
-// split operands, w.umax required).
+// split operands, w.umax required). latency != 0: the factorisation has the
+// device to itself (one frequency in flight), so the panel kernels take the
+// variant with the shorter critical path and more registers; the results are
+// the same bit for bit.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof = nullptr, int arith = kLuArithF32);
+               int* ipiv, LuProfile* prof = nullptr, int arith = kLuArithF32, int latency = 1);
 // Times the trailing-update tile shapes on the GEMM shapes of a factorisation
 // of padded order Np (once per device, precision and Np) using the Np x Np
 // scratch matrix Ar/Ai; returns the number of shape classes tuned.

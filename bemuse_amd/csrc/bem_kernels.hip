@@ -349,6 +349,90 @@ __device__ inline void cmma16(const T* ar_, const T* ai_, size_t lda, const T* b
     }
 }
 
+// The same product over 64 k (four steps of 16): all operand loads of the
+// four steps are issued before the first MFMA, so the chain waits for one
+// memory round trip instead of four (the panel kernels are latency-bound).
+// B's four k of a lane (4 g .. 4 g + 3 of a step) are one vector load:
+// callers keep b and ldb multiples of 4 elements.
+template <typename T>
+struct Ops64 {
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    T a_r[4][4], a_i[4][4];
+    v4 b_r[4], b_i[4];
+};
+
+template <typename T>
+__device__ inline void load64(Ops64<T>& o, const T* ar_, const T* ai_, size_t lda, const T* br_, const T* bi_,
+                              size_t ldb, int i, int g) {
+    typedef typename Ops64<T>::v4 v4;
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+        const size_t ob = (size_t)i * ldb + 16 * st + 4 * g;
+        o.b_r[st] = *reinterpret_cast<const v4*>(br_ + ob);
+        o.b_i[st] = *reinterpret_cast<const v4*>(bi_ + ob);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * st + 4 * g + t;
+            o.a_r[st][t] = ar_[(size_t)k * lda + i]; o.a_i[st][t] = ai_[(size_t)k * lda + i];
+        }
+    }
+}
+
+template <typename T>
+__device__ inline void mma64(const Ops64<T>& o, typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci) {
+    typedef MfmaT<T> MT;
+#pragma unroll
+    for (int st = 0; st < 4; st++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            cr = MT::mma(o.a_r[st][t], o.b_r[st][t], cr);
+            cr = MT::mma(-o.a_i[st][t], o.b_i[st][t], cr);
+            ci = MT::mma(o.a_r[st][t], o.b_i[st][t], ci);
+            ci = MT::mma(o.a_i[st][t], o.b_r[st][t], ci);
+        }
+}
+
+template <typename T>
+__device__ inline void cmma64(const T* ar_, const T* ai_, size_t lda, const T* br_, const T* bi_, size_t ldb,
+                              typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci, int i, int g) {
+    Ops64<T> o;
+    load64<T>(o, ar_, ai_, lda, br_, bi_, ldb, i, g);
+    mma64<T>(o, cr, ci);
+}
+
+// acc += A B over K (a multiple of 64) in chunks of 64; A advances by 64
+// columns (64 lda), B by 64 rows. PIPE (FP32 only): the next chunk's loads
+// are in flight while the current one feeds the MFMAs, two register stages
+// (FP64 would need twice the registers per stage). The sums are the same
+// either way, in the same order; PIPE shortens a lone factorisation's
+// critical path, but its 256 VGPRs per wave crowd the other lanes' work
+// when several frequencies share the device.
+template <typename T, bool PIPE>
+__device__ inline void cmma_k(const T* ar_, const T* ai_, size_t lda, const T* br_, const T* bi_, size_t ldb, int K,
+                              typename MfmaT<T>::acc& cr, typename MfmaT<T>::acc& ci, int i, int g) {
+    const int nch = K / LU_NB;
+    if constexpr (PIPE && sizeof(T) == 4) {
+        Ops64<T> x, y;
+        if (nch > 0) load64<T>(x, ar_, ai_, lda, br_, bi_, ldb, i, g);
+        for (int c = 0; c < nch; c += 2) {
+            if (c + 1 < nch)
+                load64<T>(y, ar_ + (size_t)(c + 1) * LU_NB * lda, ai_ + (size_t)(c + 1) * LU_NB * lda, lda,
+                          br_ + (c + 1) * LU_NB, bi_ + (c + 1) * LU_NB, ldb, i, g);
+            mma64<T>(x, cr, ci);
+            if (c + 1 < nch) {
+                if (c + 2 < nch)
+                    load64<T>(x, ar_ + (size_t)(c + 2) * LU_NB * lda, ai_ + (size_t)(c + 2) * LU_NB * lda, lda,
+                              br_ + (c + 2) * LU_NB, bi_ + (c + 2) * LU_NB, ldb, i, g);
+                mma64<T>(y, cr, ci);
+            }
+        }
+    } else {
+        for (int c = 0; c < nch; c++)
+            cmma64<T>(ar_ + (size_t)c * LU_NB * lda, ai_ + (size_t)c * LU_NB * lda, lda, br_ + c * LU_NB,
+                      bi_ + c * LU_NB, ldb, cr, ci, i, g);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Factor the NB x NB diagonal block at (k0, k0) without interchanges and form
 // the inverses of its unit-lower and upper triangles, blocked by DB = 16 so
@@ -673,41 +757,61 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 // waves read all NB columns of the rows, so they meet before overwriting.
 // Saves A21 (rows relative to k0+NB, ld Np) for the pivoting fallback and
 // flags multipliers with |l| > 1.
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
+// kp < k0: the panel is not the first of its group (whose first column is
+// kp), and its columns still lack the group's deferred update. The
+// work-group applies it to its own rows first,
+//   A(rows, k0..k0+NB) -= L(rows, kp..k0) U(kp..k0, k0..k0+NB),
+// keeps the updated rows in LDS, and saves and factors those.
+template <typename T, bool PIPE>
+__global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, int kp, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
                                                 T* save_r, T* save_i, int* flag) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
+    __shared__ T wr[LU_NB][16 + 1], wi[LU_NB][16 + 1];   // updated rows, [col][row]
     int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int rowl = blockIdx.x * 16;                          // relative to k0 + NB
     int row0 = k0 + LU_NB + rowl;
-    {
-        constexpr int PER = 16 * LU_NB / 256;            // loads first (see k_lu_diag)
-        T vr[PER], vi[PER];
-#pragma unroll
-        for (int n = 0; n < PER; n++) {
-            int t = tid + 256 * n, r = t % 16, c = t / 16;
-            size_t g = (size_t)(k0 + c) * Np + row0 + r;
-            vr[n] = Ar[g]; vi[n] = Ai[g];
-        }
-#pragma unroll
-        for (int n = 0; n < PER; n++) {
-            int t = tid + 256 * n, r = t % 16, c = t / 16;
-            save_r[(size_t)c * Np + rowl + r] = vr[n];
-            save_i[(size_t)c * Np + rowl + r] = vi[n];
-        }
-    }
     int i = lane & 15, g = lane >> 4, b = wave;
     acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
-    const T* A0r = Ar + (size_t)k0 * Np + row0;
-    const T* A0i = Ai + (size_t)k0 * Np + row0;
+    if (kp < k0) {
+        acc_t ur = acc_t{0, 0, 0, 0}, ui = acc_t{0, 0, 0, 0};
+        cmma_k<T, PIPE>(Ar + (size_t)kp * Np + row0, Ai + (size_t)kp * Np + row0, Np, Ar + (size_t)(k0 + 16 * b) * Np + kp,
+                  Ai + (size_t)(k0 + 16 * b) * Np + kp, Np, k0 - kp, ur, ui, i, g);
+        const int col = 16 * b + i;
 #pragma unroll
-    for (int kk = 0; kk < LU_NB; kk += 16)
-        cmma16<T>(A0r + (size_t)kk * Np, A0i + (size_t)kk * Np, Np, Ui_r + (16 * b) * LU_NB + kk,
-                  Ui_i + (16 * b) * LU_NB + kk, LU_NB, cr, ci, i, g);
-    __syncthreads();                                     // every wave has read the rows
+        for (int r = 0; r < 4; r++) {
+            int row = MT::row(g, r);
+            size_t o = (size_t)(k0 + col) * Np + row0 + row;
+            T vr = Ar[o] - ur[r], vi = Ai[o] - ui[r];
+            wr[col][row] = vr; wi[col][row] = vi;
+            save_r[(size_t)col * Np + rowl + row] = vr;
+            save_i[(size_t)col * Np + rowl + row] = vi;
+        }
+        __syncthreads();
+        cmma64<T>(&wr[0][0], &wi[0][0], 17, Ui_r + (16 * b) * LU_NB, Ui_i + (16 * b) * LU_NB, LU_NB, cr, ci, i, g);
+    } else {
+        {
+            constexpr int PER = 16 * LU_NB / 256;        // loads first (see k_lu_diag)
+            T vr[PER], vi[PER];
+#pragma unroll
+            for (int n = 0; n < PER; n++) {
+                int t = tid + 256 * n, r = t % 16, c = t / 16;
+                size_t o = (size_t)(k0 + c) * Np + row0 + r;
+                vr[n] = Ar[o]; vi[n] = Ai[o];
+            }
+#pragma unroll
+            for (int n = 0; n < PER; n++) {
+                int t = tid + 256 * n, r = t % 16, c = t / 16;
+                save_r[(size_t)c * Np + rowl + r] = vr[n];
+                save_i[(size_t)c * Np + rowl + r] = vi[n];
+            }
+        }
+        cmma64<T>(Ar + (size_t)k0 * Np + row0, Ai + (size_t)k0 * Np + row0, Np, Ui_r + (16 * b) * LU_NB,
+                  Ui_i + (16 * b) * LU_NB, LU_NB, cr, ci, i, g);
+        __syncthreads();                                 // every wave has read the rows
+    }
     int mybad = 0;
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -722,23 +826,45 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, T* Ar, T* Ai, co
 // U12 = Linv_p A12 in place for columns k0+NB .. Np-1. A work-group owns 16
 // columns and wave a rows 16 a .. 16 a + 15 of them; all waves read all NB
 // rows of the columns, so they meet before overwriting.
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, const T* Li_r, const T* Li_i,
-                                                float* umax) {
+// kp < k0: the panel is not the first of its group (first column kp); the
+// work-group first applies the group's deferred update to its columns of the
+// panel's rows, A(k0..k0+NB, cols) -= L(k0..k0+NB, kp..k0) U(kp..k0, cols),
+// in LDS.
+// next != 0: the next panel belongs to the same group. The four work-groups
+// whose columns are that panel's then apply the group's update, this panel's
+// included, to its diagonal block,
+//   A(k0+NB..k0+2NB, cols) -= L(k0+NB..k0+2NB, kp..k0+NB) U(kp..k0+NB, cols),
+// with this panel's U12 rows from LDS, so that the diagonal-block kernel of
+// the next panel can start at once.
+template <typename T, bool PIPE>
+__global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, int kp, int next, T* Ar, T* Ai, const T* Li_r,
+                                                const T* Li_i, float* umax) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
+    constexpr int SLD = LU_NB + 4;
+    __shared__ __align__(32) T sr[16][SLD], si[16][SLD]; // the columns' NB rows, [col][row] (vector reads)
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int col0 = k0 + LU_NB + blockIdx.x * 16;
     int i = lane & 15, g = lane >> 4, a = wave;
     acc_t cr = acc_t{0, 0, 0, 0}, ci = acc_t{0, 0, 0, 0};
-    const T* B0r = Ar + (size_t)col0 * Np + k0;
-    const T* B0i = Ai + (size_t)col0 * Np + k0;
+    if (kp < k0) {
+        acc_t ur = acc_t{0, 0, 0, 0}, ui = acc_t{0, 0, 0, 0};
+        cmma_k<T, PIPE>(Ar + (size_t)kp * Np + k0 + 16 * a, Ai + (size_t)kp * Np + k0 + 16 * a, Np, Ar + (size_t)col0 * Np + kp,
+                  Ai + (size_t)col0 * Np + kp, Np, k0 - kp, ur, ui, i, g);
 #pragma unroll
-    for (int kk = 0; kk < LU_NB; kk += 16)
-        cmma16<T>(Li_r + (size_t)kk * LU_NB + 16 * a, Li_i + (size_t)kk * LU_NB + 16 * a, LU_NB, B0r + kk,
-                  B0i + kk, Np, cr, ci, i, g);
+        for (int r = 0; r < 4; r++) {
+            int row = 16 * a + MT::row(g, r);
+            size_t o = (size_t)(col0 + i) * Np + k0 + row;
+            sr[i][row] = Ar[o] - ur[r]; si[i][row] = Ai[o] - ui[r];
+        }
+        __syncthreads();
+        cmma64<T>(Li_r + 16 * a, Li_i + 16 * a, LU_NB, &sr[0][0], &si[0][0], SLD, cr, ci, i, g);
+    } else {
+        cmma64<T>(Li_r + 16 * a, Li_i + 16 * a, LU_NB, Ar + (size_t)col0 * Np + k0, Ai + (size_t)col0 * Np + k0, Np,
+                  cr, ci, i, g);
+    }
     __syncthreads();                                     // every wave has read the columns
     float m = 0.0f;
 #pragma unroll
@@ -747,9 +873,36 @@ __global__ __launch_bounds__(256) void k_lu_u12(int Np, int k0, T* Ar, T* Ai, co
         Ar[o] = cr[r]; Ai[o] = ci[r];
         m = fmaxf(m, fmaxf(fabsf((float)cr[r]), fabsf((float)ci[r])));
     }
+    if (next && blockIdx.x < LU_NB / 16) {
+        // the next panel's diagonal block, these 16 of its columns
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int row = 16 * a + MT::row(g, r);
+            sr[i][row] = cr[r]; si[i][row] = ci[r];
+        }
+        __syncthreads();
+        const int rn = k0 + LU_NB + 16 * a;
+        acc_t dr = acc_t{0, 0, 0, 0}, di = acc_t{0, 0, 0, 0};
+        cmma_k<T, PIPE>(Ar + (size_t)kp * Np + rn, Ai + (size_t)kp * Np + rn, Np, Ar + (size_t)col0 * Np + kp,
+                  Ai + (size_t)col0 * Np + kp, Np, k0 - kp, dr, di, i, g);
+        cmma64<T>(Ar + (size_t)k0 * Np + rn, Ai + (size_t)k0 * Np + rn, Np, &sr[0][0], &si[0][0], SLD, dr, di, i, g);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            size_t o = (size_t)(col0 + i) * Np + rn + MT::row(g, r);
+            Ar[o] -= dr[r]; Ai[o] -= di[r];
+        }
+    }
     if (umax) {   // the panel's max |Re|, |Im| of U12 (non-negative floats order as their bits)
+        // one atomic per work-group: every wave's atomic on the one address
+        // serialised and doubled the kernel's duration
+        __shared__ float wmax[4];
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if (lane == 0) atomicMax(reinterpret_cast<unsigned*>(umax + k0 / LU_NB), __float_as_uint(m));
+        if (lane == 0) wmax[wave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            atomicMax(reinterpret_cast<unsigned*>(umax + k0 / LU_NB), __float_as_uint(m));
+        }
     }
 }
 
@@ -2357,13 +2510,15 @@ template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 // panel stream s2, every column right of it up to date with the groups
 // before):
 //   for j = 0 .. g-1, panel q = p + j at kj = k0 + j NB, on the main stream:
-//     (j > 0) the K = j NB update of panel q's columns by the group's panels
-//             before it, [L21_p .. L21_{q-1}] [U12 ...], then factor panel q
-//             (diagonal block, L21): nothing else could run beside them, and
-//             a hand-off between streams costs about 10 us each way;
+//     (j > 0) factor panel q (diagonal block; L21 after the K = j NB update
+//             of its rows by the group's panels before it, [L21_p ..
+//             L21_{q-1}] [U12 ...]): nothing else could run beside them, and a
+//             hand-off between streams costs about 10 us each way;
 //     fix q (pivot check / fallback + row swaps elsewhere);
-//     (j > 0) the K = j NB update of panel q's rows right of it;
-//     U12_q = Linv_q A12;
+//     U12_q = Linv_q A12 after the K = j NB update of panel q's rows right of
+//     it, and (j + 1 < g) the K = (j+1) NB update of panel q+1's diagonal
+//     block. These thin updates ride in the panel kernels that consume them
+//     instead of taking GEMM launches of their own on the critical path;
 //   main stream: the K = g NB update of the next group's first panel, which
 //   the panel stream then factors while the main stream applies the same
 //   K = g NB update to every column right of it.
@@ -2374,22 +2529,25 @@ template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 // ev[2 q], ev[2 q + 1] (q the first panel of a group) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof, int arith) {
+               int* ipiv, LuProfile* prof, int arith, int latency) {
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
     const bool split = sizeof(T) == 4 && arith == kLuArithSplitF16 && w.umax;
     float* umax = split ? w.umax : nullptr;
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv, umax);
-    auto panel = [&](hipStream_t st, int p) {
+    // kp: first column of the panel's group (kp < k0: the group's deferred
+    // update of the panel's rows below the diagonal block is applied by L21)
+    auto panel = [&](hipStream_t st, int p, int kp) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
         T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
         T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
         hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, w.sd_r, w.sd_i, Lir, Lii, Uir,
                            Uii, flag);
+        auto l21k = latency ? k_lu_l21<T, true> : k_lu_l21<T, false>;
         if (rest > 0)
-            hipLaunchKernelGGL(k_lu_l21<T>, dim3(rest / 16), dim3(256), 0, st, Np, k0, Ar, Ai, Uir, Uii, w.s21_r,
-                               w.s21_i, flag);
+            hipLaunchKernelGGL(l21k, dim3(rest / 16), dim3(256), 0, st, Np, k0, kp, Ar, Ai, Uir, Uii, w.s21_r, w.s21_i,
+                               flag);
     };
     auto fix = [&](int p) {      // pivot check and fallback of panel p, row swaps elsewhere
         int k0 = p * LU_NB;
@@ -2398,28 +2556,31 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         hipLaunchKernelGGL(k_lu_panel_gepp<T>, dim3(1), dim3(1024), 0, s, Np, k0, Ar, Ai, w.sd_r, w.sd_i, w.s21_r,
                            w.s21_i, Lir, Lii, Uir, Uii, flag, ipiv);
     };
-    auto u12 = [&](int p) {
+    // kp as for panel; next: the next panel is in the same group (its
+    // diagonal block takes the group's update here)
+    auto u12 = [&](int p, int kp, int next) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
+        auto u12k = latency ? k_lu_u12<T, true> : k_lu_u12<T, false>;
         if (rest > 0)
-            hipLaunchKernelGGL(k_lu_u12<T>, dim3(rest / 16), dim3(256), 0, s, Np, k0, Ar, Ai,
+            hipLaunchKernelGGL(u12k, dim3(rest / 16), dim3(256), 0, s, Np, k0, kp, next, Ar, Ai,
                                w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2, umax);
     };
     const int NB = LU_NB, group = lu_group<T>(Np, split);
-    panel(s, 0);
+    panel(s, 0, 0);
     for (int p = 0; p < npanel; p += group) {
         const int g = std::min(group, npanel - p), k0 = p * NB;
         if (p > 0) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);   // panel p factored on s2
         for (int j = 0; j < g; j++) {
-            const int q = p + j, kj = k0 + j * NB;
-            if (j > 0) {
-                // panel q's columns: A(kj.., q) -= L21(kj.., p..q-1) U12(p..q-1, q)
-                gemm_sub<T>(s, dev, Np, Np - kj, NB, j * NB, Ar, Ai, kj, k0, k0, kj, kj, kj, prof, 1, umax);
-                panel(s, q);   // nothing to overlap: same stream, no cross-stream hand-off
-            }
+            const int q = p + j;
+            // j > 0: the group's K = j NB update of panel q's columns was applied
+            // to its diagonal block by U12 of panel q-1 and is applied to the
+            // rows below by L21 of panel q; the main stream factors it (nothing
+            // to overlap: no cross-stream hand-off)
+            if (j > 0) panel(s, q, k0);
             fix(q);
-            if (j > 0 && Np - kj - NB > 0)   // panel q's rows right of it
-                gemm_sub<T>(s, dev, Np, NB, Np - kj - NB, j * NB, Ar, Ai, kj, k0, k0, kj + NB, kj, kj + NB, prof, 1, umax);
-            u12(q);
+            // U12 of panel q after the K = j NB update of its rows, and the
+            // K = (j+1) NB update of panel q+1's diagonal block
+            u12(q, k0, j + 1 < g);
         }
         const int kg = k0 + g * NB, restg = Np - kg;
         if (restg <= 0) break;
@@ -2427,7 +2588,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax);
         (void)hipEventRecord(ev[2 * (p + g)], s);
         (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
-        panel(s2, p + g);
+        panel(s2, p + g, kg);
         (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
         if (restg > NB)   // trailing update of everything right of it
             gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax);
@@ -2437,9 +2598,9 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                        w.Uinv_i, w.PL_r, w.PL_i, w.PU_r, w.PU_i);
 }
 template void launch_lu<double>(hipStream_t, hipStream_t, hipEvent_t*, int, double*, double*, const LuWork<double>&,
-                                 int*, int*, LuProfile*, int);
+                                 int*, int*, LuProfile*, int, int);
 template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float*, float*, const LuWork<float>&, int*,
-                                int*, LuProfile*, int);
+                                int*, LuProfile*, int, int);
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,

@@ -90,6 +90,20 @@ def test_cylinder1024_all_frequencies(cyl1024, mode):
     assert relerr(g["phiS"], refs[7]["phiS"]) < 10 * TIGHT[mode]
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_cylinder1024_lane_count_does_not_change_results(cyl1024, mode):
+    """One lane factors with the latency variant of the panel kernels (two
+    register stages of operand loads in FP32), several lanes with the
+    throughput variant; both add the same products in the same order, so the
+    results agree bit for bit."""
+    body, ob, omegas, refs = cyl1024
+    w = omegas[2:6]
+    one = body.solve_sweep(w, betas=[0.0], mode=mode, lanes=1)
+    two = body.solve_sweep(w, betas=[0.0], mode=mode, lanes=2)
+    for key in ("A", "B", "F", "FK", "SC"):
+        np.testing.assert_array_equal(one[key], two[key], err_msg=key)
+
+
 # ---------------------------------------------------------------------------
 # 6018-panel semi-sub, 40 frequencies, 5 headings
 # ---------------------------------------------------------------------------
