@@ -80,6 +80,7 @@ class SurrogateConfig(ctypes.Structure):
         ("lr_final", c_double),
         ("n_valid", ctypes.c_int64),
         ("sampling", c_int),
+        ("steps_per_launch", c_int),
     ]
 
 

@@ -1501,6 +1501,7 @@ int bem_surrogate_default_config(bem_surrogate_config* cfg) {
     cfg->lr_final = 2e-5;
     cfg->n_valid = 1000000;
     cfg->sampling = 1;
+    cfg->steps_per_launch = 0;
     return BEM_OK;
 }
 
@@ -1523,9 +1524,10 @@ int bem_surrogate_train(bem_ctx* c, const bem_surrogate_config* cfg, bem_surroga
     if (!c || !cfg) return fail(BEM_ERR_ARG, "bem_surrogate_train: null argument");
     const long long n = cfg->n_samples, nv = cfg->n_valid;
     if (n < 1 || n > (1LL << 30) || cfg->batch < 1 || cfg->batch > (1 << 20) || cfg->steps < 0 || nv < 0 ||
-        nv > (1LL << 26) || !(cfg->lr >= 0.0) || !(cfg->lr_final >= 0.0) || cfg->sampling < 0 || cfg->sampling > 1)
+        nv > (1LL << 26) || !(cfg->lr >= 0.0) || !(cfg->lr_final >= 0.0) || cfg->sampling < 0 || cfg->sampling > 1 ||
+        cfg->steps_per_launch < 0)
         return fail(BEM_ERR_ARG, "bem_surrogate_train: n_samples in [1, 2^30], batch in [1, 2^20], steps >= 0, "
-                                 "n_valid in [0, 2^26], learning rates >= 0, sampling 0 or 1");
+                                 "n_valid in [0, 2^26], learning rates >= 0, sampling 0 or 1, steps_per_launch >= 0");
     HIP_TRY(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const int nwg = sur_grad_groups(cfg->batch);
@@ -1560,18 +1562,47 @@ int bem_surrogate_train(bem_ctx* c, const bem_surrogate_config* cfg, bem_surroga
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     out.ms_sample = ms;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    HIP_TRY(hipEventRecord(e0, st));
+    // step t's learning rate (cosine decay) with Adam's bias correction
+    std::vector<float> lrt(std::max(cfg->steps, 1));
     for (int t = 0; t < cfg->steps; t++) {
         const double lr = cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(M_PI * t / cfg->steps));
-        const double lr_t = lr * std::sqrt(1.0 - std::pow((double)b2, t + 1)) / (1.0 - std::pow((double)b1, t + 1));
-        launch_sur_grad(st, P, X, T, n, (long long)t * cfg->batch, cfg->batch, perm, gpart, lpart);
-        launch_sur_adam(st, nwg, gpart, lpart, P, m, v, nullptr, loss + t, (float)lr_t, b1, b2, eps);
+        lrt[t] = (float)(lr * std::sqrt(1.0 - std::pow((double)b2, t + 1)) / (1.0 - std::pow((double)b1, t + 1)));
+    }
+    // several steps per dispatch: one cooperative launch holds every
+    // work-group of the batch and meets at a grid barrier twice per step
+    const bool coop = cfg->steps_per_launch != 1 && cfg->steps > 0 && sur_train_coop_capacity(c->device) >= nwg;
+    const int chunk = cfg->steps_per_launch > 1 ? cfg->steps_per_launch : kSurStepsPerLaunch;
+    float* lr_tab = nullptr;
+    unsigned* bar = nullptr;
+    int* berr = nullptr;
+    if (coop) {
+        if ((r = B.get(&lr_tab, (size_t)cfg->steps)) || (r = B.get(&bar, 64)) || (r = B.get(&berr, 1))) return r;
+        HIP_TRY(hipMemcpyAsync(lr_tab, lrt.data(), sizeof(float) * cfg->steps, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemsetAsync(bar, 0, sizeof(unsigned) * 64, st));
+        HIP_TRY(hipMemsetAsync(berr, 0, sizeof(int), st));
+    }
+    HIP_TRY(hipEventRecord(e0, st));
+    if (coop) {
+        for (int t0 = 0; t0 < cfg->steps; t0 += chunk)
+            if (launch_sur_train(st, nwg, t0, std::min(chunk, cfg->steps - t0), lr_tab, b1, b2, eps, X, T, n,
+                                 cfg->batch, perm, P, m, v, gpart, lpart, loss, bar, berr))
+                return fail(BEM_ERR_HIP, "bem_surrogate_train: cooperative launch refused");
+    } else {
+        for (int t = 0; t < cfg->steps; t++) {
+            launch_sur_grad(st, P, X, T, n, (long long)t * cfg->batch, cfg->batch, perm, gpart, lpart);
+            launch_sur_adam(st, nwg, gpart, lpart, P, m, v, nullptr, loss + t, lrt[t], b1, b2, eps);
+        }
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     out.ms_train = ms;
+    if (coop) {
+        int berr_h = 0;
+        HIP_TRY(hipMemcpy(&berr_h, berr, sizeof(int), hipMemcpyDeviceToHost));
+        if (berr_h) return fail(BEM_ERR_HIP, "bem_surrogate_train: a grid barrier of the training kernel timed out");
+    }
     if (cfg->steps > 0) {
         std::vector<double> lh(cfg->steps);
         HIP_TRY(hipMemcpy(lh.data(), loss, sizeof(double) * cfg->steps, hipMemcpyDeviceToHost));

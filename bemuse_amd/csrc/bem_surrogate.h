@@ -37,6 +37,7 @@ constexpr int kSurW4 = kSurB3 + kSurHid, kSurB4 = kSurW4 + kSurHid * kSurHid;
 constexpr int kSurW5 = kSurB4 + kSurHid, kSurB5 = kSurW5 + kSurOut * kSurHid;
 constexpr int kSurParams = kSurB5 + kSurOut;
 constexpr int kSurTile = 32;        // samples per work-group in training
+constexpr int kSurStepsPerLaunch = 500;   // Adam steps per cooperative training launch (default)
 constexpr int kSurJudgeStride = 61; // one pair in ~61 is also evaluated exactly during assembly
 
 __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
@@ -110,6 +111,17 @@ int sur_grad_groups(int batch);
 // squared error is stored in loss[0].
 void launch_sur_adam(hipStream_t s, int nwg, const float* gpart, const float* lpart, float* params, float* m,
                      float* v, float* grad, double* loss, float lr_t, float beta1, float beta2, float eps);
+// Training steps t0 .. t0 + nsteps - 1 in one cooperative launch of nwg
+// work-groups (gradient, grid barrier, Adam step, grid barrier per step;
+// lr_tab[t]: step t's learning rate with the bias correction). bar: two
+// counters at bar[0] and bar[32], zero before the first launch; *err is set
+// if a barrier wait times out. Returns 0, or -1 if the launch was refused.
+int launch_sur_train(hipStream_t s, int nwg, int t0, int nsteps, const float* lr_tab, float beta1, float beta2,
+                     float eps, const float* X, const float* T, long long pool, int batch,
+                     const unsigned long long* perm, float* params, float* m, float* v, float* gpart, float* lpart,
+                     double* loss, unsigned* bar, int* err);
+// Work-groups of k_sur_train that the device holds at once (0: no cooperative launch).
+int sur_train_coop_capacity(int device);
 // Network outputs y (n x 2) for features X (n x 4).
 void launch_sur_forward(hipStream_t s, const float* params, long long n, const float* X, float* Y);
 // Reconstructed GF, GFh (n x 4 doubles: Re GF, Im GF, Re GFh, Im GFh) at (h, v);

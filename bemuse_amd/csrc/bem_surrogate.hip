@@ -244,19 +244,30 @@ __global__ void k_sur_sample(uint64_t seed, int sampling, long long first, long 
 // Training: forward and backward of one 32-sample tile per work-group. The
 // loss is sum over samples and channels of (y - t)^2 / (kSurOut batch).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sur_grad(const float* __restrict__ P, const float* __restrict__ X,
-                                                  const float* __restrict__ T, long long pool, long long p0,
-                                                  int batch, const unsigned long long* __restrict__ perm,
-                                                  float dscale, float* __restrict__ gpart,
-                                                  float* __restrict__ lpart) {
+struct GradLds {
+    SmallLayers L;
+    float sx[kSurTile][4], st[kSurTile][kSurOut], sdy[kSurTile][kSurOut];
+    __align__(16) float act[4][kSurTile * LDA];
+    float sred[256];
+};
+
+// Tile `tile` of the batch starting at pool position p0: writes its gradient
+// partial (kSurParams floats at gpart + tile kSurParams) and its loss
+// partial lpart[tile].
+__device__ void sur_grad_tile(GradLds& S, int tile, const float* __restrict__ P, const float* __restrict__ X,
+                              const float* __restrict__ T, long long pool, long long p0, int batch,
+                              const unsigned long long* __restrict__ perm, float dscale, float* __restrict__ gpart,
+                              float* __restrict__ lpart) {
     constexpr int M = kSurTile;
-    __shared__ SmallLayers L;
-    __shared__ float sx[M][4], st[M][kSurOut], sdy[M][kSurOut];
-    __shared__ __align__(16) float act[4][M * LDA];
-    __shared__ float sred[256];
+    SmallLayers& L = S.L;
+    auto& sx = S.sx;
+    auto& st = S.st;
+    auto& sdy = S.sdy;
+    auto& act = S.act;
+    auto& sred = S.sred;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int j0 = blockIdx.x * M;
-    float* gp = gpart + (size_t)blockIdx.x * kSurParams;
+    const int j0 = tile * M;
+    float* gp = gpart + (size_t)tile * kSurParams;
     stage_small(P, L, tid);
     if (tid < M) {
         const int j = j0 + tid;
@@ -312,7 +323,7 @@ __global__ __launch_bounds__(256) void k_sur_grad(const float* __restrict__ P, c
     if (tid == 0) {
         float s = 0.0f;
         for (int k = 0; k < 256; k++) s += sred[k];
-        lpart[blockIdx.x] = s;
+        lpart[tile] = s;
     }
     __syncthreads();
     for (int k = tid; k < M * kSurHid; k += 256) {
@@ -340,20 +351,32 @@ __global__ __launch_bounds__(256) void k_sur_grad(const float* __restrict__ P, c
     }
 }
 
-// Sum of the work-group partials (fixed order, FP64) and one Adam step.
-__global__ void k_sur_adam(int nwg, const float* __restrict__ gpart, const float* __restrict__ lpart,
-                           float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
-                           float* __restrict__ grad, double* __restrict__ loss, float lr_t, float beta1,
-                           float beta2, float eps) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && loss) {
-        double s = 0.0;
-        for (int w = 0; w < nwg; w++) s += lpart[w];
-        loss[0] = s;
+__global__ __launch_bounds__(256) void k_sur_grad(const float* __restrict__ P, const float* __restrict__ X,
+                                                  const float* __restrict__ T, long long pool, long long p0,
+                                                  int batch, const unsigned long long* __restrict__ perm,
+                                                  float dscale, float* __restrict__ gpart,
+                                                  float* __restrict__ lpart) {
+    __shared__ GradLds S;
+    sur_grad_tile(S, blockIdx.x, P, X, T, pool, p0, batch, perm, dscale, gpart, lpart);
+}
+
+// Parameter p: the sum of the nwg work-group partials (FP64, a fixed order:
+// eight interleaved running sums over w = j mod 8, then combined pairwise, so
+// that eight loads are in flight instead of one chain) and one Adam step.
+__device__ inline void sur_adam_param(int p, int nwg, const float* __restrict__ gpart, float* __restrict__ P,
+                                      float* __restrict__ m, float* __restrict__ v, float* __restrict__ grad,
+                                      float lr_t, float beta1, float beta2, float eps) {
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int w = 0;
+    for (; w + 8 <= nwg; w += 8) {
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) x[j] = gpart[(size_t)(w + j) * kSurParams + p];
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] += x[j];
     }
-    if (p >= kSurParams) return;
-    double gs = 0.0;
-    for (int w = 0; w < nwg; w++) gs += gpart[(size_t)w * kSurParams + p];
+    for (int j = 0; w < nwg; w++, j++) acc[j] += gpart[(size_t)w * kSurParams + p];
+    const double gs = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     const float g = (float)gs;
     if (grad) grad[p] = g;
     if (m) {
@@ -362,6 +385,89 @@ __global__ void k_sur_adam(int nwg, const float* __restrict__ gpart, const float
         m[p] = mm;
         v[p] = vv;
         P[p] -= lr_t * mm / (sqrtf(vv) + eps);
+    }
+}
+
+__device__ inline double sur_loss_sum(int nwg, const float* __restrict__ lpart) {
+    double s = 0.0;
+    for (int w = 0; w < nwg; w++) s += lpart[w];
+    return s;
+}
+
+__global__ void k_sur_adam(int nwg, const float* __restrict__ gpart, const float* __restrict__ lpart,
+                           float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                           float* __restrict__ grad, double* __restrict__ loss, float lr_t, float beta1,
+                           float beta2, float eps) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && loss) loss[0] = sur_loss_sum(nwg, lpart);
+    if (p >= kSurParams) return;
+    sur_adam_param(p, nwg, gpart, P, m, v, grad, lr_t, beta1, beta2, eps);
+}
+
+// Barrier of all work-groups of a cooperative launch (every work-group is
+// resident, which the cooperative launch guarantees or refuses). bar[0]
+// counts arrivals, bar[kBarGen] is the generation the last arrival advances.
+// The wait is bounded: after kBarSpins polls the work-group sets *err and
+// gives up, so a broken barrier ends the kernel instead of hanging it.
+constexpr int kBarGen = 32;                 // the generation word on its own 128-byte line
+constexpr long long kBarSpins = 1LL << 21;  // about a second of polling; a step takes ~0.1 ms
+
+__device__ bool grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ok = 1;
+        unsigned* count = bar;
+        unsigned* gen = bar + kBarGen;
+        const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // release: this work-group's writes (and the read of g) before its arrival
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived = atomicAdd(count, 1u);
+        if (arrived == nblocks - 1) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // acquire the arrivals' writes, release them (and the reset) with the new generation
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            atomicAdd(gen, 1u);
+        } else {
+            long long spins = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kBarSpins) {
+                    atomicOr(err, 1);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        // acquire: the other work-groups' writes after the wait
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// Steps t0 .. t0 + nsteps - 1 of the training in one cooperative launch of
+// one work-group per 32-sample tile: each step is the gradient of every tile,
+// a barrier, the Adam step of all parameters spread over the work-groups
+// (the same fixed-order sums as k_sur_adam, so the parameters equal those of
+// one gradient and one Adam dispatch per step, bit for bit), and a barrier.
+__global__ __launch_bounds__(256) void k_sur_train(int t0, int nsteps, const float* __restrict__ lr_tab,
+                                                   float beta1, float beta2, float eps, const float* __restrict__ X,
+                                                   const float* __restrict__ T, long long pool, int batch,
+                                                   const unsigned long long* __restrict__ perm, float dscale,
+                                                   float* P, float* m, float* v, float* gpart, float* lpart,
+                                                   double* loss, unsigned* bar, int* err) {
+    __shared__ GradLds S;
+    const int nwg = gridDim.x;
+    for (int s = 0; s < nsteps; s++) {
+        const int t = t0 + s;
+        sur_grad_tile(S, blockIdx.x, P, X, T, pool, (long long)t * batch, batch, perm, dscale, gpart, lpart);
+        if (!grid_barrier(bar, nwg, err)) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) loss[t] = sur_loss_sum(nwg, lpart);
+        const float lr_t = lr_tab[t];
+        for (int p = blockIdx.x * 256 + threadIdx.x; p < kSurParams; p += nwg * 256)
+            sur_adam_param(p, nwg, gpart, P, m, v, nullptr, lr_t, beta1, beta2, eps);
+        if (!grid_barrier(bar, nwg, err)) return;
     }
 }
 
@@ -591,6 +697,29 @@ void launch_sur_grad(hipStream_t s, const float* params, const float* X, const f
     const float dscale = 2.0f / ((float)kSurOut * (float)batch);
     hipLaunchKernelGGL(k_sur_grad, dim3(sur_grad_groups(batch)), dim3(256), 0, s, params, X, T, pool, p0, batch,
                        perm, dscale, gpart, lpart);
+}
+
+int launch_sur_train(hipStream_t s, int nwg, int t0, int nsteps, const float* lr_tab, float beta1, float beta2,
+                     float eps, const float* X, const float* T, long long pool, int batch,
+                     const unsigned long long* perm, float* params, float* m, float* v, float* gpart, float* lpart,
+                     double* loss, unsigned* bar, int* err) {
+    float dscale = 2.0f / ((float)kSurOut * (float)batch);
+    void* args[] = {&t0, &nsteps, &lr_tab, &beta1, &beta2, &eps, &X, &T, &pool, &batch, &perm, &dscale,
+                    &params, &m, &v, &gpart, &lpart, &loss, &bar, &err};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_sur_train), dim3(nwg), dim3(256), args, 0, s) ==
+                   hipSuccess
+               ? 0
+               : -1;
+}
+
+int sur_train_coop_capacity(int device) {
+    int coop = 0, cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess || !coop) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_sur_train), 256, 0) !=
+        hipSuccess)
+        return 0;
+    return per * cus;
 }
 
 void launch_sur_adam(hipStream_t s, int nwg, const float* gpart, const float* lpart, float* params, float* m,

@@ -359,6 +359,11 @@ typedef struct bem_surrogate_config {
     int64_t n_valid;     /* held-out samples (indices after the training pool) judged exactly */
     int sampling;        /* 0: uniform in (K R, K (z + zeta)); 1: uniform in the log-scaled
                             features log1p(K R), log1p(-K (z + zeta)) (default) */
+    int steps_per_launch; /* 0 (default): up to 500 Adam steps per dispatch, one cooperative
+                             launch with a grid barrier between gradient and update, when the
+                             device holds all of the batch's work-groups at once (else as 1);
+                             1: a gradient and an Adam dispatch per step; n > 1: n steps per
+                             cooperative launch. The parameters are the same bit for bit. */
 } bem_surrogate_config;
 
 typedef struct bem_surrogate_report {
@@ -375,7 +380,8 @@ typedef struct bem_surrogate_report {
 } bem_surrogate_report;
 
 /* Defaults: seed 20261203, 20,000,000 samples, batch 8192, 24,000 steps,
- * lr 2e-3 -> 2e-5, 1,000,000 validation samples, log-scaled sampling. */
+ * lr 2e-3 -> 2e-5, 1,000,000 validation samples, log-scaled sampling,
+ * steps_per_launch 0. */
 int bem_surrogate_default_config(bem_surrogate_config* cfg);
 /* Number of network parameters (floats). */
 int bem_surrogate_param_count(void);

@@ -310,3 +310,24 @@ def test_surrogate_solve_gain(sctx, trained):
     assert all(np.isfinite(v) for v in e_tr.values())
     assert max(e_tr.values()) < 0.1 * max(e_un.values())
     assert judged < 10 * trained["rel_max"] + 1e-6
+
+
+@pytest.mark.gpu
+def test_training_steps_per_launch_bitwise(sctx, trained):
+    """Several Adam steps per dispatch (one cooperative launch with a grid
+    barrier between the gradient and the update of each step) give the
+    parameters of one gradient and one Adam dispatch per step, bit for bit:
+    the sums run in the same fixed order. Chunks of 128 steps cross launch
+    boundaries mid-training."""
+    cfg = dict(n_samples=200_000, steps=300, batch=8192, n_valid=1000)
+    reps, params = [], []
+    for spl in (1, 0, 128):
+        rep = sctx.surrogate_train(steps_per_launch=spl, **cfg)
+        reps.append(rep)
+        params.append(sctx.surrogate_get())
+    print(f"\n[gpu] training ms, per-step dispatches vs cooperative: "
+          f"{reps[0]['ms_train']:.1f} / {reps[1]['ms_train']:.1f} / {reps[2]['ms_train']:.1f}", flush=True)
+    np.testing.assert_array_equal(params[0], params[1])
+    np.testing.assert_array_equal(params[0], params[2])
+    assert reps[0]["loss_final"] == reps[1]["loss_final"] == reps[2]["loss_final"]
+    sctx.surrogate_set(trained["params"])          # the module's network for the tests after this one
