@@ -49,7 +49,7 @@ class BemError(RuntimeError):
 
 ERR_ARG, ERR_HIP, ERR_NOMEM, ERR_NUMERIC, ERR_STATE = 1, 2, 3, 4, 5
 NOALLOC_MAX_N = 512     # BEM_NOALLOC_MAX_N
-LU_FP32, LU_SPLIT_F16 = 0, 1
+LU_FP32, LU_SPLIT_F16, LU_SPLIT_F16_TILE = 0, 1, 2
 
 
 class SolveInfo(ctypes.Structure):
@@ -439,7 +439,8 @@ class Context:
         _check(lib().bem_set_refine_limit(self._h, int(max_steps)))
 
     def set_lu_arith(self, arith):
-        """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default) or LU_FP32."""
+        """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default), LU_FP32, or
+        LU_SPLIT_F16_TILE (the default's arithmetic with the split in every tile)."""
         _check(lib().bem_set_lu_arith(self._h, int(arith)))
 
     def set_graphs(self, enable):

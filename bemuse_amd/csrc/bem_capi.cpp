@@ -153,6 +153,9 @@ struct LinSys {
             (r = get(&w.PL_i, pan)) || (r = get(&w.PU_r, pan)) || (r = get(&w.PU_i, pan)))
             return r;
         if (sizeof(T) == 4 && (r = get(&w.umax, (size_t)capNp / kLuNb))) return r;
+        if (sizeof(T) == 4 && ((r = get(&w.opA, (size_t)4 * capNp * 4 * kLuNb)) ||
+                               (r = get(&w.opB, (size_t)4 * capNp * 4 * kLuNb))))
+            return r;
         return BEM_OK;
     }
     int init(int n) {
@@ -1114,9 +1117,9 @@ int bem_set_refine_limit(bem_ctx* c, int max_steps) {
 }
 
 int bem_set_lu_arith(bem_ctx* c, int arith) {
-    if (!c || (arith != BEM_LU_FP32 && arith != BEM_LU_SPLIT_F16))
-        return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32 or BEM_LU_SPLIT_F16");
-    c->lu_arith = arith == BEM_LU_FP32 ? kLuArithF32 : kLuArithSplitF16;
+    if (!c || (arith != BEM_LU_FP32 && arith != BEM_LU_SPLIT_F16 && arith != BEM_LU_SPLIT_F16_TILE))
+        return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32, BEM_LU_SPLIT_F16 or BEM_LU_SPLIT_F16_TILE");
+    c->lu_arith = arith == BEM_LU_FP32 ? kLuArithF32 : arith == BEM_LU_SPLIT_F16 ? kLuArithSplitF16 : kLuArithSplitF16Tile;
     return BEM_OK;
 }
 

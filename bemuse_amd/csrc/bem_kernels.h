@@ -44,9 +44,11 @@ struct LuProfile {
     // every timed launch, as issued (operand pointers, offsets, tile choice),
     // for replay_lu_gemms
     struct Call {
-        int dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, split;
+        int dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, split;   // split 2: operands split beforehand
         void *Ar, *Ai;
         const float* umax;
+        const _Float16 *AS, *BS;
+        int nb0, ldn;
     };
     Call call[kMax] = {};
 };
@@ -67,14 +69,19 @@ struct LuWork {
     T *Linv_r = nullptr, *Linv_i = nullptr, *Uinv_r = nullptr, *Uinv_i = nullptr;
     T *PL_r = nullptr, *PL_i = nullptr, *PU_r = nullptr, *PU_i = nullptr;   // block-solve coefficients
     float* umax = nullptr;   // per panel: max |Re|, |Im| of its U12 rows (FP32 split-arithmetic updates)
+    // FP32 split arithmetic: a panel group's update operands split into fp16
+    // once (4 parts x Np x 4 NB halves each); null: split in every tile
+    _Float16 *opA = nullptr, *opB = nullptr;
 };
 
 // Arithmetic of the FP32 factorisation's trailing updates.
-enum { kLuArithF32 = 0, kLuArithSplitF16 = 1 };
+enum { kLuArithF32 = 0, kLuArithSplitF16 = 1, kLuArithSplitF16Tile = 2 };
 
 // s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
 // arith (FP32 only): kLuArithF32 (FP32 MFMA) or kLuArithSplitF16 (fp16 MFMA on
-// split operands, w.umax required). latency != 0: the factorisation has the
+// split operands, w.umax required; operands split once per panel group when
+// w.opA, w.opB are there) or kLuArithSplitF16Tile (the same, split in every
+// tile). latency != 0: the factorisation has the
 // device to itself (one frequency in flight), so the panel kernels take the
 // variant with the shorter critical path and more registers; the results are
 // the same bit for bit.

@@ -1495,6 +1495,260 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_h2(int M, int N, int K, fl
 }
 
 // ---------------------------------------------------------------------------
+// Operands of a panel group's K = g NB updates in fp16 split form, made once
+// per group instead of in every tile of every update that reads them (the
+// split's VALU work had made the update kernel VALU-bound). For the group's
+// rows kg.. of its L21 columns (A) and columns kg.. of its U12 rows (B):
+//   AS[p][c][m][i] = part p of -2^12 A(ra + m, ca + 16 c + i),   m < M
+//   BS[p][c][n][i] = part p of 2^(10 - e) A(rb + 16 c + i, cb + n), n < N
+// with parts Re h, Re l, Im h, Im l (h = fp16(x s), l = fp16(x s - h)) and 2^e
+// above the largest |Re|, |Im| of the K / NB panels' U12 rows, the scales of
+// k_cgemm_sub_h2. Chunk-major (16 k per chunk) so that the update kernel's
+// 16-byte loads of one chunk are contiguous. The staged values equal those
+// k_cgemm_sub_h2 forms, so k_cgemm_sub_hs computes the same sums.
+// Grid: cdiv(M, 64) K/64 tiles of A, then cdiv(N, 64) K/64 tiles of B.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_split_ops(int M, int N, int K, const float* __restrict__ Pr,
+                                                   const float* __restrict__ Pi, int ld, int ra, int ca, int rb,
+                                                   int cb, const float* __restrict__ umax, _Float16* __restrict__ AS,
+                                                   _Float16* __restrict__ BS) {
+    __shared__ float tr[LU_NB][LU_NB + 1], ti[LU_NB][LU_NB + 1];   // A tile, [k][m]
+    const int tid = threadIdx.x;
+    const int kt = K / LU_NB, ta = (M + LU_NB - 1) / LU_NB * kt;
+    float um = 0.0f;
+    for (int q = 0; q < kt; q++) um = fmaxf(um, umax[rb / LU_NB + q]);
+    int eb = 0;
+    if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);
+    auto split16 = [](const float* x, float s, h8_t* hi, h8_t* lo) {
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            float v = x[t] * s;
+            _Float16 h = (_Float16)v;
+            hi[t / 8][t % 8] = h;
+            lo[t / 8][t % 8] = (_Float16)(v - (float)h);
+        }
+    };
+    const int kq = tid / 64, j = tid % 64;   // output: chunk kq of the tile, row (column) j
+    if ((int)blockIdx.x < ta) {
+        const int m0 = (blockIdx.x / kt) * LU_NB, k0 = (blockIdx.x % kt) * LU_NB;
+        for (int e = tid; e < LU_NB * LU_NB; e += 256) {
+            int m = e % LU_NB, k = e / LU_NB;
+            int mm = min(m0 + m, M - 1);
+            size_t o = (size_t)(ca + k0 + k) * ld + ra + mm;
+            tr[k][m] = Pr[o]; ti[k][m] = Pi[o];
+        }
+        __syncthreads();
+        if (m0 + j >= M) return;
+        const float sa = -(float)(1 << kSplitExpA);
+        float xr[16], xi[16];
+        for (int t = 0; t < 16; t++) { xr[t] = tr[16 * kq + t][j]; xi[t] = ti[16 * kq + t][j]; }
+        h8_t h[2], l[2];
+        const size_t pa = (size_t)M * K, o = ((size_t)(k0 / HK + kq) * M + m0 + j) * HK;
+        split16(xr, sa, h, l);
+        *reinterpret_cast<h8_t*>(AS + o) = h[0]; *reinterpret_cast<h8_t*>(AS + o + 8) = h[1];
+        *reinterpret_cast<h8_t*>(AS + pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + pa + o + 8) = l[1];
+        split16(xi, sa, h, l);
+        *reinterpret_cast<h8_t*>(AS + 2 * pa + o) = h[0]; *reinterpret_cast<h8_t*>(AS + 2 * pa + o + 8) = h[1];
+        *reinterpret_cast<h8_t*>(AS + 3 * pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + 3 * pa + o + 8) = l[1];
+    } else {
+        const int b = blockIdx.x - ta;
+        const int n0 = (b / kt) * LU_NB, k0 = (b % kt) * LU_NB;
+        if (n0 + j >= N) return;
+        const float sb = ldexpf(1.0f, kSplitExpB - eb);
+        const size_t src = (size_t)(cb + n0 + j) * ld + rb + k0 + 16 * kq;
+        float xr[16], xi[16];
+        for (int t = 0; t < 16; t += 4) {
+            float4 vr = *reinterpret_cast<const float4*>(Pr + src + t);
+            float4 vi = *reinterpret_cast<const float4*>(Pi + src + t);
+            xr[t] = vr.x; xr[t + 1] = vr.y; xr[t + 2] = vr.z; xr[t + 3] = vr.w;
+            xi[t] = vi.x; xi[t + 1] = vi.y; xi[t + 2] = vi.z; xi[t + 3] = vi.w;
+        }
+        h8_t h[2], l[2];
+        const size_t pb = (size_t)N * K, o = ((size_t)(k0 / HK + kq) * N + n0 + j) * HK;
+        split16(xr, sb, h, l);
+        *reinterpret_cast<h8_t*>(BS + o) = h[0]; *reinterpret_cast<h8_t*>(BS + o + 8) = h[1];
+        *reinterpret_cast<h8_t*>(BS + pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + pb + o + 8) = l[1];
+        split16(xi, sb, h, l);
+        *reinterpret_cast<h8_t*>(BS + 2 * pb + o) = h[0]; *reinterpret_cast<h8_t*>(BS + 2 * pb + o + 8) = h[1];
+        *reinterpret_cast<h8_t*>(BS + 3 * pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + 3 * pb + o + 8) = l[1];
+    }
+}
+
+// The update of k_cgemm_sub_h2 on operands split beforehand by k_split_ops:
+// the chunk's halves go to LDS with 16-byte loads and no conversion. A rows
+// are AS rows 0 .. M-1 (C rows rc ..); B columns are BS columns nb0 .. nb0+N-1
+// of ldn (C columns cc ..); the MFMA sequence and the epilogue are those of
+// k_cgemm_sub_h2, so the results are the same bit for bit.
+template <int WM, int WN>
+__global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, float* __restrict__ Pr,
+                                                         float* __restrict__ Pi, int ld, int rb, int rc, int cc,
+                                                         int critical, const float* __restrict__ umax,
+                                                         const _Float16* __restrict__ AS,
+                                                         const _Float16* __restrict__ BS, int nb0, int ldn) {
+    if (critical) __builtin_amdgcn_s_setprio(3);
+    constexpr int TM = 64 * WM, TN = 64 * WN;
+    __shared__ __align__(16) _Float16 As[2][4][TM][HKP];   // [buf][Re h, Re l, Im h, Im l][row][k]
+    __shared__ __align__(16) _Float16 Bs[2][4][TN][HKP];   // [buf][part][col][k]
+    int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN;
+    int ntiles = tiles_m * tiles_n;
+    constexpr int GROUP_M = 8;                     // XCD-aware grouped order, as in k_cgemm_sub
+    int bid = blockIdx.x;
+    int per = (ntiles + 7) / 8;
+    int tile = (bid % 8) * per + bid / 8;
+    if (tile >= ntiles) return;
+    int group = tile / (GROUP_M * tiles_n), first_m = group * GROUP_M;
+    int gsize = min(GROUP_M, tiles_m - first_m), in_group = tile - group * GROUP_M * tiles_n;
+    int tm = first_m + in_group % gsize, tn = in_group / gsize;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int rowt = tm * TM, colt = tn * TN;
+    const int mrows = min(TM, M - rowt), ncols = min(TN, N - colt);
+    const size_t ldl = (size_t)ld;
+    // operand scales
+    float um = 0.0f;
+    for (int q = 0; q < K / LU_NB; q++) um = fmaxf(um, umax[rb / LU_NB + q]);
+    int eb = 0;
+    if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);   // um < 2^eb
+    const float inv = ldexpf(1.0f, eb - kSplitExpB - kSplitExpA);
+
+    // staged halves: (part, row or column, k half of the chunk), 16 bytes each
+    constexpr int NA16 = 4 * TM * 2 / 256, NB16 = 4 * TN * 2 / 256;
+    h8_t ha[NA16], hb[NB16];
+    const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int l = 0; l < NA16; l++) {
+            int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
+            size_t o = part * pa + ((size_t)(k0 / HK) * M + rowt + min(row, mrows - 1)) * HK + 8 * hf;
+            ha[l] = *reinterpret_cast<const h8_t*>(AS + o);
+        }
+#pragma unroll
+        for (int l = 0; l < NB16; l++) {
+            int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
+            size_t o = part * pb + ((size_t)(k0 / HK) * ldn + nb0 + colt + min(col, ncols - 1)) * HK + 8 * hf;
+            hb[l] = *reinterpret_cast<const h8_t*>(BS + o);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int l = 0; l < NA16; l++) {
+            int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
+            *reinterpret_cast<h8_t*>(&As[buf][part][row][8 * hf]) = ha[l];
+        }
+#pragma unroll
+        for (int l = 0; l < NB16; l++) {
+            int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
+            *reinterpret_cast<h8_t*>(&Bs[buf][part][col][8 * hf]) = hb[l];
+        }
+    };
+
+    const int row0 = wm * 32 * WM, col0 = wn * 32 * WN;     // the wave's blocks within the tile
+    const bool active = row0 < mrows && col0 < ncols;
+    const int li = lane & 31, lk = 8 * (lane >> 5);          // operand row / column and k offset
+    f16x_t accR[WM][WN], accI[WM][WN];
+    float4 cR[WM][WN][4], cI[WM][WN][4];
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            accR[a][b] = f16x_t{};
+            accI[a][b] = f16x_t{};
+        }
+    const int nchunk = K / HK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+        const int buf = c & 1;
+        if (c + 1 < nchunk) gload((c + 1) * HK);
+        if (c == 0 && active) {   // C, added at the end: its loads overlap the first chunk
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) {
+                    size_t off = (size_t)(cc + colt + col0 + 32 * b + li) * ldl + rc + rowt + row0 + 32 * a +
+                                 4 * (lane >> 5);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        cR[a][b][q] = *reinterpret_cast<const float4*>(Pr + off + 8 * q);
+                        cI[a][b][q] = *reinterpret_cast<const float4*>(Pi + off + 8 * q);
+                    }
+                }
+        }
+        if (active) {
+            h8_t arh[WM], arl[WM], aih[WM], ail[WM], nih[WM], nil[WM];
+            h8_t brh[WN], brl[WN], bih[WN], bil[WN];
+#pragma unroll
+            for (int a = 0; a < WM; a++) {
+                int r = row0 + 32 * a + li;
+                arh[a] = *reinterpret_cast<const h8_t*>(&As[buf][0][r][lk]);
+                arl[a] = *reinterpret_cast<const h8_t*>(&As[buf][1][r][lk]);
+                aih[a] = *reinterpret_cast<const h8_t*>(&As[buf][2][r][lk]);
+                ail[a] = *reinterpret_cast<const h8_t*>(&As[buf][3][r][lk]);
+                nih[a] = neg8(aih[a]);
+                nil[a] = neg8(ail[a]);
+            }
+#pragma unroll
+            for (int b = 0; b < WN; b++) {
+                int cl = col0 + 32 * b + li;
+                brh[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][0][cl][lk]);
+                brl[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][1][cl][lk]);
+                bih[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][2][cl][lk]);
+                bil[b] = *reinterpret_cast<const h8_t*>(&Bs[buf][3][cl][lk]);
+            }
+            // Re += Ar Br - Ai Bi,  Im += Ar Bi + Ai Br  (three split products each),
+            // alternating the two accumulators
+#pragma unroll
+            for (int a = 0; a < WM; a++)
+#pragma unroll
+                for (int b = 0; b < WN; b++) {
+                    f16x_t R = accR[a][b], I = accI[a][b];
+                    R = mfma32(arh[a], brh[b], R);
+                    I = mfma32(arh[a], bih[b], I);
+                    R = mfma32(arh[a], brl[b], R);
+                    I = mfma32(arh[a], bil[b], I);
+                    R = mfma32(arl[a], brh[b], R);
+                    I = mfma32(arl[a], bih[b], I);
+                    R = mfma32(nih[a], bih[b], R);
+                    I = mfma32(aih[a], brh[b], I);
+                    R = mfma32(nih[a], bil[b], R);
+                    I = mfma32(aih[a], brl[b], I);
+                    R = mfma32(nil[a], bih[b], R);
+                    I = mfma32(ail[a], brh[b], I);
+                    accR[a][b] = R;
+                    accI[a][b] = I;
+                }
+        }
+        if (c + 1 < nchunk) {
+            sstore(buf ^ 1);
+            __syncthreads();
+        }
+    }
+    if (!active) return;
+    // accumulator element e of a 32 x 32 block: row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            size_t off = (size_t)(cc + colt + col0 + 32 * b + li) * ldl + rc + rowt + row0 + 32 * a + 4 * (lane >> 5);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float4 r = cR[a][b][q], m = cI[a][b][q];
+                r.x = fmaf(accR[a][b][4 * q + 0], inv, r.x);
+                r.y = fmaf(accR[a][b][4 * q + 1], inv, r.y);
+                r.z = fmaf(accR[a][b][4 * q + 2], inv, r.z);
+                r.w = fmaf(accR[a][b][4 * q + 3], inv, r.w);
+                m.x = fmaf(accI[a][b][4 * q + 0], inv, m.x);
+                m.y = fmaf(accI[a][b][4 * q + 1], inv, m.y);
+                m.z = fmaf(accI[a][b][4 * q + 2], inv, m.z);
+                m.w = fmaf(accI[a][b][4 * q + 3], inv, m.w);
+                *reinterpret_cast<float4*>(Pr + off + 8 * q) = r;
+                *reinterpret_cast<float4*>(Pi + off + 8 * q) = m;
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // Triangular solves with NB-row blocks, one persistent launch per solve.
 //
 // Block form of the lower solve (the upper one mirrors it with c > b):
@@ -2284,6 +2538,28 @@ void launch_split(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float*
 }
 int default_split(int M, int N) { return cdiv(M, 128) * cdiv(N, 64) >= 512 ? 1 : 0; }
 
+// The same updates on operands split beforehand (k_split_ops), same tile
+// choices as launch_split.
+template <int WM, int WN>
+void launch_split_pre_tile(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc, int cc,
+                           int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
+                           int ldn) {
+    constexpr int TM = 64 * WM, TN = 64 * WN;
+    int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
+    int per = (tiles + 7) / 8;
+    hipLaunchKernelGGL((k_cgemm_sub_hs<WM, WN>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc,
+                       critical, umax, AS, BS, nb0, ldn);
+}
+void launch_split_pre(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc,
+                      int cc, int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
+                      int ldn) {
+    switch (cfg) {
+    case 1: launch_split_pre_tile<2, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 2: launch_split_pre_tile<1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    default: launch_split_pre_tile<1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    }
+}
+
 // Size rule for untuned shapes: the larger FP32 tile where it still gives 2
 // work-groups per CU, else 64 x 64.
 template <typename T>
@@ -2445,7 +2721,8 @@ template int gemm_tile_choice<double>(int, int, int);
 // around the launch and the flops of the unpadded part).
 template <typename T>
 static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar, T* Ai, int ra, int ca, int rb,
-                     int cb, int rc, int cc, LuProfile* prof, int critical, const float* umax) {
+                     int cb, int rc, int cc, LuProfile* prof, int critical, const float* umax,
+                     const _Float16* AS = nullptr, const _Float16* BS = nullptr, int nb0 = 0, int ldn = 0) {
     bool timed = prof && prof->n < LuProfile::kMax;
     if (timed) {
         if (!prof->ev[2 * prof->n]) {
@@ -2454,17 +2731,20 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
         }
         (void)hipEventRecord(prof->ev[2 * prof->n], s);
     }
-    bool split = sizeof(T) == 4 && umax;
+    const bool split = sizeof(T) == 4 && umax;
+    const int mode = split ? (AS ? 2 : 1) : 0;
     int cfg = split ? pick_split(dev, M, N, K) : pick_cfg<T>(dev, M, N, K);
     if constexpr (sizeof(T) == 4) {
-        if (split) launch_split(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
+        if (mode == 2) launch_split_pre(cfg, s, M, N, K, Ar, Ai, Np, rb, rc, cc, critical, umax, AS, BS, nb0, ldn);
+        else if (mode == 1) launch_split(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
         else launch_cfg<T>(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
     } else {
         launch_cfg<T>(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical);
     }
     if (timed) {
         (void)hipEventRecord(prof->ev[2 * prof->n + 1], s);
-        prof->call[prof->n] = {dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, split ? 1 : 0, Ar, Ai, umax};
+        prof->call[prof->n] = {dev, Np, M, N, K, ra, ca, rb, cb, rc, cc, critical, cfg, mode, Ar, Ai, umax, AS, BS,
+                               nb0, ldn};
         double mu = (double)std::max(0, std::min(M, prof->N - rc));
         double nu = (double)std::max(0, std::min(N, prof->N - cc));
         double ku = (double)std::max(0, std::min(K, prof->N - ca));
@@ -2486,7 +2766,10 @@ double replay_lu_gemms(const LuProfile& prof, hipStream_t s) {
         T* Ar = static_cast<T*>(c.Ar);
         T* Ai = static_cast<T*>(c.Ai);
         if constexpr (sizeof(T) == 4) {
-            if (c.split)
+            if (c.split == 2)
+                launch_split_pre(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.rb, c.rc, c.cc, c.critical, c.umax, c.AS,
+                                 c.BS, c.nb0, c.ldn);
+            else if (c.split)
                 launch_split(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.ra, c.ca, c.rb, c.cb, c.rc, c.cc, c.critical, c.umax);
             else
                 launch_cfg<T>(c.cfg, s, c.M, c.N, c.K, Ar, Ai, c.Np, c.ra, c.ca, c.rb, c.cb, c.rc, c.cc, c.critical);
@@ -2533,7 +2816,8 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const bool split = sizeof(T) == 4 && arith == kLuArithSplitF16 && w.umax;
+    const bool split = sizeof(T) == 4 && (arith == kLuArithSplitF16 || arith == kLuArithSplitF16Tile) && w.umax;
+    const bool presplit = split && arith == kLuArithSplitF16 && w.opA && w.opB;
     float* umax = split ? w.umax : nullptr;
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv, umax);
     // kp: first column of the panel's group (kp < k0: the group's deferred
@@ -2584,14 +2868,27 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         }
         const int kg = k0 + g * NB, restg = Np - kg;
         if (restg <= 0) break;
+        // split arithmetic: the group's operands (rows kg.. of its L21, columns
+        // kg.. of its U12) in fp16 split form once, for both updates below
+        const _Float16 *AS = nullptr, *BS = nullptr;
+        if constexpr (sizeof(T) == 4) {
+            if (presplit) {
+                const int kt = g, grid = cdiv(restg, NB) * kt * 2;
+                hipLaunchKernelGGL(k_split_ops, dim3(grid), dim3(256), 0, s, restg, restg, g * NB, Ar, Ai, Np, kg, k0,
+                                   k0, kg, umax, w.opA, w.opB);
+                AS = w.opA;
+                BS = w.opB;
+            }
+        }
         // the next group's first panel, then hand it to the panel stream
-        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax);
+        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax, AS, BS, 0, restg);
         (void)hipEventRecord(ev[2 * (p + g)], s);
         (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
         panel(s2, p + g, kg);
         (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
         if (restg > NB)   // trailing update of everything right of it
-            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax);
+            gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax,
+                        AS, BS, NB, restg);
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,

@@ -186,9 +186,12 @@ int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
  *     fp16 terms each, three products per real product, FP32 accumulation
  *     (products good to about 2^-22, about 4x the FP32 rounding); the
  *     updates are then bound by their FP32 C traffic;
- *   BEM_LU_FP32: FP32 MFMA (16x16x4).
+ *   BEM_LU_FP32: FP32 MFMA (16x16x4);
+ *   BEM_LU_SPLIT_F16_TILE: the arithmetic of BEM_LU_SPLIT_F16 with the split
+ *     done in every tile of every update instead of once per panel group
+ *     (results equal to BEM_LU_SPLIT_F16 bit for bit; slower).
  * No counterpart in the reference (its LU is Eigen's). */
-enum { BEM_LU_FP32 = 0, BEM_LU_SPLIT_F16 = 1 };
+enum { BEM_LU_FP32 = 0, BEM_LU_SPLIT_F16 = 1, BEM_LU_SPLIT_F16_TILE = 2 };
 int bem_set_lu_arith(bem_ctx* ctx, int arith);
 
 /* Measured machine peaks of the context's device, for reporting beside the

@@ -538,3 +538,27 @@ def test_lu_arithmetics_agree(ctx, cyl, n):
     for arith, (X, g) in out.items():
         assert relerr(X, Xr) < 1e-9, (arith, relerr(X, Xr))
     assert relerr(out[0][0], out[1][0]) < 1e-9
+
+
+def test_split_once_per_group_equals_split_per_tile(ctx):
+    """The factorisation's K = 256 updates on operands split into fp16 once
+    per panel group (the default) and split in every tile give the same
+    factor: the same fp16 values reach the same MFMA sequence, and the
+    refinement after it is deterministic, so the refined solutions of a dense
+    mixed solve of order 1100 (padded 1152: four groups of four panels, then
+    a group of two) agree bit for bit."""
+    rng = np.random.default_rng(2026)
+    n = 1100
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 40.0 * np.eye(n)
+    B = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    out = {}
+    try:
+        for arith in (bm.LU_SPLIT_F16, bm.LU_SPLIT_F16_TILE):
+            ctx.set_lu_arith(arith)
+            X, info = ctx.dense_solve(A, B, 32)
+            assert info["fell_back_fp64"] == 0, (arith, info)
+            out[arith] = X
+    finally:
+        ctx.set_lu_arith(bm.LU_SPLIT_F16)
+    assert relerr(out[bm.LU_SPLIT_F16], np.linalg.solve(A, B)) < 1e-10
+    np.testing.assert_array_equal(out[bm.LU_SPLIT_F16], out[bm.LU_SPLIT_F16_TILE])
