@@ -28,6 +28,23 @@ __host__ __device__ inline double frcp(double x) {
 #endif
 }
 
+// a b + c for a coefficient c known at compile time, as one v_fma_f64 with c
+// in an SGPR pair. Left to itself the compiler selects v_fmac_f64, whose
+// addend is also its destination, and copies every 64-bit coefficient into
+// that VGPR pair first (two v_mov_b32 per term): in the Horner and series
+// loops of the Green function those copies were a fifth of all VALU
+// instructions. The operation is the same fused multiply-add, so results
+// are unchanged.
+__host__ __device__ inline double fma_k(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 // Coefficients of the Struve approximations (Kernels.cpp:35-116): for x <= 3
 // polynomials in (x/3)^2 (p), above rational functions of (3/x)^2 with
 // numerator a and denominator 1 + b. H0: p0..p5, a0..a3, b1..b3; H1: p1..p6,
@@ -53,8 +70,14 @@ __host__ __device__ inline void struve_h01(double x, double y0, double y1, doubl
     constexpr const double* Q = kStruve.h1p;
     if (x <= 3.0) {
         double x3 = x * (1.0 / 3.0), yy = x3 * x3;
-        h0 = (P[0] + (P[1] + (P[2] + (P[3] + (P[4] + P[5] * yy) * yy) * yy) * yy) * yy) * x3;
-        h1 = (Q[0] + (Q[1] + (Q[2] + (Q[3] + (Q[4] + Q[5] * yy) * yy) * yy) * yy) * yy) * yy;
+        double p = P[5], q = Q[5];
+#pragma unroll
+        for (int k = 4; k >= 0; k--) {
+            p = fma_k(p, yy, P[k]);
+            q = fma_k(q, yy, Q[k]);
+        }
+        h0 = p * x3;
+        h1 = q * yy;
         return;
     }
     constexpr const double* a = kStruve.h0a;
@@ -62,10 +85,10 @@ __host__ __device__ inline void struve_h01(double x, double y0, double y1, doubl
     constexpr const double* c = kStruve.h1a;
     constexpr const double* d = kStruve.h1b;
     double u = 3.0 * frcp(x), yy = u * u;
-    double c1a = 2.0 * (a[0] + (a[1] + (a[2] + a[3] * yy) * yy) * yy);
-    double c2a = kPI * x * (1.0 + (b[0] + (b[1] + b[2] * yy) * yy) * yy);
-    double c1b = 2.0 * (c[0] + (c[1] + (c[2] + c[3] * yy) * yy) * yy);
-    double c2b = kPI * (1.0 + (d[0] + (d[1] + d[2] * yy) * yy) * yy);
+    double c1a = 2.0 * fma_k(fma_k(fma_k(a[3], yy, a[2]), yy, a[1]), yy, a[0]);
+    double c2a = kPI * x * (1.0 + fma_k(fma_k(b[2], yy, b[1]), yy, b[0]) * yy);
+    double c1b = 2.0 * fma_k(fma_k(fma_k(c[3], yy, c[2]), yy, c[1]), yy, c[0]);
+    double c2b = kPI * (1.0 + fma_k(fma_k(d[2], yy, d[1]), yy, d[0]) * yy);
     double rab = frcp(c2a * c2b);
     h0 = c1a * c2b * rab + y0;
     h1 = c1b * c2a * rab + y1;
@@ -84,8 +107,8 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
         double a = kBesJ0S[15], b = kBesJ1S[15];
 #pragma unroll
         for (int k = 14; k >= 0; k--) {
-            a = fma(a, t, kBesJ0S[k]);
-            b = fma(b, t, kBesJ1S[k]);
+            a = fma_k(a, t, kBesJ0S[k]);
+            b = fma_k(b, t, kBesJ1S[k]);
         }
         J0 = a;
         J1 = 0.5 * x * b;
@@ -117,9 +140,9 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
     const double rr = r * r;
     double sp = kSinT[8], cp = kCosT[9];
 #pragma unroll
-    for (int k = 7; k >= 0; k--) sp = fma(sp, rr, kSinT[k]);
+    for (int k = 7; k >= 0; k--) sp = fma_k(sp, rr, kSinT[k]);
 #pragma unroll
-    for (int k = 8; k >= 0; k--) cp = fma(cp, rr, kCosT[k]);
+    for (int k = 8; k >= 0; k--) cp = fma_k(cp, rr, kCosT[k]);
     const double sr = r * sp, cr = cp;
     double sx, cx;
     switch (((long long)n) & 3) {
@@ -143,7 +166,7 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
 // as tabulated at Kernels.cpp:120-256, evaluated in Horner form.
 __host__ __device__ inline double horner(const double* c, int n, double t) {
     double s = c[n - 1];
-    for (int i = n - 2; i >= 0; i--) s = s * t + c[i];
+    for (int i = n - 2; i >= 0; i--) s = fma_k(s, t, c[i]);
     return s;
 }
 
