@@ -513,10 +513,9 @@ template <typename T>
 __device__ inline bool pivot_ok(T den) { return den > (T)0 && den < (T)INFINITY; }
 
 template <typename T>
-__device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad) {
+__device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad, T (&ar)[DB], T (&ai)[DB]) {
     const int rho = lane & 15;
     const bool writer = lane < DB;
-    T ar[DB], ai[DB];
 #pragma unroll
     for (int c = 0; c < DB; c++) { ar[c] = S.ar[o + c][o + rho]; ai[c] = S.ai[o + c][o + rho]; }
 #pragma unroll
@@ -545,6 +544,88 @@ __device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad) {
     if (writer) {
 #pragma unroll
         for (int c = 0; c < DB; c++) { S.ar[o + c][o + rho] = ar[c]; S.ai[o + c][o + rho] = ai[c]; }
+    }
+}
+
+// The unit-lower inverse X = L^-1 of the factored 16 x 16 block, row by row
+// in the registers factor16 left (lane rho: row rho of the factor, L(rho, k)
+// for k < rho): for k = 0 .. 14 row k of X is final, is broadcast (readlane),
+// and every row below takes X(r, c) -= L(r, k) X(k, c), c <= k. Writes
+// Li_kk to S.lr[kb] ([col][row]).
+template <typename T>
+__device__ inline void linv16_rows(DiagLds<T>& S, int kb, int lane, const T (&ar)[DB], const T (&ai)[DB]) {
+    const int rho = lane & 15;
+    T lr[DB], li[DB], xr[DB], xi[DB];
+#pragma unroll
+    for (int k = 0; k < DB; k++) {
+        lr[k] = rho > k ? ar[k] : (T)0;
+        li[k] = rho > k ? ai[k] : (T)0;
+        xr[k] = k == rho ? (T)1 : (T)0;
+        xi[k] = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < DB - 1; k++) {
+#pragma unroll
+        for (int c = 0; c <= k; c++) {
+            T br = bcast(xr[c], k), bi = bcast(xi[c], k);
+            xr[c] -= lr[k] * br - li[k] * bi;
+            xi[c] -= lr[k] * bi + li[k] * br;
+        }
+    }
+    if (lane < DB) {
+#pragma unroll
+        for (int c = 0; c < DB; c++) { S.lr[kb][c][rho] = xr[c]; S.li[kb][c][rho] = xi[c]; }
+    }
+}
+
+// The upper inverse Y = U^-1 of the factored 16 x 16 block, row by row (lane
+// rho: row rho, read from LDS): with D = diag U and N = D^-1 U - I, U^-1 =
+// (I + N)^-1 D^-1. Lane rho scales its row by 1 / U(rho, rho); then for
+// k = 15 .. 1 row k of W = (I + N)^-1 is final, is broadcast, and every row
+// above takes W(r, c) -= N(r, k) W(k, c), c >= k; last, column c of W is
+// scaled by 1 / U(c, c). Writes Ui_kk to S.ur[kb] ([col][row]).
+template <typename T>
+__device__ inline void uinv16_rows(DiagLds<T>& S, int o, int kb, int lane) {
+    const int rho = lane & 15;
+    T ur[DB], ui[DB];
+#pragma unroll
+    for (int c = 0; c < DB; c++) { ur[c] = S.ar[o + c][o + rho]; ui[c] = S.ai[o + c][o + rho]; }
+    T dr = 0, di = 0;
+#pragma unroll
+    for (int c = 0; c < DB; c++) {
+        dr = c == rho ? ur[c] : dr;
+        di = c == rho ? ui[c] : di;
+    }
+    const T den = (T)1 / cabs2(dr, di);
+    const T qr = dr * den, qi = -di * den;           // 1 / U(rho, rho)
+    T nr_[DB], ni_[DB], wr[DB], wi[DB];
+#pragma unroll
+    for (int k = 0; k < DB; k++) {
+        T a = ur[k] * qr - ui[k] * qi, b = ur[k] * qi + ui[k] * qr;
+        nr_[k] = k > rho ? a : (T)0;
+        ni_[k] = k > rho ? b : (T)0;
+        wr[k] = k == rho ? (T)1 : (T)0;
+        wi[k] = 0;
+    }
+#pragma unroll
+    for (int k = DB - 1; k >= 1; k--) {
+#pragma unroll
+        for (int c = k; c < DB; c++) {
+            T br = bcast(wr[c], k), bi = bcast(wi[c], k);
+            wr[c] -= nr_[k] * br - ni_[k] * bi;
+            wi[c] -= nr_[k] * bi + ni_[k] * br;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DB; c++) {
+        T sr = bcast(qr, c), si = bcast(qi, c);      // 1 / U(c, c)
+        T a = wr[c] * sr - wi[c] * si, b = wr[c] * si + wi[c] * sr;
+        wr[c] = a;
+        wi[c] = b;
+    }
+    if (lane < DB) {
+#pragma unroll
+        for (int c = 0; c < DB; c++) { S.ur[kb][c][rho] = wr[c]; S.ui[kb][c][rho] = wi[c]; }
     }
 }
 
@@ -685,10 +766,12 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     int mybad = 0;
     for (int kb = 0; kb < 4; kb++) {
         const int o = DB * kb;
-        if (wave == 0) factor16<T>(S, o, lane, mybad);
+        T far_[DB], fai_[DB];                        // wave 0: its rows of the factored block
+        if (wave == 0) factor16<T>(S, o, lane, mybad, far_, fai_);
         __syncthreads();
-        if (wave == 0) linv16<T>(S, o, kb, lane);
-        else if (wave == 1) uinv16<T>(S, o, kb, lane);
+        // the two triangle inverses side by side, each in one wave's registers
+        if (wave == 0) linv16_rows<T>(S, kb, lane, far_, fai_);
+        else if (wave == 1) uinv16_rows<T>(S, o, kb, lane);
         __syncthreads();
         // L_ik = A_ik Ui_kk (i > kb), U_kj = Li_kk A_kj (j > kb), one block per wave in turn
         const int nrest = 3 - kb;
@@ -1893,10 +1976,10 @@ __device__ inline void stage_block(unsigned long long* pub, unsigned epoch, int 
             }
             return;
         }
-        // not out yet: one lane waits on a single word of the block before the
-        // batch is read again (every waiting work-group polling every word of
-        // a block congests its memory channel)
-        if (lane == 0) pub_wait(pub + ((size_t)(c * MV_MAXR) * LU_NB) * 2 * W, epoch);
+        // not out yet: the batch is read again. Only the work-group that waits
+        // for the block finishing its own (the chain) polls a block that is not
+        // out; the others find theirs published when they get to them.
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -1937,16 +2020,49 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
         ci[r] = i < nr ? Xi[x] : (TX)0;
     }
     int ndep = (lower ? b : nblk - 1 - b) - 1;      // all dependencies but `prev`
-    for (int d = 0; d < ndep; d++) {
-        int c = lower ? d : nblk - 1 - d;
-        T ma_r[16], ma_i[16];                       // -M_bc: A(i, k = 4s + g)
+    // Dependency d: -M_bc (A(i, k = 4s + g)) and the published words of X_c.
+    // Both are loaded one dependency ahead, while the current one feeds the
+    // MFMAs, so each step of this loop waits for no memory round trip of its
+    // own; words that were not out yet at the early load are polled again.
+    struct Dep {
+        T ma_r[16], ma_i[16];
+        TX xv[8];
+        int ok;
+    };
+    auto load_dep = [&](int d, Dep& q) {
+        const int c = lower ? d : nblk - 1 - d;
 #pragma unroll
         for (int s = 0; s < 16; s++) {
             size_t o = (size_t)(c * LU_NB + 4 * s + g) * Np + w0 + i;
-            ma_r[s] = -Ar[o]; ma_i[s] = -Ai[o];
+            q.ma_r[s] = -Ar[o]; q.ma_i[s] = -Ai[o];
         }
+        constexpr int W = PubWords<TX>::W;
+        q.ok = 1;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {               // value t: rhs t / 128, row (t % 128) / 2, part t % 2
+            int t = tid + 256 * u;
+            q.xv[u] = 0;
+            if (t < 128 * nr) {
+                size_t o = ((size_t)(c * MV_MAXR + (t >> 7)) * LU_NB) * 2 * W + (size_t)(t & 127) * W;
+                q.ok &= pub_load(pub + o, epoch, q.xv[u]) ? 1 : 0;
+            }
+        }
+    };
+    auto use_dep = [&](int d, const Dep& q) {
+        const int c = lower ? d : nblk - 1 - d;
         __syncthreads();                            // the previous block's operands are read
-        stage_block<TX>(pub, epoch, c, nr, tid, lane, sr, si);
+        if (__all(q.ok)) {                          // this wave's words were all out at the early load
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                int t = tid + 256 * u;
+                if (t < 128 * nr) {
+                    if (t & 1) si[t >> 7][(t & 127) >> 1] = q.xv[u];
+                    else sr[t >> 7][(t & 127) >> 1] = q.xv[u];
+                }
+            }
+        } else {
+            stage_block<TX>(pub, epoch, c, nr, tid, lane, sr, si);
+        }
         __syncthreads();
         T xb_r[16], xb_i[16];
 #pragma unroll
@@ -1955,7 +2071,19 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
             xb_i[s] = i < nr ? si[i][4 * s + g] : (TX)0;
         }
 #pragma unroll
-        for (int s = 0; s < 16; s++) cmac<T>(cr, ci, ma_r[s], ma_i[s], xb_r[s], xb_i[s]);
+        for (int s = 0; s < 16; s++) cmac<T>(cr, ci, q.ma_r[s], q.ma_i[s], xb_r[s], xb_i[s]);
+    };
+    {
+        Dep q0, q1;
+        if (ndep > 0) load_dep(0, q0);
+        for (int d = 0; d < ndep; d += 2) {
+            if (d + 1 < ndep) load_dep(d + 1, q1);
+            use_dep(d, q0);
+            if (d + 1 < ndep) {
+                if (d + 2 < ndep) load_dep(d + 2, q0);
+                use_dep(d + 1, q1);
+            }
+        }
     }
     // y = B_b - sum into LDS as the B operand of Inv_b
     T va_r[16], va_i[16];
