@@ -2996,27 +2996,30 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         }
         const int kg = k0 + g * NB, restg = Np - kg;
         if (restg <= 0) break;
-        // split arithmetic: the group's operands (rows kg.. of its L21, columns
-        // kg.. of its U12) in fp16 split form once, for both updates below
-        const _Float16 *AS = nullptr, *BS = nullptr;
-        if constexpr (sizeof(T) == 4) {
-            if (presplit) {
-                const int kt = g, grid = cdiv(restg, NB) * kt * 2;
-                hipLaunchKernelGGL(k_split_ops, dim3(grid), dim3(256), 0, s, restg, restg, g * NB, Ar, Ai, Np, kg, k0,
-                                   k0, kg, umax, w.opA, w.opB);
-                AS = w.opA;
-                BS = w.opB;
-            }
-        }
-        // the next group's first panel, then hand it to the panel stream
-        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax, AS, BS, 0, restg);
+        // the next group's first panel (thin: its operands split in the tiles),
+        // then hand it to the panel stream
+        gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax);
         (void)hipEventRecord(ev[2 * (p + g)], s);
         (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
         panel(s2, p + g, kg);
         (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
-        if (restg > NB)   // trailing update of everything right of it
+        if (restg > NB) {   // trailing update of everything right of it
+            // split arithmetic: its operands (rows kg.. of the group's L21,
+            // columns kg+NB.. of its U12) in fp16 split form once, beside the
+            // panel stream's work
+            const _Float16 *AS = nullptr, *BS = nullptr;
+            if constexpr (sizeof(T) == 4) {
+                if (presplit) {
+                    const int grid = (cdiv(restg, NB) + cdiv(restg - NB, NB)) * g;
+                    hipLaunchKernelGGL(k_split_ops, dim3(grid), dim3(256), 0, s, restg, restg - NB, g * NB, Ar, Ai, Np,
+                                       kg, k0, k0, kg + NB, umax, w.opA, w.opB);
+                    AS = w.opA;
+                    BS = w.opB;
+                }
+            }
             gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax,
-                        AS, BS, NB, restg);
+                        AS, BS, 0, restg - NB);
+        }
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,
