@@ -505,6 +505,20 @@ __device__ inline void blk_mma_rb(const T* ar_, const T* ai_, int lda, const typ
     }
 }
 
+// 1 / x for a pivot's squared modulus: the hardware reciprocal refined by
+// Newton steps (one in FP32, two in FP64), about 1 ulp, in place of the IEEE
+// division sequence on each pivot step's critical path. x = 0 gives inf and
+// x = NaN gives NaN, as the division would (the pivot checks flag both).
+__device__ inline float rcp_pivot(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+}
+__device__ inline double rcp_pivot(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Wave 0: the 16 x 16 diagonal block at (o, o) factored in registers (lane
 // rho & 15 holds row rho, the pivot row broadcast with readlane), written
 // back to LDS.
@@ -525,7 +539,7 @@ __device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad, T (&
         // a zero or non-finite pivot: the pivoting fallback decides (it flags
         // the panel numerically singular if no usable pivot exists)
         if (!pivot_ok(den)) mybad |= 1;
-        T inv = (T)1 / den;
+        T inv = rcp_pivot(den);
         T qr = pr * inv, qi = -pi * inv;
         const bool below = rho > j;
         T mr = ar[j] * qr - ai[j] * qi, mi = ar[j] * qi + ai[j] * qr;
@@ -596,7 +610,7 @@ __device__ inline void uinv16_rows(DiagLds<T>& S, int o, int kb, int lane) {
         dr = c == rho ? ur[c] : dr;
         di = c == rho ? ui[c] : di;
     }
-    const T den = (T)1 / cabs2(dr, di);
+    const T den = rcp_pivot(cabs2(dr, di));
     const T qr = dr * den, qi = -di * den;           // 1 / U(rho, rho)
     T nr_[DB], ni_[DB], wr[DB], wi[DB];
 #pragma unroll
