@@ -161,6 +161,7 @@ _SIGS = {
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
     "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
     "bem_set_lu_arith": (c_int, [c_void_p, c_int]),
+    "bem_set_wave_far32": (c_int, [c_void_p, c_double]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
     "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
@@ -419,12 +420,13 @@ class Context:
         except Exception:
             pass
 
-    def eval_havelock(self, X, Y, V):
+    def eval_havelock(self, X, Y, V, precision=64):
+        """HavelockGF on the device; precision 32 evaluates the far-pair FP32 variant."""
         X = np.ascontiguousarray(X, dtype=np.float64)
         Y = np.ascontiguousarray(Y, dtype=np.float64)
         V = np.ascontiguousarray(V, dtype=np.float64)
         out = np.zeros((len(X), 8))
-        _check(lib().bem_eval_havelock(self._h, 64, len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
+        _check(lib().bem_eval_havelock(self._h, int(precision), len(X), _dp(X), _dp(Y), _dp(V), _dp(out)))
         return out
 
     def measure_peaks(self):
@@ -437,6 +439,11 @@ class Context:
         """Correction steps allowed to the mixed modes' refinement (0..4, default 4);
         0 sends every mixed frequency to the FP64 re-solve."""
         _check(lib().bem_set_refine_limit(self._h, int(max_steps)))
+
+    def set_wave_far32(self, near_radius):
+        """Measurement option: the wave term of pairs farther than near_radius (m) with FP32
+        special functions; near_radius < 0 (default) turns it off."""
+        _check(lib().bem_set_wave_far32(self._h, float(near_radius)))
 
     def set_lu_arith(self, arith):
         """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default), LU_FP32, or

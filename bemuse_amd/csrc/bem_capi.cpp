@@ -100,6 +100,7 @@ struct bem_ctx {
     int device = 0;
     int refine_max = kRefineMaxIter;   // correction steps allowed (bem_set_refine_limit)
     int lu_arith = kLuArithSplitF16;   // arithmetic of the FP32 factorisation's updates (bem_set_lu_arith)
+    double wave_far_r2 = -1.0;         // bem_set_wave_far32: squared radius beyond which the wave term runs FP32 special functions
     // streams and events of bem_dense_solve_noalloc, made here so that the
     // entry point itself creates nothing
     hipStream_t na_s2 = nullptr;
@@ -192,9 +193,10 @@ struct LinSys {
 struct GraphKey {
     unsigned long long body = 0;
     int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1, arith = -1;
+    double far_r2 = -2.0;
     bool operator==(const GraphKey& o) const {
         return body == o.body && Np == o.Np && mode == o.mode && nb == o.nb && rad_only == o.rad_only &&
-               compose == o.compose && sur == o.sur && refine == o.refine && arith == o.arith;
+               compose == o.compose && sur == o.sur && refine == o.refine && arith == o.arith && far_r2 == o.far_r2;
     }
 };
 
@@ -851,7 +853,8 @@ void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
         return;
     }
     launch_wave_compose(L.s1, b->N, b->Np, L.dout->freq, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
-                        L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr);
+                        L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr,
+                        b->ctx->wave_far_r2);
 }
 
 // Issues one frequency on a lane (all asynchronous on the lane's streams).
@@ -908,7 +911,8 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     L.hout->freq[0] = omega;
     L.hout->freq[1] = kappa;
     const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
-    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max, L.lu_arith};
+    const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max, L.lu_arith,
+                       b->ctx->wave_far_r2};
     const bool use_graph = b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
     L.last_key = key;
     L.graphed = use_graph;
@@ -1113,6 +1117,12 @@ int bem_set_refine_limit(bem_ctx* c, int max_steps) {
     if (!c || max_steps < 0 || max_steps > kRefineMaxIter)
         return fail(BEM_ERR_ARG, "bem_set_refine_limit: max_steps must be in [0, 4]");
     c->refine_max = max_steps;
+    return BEM_OK;
+}
+
+int bem_set_wave_far32(bem_ctx* c, double near_radius) {
+    if (!c || std::isnan(near_radius)) return fail(BEM_ERR_ARG, "bem_set_wave_far32: null context or NaN radius");
+    c->wave_far_r2 = near_radius < 0.0 ? -1.0 : near_radius * near_radius;
     return BEM_OK;
 }
 
@@ -1443,7 +1453,7 @@ int bem_struve_coefficients(double* out, int n) {
 int bem_eval_havelock(bem_ctx* c, int precision, int n, const double* X, const double* Y, const double* V,
                       double* out) {
     if (!c || n <= 0 || !X || !Y || !V || !out) return fail(BEM_ERR_ARG, "bem_eval_havelock: bad arguments");
-    if (precision != 64) return fail(BEM_ERR_ARG, "bem_eval_havelock: precision must be 64");
+    if (precision != 64 && precision != 32) return fail(BEM_ERR_ARG, "bem_eval_havelock: precision must be 64 or 32");
     HIP_TRY(hipSetDevice(c->device));
     double *dx = nullptr, *dy = nullptr, *dv = nullptr, *dout = nullptr;
     int r = BEM_OK;
@@ -1454,7 +1464,7 @@ int bem_eval_havelock(bem_ctx* c, int precision, int n, const double* X, const d
     (void)hipMemcpy(dx, X, n * 8, hipMemcpyHostToDevice);
     (void)hipMemcpy(dy, Y, n * 8, hipMemcpyHostToDevice);
     (void)hipMemcpy(dv, V, n * 8, hipMemcpyHostToDevice);
-    launch_havelock_eval(c->stream, n, dx, dy, dv, dout);
+    launch_havelock_eval(c->stream, n, dx, dy, dv, dout, precision == 32);
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * (size_t)n * 8, hipMemcpyDeviceToHost);
     (void)hipFree(dx); (void)hipFree(dy); (void)hipFree(dv); (void)hipFree(dout);

@@ -101,17 +101,21 @@ __host__ __device__ inline void struve_h01(double x, double y0, double y1, doubl
 // gen_bessel.py) and one shared sin/cos of x (three-piece pi/2 reduction,
 // Taylor series on |r| <= pi/4). Beyond x = 1.6e6 the reduction is no longer
 // exact and the device library functions are used.
+__host__ __device__ inline void bessel01_series(double x, double& J0, double& J1) {
+    const double t = -0.25 * x * x;
+    double a = kBesJ0S[15], b = kBesJ1S[15];
+#pragma unroll
+    for (int k = 14; k >= 0; k--) {
+        a = fma_k(a, t, kBesJ0S[k]);
+        b = fma_k(b, t, kBesJ1S[k]);
+    }
+    J0 = a;
+    J1 = 0.5 * x * b;
+}
+
 __host__ __device__ inline void bessel01(double x, double& J0, double& J1, double& Y0, double& Y1) {
     if (x <= 3.0) {
-        const double t = -0.25 * x * x;
-        double a = kBesJ0S[15], b = kBesJ1S[15];
-#pragma unroll
-        for (int k = 14; k >= 0; k--) {
-            a = fma_k(a, t, kBesJ0S[k]);
-            b = fma_k(b, t, kBesJ1S[k]);
-        }
-        J0 = a;
-        J1 = 0.5 * x * b;
+        bessel01_series(x, J0, J1);
         Y0 = 0.0;
         Y1 = 0.0;
         return;
@@ -160,6 +164,81 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
     Y1 = A * (Q1 * sc - P1 * cc);
 }
 
+// FP32 versions of the special functions above, for the wave term of far
+// pairs (a measured option; the exact path stays FP64). Same methods, each
+// series cut where its terms fall below FP32 resolution: the Struve power
+// series at 6 terms, the Chebyshev series at degree 10, sin/cos of the reduced
+// argument at degree 9/10; the reduction x = n pi/2 + r stays FP64 (an FP32
+// argument of K R ~ 100 would already carry a phase error of 1e-5).
+// bessel01_f32 covers x > 3 only (below, the caller takes the FP64 series).
+__device__ inline void bessel01_f32(double xd, float& J0, float& J1, float& Y0, float& Y1) {
+    const float x = (float)xd;
+    const float u = 3.0f / x, t = 2.0f * u - 1.0f, t2 = 2.0f * t;
+    float p1 = 0.0f, p2 = 0.0f, q1 = 0.0f, q2 = 0.0f, r1 = 0.0f, r2 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int k = 10; k >= 1; k--) {
+        float tp = fmaf(t2, p1, (float)kBesP0[k] - p2); p2 = p1; p1 = tp;
+        float tq = fmaf(t2, q1, (float)kBesQ0[k] - q2); q2 = q1; q1 = tq;
+        float tr = fmaf(t2, r1, (float)kBesP1[k] - r2); r2 = r1; r1 = tr;
+        float ts = fmaf(t2, s1, (float)kBesQ1[k] - s2); s2 = s1; s1 = ts;
+    }
+    const float P0 = fmaf(t, p1, (float)kBesP0[0] - p2), Q0 = fmaf(t, q1, (float)kBesQ0[0] - q2);
+    const float P1 = fmaf(t, r1, (float)kBesP1[0] - r2), Q1 = fmaf(t, s1, (float)kBesQ1[0] - s2);
+    const double n = rint(xd * kTwoOverPi);
+    double rd = fma(-n, kPio2A, xd);
+    rd = fma(-n, kPio2B, rd);
+    const float r = (float)rd, rr = r * r;
+    float sp = (float)kSinT[4], cp = (float)kCosT[5];
+#pragma unroll
+    for (int k = 3; k >= 0; k--) sp = fmaf(sp, rr, (float)kSinT[k]);
+#pragma unroll
+    for (int k = 4; k >= 0; k--) cp = fmaf(cp, rr, (float)kCosT[k]);
+    const float sr = r * sp, cr = cp;
+    float sx, cx;
+    switch (((long long)n) & 3) {
+        case 0: sx = sr; cx = cr; break;
+        case 1: sx = cr; cx = -sr; break;
+        case 2: sx = -sr; cx = -cr; break;
+        default: sx = -cr; cx = sr; break;
+    }
+    const float cc = (cx + sx) * (float)kSqrtHalf, sc = (sx - cx) * (float)kSqrtHalf;
+    const float A = sqrtf((float)kTwoOver3Pi * u);
+    J0 = A * (P0 * cc - Q0 * sc);
+    Y0 = A * (P0 * sc + Q0 * cc);
+    J1 = A * (P1 * sc + Q1 * cc);
+    Y1 = A * (Q1 * sc - P1 * cc);
+}
+
+__device__ inline void struve_h01_f32(double xd, float y0, float y1, float& h0, float& h1) {
+    constexpr const double* P = kStruve.h0p;
+    constexpr const double* Q = kStruve.h1p;
+    const float x = (float)xd;
+    if (xd <= 3.0) {
+        float x3 = x * (1.0f / 3.0f), yy = x3 * x3;
+        float p = (float)P[5], q = (float)Q[5];
+#pragma unroll
+        for (int k = 4; k >= 0; k--) {
+            p = fmaf(p, yy, (float)P[k]);
+            q = fmaf(q, yy, (float)Q[k]);
+        }
+        h0 = p * x3;
+        h1 = q * yy;
+        return;
+    }
+    constexpr const double* a = kStruve.h0a;
+    constexpr const double* b = kStruve.h0b;
+    constexpr const double* c = kStruve.h1a;
+    constexpr const double* d = kStruve.h1b;
+    const float u = 3.0f / x, yy = u * u;
+    const float c1a = 2.0f * fmaf(fmaf(fmaf((float)a[3], yy, (float)a[2]), yy, (float)a[1]), yy, (float)a[0]);
+    const float c2a = (float)kPI * x * (1.0f + fmaf(fmaf((float)b[2], yy, (float)b[1]), yy, (float)b[0]) * yy);
+    const float c1b = 2.0f * fmaf(fmaf(fmaf((float)c[3], yy, (float)c[2]), yy, (float)c[1]), yy, (float)c[0]);
+    const float c2b = (float)kPI * (1.0f + fmaf(fmaf((float)d[2], yy, (float)d[1]), yy, (float)d[0]) * yy);
+    const float rab = 1.0f / (c2a * c2b);
+    h0 = c1a * c2b * rab + y0;
+    h1 = c1b * c2a * rab + y1;
+}
+
 // Polynomials A..D (local flow) and dA..dC (its horizontal derivative) of the
 // Wu/Zhang/Zhu/Li/Wan/Noblesse global approximation (Eur. J. Mech. B 65 (2017)
 // eq. 14; Liang/Wu/Noblesse, Appl. Ocean Res. 74 (2018) eq. 16), coefficients
@@ -190,7 +269,17 @@ struct GFVal { double gr, gi, hr, hi, h; };
 
 // Quotients are products with frcp reciprocals (1/d shared by alpha and beta,
 // rho = d/(1+d) and 1 - rho from one reciprocal).
-__host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
+// S32: exp, log, Bessel, Struve in FP32 (the far-pair option); the
+// polynomials, reciprocals and the combination stay FP64 either way (the
+// polynomials' large alternating coefficients lose 1e-4 .. 1e-3 of their
+// value in FP32). Where h <= 3 the imaginary part's factors e^V, J0 and J1
+// stay FP64 too (power series, no Y needed there): at low frequency every
+// pair has small h, and the damping of a body small against the wavelength
+// is what is left of e^V J0(h) ~ 1 + V + (V^2/2 - h^2/4) after its leading
+// terms cancel over the hull, so FP32 rounding of those factors (6e-8 of 1)
+// would swamp it.
+template <bool S32>
+__host__ __device__ inline GFVal havelock_gf_t(double X, double Y, double vv) {
     double hh = sqrt(X * X + Y * Y);
     double dd = sqrt(hh * hh + vv * vv);
     double rdd = frcp(dd);
@@ -200,25 +289,52 @@ __host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     double rho = dd * omr;
     double rw = rho * omr * omr * omr;
     double dd3 = dd * dd * dd;
-    double ev = exp(vv);
+    double ev, lg, emd;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (S32) {
+        ev = hh <= 3.0 ? exp(vv) : (double)expf((float)vv);
+        lg = (double)logf((float)(0.5 * (dd - vv)));
+        emd = (double)expf((float)-dd);
+    } else
+#endif
+    {
+        ev = exp(vv);
+        lg = log(0.5 * (dd - vv));
+        emd = exp(-dd);
+    }
     double inv1 = frcp(1.0 + dd3);
     // local flow L0 (GF_Func_L0, :258-282)
-    double PP = (log(0.5 * (dd - vv)) + kEUL - 2.0 * dd * dd) * ev + dd * dd - vv;
+    double PP = (lg + kEUL - 2.0 * dd * dd) * ev + dd * dd - vv;
     double A = horner(kGF.A, 10, rho), B = horner(kGF.B, 10, rho);
     double C = horner(kGF.C, 8, rho), D = horner(kGF.D, 10, rho);
     double RR = (1.0 - be) * A - be * B - al * C * frcp(1.0 + 6.0 * al * rho * omr) + be * (1.0 - be) * D;
     double L0 = 2.0 * PP * inv1 + 2.0 * rw * RR;
     // derivative local flow Ls (GF_Func_Ls, :297-325)
     double PS = (be + hh) * frcp(dd - vv) - 2.0 * be + 2.0 * dd * ev - hh;
-    double QS = exp(-dd) * (1.0 - be) * (1.0 + dd * inv1);
+    double QS = emd * (1.0 - be) * (1.0 + dd * inv1);
     double dA = horner(kGF.dA, 10, rho), dB = horner(kGF.dB, 10, rho), dC = horner(kGF.dC, 6, rho);
     double R2 = be * dA - (1.0 - al) * dB + be * (1.0 - be) * rho * (1.0 - 2.0 * rho) * dC;
     double Ls = 2.0 * PS * inv1 - 4.0 * QS + 2.0 * rw * R2;
     // wave parts W, Wh (GF_Func_W / GF_Func_Wh, :284-295, :327-338)
-    double j0v, j1v, y0v, y1v;
-    bessel01(hh, j0v, j1v, y0v, y1v);
-    double h0v, h1v;
-    struve_h01(hh, y0v, y1v, h0v, h1v);
+    double j0v, j1v, y0v, y1v, h0v, h1v;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (S32) {
+        float y0f = 0.0f, y1f = 0.0f, h0f, h1f;
+        if (hh <= 3.0) {
+            bessel01_series(hh, j0v, j1v);
+        } else {
+            float j0f, j1f;
+            bessel01_f32(hh, j0f, j1f, y0f, y1f);
+            j0v = j0f; j1v = j1f;
+        }
+        struve_h01_f32(hh, y0f, y1f, h0f, h1f);
+        h0v = h0f; h1v = h1f;
+    } else
+#endif
+    {
+        bessel01(hh, j0v, j1v, y0v, y1v);
+        struve_h01(hh, y0v, y1v, h0v, h1v);
+    }
     double E = 2.0 * kPI * ev;
     GFVal o;
     o.gr = L0 + E * h0v;
@@ -228,10 +344,12 @@ __host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) {
     o.h = hh;
     return o;
 }
+__host__ __device__ inline GFVal havelock_gf(double X, double Y, double vv) { return havelock_gf_t<false>(X, Y, vv); }
 
 // Wave S/D influence of one mirrored panel at receiver (xr, yr, zr): sum over
 // its quadrature nodes of w G and w (grad_R G . n_q) (Panel.cpp:601-636,776-794).
 // q points to the panel's 32-double quadrature record.
+template <bool S32 = false>
 __host__ __device__ inline void wave_panel(const double* q, double xr, double yr, double zr, double K,
                                   double& sr, double& si, double& dr, double& di) {
     sr = si = dr = di = 0.0;
@@ -241,7 +359,7 @@ __host__ __device__ inline void wave_panel(const double* q, double xr, double yr
     for (int k = 0; k < nq; k++) {
         const double* p = q + 7 * k;
         double X = (xr - p[0]) * K, Y = (yr - p[1]) * K, V = (zr - p[2]) * K;
-        GFVal g = havelock_gf(X, Y, V);
+        GFVal g = havelock_gf_t<S32>(X, Y, V);
         double w = p[6];
         sr += w * (g.gr * s1);
         si += w * (g.gi * s1);

@@ -17,8 +17,9 @@ void launch_rankine(hipStream_t s, int N, int Np, const double* body, const doub
 void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
                          const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                          const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
-                         float* D32i);
-void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out);
+                         float* D32i, double far_r2 = -1.0);
+void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out,
+                          bool far32 = false);
 // FP32 and FP64 MFMA TFLOP/s and streaming-copy GB/s measured on the current device; 0 on success.
 int measure_peaks(hipStream_t s, double* out);
 void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,

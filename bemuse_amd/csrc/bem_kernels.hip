@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const doubl
                                                       const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
                                                       const double* __restrict__ Dsrc, const double* __restrict__ Drefl,
                                                       double* Sr, double* Si, double* Dr, double* Di,
-                                                      float* D32r, float* D32i) {
+                                                      float* D32r, float* D32i, double far_r2) {
     __shared__ double sq[AS_TS][32];
     const double K = freq[1];
     int r0 = blockIdx.x * AS_TR, s0 = blockIdx.y * AS_TS;
@@ -99,7 +99,13 @@ __global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const doubl
             double ss = Ssrc[idx], sf = Srefl[idx], ds = Dsrc[idx], df = Drefl[idx];
             if (compose == 0) {
                 double wsr, wsi, wdr, wdi;
-                wave_panel(sq[sl], px, py, pz, K, wsr, wsi, wdr, wdi);
+                // far_r2 >= 0: pairs whose first mirrored node lies farther than
+                // sqrt(far_r2) from the receiver take the FP32 special functions
+                const double ex = px - sq[sl][0], ey = py - sq[sl][1], ez = pz - sq[sl][2];
+                if (far_r2 >= 0.0 && ex * ex + ey * ey + ez * ez > far_r2)
+                    wave_panel<true>(sq[sl], px, py, pz, K, wsr, wsi, wdr, wdi);
+                else
+                    wave_panel<false>(sq[sl], px, py, pz, K, wsr, wsi, wdr, wdi);
                 sre = (ss + sf) - wsr;
                 sim = -wsi;
                 dre = (ds + df) + (wdr + sf * sq[sl][29] * K);
@@ -117,10 +123,11 @@ __global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const doubl
     }
 }
 
+template <bool S32>
 __global__ void k_havelock_eval(int n, const double* X, const double* Y, const double* V, double* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    GFVal g = havelock_gf(X[i], Y[i], V[i]);
+    GFVal g = havelock_gf_t<S32>(X[i], Y[i], V[i]);
     double cx = 0.0, cy = 0.0;
     if (g.h > 1.0e-6) { double rh = frcp(g.h); cx = X[i] * rh; cy = Y[i] * rh; }
     double* o = out + 8 * (size_t)i;
@@ -2421,14 +2428,16 @@ void launch_rankine(hipStream_t s, int N, int Np, const double* body, const doub
 void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
                          const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                          const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
-                         float* D32i) {
+                         float* D32i, double far_r2) {
     dim3 grid(Np / AS_TR, Np / AS_TS);
     hipLaunchKernelGGL(k_wave_compose, grid, dim3(256), 0, s, N, Np, freq, compose, quad, recv, Ssrc, Srefl, Dsrc,
-                       Drefl, Sr, Si, Dr, Di, D32r, D32i);
+                       Drefl, Sr, Si, Dr, Di, D32r, D32i, far_r2);
 }
 
-void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out) {
-    hipLaunchKernelGGL(k_havelock_eval, dim3(cdiv(n, 256)), dim3(256), 0, s, n, X, Y, V, out);
+void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out,
+                          bool far32) {
+    auto k = far32 ? k_havelock_eval<true> : k_havelock_eval<false>;
+    hipLaunchKernelGGL(k, dim3(cdiv(n, 256)), dim3(256), 0, s, n, X, Y, V, out);
 }
 
 void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, const double* gen, const double* betas,

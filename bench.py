@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=8, help="frequencies in flight (concurrent solve lanes)")
     ap.add_argument("--surrogate", type=int, default=1, help="1: also train the wave-term surrogate and time "
                     "MIXED_SURROGATE solves of every 5th frequency (reported beside the headline)")
+    ap.add_argument("--far32", type=int, default=1, help="1: also time mixed solves with the far-pair FP32 wave "
+                    "term at several radii (reported beside the headline)")
     ap.add_argument("--workload", default="semisub6k", choices=["semisub6k", "batch64", "spar20k"],
                     help="semisub6k: the headline line; batch64: 64 seeded uneven bodies x 16 frequencies; "
                          "spar20k: 20736-panel cone spar x 24 frequencies (extra lines, not the headline)")
@@ -228,6 +230,14 @@ def main():
             surrogate = {"error": f"{type(e).__name__}: {e}"}
             print(f"[bench] surrogate leg failed: {surrogate['error']}", file=sys.stderr, flush=True)
 
+    far32 = None
+    if args.far32:
+        try:
+            far32 = far32_leg(bm, ctx, mesh, body, omegas, betas, fp64_by_freq, args.lanes)
+        except Exception as e:
+            far32 = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] far-pair FP32 leg failed: {far32['error']}", file=sys.stderr, flush=True)
+
     value = args.steps / dtmx
     line = {
         "metric": METRIC,
@@ -260,6 +270,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "surrogate": surrogate,
+        "far32": far32,
     }
     print(json.dumps(line), flush=True)
 
@@ -332,6 +343,56 @@ def surrogate_leg(bm, ctx, body, omegas, betas, fp64_by_freq, lanes):
             "judged_pair_rel_err": float(max(i["surrogate_err"] for i in infos)),
             "train_s": t_train, "train": {k: rep[k] for k in ("loss_initial", "loss_final", "rel_rms", "rel_p99",
                                                               "rel_max", "ms_sample", "ms_train")}}
+
+
+def far_pair_fraction(mesh, radius):
+    """Share of receiver / source pairs farther apart than radius, receiver
+    centroid against the mirrored source centroid (the kernel tests the
+    source's first mirrored quadrature node, so this is close, not exact)."""
+    nodes, elems = mesh.arrays()
+    c = nodes[elems].mean(axis=1)
+    m = c * np.array([1.0, 1.0, -1.0])
+    far = 0
+    for i in range(0, len(c), 512):
+        d2 = ((c[i:i + 512, None, :] - m[None, :, :]) ** 2).sum(axis=2)
+        far += int((d2 > radius * radius).sum())
+    return far / float(len(c)) ** 2
+
+
+def far32_leg(bm, ctx, mesh, body, omegas, betas, fp64_by_freq, lanes):
+    """Mixed solves of every 5th frequency with the wave term of far pairs on
+    FP32 special functions (Context.set_wave_far32), at several radii beside
+    the exact setting (radius < 0): sweep throughput, the assembly time of one
+    isolated solve, and the A, B, F errors against the FP64 GPU results of the
+    same frequencies. A measurement, not the headline: the headline keeps the
+    exact FP64 wave term."""
+    idx = np.arange(0, len(omegas), 5)
+    missing = [int(j) for j in idx if int(j) not in fp64_by_freq]
+    if missing:
+        sw64 = body.solve_sweep(omegas[missing], betas=betas, mode=bm.MODE_FP64, lanes=lanes)
+        for i, j in enumerate(missing):
+            fp64_by_freq[j] = dict(A=sw64["A"][i], B=sw64["B"][i], F=sw64["F"][i])
+    rows = []
+    try:
+        for radius in (-1.0, 40.0, 25.0, 10.0, 0.0):
+            ctx.set_wave_far32(radius)
+            body.solve_sweep(omegas[idx[:2]], betas=betas, mode=bm.MODE_MIXED, lanes=lanes)
+            t0 = time.perf_counter()
+            sw = body.solve_sweep(omegas[idx], betas=betas, mode=bm.MODE_MIXED, lanes=lanes)
+            dt = time.perf_counter() - t0
+            per = {k: max(rel(sw[k][i], fp64_by_freq[int(j)][k]) for i, j in enumerate(idx)) for k in ("A", "B", "F")}
+            iso = [body.solve(omegas[j], betas=betas, mode=bm.MODE_MIXED)["info"]["ms_assemble"] for j in idx[3:6]]
+            rows.append({"near_radius_m": radius,
+                         "far_pair_share": 0.0 if radius < 0 else far_pair_fraction(mesh, radius),
+                         "value": len(idx) / dt, "ms_assemble_isolated": float(np.mean(iso)),
+                         "max_rel_err_vs_fp64_gpu": per,
+                         "fallbacks": int(sum(i["fell_back_fp64"] for i in sw["info"]))})
+            print(f"[bench] far32 radius {radius:g}: {len(idx) / dt:.2f}/s, assembly {np.mean(iso):.2f} ms, "
+                  f"max err {max(per.values()):.2e}", file=sys.stderr, flush=True)
+    finally:
+        ctx.set_wave_far32(-1.0)
+    return {"unit": "frequency-solves/s", "frequencies": [float(w) for w in omegas[idx]], "lanes": lanes,
+            "mode": "mixed", "radii": rows}
 
 
 def other_workload(args, bm):

@@ -194,6 +194,18 @@ int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
 enum { BEM_LU_FP32 = 0, BEM_LU_SPLIT_F16 = 1, BEM_LU_SPLIT_F16_TILE = 2 };
 int bem_set_lu_arith(bem_ctx* ctx, int arith);
 
+/* Measurement option, off by default: the wave term of receiver / source
+ * pairs whose first mirrored quadrature node lies farther than near_radius
+ * (metres) from the receiver is evaluated with FP32 exp, log, Bessel, Struve
+ * and sin/cos (the Green function's polynomials and combination stay FP64),
+ * in the FP64 and mixed modes (the surrogate mode's exact pairs and the
+ * field-point evaluation keep FP64). near_radius < 0 turns it off (the exact FP64 wave term
+ * everywhere, the default); 0 sends every pair to it. Results then differ
+ * from the exact ones by the FP32 error of those functions (about 1e-7
+ * relative per term). No counterpart in the reference (its wave term is
+ * FP64, or FP32 throughout in its single-precision build). */
+int bem_set_wave_far32(bem_ctx* ctx, double near_radius);
+
 /* Measured machine peaks of the context's device, for reporting beside the
  * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
  * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +
@@ -272,7 +284,9 @@ int bem_get_rankine(bem_ctx* ctx, bem_body* b, double* Ssrc, double* Srefl, doub
  * as produced by the device assembly in the given mode. */
 int bem_get_system(bem_ctx* ctx, bem_body* b, int mode, double kappa, int compose, double* SMat, double* DMat);
 /* Device evaluation of HavelockGF (Kernels.cpp:342-412) at n points;
- * out: 8 doubles per point (GF, DGFX, DGFY, DGFZ as re, im). precision: 64. */
+ * out: 8 doubles per point (GF, DGFX, DGFY, DGFZ as re, im). precision: 64
+ * (the exact kernel), or 32 for the function the far-pair option of
+ * bem_set_wave_far32 evaluates (FP32 special functions). */
 int bem_eval_havelock(bem_ctx* ctx, int precision, int n, const double* X, const double* Y,
                       const double* V, double* out);
 /* The constant tables the device HavelockGF is compiled with, for checking

@@ -71,6 +71,34 @@ def test_havelock_device_vs_oracle(ctx):
         assert err < 1e-11, (k, err)
 
 
+def test_havelock_far32_vs_oracle(ctx):
+    """The far-pair variant of the Green function (FP32 exp, log, Bessel and
+    Struve above h = 3; FP64 polynomials, and FP64 e^V, J0, J1 below h = 3)
+    against the FP64 oracle: within 3e-6 of max(|value|, 1) per component.
+    The wave parts are sums of terms up to 2 pi in size, each evaluated in
+    FP32 (half an ulp of 2 pi is 2.4e-7), and 2/pi - H1 cancels to |Y1| at
+    large h; a handful of such roundings is about 1e-6. The exact function's
+    own test holds 1e-11, so this also shows the variant ran."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    h = rng.uniform(0, 60, n) * rng.uniform(0, 1, n) ** 2
+    th = rng.uniform(0, 2 * np.pi, n)
+    X, Y = h * np.cos(th), h * np.sin(th)
+    V = -rng.uniform(0, 40, n) * rng.uniform(0, 1, n) ** 2 - 1e-3
+    gpu = ctx.eval_havelock(X, Y, V, precision=32)
+    ref = oracle.havelock(X, Y, V)
+    g = gpu[:, 0::2] + 1j * gpu[:, 1::2]
+    errs = []
+    for k in range(4):
+        scale = np.maximum(np.abs(ref[:, k]), 1.0)
+        errs.append(float((np.abs(g[:, k] - ref[:, k]) / scale).max()))
+    print(f"\n[gpu] far32 Green function max errors {errs}", flush=True)
+    assert max(errs) < 3e-6, errs
+    assert max(errs) > 1e-12, errs
+    with pytest.raises(bm.BemError):
+        ctx.eval_havelock(X[:4], Y[:4], V[:4], precision=16)
+
+
 def test_rankine_matrices(cyl):
     mesh, body, ob = cyl
     R = body.get_rankine()
@@ -209,6 +237,40 @@ def test_compose_cases(cyl):
         r = ob.solve(0.0 if compose == bm.COMPOSE_ZERO else -1.0, kappa=0.0, compose=compose, rad_only=True)
         assert relerr(g["A"], r["A"]) < 1e-8
         assert relerr(g["B"] + 1.0, r["B"] + 1.0) < 1e-8
+
+
+@pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
+def test_wave_far32_option(ctx, cyl, mode):
+    """Far pairs on FP32 special functions: off (radius < 0) and a radius
+    beyond every pair (the cylinder's mirrored pairs lie within 20 m) give the
+    default results bit for bit; radius 0 sends every pair to FP32 exp, log,
+    Bessel, Struve and sin/cos, which must change the results (the path
+    ran) and keep A, B, F within 1e-6 of the oracle (FP32 rounding of those
+    functions, about 1e-7 relative each, through the wave term): the mixed
+    mode's 1e-6 bar."""
+    mesh, body, ob = cyl
+    betas = [0.0, 1.2]
+    w = 1.1
+    ref = body.solve(w, betas=betas, mode=mode)
+    r = ob.solve(w, betas=betas)
+    try:
+        for radius in (-1.0, 100.0):
+            ctx.set_wave_far32(radius)
+            g = body.solve(w, betas=betas, mode=mode)
+            for key in ("A", "B", "F"):
+                assert np.array_equal(g[key], ref[key]), (radius, key)
+        ctx.set_wave_far32(0.0)
+        g = body.solve(w, betas=betas, mode=mode)
+        errs = {key: relerr(g[key], r[key]) for key in ("A", "B", "F")}
+        print(f"\n[gpu] far32 radius 0, mode {mode}: A/B/F rel err vs oracle {errs}", flush=True)
+        assert any(not np.array_equal(g[key], ref[key]) for key in ("A", "B", "F"))
+        for key, e in errs.items():
+            assert e < 1e-6, (key, e)
+        assert g["info"]["fell_back_fp64"] == 0
+    finally:
+        ctx.set_wave_far32(-1.0)
+    with pytest.raises(bm.BemError):
+        ctx.set_wave_far32(float("nan"))
 
 
 @pytest.mark.parametrize("mode", [bm.MODE_FP64, bm.MODE_MIXED])
