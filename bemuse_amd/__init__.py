@@ -64,6 +64,7 @@ class SolveInfo(ctypes.Structure):
         ("surrogate_points", ctypes.c_int64),
         ("exact_points", ctypes.c_int64),
         ("surrogate_err", c_double),
+        ("lanes", c_int),
     ]
 
     def as_dict(self):
@@ -161,6 +162,7 @@ _SIGS = {
     "bem_set_graphs": (c_int, [c_void_p, c_int]),
     "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
     "bem_set_lu_arith": (c_int, [c_void_p, c_int]),
+    "bem_set_lu_tile": (c_int, [c_void_p, c_int]),
     "bem_set_wave_far32": (c_int, [c_void_p, c_double]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
@@ -449,6 +451,13 @@ class Context:
         """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default), LU_FP32, or
         LU_SPLIT_F16_TILE (the default's arithmetic with the split in every tile)."""
         _check(lib().bem_set_lu_arith(self._h, int(arith)))
+
+    def set_lu_tile(self, tile):
+        """Tile of the LU_SPLIT_F16 trailing updates: -1 autotuned (default), 0-2 the
+        32x32-wave-block tiles (64x64, 128x64, 64x128) with loads one chunk ahead, 5-7 the
+        same two chunks ahead, 3-4 the 128x128 tiles with 64x64 wave blocks (one or two
+        LDS chunks). Results do not depend on it."""
+        _check(lib().bem_set_lu_tile(self._h, int(tile)))
 
     def set_graphs(self, enable):
         """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""

@@ -825,7 +825,9 @@ int ensure_lanes(bem_body* b, int n, bool need32) {
     Lane& L0 = *b->lanes[0];
     if (L0.pending < 0) {
         tune_lu_gemms<double>(L0.s1, b->Np, L0.sys.Dr, L0.sys.Di);
-        if (need32) tune_lu_gemms<float>(L0.s1, b->Np, L0.sys.D32r, L0.sys.D32i, L0.sys.w32.umax);
+        if (need32)
+            tune_lu_gemms<float>(L0.s1, b->Np, L0.sys.D32r, L0.sys.D32i, L0.sys.w32.umax, L0.sys.w32.opA,
+                                 L0.sys.w32.opB);
         HIP_TRY(hipStreamSynchronize(L0.s1));
     }
     return BEM_OK;
@@ -1081,6 +1083,7 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
             size_t f = (size_t)t.out;
             r = harvest(t.b, *lanes[pick], A36 + 36 * f, B36 + 36 * f, F ? F + fs * f : nullptr,
                         FK ? FK + fs * f : nullptr, SC ? SC + fs * f : nullptr, infos ? infos + f : nullptr);
+            if (infos) infos[f].lanes = nl;
             busy[pick] = -1;
             if (r == BEM_ERR_NUMERIC) {   // that frequency's results are NaN; the others go on
                 if (numeric == BEM_OK) { numeric = r; numeric_msg = g_err; }
@@ -1129,7 +1132,15 @@ int bem_set_wave_far32(bem_ctx* c, double near_radius) {
 int bem_set_lu_arith(bem_ctx* c, int arith) {
     if (!c || (arith != BEM_LU_FP32 && arith != BEM_LU_SPLIT_F16 && arith != BEM_LU_SPLIT_F16_TILE))
         return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32, BEM_LU_SPLIT_F16 or BEM_LU_SPLIT_F16_TILE");
-    c->lu_arith = arith == BEM_LU_FP32 ? kLuArithF32 : arith == BEM_LU_SPLIT_F16 ? kLuArithSplitF16 : kLuArithSplitF16Tile;
+    const int base = arith == BEM_LU_FP32 ? kLuArithF32 : arith == BEM_LU_SPLIT_F16 ? kLuArithSplitF16 : kLuArithSplitF16Tile;
+    c->lu_arith = (c->lu_arith & ~kLuArithMask) | base;
+    return BEM_OK;
+}
+
+int bem_set_lu_tile(bem_ctx* c, int tile) {
+    if (!c || tile < -1 || tile >= kLuTileCfgs)
+        return fail(BEM_ERR_ARG, "bem_set_lu_tile: tile in [-1, 7]");
+    c->lu_arith = (c->lu_arith & kLuArithMask) | ((tile + 1) << kLuTileShift);
     return BEM_OK;
 }
 
@@ -1161,14 +1172,42 @@ int bem_solve_frequency(bem_ctx* c, bem_body* b, int mode, double omega, double 
     return BEM_OK;
 }
 
+// Lanes for ntask tasks of padded order up to Np when the caller leaves the
+// choice to the library: at most `pool`, spread so that every round of tasks
+// fills the lanes evenly (20 tasks: 10 lanes twice, not 8 + 8 + 4; the last,
+// partly filled round runs with the device half idle).
+constexpr int kAutoLanes = 10;
+int auto_lanes(int ntask, int pool) {
+    int lanes = std::max(1, std::min(pool, ntask));
+    const int rounds = (ntask + lanes - 1) / lanes;
+    return std::max(1, (ntask + rounds - 1) / rounds);
+}
+// Lanes the library keeps for such calls: kAutoLanes, or fewer where half the
+// free device memory would not hold them (a lane keeps about 48 Np^2 bytes:
+// FP64 S and D, the FP32 copy of D, workspaces), counting lanes already
+// there. They are made on the first call that leaves the choice to the
+// library, whatever its task count, so that a short first sweep (a warm-up)
+// does not leave the allocation of the rest to the next one.
+int auto_pool(int Np, int have) {
+    size_t freeb = 0, total = 0;
+    if (hipMemGetInfo(&freeb, &total) != hipSuccess) return std::max(1, std::min(kAutoLanes, have));
+    const size_t per = (size_t)48 * Np * Np;
+    return std::max(1, std::min(kAutoLanes, have + (int)(freeb / 2 / per)));
+}
+
 int bem_solve_sweep(bem_ctx* c, bem_body* b, int mode, int nf, const double* omegas, const double* kappas,
                     int compose, int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
                     double* FK, double* SC, bem_solve_info* infos) {
     int r = check_solve_args(c, b, mode, compose, nb, betas);
     if (r) return r;
-    if (nf < 0 || !omegas || !A36 || !B36 || lanes < 1 || lanes > 16)
-        return fail(BEM_ERR_ARG, "bem_solve_sweep: bad arguments (lanes in [1, 16])");
+    if (nf < 0 || !omegas || !A36 || !B36 || lanes < 0 || lanes > 16)
+        return fail(BEM_ERR_ARG, "bem_solve_sweep: bad arguments (lanes in [0, 16])");
     HIP_TRY(hipSetDevice(c->device));
+    if (lanes == 0) {
+        const int pool = auto_pool(b->Np, (int)b->lanes.size());
+        if ((r = ensure_lanes(b, pool, mode != BEM_MODE_FP64))) return r;
+        lanes = auto_lanes(nf, pool);
+    }
     if ((r = ensure_lanes(b, lanes, mode != BEM_MODE_FP64))) return r;
     for (int l = 0; l < lanes; l++)
         if ((r = upload_betas(*b->lanes[l], nb, betas))) return r;
@@ -1185,8 +1224,8 @@ int bem_solve_sweep(bem_ctx* c, bem_body* b, int mode, int nf, const double* ome
 int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* nfreq, const double* omegas, int mode,
                     int nb, const double* betas, int lanes, double* A36, double* B36, double* F,
                     bem_solve_info* infos) {
-    if (!c || nbody < 1 || !bodies || !nfreq || !omegas || !A36 || !B36 || lanes < 1 || lanes > 16)
-        return fail(BEM_ERR_ARG, "bem_solve_batch: bad arguments (lanes in [1, 16])");
+    if (!c || nbody < 1 || !bodies || !nfreq || !omegas || !A36 || !B36 || lanes < 0 || lanes > 16)
+        return fail(BEM_ERR_ARG, "bem_solve_batch: bad arguments (lanes in [0, 16])");
     int capNp = 0, total = 0, r = BEM_OK;
     for (int i = 0; i < nbody; i++) {
         if ((r = check_solve_args(c, bodies[i], mode, BEM_COMPOSE_WAVE, nb, betas))) return r;
@@ -1199,11 +1238,21 @@ int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* n
             return fail(BEM_ERR_ARG, "bem_solve_batch: omegas must be finite");
     HIP_TRY(hipSetDevice(c->device));
     bool need32 = mode != BEM_MODE_FP64;
+    int make = lanes;
+    if (lanes == 0) {
+        const bool fits = !c->pool.empty() && c->pool[0]->sys.capNp >= capNp;
+        if (!fits) {   // the pool is remade for this batch: its memory is free then
+            for (Lane* l : c->pool) delete l;
+            c->pool.clear();
+        }
+        make = auto_pool(capNp, (int)c->pool.size());
+        lanes = auto_lanes(total, make);
+    }
     // lane pool of the context, sized for the largest body of the batch
-    if ((int)c->pool.size() < lanes || (!c->pool.empty() && c->pool[0]->sys.capNp < capNp)) {
+    if ((int)c->pool.size() < make || (!c->pool.empty() && c->pool[0]->sys.capNp < capNp)) {
         for (Lane* l : c->pool) delete l;
         c->pool.clear();
-        while ((int)c->pool.size() < lanes) {
+        while ((int)c->pool.size() < make) {
             Lane* L = new Lane;
             if ((r = L->init(capNp))) { delete L; return r; }
             c->pool.push_back(L);
@@ -1221,7 +1270,9 @@ int bem_solve_batch(bem_ctx* c, int nbody, bem_body* const* bodies, const int* n
     nps.erase(std::unique(nps.begin(), nps.end()), nps.end());
     for (int np : nps) {
         tune_lu_gemms<double>(ls[0]->s1, np, ls[0]->sys.Dr, ls[0]->sys.Di);
-        if (need32) tune_lu_gemms<float>(ls[0]->s1, np, ls[0]->sys.D32r, ls[0]->sys.D32i, ls[0]->sys.w32.umax);
+        if (need32)
+            tune_lu_gemms<float>(ls[0]->s1, np, ls[0]->sys.D32r, ls[0]->sys.D32i, ls[0]->sys.w32.umax,
+                                 ls[0]->sys.w32.opA, ls[0]->sys.w32.opB);
     }
     HIP_TRY(hipStreamSynchronize(ls[0]->s1));
     // tasks, largest body first (the long factorisations start early)

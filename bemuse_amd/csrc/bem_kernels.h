@@ -75,14 +75,18 @@ struct LuWork {
     _Float16 *opA = nullptr, *opB = nullptr;
 };
 
-// Arithmetic of the FP32 factorisation's trailing updates.
+// Arithmetic of the FP32 factorisation's trailing updates. The bits above
+// kLuArithMask carry a forced tile configuration of the updates on pre-split
+// operands: (cfg + 1) << 4, 0 = autotuned.
 enum { kLuArithF32 = 0, kLuArithSplitF16 = 1, kLuArithSplitF16Tile = 2 };
+constexpr int kLuArithMask = 15, kLuTileShift = 4, kLuTileCfgs = 8;
 
 // s2: panel (lookahead) stream; ev: 2 events per panel (Np / NB panels).
 // arith (FP32 only): kLuArithF32 (FP32 MFMA) or kLuArithSplitF16 (fp16 MFMA on
 // split operands, w.umax required; operands split once per panel group when
 // w.opA, w.opB are there) or kLuArithSplitF16Tile (the same, split in every
-// tile). latency != 0: the factorisation has the
+// tile); a forced tile configuration in the bits above kLuArithMask.
+// latency != 0: the factorisation has the
 // device to itself (one frequency in flight), so the panel kernels take the
 // variant with the shorter critical path and more registers; the results are
 // the same bit for bit.
@@ -95,7 +99,8 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
 // umax (FP32: Np / 64 floats, overwritten): the panel maxima the split-
 // arithmetic tiles read; with it the split tiles are tuned too.
 template <typename T>
-int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax = nullptr);
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax = nullptr, const _Float16* AS = nullptr,
+                  const _Float16* BS = nullptr);
 // Tile chosen for an M x N x K update on the current device: 1000 * rows + cols.
 template <typename T>
 int gemm_tile_choice(int M, int N, int K);

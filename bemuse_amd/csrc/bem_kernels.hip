@@ -1683,7 +1683,7 @@ __global__ __launch_bounds__(256) void k_split_ops(int M, int N, int K, const fl
 // are AS rows 0 .. M-1 (C rows rc ..); B columns are BS columns nb0 .. nb0+N-1
 // of ldn (C columns cc ..); the MFMA sequence and the epilogue are those of
 // k_cgemm_sub_h2, so the results are the same bit for bit.
-template <int WM, int WN>
+template <int WM, int WN, int PD>
 __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, float* __restrict__ Pr,
                                                          float* __restrict__ Pi, int ld, int rb, int rc, int cc,
                                                          int critical, const float* __restrict__ umax,
@@ -1717,32 +1717,34 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, fl
 
     // staged halves: (part, row or column, k half of the chunk), 16 bytes each
     constexpr int NA16 = 4 * TM * 2 / 256, NB16 = 4 * TN * 2 / 256;
-    h8_t ha[NA16], hb[NB16];
+    // PD register sets: chunk c waits in set c % PD from its load, issued PD
+    // chunks ahead, until its store to LDS at the end of chunk c - 1
+    h8_t ha[PD][NA16], hb[PD][NB16];
     const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
-    auto gload = [&](int k0) {
+    auto gload = [&](int k0, int set) {
 #pragma unroll
         for (int l = 0; l < NA16; l++) {
             int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
             size_t o = part * pa + ((size_t)(k0 / HK) * M + rowt + min(row, mrows - 1)) * HK + 8 * hf;
-            ha[l] = *reinterpret_cast<const h8_t*>(AS + o);
+            ha[set][l] = *reinterpret_cast<const h8_t*>(AS + o);
         }
 #pragma unroll
         for (int l = 0; l < NB16; l++) {
             int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
             size_t o = part * pb + ((size_t)(k0 / HK) * ldn + nb0 + colt + min(col, ncols - 1)) * HK + 8 * hf;
-            hb[l] = *reinterpret_cast<const h8_t*>(BS + o);
+            hb[set][l] = *reinterpret_cast<const h8_t*>(BS + o);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, int set) {
 #pragma unroll
         for (int l = 0; l < NA16; l++) {
             int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
-            *reinterpret_cast<h8_t*>(&As[buf][part][row][8 * hf]) = ha[l];
+            *reinterpret_cast<h8_t*>(&As[buf][part][row][8 * hf]) = ha[set][l];
         }
 #pragma unroll
         for (int l = 0; l < NB16; l++) {
             int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
-            *reinterpret_cast<h8_t*>(&Bs[buf][part][col][8 * hf]) = hb[l];
+            *reinterpret_cast<h8_t*>(&Bs[buf][part][col][8 * hf]) = hb[set][l];
         }
     };
 
@@ -1759,12 +1761,18 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, fl
             accI[a][b] = f16x_t{};
         }
     const int nchunk = K / HK;
-    gload(0);
-    sstore(0);
+#pragma unroll
+    for (int u = 0; u < PD; u++)
+        if (u < nchunk) gload(u * HK, u);
+    sstore(0, 0);
     __syncthreads();
-    for (int c = 0; c < nchunk; c++) {
+    for (int c0 = 0; c0 < nchunk; c0 += PD)
+#pragma unroll
+    for (int u = 0; u < PD; u++) {
+        const int c = c0 + u;   // its halves came from set u
+        if (c >= nchunk) break;
         const int buf = c & 1;
-        if (c + 1 < nchunk) gload((c + 1) * HK);
+        if (c + PD < nchunk) gload((c + PD) * HK, u);
         if (c == 0 && active) {   // C, added at the end: its loads overlap the first chunk
 #pragma unroll
             for (int a = 0; a < WM; a++)
@@ -1824,7 +1832,7 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, fl
                 }
         }
         if (c + 1 < nchunk) {
-            sstore(buf ^ 1);
+            sstore(buf ^ 1, (u + 1) % PD);
             __syncthreads();
         }
     }
@@ -1838,6 +1846,159 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, fl
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 float4 r = cR[a][b][q], m = cI[a][b][q];
+                r.x = fmaf(accR[a][b][4 * q + 0], inv, r.x);
+                r.y = fmaf(accR[a][b][4 * q + 1], inv, r.y);
+                r.z = fmaf(accR[a][b][4 * q + 2], inv, r.z);
+                r.w = fmaf(accR[a][b][4 * q + 3], inv, r.w);
+                m.x = fmaf(accI[a][b][4 * q + 0], inv, m.x);
+                m.y = fmaf(accI[a][b][4 * q + 1], inv, m.y);
+                m.z = fmaf(accI[a][b][4 * q + 2], inv, m.z);
+                m.w = fmaf(accI[a][b][4 * q + 3], inv, m.w);
+                *reinterpret_cast<float4*>(Pr + off + 8 * q) = r;
+                *reinterpret_cast<float4*>(Pi + off + 8 * q) = m;
+            }
+        }
+}
+
+// Wide tiles for k_cgemm_sub_hs's updates: 128 x 128 per work-group, four
+// waves in a 2 x 2 grid with 2 x 2 blocks of 32 x 32 each (64 x 64 per wave).
+// Every 16-byte operand read from LDS then feeds two blocks' MFMAs, half the
+// LDS reads per MFMA of the 32 x 64 wave tile, and the work-group reads 32 KB
+// of operands per 192 MFMAs instead of 24 KB per 96.
+// - C is read after the products: its registers do not fit beside the four
+//   blocks' accumulators. The other work-group on the CU runs meanwhile.
+// - DB = false: the LDS holds one chunk (49 KB, two work-groups per CU); the
+//   next chunk's halves wait in registers while the current one is
+//   multiplied, with a barrier before and after the store. DB = true: two
+//   chunks in LDS (98 KB, one work-group per CU), one barrier per chunk.
+// - The products of each accumulator are those of k_cgemm_sub_hs in the same
+//   order, and the epilogue is the same: equal results bit for bit. The four
+//   accumulators of a 32-row block pair take their products interleaved, so
+//   no MFMA waits on the one before it.
+// M and N are multiples of 64, so a wave's 64 x 64 block lies wholly inside
+// the update or wholly outside (the host checks).
+template <bool DB>
+__global__ __launch_bounds__(256, DB ? 1 : 2) void k_cgemm_sub_hw(int M, int N, int K, float* __restrict__ Pr,
+                                                                  float* __restrict__ Pi, int ld, int rb, int rc,
+                                                                  int cc, int critical,
+                                                                  const float* __restrict__ umax,
+                                                                  const _Float16* __restrict__ AS,
+                                                                  const _Float16* __restrict__ BS, int nb0, int ldn) {
+    if (critical) __builtin_amdgcn_s_setprio(3);
+    constexpr int TM = 128, TN = 128, NBUF = DB ? 2 : 1;
+    __shared__ __align__(16) _Float16 As[NBUF][4][TM][HKP];   // [buf][Re h, Re l, Im h, Im l][row][k]
+    __shared__ __align__(16) _Float16 Bs[NBUF][4][TN][HKP];   // [buf][part][col][k]
+    const int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN, ntiles = tiles_m * tiles_n;
+    constexpr int GROUP_M = 8;                     // XCD-aware grouped order, as in k_cgemm_sub
+    const int bid = blockIdx.x, per = (ntiles + 7) / 8;
+    const int tile = (bid % 8) * per + bid / 8;
+    if (tile >= ntiles) return;
+    const int group = tile / (GROUP_M * tiles_n), first_m = group * GROUP_M;
+    const int gsize = min(GROUP_M, tiles_m - first_m), in_group = tile - group * GROUP_M * tiles_n;
+    const int tm = first_m + in_group % gsize, tn = in_group / gsize;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rowt = tm * TM, colt = tn * TN;
+    const int mrows = min(TM, M - rowt), ncols = min(TN, N - colt);
+    float um = 0.0f;
+    for (int q = 0; q < K / LU_NB; q++) um = fmaxf(um, umax[rb / LU_NB + q]);
+    int eb = 0;
+    if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);   // um < 2^eb
+    const float inv = ldexpf(1.0f, eb - kSplitExpB - kSplitExpA);
+
+    // staging: load l is part l of row (column) tid / 2, k half tid % 2 of the chunk
+    const int srow = tid >> 1, shf = tid & 1;
+    const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
+    const size_t arow = (size_t)(rowt + min(srow, mrows - 1)), bcol = (size_t)(nb0 + colt + min(srow, ncols - 1));
+    h8_t ha[4], hb[4];
+    auto gload = [&](int c) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            ha[l] = *reinterpret_cast<const h8_t*>(AS + l * pa + ((size_t)c * M + arow) * HK + 8 * shf);
+            hb[l] = *reinterpret_cast<const h8_t*>(BS + l * pb + ((size_t)c * ldn + bcol) * HK + 8 * shf);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            *reinterpret_cast<h8_t*>(&As[buf][l][srow][8 * shf]) = ha[l];
+            *reinterpret_cast<h8_t*>(&Bs[buf][l][srow][8 * shf]) = hb[l];
+        }
+    };
+
+    const int row0 = (wave >> 1) * 64, col0 = (wave & 1) * 64;
+    const bool active = row0 < mrows && col0 < ncols;
+    const int li = lane & 31, lk = 8 * (lane >> 5);
+    f16x_t accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            accR[a][b] = f16x_t{};
+            accI[a][b] = f16x_t{};
+        }
+    const int nchunk = K / HK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; c++) {
+        const int buf = DB ? (c & 1) : 0;
+        if (c + 1 < nchunk) gload(c + 1);
+        if (active) {
+            h8_t bo[2][4];   // [b][Re h, Re l, Im h, Im l]
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+                    bo[b][p] = *reinterpret_cast<const h8_t*>(&Bs[buf][p][col0 + 32 * b + li][lk]);
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                h8_t ao[4];
+#pragma unroll
+                for (int p = 0; p < 4; p++)
+                    ao[p] = *reinterpret_cast<const h8_t*>(&As[buf][p][row0 + 32 * a + li][lk]);
+                const h8_t nih = neg8(ao[2]), nil = neg8(ao[3]);
+                // Re: arh brh, arh brl, arl brh, -aih bih, -aih bil, -ail bih
+                // Im: arh bih, arh bil, arl bih,  aih brh,  aih brl,  ail brh
+                const h8_t ra[6] = {ao[0], ao[0], ao[1], nih, nih, nil};
+                const h8_t ia[6] = {ao[0], ao[0], ao[1], ao[2], ao[2], ao[3]};
+                constexpr int rbp[6] = {0, 1, 0, 2, 3, 2}, ibp[6] = {2, 3, 2, 0, 1, 0};
+#pragma unroll
+                for (int s = 0; s < 6; s++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        accR[a][b] = mfma32(ra[s], bo[b][rbp[s]], accR[a][b]);
+                        accI[a][b] = mfma32(ia[s], bo[b][ibp[s]], accI[a][b]);
+                    }
+            }
+        }
+        if (c + 1 < nchunk) {
+            if constexpr (DB) {
+                sstore(buf ^ 1);
+                __syncthreads();
+            } else {
+                __syncthreads();
+                sstore(0);
+                __syncthreads();
+            }
+        }
+    }
+    if (!active) return;
+    // accumulator element e of a 32 x 32 block: row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const size_t off =
+                (size_t)(cc + colt + col0 + 32 * b + li) * ld + rc + rowt + row0 + 32 * a + 4 * (lane >> 5);
+            float4 cr[4], ci[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cr[q] = *reinterpret_cast<const float4*>(Pr + off + 8 * q);
+                ci[q] = *reinterpret_cast<const float4*>(Pi + off + 8 * q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float4 r = cr[q], m = ci[q];
                 r.x = fmaf(accR[a][b][4 * q + 0], inv, r.x);
                 r.y = fmaf(accR[a][b][4 * q + 1], inv, r.y);
                 r.z = fmaf(accR[a][b][4 * q + 2], inv, r.z);
@@ -2629,11 +2790,13 @@ template <typename T> constexpr TileCfg cfg_of(int i) { return kCfg[i]; }
 constexpr int kMaxDev = 16, kClasses = 11;
 struct TuneTable {
     signed char pick[2][kClasses][kClasses][3];   // [prec][M class][N class][K class: 64, 128, 192-256]
-    signed char split[kClasses][kClasses][3];     // split-arithmetic FP32 updates
-    bool tuned_np[3][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128): FP32, FP64, split
+    signed char split[kClasses][kClasses][3];     // split-arithmetic FP32 updates, split in the tiles
+    signed char pre[kClasses][kClasses][3];       // split-arithmetic FP32 updates on pre-split operands
+    bool tuned_np[4][kMaxPanels / 2 + 1];          // padded orders already tuned (Np / 128): FP32, FP64, split, pre
     TuneTable() {
         std::memset(pick, -1, sizeof(pick));
         std::memset(split, -1, sizeof(split));
+        std::memset(pre, -1, sizeof(pre));
         std::memset(tuned_np, 0, sizeof(tuned_np));
     }
 };
@@ -2689,25 +2852,42 @@ void launch_split(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float*
 }
 int default_split(int M, int N) { return cdiv(M, 128) * cdiv(N, 64) >= 512 ? 1 : 0; }
 
-// The same updates on operands split beforehand (k_split_ops), same tile
-// choices as launch_split.
-template <int WM, int WN>
+// The same updates on operands split beforehand (k_split_ops): the tiles of
+// launch_split (configurations 0-2), and the 128 x 128 tiles of
+// k_cgemm_sub_hw with one LDS chunk (3) or two (4), for M and N multiples of
+// 64 (else configuration 0).
+template <int WM, int WN, int PD>
 void launch_split_pre_tile(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc, int cc,
                            int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
                            int ldn) {
     constexpr int TM = 64 * WM, TN = 64 * WN;
     int tiles = (int)(cdiv(M, TM) * cdiv(N, TN));
     int per = (tiles + 7) / 8;
-    hipLaunchKernelGGL((k_cgemm_sub_hs<WM, WN>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc,
+    hipLaunchKernelGGL((k_cgemm_sub_hs<WM, WN, PD>), dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc,
                        critical, umax, AS, BS, nb0, ldn);
 }
+template <bool DB>
+void launch_split_wide(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc, int cc,
+                       int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0, int ldn) {
+    int tiles = (int)(cdiv(M, 128) * cdiv(N, 128));
+    int per = (tiles + 7) / 8;
+    hipLaunchKernelGGL(k_cgemm_sub_hw<DB>, dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical,
+                       umax, AS, BS, nb0, ldn);
+}
+constexpr int kSplitPreCfgs = 8;
 void launch_split_pre(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc,
                       int cc, int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
                       int ldn) {
+    if (cfg >= 3 && (M % 64 != 0 || N % 64 != 0)) cfg = 0;
     switch (cfg) {
-    case 1: launch_split_pre_tile<2, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
-    case 2: launch_split_pre_tile<1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
-    default: launch_split_pre_tile<1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 3: launch_split_wide<false>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 4: launch_split_wide<true>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 5: launch_split_pre_tile<1, 1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 6: launch_split_pre_tile<2, 1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 7: launch_split_pre_tile<1, 2, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 1: launch_split_pre_tile<2, 1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 2: launch_split_pre_tile<1, 2, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    default: launch_split_pre_tile<1, 1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     }
 }
 
@@ -2731,6 +2911,21 @@ int pick_cfg(int dev, int M, int N, int K) {
 int pick_split(int dev, int M, int N, int K) {
     int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].split[size_class(M)][size_class(N)][k_class(K)] : -1;
     return v >= 0 ? v : default_split(M, N);
+}
+
+int pick_split_pre(int dev, int M, int N, int K) {
+    int v = (dev >= 0 && dev < kMaxDev) ? g_tune[dev].pre[size_class(M)][size_class(N)][k_class(K)] : -1;
+    return v >= 0 ? v : default_split(M, N);
+}
+
+// BEM_TUNE_LOG=1 in the environment: the tuner prints each shape's timings
+// to stderr.
+bool tune_log() {
+    static const bool on = [] {
+        const char* e = getenv("BEM_TUNE_LOG");
+        return e && *e && *e != '0';
+    }();
+    return on;
 }
 
 // The (M, N, K) of every update launch of launch_lu at padded order Np, in
@@ -2801,17 +2996,73 @@ int tune_split(hipStream_t s, int Np, float* Ar, float* Ai, float* umax) {
 }
 }  // namespace
 
+namespace {
+// The pre-split update kernels (launch_split_pre) timed on the trailing-update
+// shapes of a split-arithmetic factorisation of order Np, with operands from
+// the lane's split buffers AS, BS (their contents do not change the timing).
+int tune_split_pre(hipStream_t s, int Np, float* Ar, float* Ai, float* umax, const _Float16* AS,
+                   const _Float16* BS) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 0;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_tune[dev].tuned_np[3][Np / 128]) return 0;
+    }
+    if (hipMemsetAsync(umax, 0, sizeof(float) * (Np / LU_NB), s) != hipSuccess) return 0;
+    const int group = lu_group<float>(Np, true);
+    std::map<std::array<int, 3>, std::array<int, 3>> reps;
+    for (int p = 0; p < Np / LU_NB; p += group) {   // the trailing updates of launch_lu
+        int kg = (p + group) * LU_NB, restg = Np - kg;
+        if (restg <= LU_NB) break;
+        std::array<int, 3> sh = {restg, restg - LU_NB, group * LU_NB};
+        std::array<int, 3> key = {size_class(sh[0]), size_class(sh[1]), k_class(sh[2])};
+        auto it = reps.find(key);
+        if (it == reps.end() || (double)sh[0] * sh[1] > (double)it->second[0] * it->second[1]) reps[key] = sh;
+    }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 0;
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return 0; }
+    std::vector<std::pair<std::array<int, 3>, int>> picks;
+    for (auto& kv : reps) {
+        int M = std::min(kv.second[0], Np - 128), N = std::min(kv.second[1], Np - 128), K = kv.second[2];
+        if (M < 64 || N < 64) continue;
+        float best = 1e30f;
+        int bi = default_split(M, N);
+        for (int c = 0; c < kSplitPreCfgs; c++) {
+            launch_split_pre(c, s, M, N, K, Ar, Ai, Np, 0, 128, 128, 0, umax, AS, BS, 0, N);   // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; r < 3; r++) launch_split_pre(c, s, M, N, K, Ar, Ai, Np, 0, 128, 128, 0, umax, AS, BS, 0, N);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (tune_log())
+                fprintf(stderr, "[bem tune] pre-split M=%d N=%d K=%d cfg %d: %.1f us\n", M, N, K, c, ms * 1e3f / 3);
+            if (ms < best) { best = ms; bi = c; }
+        }
+        picks.push_back({kv.first, bi});
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& pk : picks) g_tune[dev].pre[pk.first[0]][pk.first[1]][pk.first[2]] = (signed char)pk.second;
+    g_tune[dev].tuned_np[3][Np / 128] = true;
+    return (int)picks.size();
+}
+}  // namespace
+
 // Times every tile shape on the shapes a factorisation of order Np launches
 // (one representative per class, the largest of the class) and records the
 // fastest. Ar, Ai: an Np x Np scratch matrix (its contents are overwritten).
 template <typename T>
-int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax) {
+int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax, const _Float16* AS, const _Float16* BS) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDev) return 0;
     int prec = sizeof(T) == 8;
     int tuned = 0;
     if constexpr (sizeof(T) == 4) {
         if (umax) tuned += tune_split(s, Np, Ar, Ai, umax);
+        if (umax && AS && BS) tuned += tune_split_pre(s, Np, Ar, Ai, umax, AS, BS);
     }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2854,8 +3105,8 @@ int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax) {
     g_tune[dev].tuned_np[prec][Np / 128] = true;
     return (int)picks.size();
 }
-template int tune_lu_gemms<float>(hipStream_t, int, float*, float*, float*);
-template int tune_lu_gemms<double>(hipStream_t, int, double*, double*, float*);
+template int tune_lu_gemms<float>(hipStream_t, int, float*, float*, float*, const _Float16*, const _Float16*);
+template int tune_lu_gemms<double>(hipStream_t, int, double*, double*, float*, const _Float16*, const _Float16*);
 
 template <typename T>
 int gemm_tile_choice(int M, int N, int K) {
@@ -2873,7 +3124,8 @@ template int gemm_tile_choice<double>(int, int, int);
 template <typename T>
 static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar, T* Ai, int ra, int ca, int rb,
                      int cb, int rc, int cc, LuProfile* prof, int critical, const float* umax,
-                     const _Float16* AS = nullptr, const _Float16* BS = nullptr, int nb0 = 0, int ldn = 0) {
+                     const _Float16* AS = nullptr, const _Float16* BS = nullptr, int nb0 = 0, int ldn = 0,
+                     int force_cfg = -1) {
     bool timed = prof && prof->n < LuProfile::kMax;
     if (timed) {
         if (!prof->ev[2 * prof->n]) {
@@ -2884,7 +3136,9 @@ static void gemm_sub(hipStream_t s, int dev, int Np, int M, int N, int K, T* Ar,
     }
     const bool split = sizeof(T) == 4 && umax;
     const int mode = split ? (AS ? 2 : 1) : 0;
-    int cfg = split ? pick_split(dev, M, N, K) : pick_cfg<T>(dev, M, N, K);
+    int cfg = split ? (mode == 2 ? (force_cfg >= 0 ? force_cfg : pick_split_pre(dev, M, N, K))
+                                 : pick_split(dev, M, N, K))
+                    : pick_cfg<T>(dev, M, N, K);
     if constexpr (sizeof(T) == 4) {
         if (mode == 2) launch_split_pre(cfg, s, M, N, K, Ar, Ai, Np, rb, rc, cc, critical, umax, AS, BS, nb0, ldn);
         else if (mode == 1) launch_split(cfg, s, M, N, K, Ar, Ai, Np, ra, ca, rb, cb, rc, cc, critical, umax);
@@ -2967,6 +3221,8 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
+    const int force_cfg = (arith >> kLuTileShift) - 1;   // forced tile of the pre-split updates, -1: tuned
+    arith &= kLuArithMask;
     const bool split = sizeof(T) == 4 && (arith == kLuArithSplitF16 || arith == kLuArithSplitF16Tile) && w.umax;
     const bool presplit = split && arith == kLuArithSplitF16 && w.opA && w.opB;
     float* umax = split ? w.umax : nullptr;
@@ -3041,7 +3297,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                 }
             }
             gemm_sub<T>(s, dev, Np, restg, restg - NB, g * NB, Ar, Ai, kg, k0, k0, kg + NB, kg, kg + NB, prof, 0, umax,
-                        AS, BS, 0, restg - NB);
+                        AS, BS, 0, restg - NB, force_cfg);
         }
     }
     hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);

@@ -68,6 +68,7 @@ typedef struct bem_solve_info {
     int64_t exact_points;     /* wave-kernel evaluations served by the exact kernel */
     double surrogate_err;     /* largest relative deviation of the surrogate from the exact
                                  kernel over the pairs judged during assembly (one in 61) */
+    int lanes;                /* solve lanes the call ran its frequencies on */
 } bem_solve_info;
 
 const char* bem_last_error(void);
@@ -194,6 +195,16 @@ int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
 enum { BEM_LU_FP32 = 0, BEM_LU_SPLIT_F16 = 1, BEM_LU_SPLIT_F16_TILE = 2 };
 int bem_set_lu_arith(bem_ctx* ctx, int arith);
 
+/* Tile of the BEM_LU_SPLIT_F16 trailing updates (the products on operands
+ * split once per panel group): -1 (default) takes the autotuner's choice per
+ * update shape; 0, 1, 2 force 64x64, 128x64, 64x128 tiles of 32x32 wave
+ * blocks with operand loads one 16-k chunk ahead, 5, 6, 7 the same tiles with
+ * loads two chunks ahead; 3, 4 force 128x128 tiles of 64x64 wave blocks with
+ * one or two operand chunks in LDS. Every tile adds the same products in the
+ * same order, so the choice never changes results. No counterpart in the
+ * reference. */
+int bem_set_lu_tile(bem_ctx* ctx, int tile);
+
 /* Measurement option, off by default: the wave term of receiver / source
  * pairs whose first mirrored quadrature node lies farther than near_radius
  * (metres) from the receiver is evaluated with FP32 exp, log, Bessel, Struve
@@ -257,7 +268,10 @@ int bem_solve_frequency(bem_ctx* ctx, bem_body* b, int mode, double omega, doubl
  * src/BEMUser_Console.cpp:252-257, each iteration Solve()), pipelined over
  * `lanes` (1..16) concurrent solve lanes, each with its own HIP streams and
  * workspaces, so that one frequency's dense MFMA work overlaps another's
- * latency-bound panel factorisations and triangular solves. kappas may be
+ * latency-bound panel factorisations and triangular solves. lanes = 0 lets
+ * the library choose: at most 10, spread so that every round of frequencies
+ * fills the lanes evenly, within half the free device memory (infos[f].lanes
+ * reports the count). kappas may be
  * NULL (Omega^2 / 9.81). Outputs per frequency f: A36 + 36 f, B36 + 36 f,
  * F/FK/SC + 12 nb f (row-major 6 x nb complex), infos + f (may be NULL).
  * Results are identical to nf calls of bem_solve_frequency. */
@@ -269,6 +283,7 @@ int bem_solve_sweep(bem_ctx* ctx, bem_body* b, int mode, int nf, const double* o
  * runs one Solver per body, BEMUser_Console.cpp:227-260): nfreq[i] frequencies
  * of body i, concatenated in omegas; results in the same order (A36 + 36 k,
  * B36 + 36 k, F + 12 nb k). All tasks share a pool of `lanes` solve lanes
+ * (0: chosen as for bem_solve_sweep over all tasks)
  * owned by the context, sized for the largest body; larger bodies are started
  * first, and each task goes to the first lane that is free. Same headings for
  * all bodies; compose WAVE. Results equal those of per-body sweeps. */

@@ -624,3 +624,35 @@ def test_split_once_per_group_equals_split_per_tile(ctx):
         ctx.set_lu_arith(bm.LU_SPLIT_F16)
     assert relerr(out[bm.LU_SPLIT_F16], np.linalg.solve(A, B)) < 1e-10
     np.testing.assert_array_equal(out[bm.LU_SPLIT_F16], out[bm.LU_SPLIT_F16_TILE])
+
+
+def test_update_tiles_agree_bitwise(ctx):
+    """Every tile of the pre-split trailing updates (64x64, 128x64 and 64x128
+    tiles of 32x32 wave blocks, operand loads one or two chunks ahead;
+    128x128 tiles of 64x64 wave blocks with one or two LDS chunks) adds the same products in the same order: the refined
+    solutions of a mixed dense solve of order 1100 (padded 1152; trailing
+    updates 896 x 832 .. 128 x 64, so the 128x128 tiles have half-empty edge
+    columns) agree bit for bit with the autotuned choice and with the split in
+    every tile."""
+    rng = np.random.default_rng(77)
+    n = 1100
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 40.0 * np.eye(n)
+    B = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    out = {}
+    try:
+        for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7):
+            ctx.set_lu_tile(tile)
+            X, info = ctx.dense_solve(A, B, 32)
+            assert info["fell_back_fp64"] == 0, (tile, info)
+            out[tile] = X
+        ctx.set_lu_tile(-1)
+        ctx.set_lu_arith(bm.LU_SPLIT_F16_TILE)
+        out["tile"] = ctx.dense_solve(A, B, 32)[0]
+    finally:
+        ctx.set_lu_tile(-1)
+        ctx.set_lu_arith(bm.LU_SPLIT_F16)
+    assert relerr(out[-1], np.linalg.solve(A, B)) < 1e-10
+    for k, X in out.items():
+        np.testing.assert_array_equal(X, out[-1], err_msg=f"tile {k}")
+    with pytest.raises(bm.BemError):
+        ctx.set_lu_tile(8)

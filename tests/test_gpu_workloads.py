@@ -160,6 +160,12 @@ def test_semisub6018_mixed_vs_fp64_all_frequencies(semisub):
     for i, j in enumerate((0, 20, 39)):
         assert_close(at(smx, j), {k: fx[k][i] for k in ("A", "B", "F")}, bm.MODE_MIXED, what=f"sweep {j}")
         assert_close(at(s64, j), {k: fx[k][i] for k in ("A", "B", "F")}, bm.MODE_FP64, what=f"sweep {j}")
+    # lanes chosen by the library: 40 frequencies run as 4 rounds of 10, with
+    # the results of any other lane count above one, bit for bit
+    sauto = body.solve_sweep(omegas, betas=betas, mode=bm.MODE_MIXED, lanes=0)
+    assert all(i["lanes"] == 10 for i in sauto["info"]), sauto["info"][0]
+    for k in ("A", "B", "F"):
+        np.testing.assert_array_equal(sauto[k], smx[k], err_msg=k)
 
 
 # ---------------------------------------------------------------------------
