@@ -432,10 +432,11 @@ class Context:
         return out
 
     def measure_peaks(self):
-        """Measured FP32 / FP64 MFMA TFLOP/s and streaming-copy GB/s of the device."""
-        out = np.zeros(3)
+        """Measured FP32 / FP64 / fp16 MFMA TFLOP/s and streaming-copy GB/s of the device."""
+        out = np.zeros(4)
         _check(lib().bem_measure_peaks(self._h, _dp(out)))
-        return dict(fp32_mfma_tflops=float(out[0]), fp64_mfma_tflops=float(out[1]), copy_gbs=float(out[2]))
+        return dict(fp32_mfma_tflops=float(out[0]), fp64_mfma_tflops=float(out[1]), copy_gbs=float(out[2]),
+                    fp16_mfma_tflops=float(out[3]))
 
     def set_refine_limit(self, max_steps):
         """Correction steps allowed to the mixed modes' refinement (0..4, default 4);

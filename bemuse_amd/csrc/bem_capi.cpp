@@ -154,8 +154,8 @@ struct LinSys {
             (r = get(&w.PL_i, pan)) || (r = get(&w.PU_r, pan)) || (r = get(&w.PU_i, pan)))
             return r;
         if (sizeof(T) == 4 && (r = get(&w.umax, (size_t)capNp / kLuNb))) return r;
-        if (sizeof(T) == 4 && ((r = get(&w.opA, (size_t)4 * capNp * 4 * kLuNb)) ||
-                               (r = get(&w.opB, (size_t)4 * capNp * 4 * kLuNb))))
+        if (sizeof(T) == 4 && ((r = get(&w.opA, (size_t)4 * capNp * kSplitGroupMax * kLuNb)) ||
+                               (r = get(&w.opB, (size_t)4 * capNp * kSplitGroupMax * kLuNb))))
             return r;
         return BEM_OK;
     }
@@ -1139,7 +1139,7 @@ int bem_set_lu_arith(bem_ctx* c, int arith) {
 
 int bem_set_lu_tile(bem_ctx* c, int tile) {
     if (!c || tile < -1 || tile >= kLuTileCfgs)
-        return fail(BEM_ERR_ARG, "bem_set_lu_tile: tile in [-1, 7]");
+        return fail(BEM_ERR_ARG, "bem_set_lu_tile: tile in [-1, 8]");
     c->lu_arith = (c->lu_arith & kLuArithMask) | ((tile + 1) << kLuTileShift);
     return BEM_OK;
 }

@@ -258,7 +258,10 @@ constexpr int NB2 = LU_NB * LU_NB;
 // panels; measured on MI355X, that pays in FP64 and for large orders, while
 // FP32 at N ~ 6000 is faster with pairs (74.9 vs 71.4 frequency-solves/s).
 template <typename T>
-inline int lu_group(int Np, bool split = false) { return (sizeof(T) == 8 || split || Np >= 12288) ? 4 : 2; }
+inline int lu_group(int Np, bool split = false) {
+    if (split && Np >= kSplitGroup8Np) return 8;
+    return (sizeof(T) == 8 || split || Np >= 12288) ? 4 : 2;
+}
 
 template <typename T>
 __device__ inline T cabs2(T r, T i) { return r * r + i * i; }
@@ -1400,6 +1403,7 @@ constexpr int kSplitExpB = 10;  // max |B| s_b < 2^10
 
 __device__ inline h8_t neg8(h8_t v) { return -v; }
 
+
 __device__ inline f16x_t mfma32(h8_t a, h8_t b, f16x_t c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
@@ -1603,13 +1607,16 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_h2(int M, int N, int K, fl
 // per group instead of in every tile of every update that reads them (the
 // split's VALU work had made the update kernel VALU-bound). For the group's
 // rows kg.. of its L21 columns (A) and columns kg.. of its U12 rows (B):
-//   AS[p][c][m][i] = part p of -2^12 A(ra + m, ca + 16 c + i),   m < M
-//   BS[p][c][n][i] = part p of 2^(10 - e) A(rb + 16 c + i, cb + n), n < N
+//   AS[p][c][q][m][i] = part p of -2^12 A(ra + m, ca + 16 c + 8 q + i),   m < M
+//   BS[p][c][q][n][i] = part p of 2^(10 - e) A(rb + 16 c + 8 q + i, cb + n), n < N
 // with parts Re h, Re l, Im h, Im l (h = fp16(x s), l = fp16(x s - h)) and 2^e
 // above the largest |Re|, |Im| of the K / NB panels' U12 rows, the scales of
-// k_cgemm_sub_h2. Chunk-major (16 k per chunk) so that the update kernel's
-// 16-byte loads of one chunk are contiguous. The staged values equal those
-// k_cgemm_sub_h2 forms, so k_cgemm_sub_hs computes the same sums.
+// k_cgemm_sub_h2. Chunk-major (16 k per chunk), then the chunk's two 8-k
+// halves q: an MFMA operand (8 consecutive k of one row or column) is one
+// 16-byte unit, and the units of consecutive rows of one (part, chunk, half)
+// are contiguous, so a wave's 64 16-byte loads read 1 KB in one piece. The
+// staged values equal those k_cgemm_sub_h2 forms, so the update kernels on
+// them compute the same sums.
 // Grid: cdiv(M, 64) K/64 tiles of A, then cdiv(N, 64) K/64 tiles of B.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_split_ops(int M, int N, int K, const float* __restrict__ Pr,
@@ -1647,13 +1654,14 @@ __global__ __launch_bounds__(256) void k_split_ops(int M, int N, int K, const fl
         float xr[16], xi[16];
         for (int t = 0; t < 16; t++) { xr[t] = tr[16 * kq + t][j]; xi[t] = ti[16 * kq + t][j]; }
         h8_t h[2], l[2];
-        const size_t pa = (size_t)M * K, o = ((size_t)(k0 / HK + kq) * M + m0 + j) * HK;
+        // unit (chunk, half 0, row) and the half-1 unit M rows later
+        const size_t pa = (size_t)M * K, o = ((size_t)(k0 / HK + kq) * 2 * M + m0 + j) * 8, o1 = o + (size_t)M * 8;
         split16(xr, sa, h, l);
-        *reinterpret_cast<h8_t*>(AS + o) = h[0]; *reinterpret_cast<h8_t*>(AS + o + 8) = h[1];
-        *reinterpret_cast<h8_t*>(AS + pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + pa + o + 8) = l[1];
+        *reinterpret_cast<h8_t*>(AS + o) = h[0]; *reinterpret_cast<h8_t*>(AS + o1) = h[1];
+        *reinterpret_cast<h8_t*>(AS + pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + pa + o1) = l[1];
         split16(xi, sa, h, l);
-        *reinterpret_cast<h8_t*>(AS + 2 * pa + o) = h[0]; *reinterpret_cast<h8_t*>(AS + 2 * pa + o + 8) = h[1];
-        *reinterpret_cast<h8_t*>(AS + 3 * pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + 3 * pa + o + 8) = l[1];
+        *reinterpret_cast<h8_t*>(AS + 2 * pa + o) = h[0]; *reinterpret_cast<h8_t*>(AS + 2 * pa + o1) = h[1];
+        *reinterpret_cast<h8_t*>(AS + 3 * pa + o) = l[0]; *reinterpret_cast<h8_t*>(AS + 3 * pa + o1) = l[1];
     } else {
         const int b = blockIdx.x - ta;
         const int n0 = (b / kt) * LU_NB, k0 = (b % kt) * LU_NB;
@@ -1668,13 +1676,13 @@ __global__ __launch_bounds__(256) void k_split_ops(int M, int N, int K, const fl
             xi[t] = vi.x; xi[t + 1] = vi.y; xi[t + 2] = vi.z; xi[t + 3] = vi.w;
         }
         h8_t h[2], l[2];
-        const size_t pb = (size_t)N * K, o = ((size_t)(k0 / HK + kq) * N + n0 + j) * HK;
+        const size_t pb = (size_t)N * K, o = ((size_t)(k0 / HK + kq) * 2 * N + n0 + j) * 8, o1 = o + (size_t)N * 8;
         split16(xr, sb, h, l);
-        *reinterpret_cast<h8_t*>(BS + o) = h[0]; *reinterpret_cast<h8_t*>(BS + o + 8) = h[1];
-        *reinterpret_cast<h8_t*>(BS + pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + pb + o + 8) = l[1];
+        *reinterpret_cast<h8_t*>(BS + o) = h[0]; *reinterpret_cast<h8_t*>(BS + o1) = h[1];
+        *reinterpret_cast<h8_t*>(BS + pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + pb + o1) = l[1];
         split16(xi, sb, h, l);
-        *reinterpret_cast<h8_t*>(BS + 2 * pb + o) = h[0]; *reinterpret_cast<h8_t*>(BS + 2 * pb + o + 8) = h[1];
-        *reinterpret_cast<h8_t*>(BS + 3 * pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + 3 * pb + o + 8) = l[1];
+        *reinterpret_cast<h8_t*>(BS + 2 * pb + o) = h[0]; *reinterpret_cast<h8_t*>(BS + 2 * pb + o1) = h[1];
+        *reinterpret_cast<h8_t*>(BS + 3 * pb + o) = l[0]; *reinterpret_cast<h8_t*>(BS + 3 * pb + o1) = l[1];
     }
 }
 
@@ -1721,29 +1729,31 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_sub_hs(int M, int N, int K, fl
     // chunks ahead, until its store to LDS at the end of chunk c - 1
     h8_t ha[PD][NA16], hb[PD][NB16];
     const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
+    // load q: row (column) q % TM of k half (q / TM) % 2 of part q / (2 TM):
+    // consecutive threads read consecutive 16-byte units of the split layout
     auto gload = [&](int k0, int set) {
 #pragma unroll
         for (int l = 0; l < NA16; l++) {
-            int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
-            size_t o = part * pa + ((size_t)(k0 / HK) * M + rowt + min(row, mrows - 1)) * HK + 8 * hf;
+            int q = tid + 256 * l, row = q % TM, hf = (q / TM) & 1, part = q / (2 * TM);
+            size_t o = part * pa + ((size_t)(k0 / HK) * 2 * M + (size_t)hf * M + rowt + min(row, mrows - 1)) * 8;
             ha[set][l] = *reinterpret_cast<const h8_t*>(AS + o);
         }
 #pragma unroll
         for (int l = 0; l < NB16; l++) {
-            int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
-            size_t o = part * pb + ((size_t)(k0 / HK) * ldn + nb0 + colt + min(col, ncols - 1)) * HK + 8 * hf;
+            int q = tid + 256 * l, col = q % TN, hf = (q / TN) & 1, part = q / (2 * TN);
+            size_t o = part * pb + ((size_t)(k0 / HK) * 2 * ldn + (size_t)hf * ldn + nb0 + colt + min(col, ncols - 1)) * 8;
             hb[set][l] = *reinterpret_cast<const h8_t*>(BS + o);
         }
     };
     auto sstore = [&](int buf, int set) {
 #pragma unroll
         for (int l = 0; l < NA16; l++) {
-            int q = tid + 256 * l, hf = q & 1, row = (q >> 1) % TM, part = q / (2 * TM);
+            int q = tid + 256 * l, row = q % TM, hf = (q / TM) & 1, part = q / (2 * TM);
             *reinterpret_cast<h8_t*>(&As[buf][part][row][8 * hf]) = ha[set][l];
         }
 #pragma unroll
         for (int l = 0; l < NB16; l++) {
-            int q = tid + 256 * l, hf = q & 1, col = (q >> 1) % TN, part = q / (2 * TN);
+            int q = tid + 256 * l, col = q % TN, hf = (q / TN) & 1, part = q / (2 * TN);
             *reinterpret_cast<h8_t*>(&Bs[buf][part][col][8 * hf]) = hb[set][l];
         }
     };
@@ -1905,16 +1915,16 @@ __global__ __launch_bounds__(256, DB ? 1 : 2) void k_cgemm_sub_hw(int M, int N, 
     if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);   // um < 2^eb
     const float inv = ldexpf(1.0f, eb - kSplitExpB - kSplitExpA);
 
-    // staging: load l is part l of row (column) tid / 2, k half tid % 2 of the chunk
-    const int srow = tid >> 1, shf = tid & 1;
+    // staging: load l is part l of row (column) tid % 128, k half tid / 128 of the chunk
+    const int srow = tid & 127, shf = tid >> 7;
     const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
     const size_t arow = (size_t)(rowt + min(srow, mrows - 1)), bcol = (size_t)(nb0 + colt + min(srow, ncols - 1));
     h8_t ha[4], hb[4];
     auto gload = [&](int c) {
 #pragma unroll
         for (int l = 0; l < 4; l++) {
-            ha[l] = *reinterpret_cast<const h8_t*>(AS + l * pa + ((size_t)c * M + arow) * HK + 8 * shf);
-            hb[l] = *reinterpret_cast<const h8_t*>(BS + l * pb + ((size_t)c * ldn + bcol) * HK + 8 * shf);
+            ha[l] = *reinterpret_cast<const h8_t*>(AS + l * pa + ((size_t)c * 2 * M + (size_t)shf * M + arow) * 8);
+            hb[l] = *reinterpret_cast<const h8_t*>(BS + l * pb + ((size_t)c * 2 * ldn + (size_t)shf * ldn + bcol) * 8);
         }
     };
     auto sstore = [&](int buf) {
@@ -1981,6 +1991,161 @@ __global__ __launch_bounds__(256, DB ? 1 : 2) void k_cgemm_sub_hw(int M, int N, 
                 __syncthreads();
             }
         }
+    }
+    if (!active) return;
+    // accumulator element e of a 32 x 32 block: row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const size_t off =
+                (size_t)(cc + colt + col0 + 32 * b + li) * ld + rc + rowt + row0 + 32 * a + 4 * (lane >> 5);
+            float4 cr[4], ci[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cr[q] = *reinterpret_cast<const float4*>(Pr + off + 8 * q);
+                ci[q] = *reinterpret_cast<const float4*>(Pi + off + 8 * q);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float4 r = cr[q], m = ci[q];
+                r.x = fmaf(accR[a][b][4 * q + 0], inv, r.x);
+                r.y = fmaf(accR[a][b][4 * q + 1], inv, r.y);
+                r.z = fmaf(accR[a][b][4 * q + 2], inv, r.z);
+                r.w = fmaf(accR[a][b][4 * q + 3], inv, r.w);
+                m.x = fmaf(accI[a][b][4 * q + 0], inv, m.x);
+                m.y = fmaf(accI[a][b][4 * q + 1], inv, m.y);
+                m.z = fmaf(accI[a][b][4 * q + 2], inv, m.z);
+                m.w = fmaf(accI[a][b][4 * q + 3], inv, m.w);
+                *reinterpret_cast<float4*>(Pr + off + 8 * q) = r;
+                *reinterpret_cast<float4*>(Pi + off + 8 * q) = m;
+            }
+        }
+}
+
+// 128 x 128 tiles of the pre-split updates with the operands copied to LDS by
+// the memory path itself: global_load_lds_dwordx4 writes each lane's 16 bytes
+// at the wave's LDS base + 16 lane, so the copy takes no registers and no LDS
+// store instructions.
+// - LDS per buffer: A and B each as [part][k half][128 rows][8 halves]
+//   (16 KB); two buffers (64 KB), two work-groups per CU.
+// - A copy instruction moves one (part, k half, 64-row block): 1 KB in one
+//   piece in the split layout and in LDS. Each wave issues four for A and four
+//   for B per chunk.
+// - Chunk c + 1 is copied while chunk c is multiplied; each wave waits for its
+//   own copies (vmcnt) before the one barrier per chunk, which makes them
+//   visible and frees the buffer that the copies after it overwrite.
+// - MFMA operand reads of 32 consecutive rows are 32 consecutive 16-byte
+//   units: no bank conflicts.
+// Waves, products and epilogue as k_cgemm_sub_hw: equal results bit for bit.
+// M and N multiples of 64 (the host checks).
+typedef _Float16 OpBuf[2][4][2][128][8];   // [A, B][part][k half][row][8 halves]
+
+__device__ inline void lds_copy16(const _Float16* src, _Float16* dst) {
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+__global__ __launch_bounds__(256, 2) void k_cgemm_sub_hd(int M, int N, int K, float* __restrict__ Pr,
+                                                         float* __restrict__ Pi, int ld, int rb, int rc, int cc,
+                                                         int critical, const float* __restrict__ umax,
+                                                         const _Float16* __restrict__ AS,
+                                                         const _Float16* __restrict__ BS, int nb0, int ldn) {
+    if (critical) __builtin_amdgcn_s_setprio(3);
+    constexpr int TM = 128, TN = 128;
+    __shared__ __align__(1024) OpBuf L0;
+    __shared__ __align__(1024) OpBuf L1;
+    const int tiles_m = (M + TM - 1) / TM, tiles_n = (N + TN - 1) / TN, ntiles = tiles_m * tiles_n;
+    constexpr int GROUP_M = 8;                     // XCD-aware grouped order, as in k_cgemm_sub
+    const int bid = blockIdx.x, per = (ntiles + 7) / 8;
+    const int tile = (bid % 8) * per + bid / 8;
+    if (tile >= ntiles) return;
+    const int group = tile / (GROUP_M * tiles_n), first_m = group * GROUP_M;
+    const int gsize = min(GROUP_M, tiles_m - first_m), in_group = tile - group * GROUP_M * tiles_n;
+    const int tm = first_m + in_group % gsize, tn = in_group / gsize;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rowt = tm * TM, colt = tn * TN;
+    const int mrows = min(TM, M - rowt), ncols = min(TN, N - colt);
+    float um = 0.0f;
+    for (int q = 0; q < K / LU_NB; q++) um = fmaxf(um, umax[rb / LU_NB + q]);
+    int eb = 0;
+    if (um > 0.0f && um < INFINITY) (void)frexpf(um, &eb);   // um < 2^eb
+    const float inv = ldexpf(1.0f, eb - kSplitExpB - kSplitExpA);
+
+    // the wave's copies i = 0..3: j = 4 wave + i is (part j / 4, k half j / 2 % 2,
+    // rows 64 (j % 2) ..); lane l takes row 64 (j % 2) + l (clamped into the update)
+    const size_t pa = (size_t)M * K, pb = (size_t)ldn * K;
+    const size_t astep = (size_t)16 * M, bstep = (size_t)16 * ldn;   // halves per chunk in one part
+    size_t asrc[4], bsrc[4];
+    int lofs[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int j = 4 * wave + i, p = j >> 2, h = (j >> 1) & 1, r = 64 * (j & 1) + lane;
+        asrc[i] = p * pa + ((size_t)h * M + rowt + min(r, mrows - 1)) * 8;
+        bsrc[i] = p * pb + ((size_t)h * ldn + nb0 + colt + min(r, ncols - 1)) * 8;
+        lofs[i] = ((p * 2 + h) * TM + 64 * (j & 1)) * 8;   // halves; the lane's unit is added by the copy
+    }
+    auto copy = [&](OpBuf& buf, int c) {
+        _Float16* a = &buf[0][0][0][0][0];
+        _Float16* b = &buf[1][0][0][0][0];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            lds_copy16(AS + asrc[i] + c * astep, a + lofs[i]);
+            lds_copy16(BS + bsrc[i] + c * bstep, b + lofs[i]);
+        }
+    };
+    auto copies_done = [] { __builtin_amdgcn_s_waitcnt(0x0F70); };   // vmcnt(0)
+
+    const int row0 = (wave >> 1) * 64, col0 = (wave & 1) * 64;
+    const bool active = row0 < mrows && col0 < ncols;
+    const int li = lane & 31, lh = lane >> 5;
+    f16x_t accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            accR[a][b] = f16x_t{};
+            accI[a][b] = f16x_t{};
+        }
+    auto multiply = [&](const OpBuf& buf) {
+        if (!active) return;
+        h8_t bo[2][4];   // [b][Re h, Re l, Im h, Im l]
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                bo[b][p] = *reinterpret_cast<const h8_t*>(&buf[1][p][lh][col0 + 32 * b + li][0]);
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            h8_t ao[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) ao[p] = *reinterpret_cast<const h8_t*>(&buf[0][p][lh][row0 + 32 * a + li][0]);
+            const h8_t nih = neg8(ao[2]), nil = neg8(ao[3]);
+            const h8_t ra[6] = {ao[0], ao[0], ao[1], nih, nih, nil};
+            const h8_t ia[6] = {ao[0], ao[0], ao[1], ao[2], ao[2], ao[3]};
+            constexpr int rbp[6] = {0, 1, 0, 2, 3, 2}, ibp[6] = {2, 3, 2, 0, 1, 0};
+#pragma unroll
+            for (int s = 0; s < 6; s++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    accR[a][b] = mfma32(ra[s], bo[b][rbp[s]], accR[a][b]);
+                    accI[a][b] = mfma32(ia[s], bo[b][ibp[s]], accI[a][b]);
+                }
+        }
+    };
+    const int nchunk = K / HK;
+    copy(L0, 0);
+    copies_done();
+    __syncthreads();
+    for (int c = 0; c < nchunk; c += 2) {
+        if (c + 1 < nchunk) copy(L1, c + 1);
+        multiply(L0);
+        copies_done();
+        __syncthreads();
+        if (c + 1 >= nchunk) break;
+        if (c + 2 < nchunk) copy(L0, c + 2);
+        multiply(L1);
+        copies_done();
+        __syncthreads();
     }
     if (!active) return;
     // accumulator element e of a 32 x 32 block: row 8 (e / 4) + 4 (lane / 32) + e % 4, column lane % 32
@@ -2692,6 +2857,28 @@ __global__ __launch_bounds__(256) void k_peak_mfma(int iters, T* sink) {
     if (v == (T)-1.2345) sink[blockIdx.x] = v;    // never true; keeps the chains live
 }
 
+// fp16 MFMA 32x32x16 chains (the split-operand updates' instruction): eight
+// independent accumulators per wave.
+__global__ __launch_bounds__(256) void k_peak_mfma_f16(int iters, float* sink) {
+    h8_t a, b;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        a[q] = (_Float16)((threadIdx.x + q) & 7) * (_Float16)0.125f;
+        b[q] = (_Float16)((threadIdx.x >> 3) & 7) * (_Float16)0.0625f;
+    }
+    f16x_t c[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[q] = f16x_t{(float)q};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) c[q] = mfma32(a, b, c[q]);
+    }
+    float v = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) v += c[q][0] + c[q][15];
+    if (v == -1.2345f) sink[blockIdx.x] = v;    // never true; keeps the chains live
+}
+
 __global__ __launch_bounds__(256) void k_peak_copy(size_t n4, const float4* __restrict__ src, float4* dst) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride) dst[t] = src[t];
@@ -2736,9 +2923,14 @@ int measure_peaks(hipStream_t s, double* out) {
         double tc = time_launch([&] {
             hipLaunchKernelGGL(k_peak_copy, dim3(ncu * 16), dim3(256), 0, s, n4, (const float4*)a, b);
         });
+        const int iters16 = 2048;
+        double t16 = time_launch([&] {
+            hipLaunchKernelGGL(k_peak_mfma_f16, dim3(wgs), dim3(256), 0, s, iters16, (float*)sink);
+        });
         out[0] = mfma_flops / t32 / 1e12;
         out[1] = mfma_flops / t64 / 1e12;
         out[2] = 2.0 * (double)bytes / tc / 1e9;
+        out[3] = (double)wgs * 4 * iters16 * 8 * (32.0 * 32 * 16 * 2) / t16 / 1e12;
         if (hipStreamSynchronize(s) != hipSuccess) rc = 1;
     }
     if (sink) (void)hipFree(sink);
@@ -2853,9 +3045,9 @@ void launch_split(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float*
 int default_split(int M, int N) { return cdiv(M, 128) * cdiv(N, 64) >= 512 ? 1 : 0; }
 
 // The same updates on operands split beforehand (k_split_ops): the tiles of
-// launch_split (configurations 0-2), and the 128 x 128 tiles of
-// k_cgemm_sub_hw with one LDS chunk (3) or two (4), for M and N multiples of
-// 64 (else configuration 0).
+// launch_split with loads one chunk ahead (configurations 0-2) or two (5-7),
+// and the 128 x 128 tiles of k_cgemm_sub_hw with one LDS chunk (3) or two (4)
+// and of k_cgemm_sub_hd (8), for M and N multiples of 64 (else configuration 0).
 template <int WM, int WN, int PD>
 void launch_split_pre_tile(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc, int cc,
                            int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
@@ -2874,7 +3066,14 @@ void launch_split_wide(hipStream_t s, int M, int N, int K, float* Ar, float* Ai,
     hipLaunchKernelGGL(k_cgemm_sub_hw<DB>, dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical,
                        umax, AS, BS, nb0, ldn);
 }
-constexpr int kSplitPreCfgs = 8;
+void launch_split_direct(hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc, int cc,
+                         int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0, int ldn) {
+    int tiles = (int)(cdiv(M, 128) * cdiv(N, 128));
+    int per = (tiles + 7) / 8;
+    hipLaunchKernelGGL(k_cgemm_sub_hd, dim3(per * 8), dim3(256), 0, s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical,
+                       umax, AS, BS, nb0, ldn);
+}
+constexpr int kSplitPreCfgs = 9;
 void launch_split_pre(int cfg, hipStream_t s, int M, int N, int K, float* Ar, float* Ai, int ld, int rb, int rc,
                       int cc, int critical, const float* umax, const _Float16* AS, const _Float16* BS, int nb0,
                       int ldn) {
@@ -2885,6 +3084,7 @@ void launch_split_pre(int cfg, hipStream_t s, int M, int N, int K, float* Ar, fl
     case 5: launch_split_pre_tile<1, 1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     case 6: launch_split_pre_tile<2, 1, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     case 7: launch_split_pre_tile<1, 2, 2>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
+    case 8: launch_split_direct(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     case 1: launch_split_pre_tile<2, 1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     case 2: launch_split_pre_tile<1, 2, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;
     default: launch_split_pre_tile<1, 1, 1>(s, M, N, K, Ar, Ai, ld, rb, rc, cc, critical, umax, AS, BS, nb0, ldn); return;

@@ -27,11 +27,13 @@ the wave term on FP32 special functions as well (not the default: near the
 semi-sub's irregular frequencies and at 2.5 rad/s its B and F exceed 1e-6).
 
 Order: the headline legs, the roofline of the dominant kernel, the CPU
-baseline, then the optional MIXED_SURROGATE leg (wave term from the trained
+baseline, then the optional legs: the far-pair FP32 wave term over all 40
+frequencies ("far32"), and last MIXED_SURROGATE (wave term from the trained
 MLP, every 5th frequency, reported in the "surrogate" object: it does not
 reach the 1e-6 bar and is slower than the exact kernel, so it is not the
-headline value). A failing optional leg is reported in its object's "error"
-field; the line is still printed.
+headline value; its training's sustained load is left to the end). A failing
+optional leg is reported in its object's "error" field; the line is still
+printed.
 
 Prints one JSON line (rank 0, N=1 only).
 """
@@ -243,14 +245,6 @@ def main():
             cpu = {"error": f"{type(e).__name__}: {e}"}
             print(f"[bench] CPU baseline failed: {cpu['error']}", file=sys.stderr, flush=True)
 
-    surrogate = None
-    if args.surrogate:
-        try:
-            surrogate = surrogate_leg(bm, ctx, body, omegas, betas, fp64_by_freq, args.lanes)
-        except Exception as e:
-            surrogate = {"error": f"{type(e).__name__}: {e}"}
-            print(f"[bench] surrogate leg failed: {surrogate['error']}", file=sys.stderr, flush=True)
-
     far32 = None
     if args.far32:
         try:
@@ -258,6 +252,14 @@ def main():
         except Exception as e:
             far32 = {"error": f"{type(e).__name__}: {e}"}
             print(f"[bench] far-pair FP32 leg failed: {far32['error']}", file=sys.stderr, flush=True)
+
+    surrogate = None
+    if args.surrogate:
+        try:
+            surrogate = surrogate_leg(bm, ctx, body, omegas, betas, fp64_by_freq, args.lanes)
+        except Exception as e:
+            surrogate = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] surrogate leg failed: {surrogate['error']}", file=sys.stderr, flush=True)
 
     value = args.steps / dtmx
     line = {

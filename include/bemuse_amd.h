@@ -220,8 +220,9 @@ int bem_set_wave_far32(bem_ctx* ctx, double near_radius);
 /* Measured machine peaks of the context's device, for reporting beside the
  * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
  * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +
- * 1 GiB written). Allocates 2 GiB for the copy while it runs. No counterpart
- * in the reference (a measurement aid). */
+ * 1 GiB written), out[3] fp16 MFMA TFLOP/s (32x32x16 chains). Allocates 2 GiB
+ * for the copy while it runs. No counterpart in the reference (a measurement
+ * aid). */
 int bem_measure_peaks(bem_ctx* ctx, double* out);
 
 /* Uploads the geometry of a mesh (Hydrodynamic_Radiation_Solver::Setup,

@@ -626,10 +626,34 @@ def test_split_once_per_group_equals_split_per_tile(ctx):
     np.testing.assert_array_equal(out[bm.LU_SPLIT_F16], out[bm.LU_SPLIT_F16_TILE])
 
 
+def test_split_groups_of_eight(ctx):
+    """Padded orders from 12288 up defer the split-arithmetic trailing updates
+    over groups of eight panels (K = 512, thin updates up to K = 448 inside
+    the panel kernels): a mixed dense solve of order 12300 (padded 12416:
+    twenty-four groups of eight, then a group of two) against the FP64 device
+    solve, and bit for bit against the split in every tile."""
+    rng = np.random.default_rng(12300)
+    n = 12300
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A += 6.0 * np.sqrt(n) * np.eye(n)                # no interchanges under partial pivoting
+    B = rng.standard_normal((n, 6)) + 1j * rng.standard_normal((n, 6))
+    X, info = ctx.dense_solve(A, B, 32)
+    assert info["fell_back_fp64"] == 0 and info["pivot_fallback"] == 0, info
+    X64, _ = ctx.dense_solve(A, B, 64)
+    assert relerr(X, X64) < 1e-11
+    try:
+        ctx.set_lu_arith(bm.LU_SPLIT_F16_TILE)
+        Xt, _ = ctx.dense_solve(A, B, 32)
+    finally:
+        ctx.set_lu_arith(bm.LU_SPLIT_F16)
+    np.testing.assert_array_equal(X, Xt)
+
+
 def test_update_tiles_agree_bitwise(ctx):
     """Every tile of the pre-split trailing updates (64x64, 128x64 and 64x128
     tiles of 32x32 wave blocks, operand loads one or two chunks ahead;
-    128x128 tiles of 64x64 wave blocks with one or two LDS chunks) adds the same products in the same order: the refined
+    128x128 tiles of 64x64 wave blocks with one or two LDS chunks, staged
+    through registers or copied to LDS directly) adds the same products in the same order: the refined
     solutions of a mixed dense solve of order 1100 (padded 1152; trailing
     updates 896 x 832 .. 128 x 64, so the 128x128 tiles have half-empty edge
     columns) agree bit for bit with the autotuned choice and with the split in
@@ -640,7 +664,7 @@ def test_update_tiles_agree_bitwise(ctx):
     B = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
     out = {}
     try:
-        for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7):
+        for tile in (-1, 0, 1, 2, 3, 4, 5, 6, 7, 8):
             ctx.set_lu_tile(tile)
             X, info = ctx.dense_solve(A, B, 32)
             assert info["fell_back_fp64"] == 0, (tile, info)
@@ -655,4 +679,4 @@ def test_update_tiles_agree_bitwise(ctx):
     for k, X in out.items():
         np.testing.assert_array_equal(X, out[-1], err_msg=f"tile {k}")
     with pytest.raises(bm.BemError):
-        ctx.set_lu_tile(8)
+        ctx.set_lu_tile(9)
