@@ -571,6 +571,15 @@ def test_spar_20k_panels(ctx):
     assert abs(abs(F[2, 0]) - abs(F[2, 2])) < 1e-6 * np.abs(F).max()     # heave at 0 = heave at 180 deg
     K = w * w / 9.81
     assert abs(B[2, 2] / (K * abs(F[2, 0]) ** 2 / (2 * 9.81 ** 2)) - 1) < 1e-2
+    # one lane (the latency variants of the panel kernels) against two (at this
+    # order the wide 64-row / 64-column variants, and groups of eight panels):
+    # the same products in the same order, so equal bit for bit
+    ws = np.array([0.6, 1.1])
+    one = body.solve_sweep(ws, betas=betas, mode=bm.MODE_MIXED, lanes=1)
+    two = body.solve_sweep(ws, betas=betas, mode=bm.MODE_MIXED, lanes=2)
+    for key in ("A", "B", "F"):
+        np.testing.assert_array_equal(one[key], two[key], err_msg=key)
+        np.testing.assert_array_equal(one[key][0], gmx[key], err_msg=key)
     body.close()
 
 
