@@ -210,8 +210,9 @@ def main():
     s_launch = prof_mx["ms_gemm_per_launch_replay"] * 1e-3
     achieved_bw = prof_mx["bytes_per_launch"] / s_launch
     roofline = {
-        "kernel": "k_cgemm_sub_h2<WM, WN> (LU trailing updates C -= A B on FP32 planar complex; operands split into "
-                  "two scaled fp16 terms, v_mfma_f32_32x32x16_f16, FP32 accumulation; K = 64..256, autotuned tiles)",
+        "kernel": "k_cgemm_sub_hs / k_cgemm_sub_hd (trailing updates) and k_cgemm_sub_h2 (next group's first panel): "
+                  "the mixed LU's updates C -= A B on FP32 planar complex, operands split into two scaled fp16 terms, "
+                  "v_mfma_f32_32x32x16_f16, FP32 accumulation, K = 256, autotuned tiles",
         "bound": "hbm", "achieved": achieved_bw / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
         "frac": achieved_bw / PEAK_HBM, "traffic": measured_traffic(),
         "bytes_per_launch": prof_mx["bytes_per_launch"], "flops_per_launch": prof_mx["flops_per_launch"],
