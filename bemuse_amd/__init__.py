@@ -457,7 +457,8 @@ class Context:
         """Tile of the LU_SPLIT_F16 trailing updates: -1 autotuned (default), 0-2 the
         32x32-wave-block tiles (64x64, 128x64, 64x128) with loads one chunk ahead, 5-7 the
         same two chunks ahead, 3-4 the 128x128 tiles with 64x64 wave blocks (one or two
-        LDS chunks). Results do not depend on it."""
+        LDS chunks), 8 the 128x128 tile with direct memory-to-LDS operand copies. Results do
+        not depend on it."""
         _check(lib().bem_set_lu_tile(self._h, int(tile)))
 
     def set_graphs(self, enable):

@@ -200,9 +200,10 @@ int bem_set_lu_arith(bem_ctx* ctx, int arith);
  * update shape; 0, 1, 2 force 64x64, 128x64, 64x128 tiles of 32x32 wave
  * blocks with operand loads one 16-k chunk ahead, 5, 6, 7 the same tiles with
  * loads two chunks ahead; 3, 4 force 128x128 tiles of 64x64 wave blocks with
- * one or two operand chunks in LDS. Every tile adds the same products in the
- * same order, so the choice never changes results. No counterpart in the
- * reference. */
+ * one or two operand chunks in LDS; 8 the 128x128 tile whose operands go
+ * from memory straight to LDS (double-buffered). Every tile adds the same
+ * products in the same order, so the choice never changes results. No
+ * counterpart in the reference. */
 int bem_set_lu_tile(bem_ctx* ctx, int tile);
 
 /* Measurement option, off by default: the wave term of receiver / source
