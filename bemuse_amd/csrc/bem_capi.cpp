@@ -90,6 +90,11 @@ struct SurState {
 // F orders of magnitude inside the 1e-6 bar while saving correction sweeps.
 constexpr int kRefineMaxIter = 4;
 constexpr double kRefineTol = 1e-13;
+// Correction steps a solve lane enqueues with the frequency; the rest, when
+// its refinement needs them, follow once the lane's results are in (every
+// enqueued step that the device skips still costs a launch on the lane's
+// chain). The benchmark systems converge in two.
+constexpr int kRefineEager = 2;
 
 // Largest order of bem_dense_solve_noalloc and its padded order.
 constexpr int kNaMaxN = BEM_NOALLOC_MAX_N;
@@ -233,6 +238,11 @@ struct Lane {
     int refine_max = kRefineMaxIter;
     int lu_arith = kLuArithSplitF16;
     int lu_latency = 1;    // set per run: 1 when this lane is the only one in flight
+    // set per frequency by issue_frequency (other users of a lane keep the
+    // full path): the factorisation without its pivoting fallback
+    // (speculative), and the correction steps enqueued with the solve
+    int lu_spec = 0;
+    int refine_eager = kRefineMaxIter;
     bool sur_used = false;
     double omega = 0, kappa = 0;
     // captured frequency graph and the configuration it was captured for
@@ -327,6 +337,8 @@ struct SolveView {
     int refine_max = kRefineMaxIter;
     int lu_arith = kLuArithSplitF16;
     int lu_latency = 1;    // one frequency in flight (launch_lu's latency)
+    int lu_spec = 0;       // speculative factorisation (kLuSpeculative): the caller re-runs flagged systems
+    int refine_eager = kRefineMaxIter;   // correction steps enqueued with the solve (the rest: refine_tail)
 };
 
 SolveView lane_view(Lane& L) {
@@ -334,16 +346,20 @@ SolveView lane_view(Lane& L) {
     v.s1 = L.s1; v.s2 = L.s2; v.ev = L.lu_events.data(); v.sys = &L.sys;
     v.counter = L.counter; v.pub = L.pub; v.ctrl = L.ctrl; v.refine_max = L.refine_max; v.lu_arith = L.lu_arith;
     v.lu_latency = L.lu_latency;
+    v.lu_spec = L.lu_spec;
+    v.refine_eager = L.refine_eager;
     return v;
 }
 
 // Device LU in the given precision (FP64 in place, or FP32 on the FP32 copy).
 void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr) {
     LinSys& s = *V.sys;
+    const int spec = V.lu_spec ? kLuSpeculative : 0;
     if (precision == 64)
-        launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof, kLuArithF32, V.lu_latency);
+        launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof, kLuArithF32 | spec,
+                          V.lu_latency);
     else
-        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith,
+        launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith | spec,
                          V.lu_latency);
 }
 void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
@@ -352,37 +368,54 @@ void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_s
 // FP32 LU solve followed by FP64 iterative refinement against the FP64
 // matrix s.Dr/s.Di, controlled on the device (no host round trip), with at
 // most V.refine_max correction steps.
+// Refinement iterations it0 .. it1 of an FP32 solve: each is a correction
+// (it > 0: d = LU32^-1 r, x += d) followed by the residual r = b - A x in
+// FP64 (the first also yields |A|_inf) and the convergence check; all of
+// them are no-ops once the device marked the solve done.
+void refine_steps(const SolveView& V, int nr, int it0, int it1) {
+    LinSys& s = *V.sys;
+    hipStream_t st = V.s1;
+    const size_t n = (size_t)s.Np * nr;
+    const int* fany = V.lu_spec ? nullptr : s.flag32 + s.Np / kLuNb;
+    const int* skip = &V.ctrl->done;
+    const int max_iter = std::max(0, std::min(V.refine_max, kRefineMaxIter));
+    for (int it = it0; it <= it1; it++) {
+        if (it > 0) {
+            launch_to_f32(st, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
+            launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
+                                          V.counter, V.pub, skip);
+            launch_add_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
+        }
+        launch_matvec<double>(st, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
+                              it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
+        launch_refine_check(st, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, V.ctrl, kRefineTol, max_iter);
+    }
+}
+
+// Iterations that V.refine_eager left for later (refine_tail): from
+// min(refine_eager, max) + 1 to max.
+int refine_last(const SolveView& V) { return std::max(0, std::min(V.refine_max, kRefineMaxIter)); }
+int refine_first_tail(const SolveView& V) { return std::min(V.refine_eager, refine_last(V)) + 1; }
+
 void solve_system(const SolveView& V, int precision, int nr) {
     LinSys& s = *V.sys;
     hipStream_t st = V.s1;
     size_t n = (size_t)s.Np * nr;
     if (precision == 64) {
         launch_copy2(st, n, s.Br, s.Bi, s.Xr, s.Xi);
-        launch_lu_solve<double, double>(st, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv, s.flag + s.Np / kLuNb, s.Xr, s.Xi,
-                                        V.counter, V.pub);
+        launch_lu_solve<double, double>(st, s.Np, nr, s.Dr, s.Di, s.w64, s.ipiv,
+                                        V.lu_spec ? nullptr : s.flag + s.Np / kLuNb, s.Xr, s.Xi, V.counter, V.pub);
         return;
     }
-    const int* fany = s.flag32 + s.Np / kLuNb;
-    const int* skip = &V.ctrl->done;
     launch_refine_reset(st, V.ctrl);
     launch_to_f32(st, n, s.Br, s.Bi, s.X32r, s.X32i);
-    launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i, V.counter,
-                                  V.pub);
+    launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32,
+                                  V.lu_spec ? nullptr : s.flag32 + s.Np / kLuNb, s.X32r, s.X32i, V.counter, V.pub);
     launch_from_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i);
-    const int max_iter = std::max(0, std::min(V.refine_max, kRefineMaxIter));
-    for (int it = 0; it <= max_iter; it++) {
-        // r = b - A x in FP64; the first residual also yields |A|_inf
-        launch_matvec<double>(st, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
-                              it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
-        launch_refine_check(st, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, V.ctrl, kRefineTol, max_iter);
-        if (it == max_iter) break;
-        // d = LU32^-1 r, x += d (no-ops once the device marked the solve done)
-        launch_to_f32(st, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
-        launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                      V.counter, V.pub, skip);
-        launch_add_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
-    }
+    refine_steps(V, nr, 0, refine_first_tail(V) - 1);
 }
+// The refinement iterations solve_system left out (refine_eager below the limit).
+void refine_tail(const SolveView& V, int nr) { refine_steps(V, nr, refine_first_tail(V), refine_last(V)); }
 void solve_system(Lane& L, int precision, int nr) { solve_system(lane_view(L), precision, nr); }
 
 void planar_to_interleaved(const std::vector<double>& re, const std::vector<double>& im, int N, int Np, int ncol,
@@ -859,8 +892,8 @@ void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
                         b->ctx->wave_far_r2);
 }
 
-// Issues one frequency on a lane (all asynchronous on the lane's streams).
-// The betas must already be on the device (L.d_betas).
+int enqueue_results(bem_body* b, Lane& L, bool timing);
+
 // Everything one frequency puts on the lane's streams, from the frequency
 // copy to the result copies. timing: per-stage events (not inside a graph).
 int enqueue_frequency(bem_body* b, Lane& L, bool timing) {
@@ -877,6 +910,14 @@ int enqueue_frequency(bem_body* b, Lane& L, bool timing) {
                     L.PhiIr, L.PhiIi, L.DAr, L.DAi);
     launch_matvec<double>(st, b->Np, nr, L.Sr, L.Si, L.Xinr, L.Xini, s.Pr, s.Pi, nullptr, nullptr, 0, s.Br, s.Bi);
     solve_system(L, L.prec, nr);
+    return enqueue_results(b, L, timing);
+}
+
+// The reductions and the result copies of the lane's frequency.
+int enqueue_results(bem_body* b, Lane& L, bool timing) {
+    LinSys& s = L.sys;
+    hipStream_t st = L.s1;
+    const int nb = L.nb;
     // reductions over the body panels only (Prepare_PostProcessing_Mats :145-165, potentials cut to NPA :450-458)
     launch_reduce_coeffs(st, b->NPA, b->Np, nb, b->d_gen, s.Xr, s.Xi, L.PhiIr, L.PhiIi, L.DAr, L.DAi, L.dout->coeffs);
     if (timing) HIP_TRY(hipEventRecord(L.t[3], st));
@@ -898,9 +939,14 @@ int enqueue_frequency(bem_body* b, Lane& L, bool timing) {
 // enqueue sequence into a graph, and later ones replay it: the frequency
 // reaches the kernels through the captured host-to-device copy, and every
 // other input is at a fixed device address.
+// pivot: factor with the pivoting fallback inside the LU (a re-run of a
+// frequency whose speculative factorisation flagged a panel); otherwise the
+// LU is speculative and harvest re-runs flagged frequencies this way.
 int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, double kappa, int compose, int rad_only,
-                    int nb) {
+                    int nb, bool pivot = false) {
     if (b->Np > L.sys.capNp) return fail(BEM_ERR_STATE, "lane workspace smaller than the body");
+    L.lu_spec = pivot ? 0 : 1;
+    L.refine_eager = kRefineEager;
     LinSys& s = L.sys;
     s.N = b->N;
     s.Np = b->Np;
@@ -915,7 +961,7 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
     const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max, L.lu_arith,
                        b->ctx->wave_far_r2};
-    const bool use_graph = b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
+    const bool use_graph = !pivot && b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
     L.last_key = key;
     L.graphed = use_graph;
     if (!use_graph) {
@@ -952,6 +998,26 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
 // silently).
 int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* info) {
     HIP_TRY(hipEventSynchronize(L.done));
+    int np = b->Np / kLuNb;
+    if (L.lu_spec) {
+        // a panel of the speculative factorisation wanted interchanges: the
+        // frequency again, with the pivoting fallback
+        bool flagged = false;
+        for (int i = 0; i < np; i++) flagged = flagged || (L.hout->flags[i] & 1);
+        if (flagged) {
+            int r = issue_frequency(b, L, L.pending, L.mode, L.omega, L.kappa, L.compose, L.rad_only, L.nb, true);
+            if (r) return r;
+            HIP_TRY(hipEventSynchronize(L.done));
+        }
+    }
+    if (L.prec == 32 && !L.hout->ctrl.done) {
+        // the refinement needs the correction steps not enqueued with it
+        refine_tail(lane_view(L), 6 + L.nb);
+        int r = enqueue_results(b, L, !L.graphed);
+        if (r) return r;
+        HIP_TRY(hipEventRecord(L.done, L.s1));
+        HIP_TRY(hipEventSynchronize(L.done));
+    }
     bem_solve_info loc{};
     int fell_back = 0;
     if (L.prec == 32) {
@@ -974,7 +1040,6 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
     } else if (L.compose == BEM_COMPOSE_WAVE && L.kappa != 0.0) {
         loc.exact_points = (int64_t)b->N * b->nq_total;
     }
-    int np = b->Np / kLuNb;
     for (int i = 0; i < np; i++) loc.pivot_fallback += (L.hout->flags[i] & 1) ? 1 : 0;
     float t01 = 0, t12 = 0, t23 = 0;
     if (L.graphed) {   // stages are not timed inside a graph: the whole frequency is in ms_solve
@@ -1345,6 +1410,7 @@ int bem_profile_lu(bem_ctx* c, bem_body* b, int mode, double kappa, double* out)
     Lane& L = *b->lanes[0];
     L.lu_arith = c->lu_arith;
     L.lu_latency = 1;   // one factorisation alone on the device
+    L.lu_spec = 0;      // the full path, pivoting fallback included
     set_frequency(L, 0.0, kappa);
     assemble(b, L, mode, kappa, BEM_COMPOSE_WAVE);
     LuProfile* prof = new LuProfile;

@@ -79,7 +79,13 @@ struct LuWork {
 // kLuArithMask carry a forced tile configuration of the updates on pre-split
 // operands: (cfg + 1) << 4, 0 = autotuned.
 enum { kLuArithF32 = 0, kLuArithSplitF16 = 1, kLuArithSplitF16Tile = 2 };
-constexpr int kLuArithMask = 15, kLuTileShift = 4, kLuTileCfgs = 9;
+constexpr int kLuArithMask = 15, kLuTileShift = 4, kLuTileMask = 255, kLuTileCfgs = 9;
+// With this bit in arith the factorisation runs without its pivoting
+// fallback: no per-panel pivot-check launch and no interchanges (the panel
+// flags still record multipliers with |l| > 1). The caller then re-runs a
+// flagged system with the fallback; an unflagged one has the LU of the full
+// path bit for bit.
+constexpr int kLuSpeculative = 1 << 12;
 // Split-arithmetic factorisations of padded order >= kSplitGroup8Np defer
 // their trailing updates over groups of eight panels (K = 512: half the
 // passes over C, longer thin updates inside the panel kernels); the split
@@ -91,7 +97,8 @@ constexpr int kSplitGroup8Np = 12288, kSplitGroupMax = 8;
 // arith (FP32 only): kLuArithF32 (FP32 MFMA) or kLuArithSplitF16 (fp16 MFMA on
 // split operands, w.umax required; operands split once per panel group when
 // w.opA, w.opB are there) or kLuArithSplitF16Tile (the same, split in every
-// tile); a forced tile configuration in the bits above kLuArithMask.
+// tile); a forced tile configuration in the bits above kLuArithMask;
+// kLuSpeculative (both precisions).
 // latency != 0: the factorisation has the
 // device to itself (one frequency in flight), so the panel kernels take the
 // variant with the shorter critical path and more registers; the results are
@@ -110,6 +117,9 @@ int tune_lu_gemms(hipStream_t s, int Np, T* Ar, T* Ai, float* umax = nullptr, co
 // Tile chosen for an M x N x K update on the current device: 1000 * rows + cols.
 template <typename T>
 int gemm_tile_choice(int M, int N, int K);
+// flag_any: the factorisation's any-interchange flag (the row interchanges
+// are applied to X when it is set); nullptr for a speculative factorisation,
+// which has none (no launch for them).
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
                      const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,

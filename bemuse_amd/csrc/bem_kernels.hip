@@ -3648,7 +3648,8 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const int force_cfg = (arith >> kLuTileShift) - 1;   // forced tile of the pre-split updates, -1: tuned
+    const int force_cfg = ((arith >> kLuTileShift) & kLuTileMask) - 1;   // forced tile of the pre-split updates, -1: tuned
+    const bool spec = (arith & kLuSpeculative) != 0;    // no pivoting fallback (the caller re-runs flagged systems)
     arith &= kLuArithMask;
     const bool split = sizeof(T) == 4 && (arith == kLuArithSplitF16 || arith == kLuArithSplitF16Tile) && w.umax;
     const bool presplit = split && arith == kLuArithSplitF16 && w.opA && w.opB;
@@ -3709,7 +3710,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
             // rows below by L21 of panel q; the main stream factors it (nothing
             // to overlap: no cross-stream hand-off)
             if (j > 0) panel(s, q, k0);
-            fix(q);
+            if (!spec) fix(q);
             // U12 of panel q after the K = j NB update of its rows, and the
             // K = (j+1) NB update of panel q+1's diagonal block
             u12(q, k0, j + 1 < g);
@@ -3741,7 +3742,7 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                         AS, BS, 0, restg - NB, force_cfg);
         }
     }
-    hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
+    if (!spec) hipLaunchKernelGGL(k_flags_any, dim3(1), dim3(256), 0, s, npanel, flag);
     hipLaunchKernelGGL(k_trsm_pcoef<T>, dim3(npanel), dim3(256), 0, s, Np, Ar, Ai, w.Linv_r, w.Linv_i, w.Uinv_r,
                        w.Uinv_i, w.PL_r, w.PL_i, w.PU_r, w.PU_i);
 }
@@ -3754,7 +3755,7 @@ template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
                      const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,
                      const int* skip) {
-    hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
+    if (flag_any) hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
     int nblk = Np / LU_NB;
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i,
                        w.PL_r, w.PL_i, Xr, Xi, pub, counter, skip);
