@@ -372,23 +372,23 @@ void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_s
 // (it > 0: d = LU32^-1 r, x += d) followed by the residual r = b - A x in
 // FP64 (the first also yields |A|_inf) and the convergence check; all of
 // them are no-ops once the device marked the solve done.
+// The residual launch also leaves fp32(r) in X32 (the correction solve's
+// right-hand side), and the correction's upper-triangular solve adds d to x
+// as it writes d.
 void refine_steps(const SolveView& V, int nr, int it0, int it1) {
     LinSys& s = *V.sys;
     hipStream_t st = V.s1;
-    const size_t n = (size_t)s.Np * nr;
     const int* fany = V.lu_spec ? nullptr : s.flag32 + s.Np / kLuNb;
     const int* skip = &V.ctrl->done;
     const int max_iter = std::max(0, std::min(V.refine_max, kRefineMaxIter));
     for (int it = it0; it <= it1; it++) {
-        if (it > 0) {
-            launch_to_f32(st, n, s.Rr, s.Ri, s.X32r, s.X32i, skip);
+        if (it > 0)
             launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32, fany, s.X32r, s.X32i,
-                                          V.counter, V.pub, skip);
-            launch_add_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i, skip);
-        }
-        launch_matvec<double>(st, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, s.Br, s.Bi, 1, s.Rr, s.Ri,
-                              it == 0 ? s.Pabs : nullptr, it == 0 ? s.anorm : nullptr, skip);
-        launch_refine_check(st, s.N, s.Np, nr, s.Rr, s.Ri, s.Xr, s.Xi, s.anorm, V.ctrl, kRefineTol, max_iter);
+                                          V.counter, V.pub, skip, s.Xr, s.Xi, 1);
+        double* pabs = it == 0 ? s.Pabs : nullptr;
+        launch_matvec_partials<double>(st, s.Np, nr, s.Dr, s.Di, s.Xr, s.Xi, s.Pr, s.Pi, pabs, skip);
+        launch_refine_reduce(st, s.N, s.Np, nr, s.Pr, s.Pi, s.Br, s.Bi, s.Rr, s.Ri, s.X32r, s.X32i, s.Xr, s.Xi, pabs,
+                             s.anorm, V.ctrl, kRefineTol, max_iter);
     }
 }
 
@@ -407,11 +407,11 @@ void solve_system(const SolveView& V, int precision, int nr) {
                                         V.lu_spec ? nullptr : s.flag + s.Np / kLuNb, s.Xr, s.Xi, V.counter, V.pub);
         return;
     }
-    launch_refine_reset(st, V.ctrl);
-    launch_to_f32(st, n, s.Br, s.Bi, s.X32r, s.X32i);
+    // x = LU32^-1 b (the refinement state cleared on the way), into X32 and X
+    launch_to_f32(st, n, s.Br, s.Bi, s.X32r, s.X32i, nullptr, V.ctrl);
     launch_lu_solve<float, float>(st, s.Np, nr, s.D32r, s.D32i, s.w32, s.ipiv32,
-                                  V.lu_spec ? nullptr : s.flag32 + s.Np / kLuNb, s.X32r, s.X32i, V.counter, V.pub);
-    launch_from_f32(st, n, s.Xr, s.Xi, s.X32r, s.X32i);
+                                  V.lu_spec ? nullptr : s.flag32 + s.Np / kLuNb, s.X32r, s.X32i, V.counter, V.pub,
+                                  nullptr, s.Xr, s.Xi, 0);
     refine_steps(V, nr, 0, refine_first_tail(V) - 1);
 }
 // The refinement iterations solve_system left out (refine_eager below the limit).

@@ -26,12 +26,18 @@ void launch_incident(hipStream_t s, int N, int Np, int nb, const double* body, c
                      const double* freq, double* Xr, double* Xi, int ldx, double* PhiIr, double* PhiIi,
                      double* DAr, double* DAi);
 int matvec_kslabs(int Np);
-// Y = M X (mode 0) or Y = B - M X (mode 1); with Pabs (nks x Np scratch) and
-// norm, also norm[0] = max row sum of |re| + |im| of M.
+// Y = M X (mode 0) or Y = B - M X (mode 1).
 template <typename TM>
 void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
                    double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
-                   double* Pabs = nullptr, double* norm = nullptr, const int* skip = nullptr);
+                   const int* skip = nullptr);
+// The partial products of launch_matvec only (and the row sums of |re| +
+// |im| per slab into Pabs, nks x Np, when given); the reduction is left to the
+// caller.
+template <typename TM>
+void launch_matvec_partials(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr,
+                            const double* Xi, double* Pr, double* Pi, double* Pabs = nullptr,
+                            const int* skip = nullptr);
 // Optional timing of the trailing-update GEMM launches of one factorisation:
 // events bracket every GEMM launch; useful_flops counts 8 M N K over the
 // unpadded part (order N) of each launch.
@@ -120,14 +126,19 @@ int gemm_tile_choice(int M, int N, int K);
 // flag_any: the factorisation's any-interchange flag (the row interchanges
 // are applied to X when it is set); nullptr for a speculative factorisation,
 // which has none (no launch for them).
+// X64r/X64i (FP32 solves, optional): the solution also into FP64 arrays as
+// the upper-triangular solve writes it, stored (x64_add = 0) or added to
+// them (x64_add = 1).
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
                      const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,
-                     const int* skip = nullptr);
+                     const int* skip = nullptr, double* X64r = nullptr, double* X64i = nullptr, int x64_add = 0);
 // Words of the triangular-solve publication buffer (enough for FP64).
 inline size_t lu_solve_pub_words(int Np) { return (size_t)(Np / kLuNb) * kMaxRhs * kLuNb * 2 * 2; }
+struct RefineCtrl;
+// reset (optional): a refinement state to clear on the way.
 void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb,
-                   const int* skip = nullptr);
+                   const int* skip = nullptr, RefineCtrl* reset = nullptr);
 // Interleaved complex n_rows x ncol (ld n_rows) -> planar Np x ncolp (ld Np),
 // zero beyond the source except a unit diagonal when ident = 1.
 void launch_pack_planar(hipStream_t s, int n_rows, int ncol, int Np, int ncolp, const double* src, double* re,
@@ -138,19 +149,28 @@ void launch_unpack_planar(hipStream_t s, int n_rows, int ncol, int Np, const dou
 // Triangular-solve state as a new lane starts it: counter at epoch 1,
 // nwords publication words at epoch 0.
 void launch_solve_state_init(hipStream_t s, unsigned long long* counter, unsigned long long* pub, size_t nwords);
-void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb,
-                    const int* skip = nullptr);
 
 // Device-resident refinement state (one per solve lane); `done` is the first
 // member so kernels can take &ctrl->done as their skip flag.
 struct RefineCtrl {
     int done, iters, converged, pad;
     double rel, prev;
+    // posted by the work-groups of k_refine_reduce, cleared by its last one:
+    // per column the maxima |r|, |x|, |Im r|, |Im x| and the matrix norm, as
+    // the bit patterns of non-negative doubles; the finished work-groups
+    unsigned long long mx[4][kMaxRhs];
+    unsigned long long an;
+    unsigned count, pad2;
 };
-void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl);
-void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
-                         const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter);
-void launch_from_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb);
+// One refinement residual and convergence check after
+// launch_matvec_partials(M = A, X): R = B - A X, R32 = fp32(R), and the
+// decision in ctrl (done, converged, iters, rel) from the max-norm relative
+// residual against tol; with Pabs (the partials' row sums) anorm[0] =
+// |A|_inf first, else anorm[0] is read.
+void launch_refine_reduce(hipStream_t s, int N, int Np, int nr, const double* Pr, const double* Pi, const double* Br,
+                          const double* Bi, double* Rr, double* Ri, float* R32r, float* R32i, const double* Xr,
+                          const double* Xi, const double* Pabs, double* anorm, RefineCtrl* ctrl, double tol,
+                          int max_iter);
 void launch_copy2(hipStream_t s, size_t n, const double* a, const double* b, double* ca, double* cb);
 void launch_colmax(hipStream_t s, int N, int Np, int nr, const double* rr, const double* ri, const double* xr,
                    const double* xi, double* out);

@@ -187,8 +187,7 @@ __global__ void k_incident(int N, int Np, int nb, const double* __restrict__ bod
 // ---------------------------------------------------------------------------
 constexpr int MV_ROWS = 64, MV_MAXR = 16, MV_KC = 64;
 
-// Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1); with Pabs, also the max
-// row sum of |re| + |im| into norm[0] (one work-group) -- see k_rownorm.
+// Y = sum_ks P (mode 0) or Y = B - sum_ks P (mode 1).
 __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restrict__ Pr, const double* __restrict__ Pi,
                                 const double* Br, const double* Bi, int mode, double* Yr, double* Yi, const int* skip) {
     if (skip && *skip) return;
@@ -203,37 +202,6 @@ __global__ void k_matvec_reduce(int Np, int nr, int nks, const double* __restric
     size_t y = (size_t)j * Np + r;
     if (mode == 1) { sr = Br[y] - sr; si = Bi[y] - si; }
     Yr[y] = sr; Yi[y] = si;
-}
-
-// norm[0] = max over rows of sum_ks Pabs (the infinity norm with |re|+|im|).
-// One work-group of RN_THREADS; four independent partial sums per row keep
-// several loads in flight per thread (a single running sum made every load
-// wait for the previous add).
-constexpr int RN_THREADS = 1024;
-__global__ __launch_bounds__(RN_THREADS) void k_rownorm(int Np, int nks, const double* Pabs, double* norm,
-                                                        const int* skip) {
-    if (skip && *skip) return;
-    __shared__ double s[RN_THREADS];
-    double m = 0.0;
-    for (int r = threadIdx.x; r < Np; r += RN_THREADS) {
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
-        int ks = 0;
-        for (; ks + 4 <= nks; ks += 4) {
-            v0 += Pabs[(size_t)ks * Np + r];
-            v1 += Pabs[(size_t)(ks + 1) * Np + r];
-            v2 += Pabs[(size_t)(ks + 2) * Np + r];
-            v3 += Pabs[(size_t)(ks + 3) * Np + r];
-        }
-        for (; ks < nks; ks++) v0 += Pabs[(size_t)ks * Np + r];
-        m = fmax(m, (v0 + v1) + (v2 + v3));
-    }
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int h = RN_THREADS / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + h]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) norm[0] = s[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -2563,7 +2531,7 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
                                                       const T* __restrict__ Inv_i, const T* __restrict__ P_r,
                                                       const T* __restrict__ P_i, TX* Xr, TX* Xi,
                                                       unsigned long long* pub, unsigned long long* counter,
-                                                      const int* skip) {
+                                                      const int* skip, double* X64r, double* X64i, int x64_add) {
     static_assert(sizeof(T) == sizeof(TX), "factor and solution share one precision");
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
@@ -2707,8 +2675,14 @@ __global__ __launch_bounds__(256) void k_trsm_persist(int Np, int nr, int lower,
             int row = wave * 16 + MT::row(g, r);
             pub_store(pub + pub_off<TX>(b, i, row, 0), zr[r], epoch);
             pub_store(pub + pub_off<TX>(b, i, row, 1), zi[r], epoch);
-            Xr[(size_t)i * Np + r0 + row] = zr[r];
-            Xi[(size_t)i * Np + r0 + row] = zi[r];
+            const size_t o = (size_t)i * Np + r0 + row;
+            const TX xr = zr[r], xi = zi[r];
+            Xr[o] = xr;
+            Xi[o] = xi;
+            if (X64r) {   // the FP64 solution: stored, or the correction added to it
+                if (x64_add) { X64r[o] += (double)xr; X64i[o] += (double)xi; }
+                else { X64r[o] = (double)xr; X64i[o] = (double)xi; }
+            }
         }
     }
 }
@@ -2755,19 +2729,20 @@ __global__ void k_init_lu(int Np, int npanel, int* flag, int* ipiv, float* umax)
 // ---------------------------------------------------------------------------
 // Refinement helpers
 // ---------------------------------------------------------------------------
-__global__ void k_to_f32(size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
+__device__ inline void refine_clear(RefineCtrl* ctrl) {
+    ctrl->done = 0; ctrl->iters = 0; ctrl->converged = 0;
+    ctrl->rel = 1e300; ctrl->prev = 1e300;
+    for (int q = 0; q < 4; q++)
+        for (int j = 0; j < kMaxRhs; j++) ctrl->mx[q][j] = 0ULL;
+    ctrl->an = 0ULL;
+    ctrl->count = 0u;
+}
+__global__ void k_to_f32(size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip,
+                         RefineCtrl* reset) {
     if (skip && *skip) return;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (reset && i == 0) refine_clear(reset);
     if (i < n) { fa[i] = (float)a[i]; fb[i] = (float)b[i]; }
-}
-__global__ void k_add_f32(size_t n, double* a, double* b, const float* fa, const float* fb, const int* skip) {
-    if (skip && *skip) return;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) { a[i] += (double)fa[i]; b[i] += (double)fb[i]; }
-}
-__global__ void k_from_f32(size_t n, double* a, double* b, const float* fa, const float* fb) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) { a[i] = (double)fa[i]; b[i] = (double)fb[i]; }
 }
 // Interleaved complex n_rows x ncol (column-major, ld n_rows) to planar
 // Np x ncolp (ld Np): rows and columns beyond the source are zero, except the
@@ -2831,16 +2806,8 @@ __global__ void k_colmax(int N, int Np, const double* rr, const double* ri, cons
 }
 
 // Device-side control of iterative refinement (no host round trip inside a
-// frequency solve). ctrl: done, iterations, converged; rel[0] = last
-// max-norm relative residual, rel[1] = previous one, rel[2] = |A|_inf.
-__global__ void k_refine_reset(RefineCtrl* ctrl) {
-    if (threadIdx.x == 0) {
-        ctrl->done = 0; ctrl->iters = 0; ctrl->converged = 0;
-        ctrl->rel = 1e300; ctrl->prev = 1e300;
-    }
-}
-
-// After a residual R = B - A X, per right-hand side j over rows < N:
+// frequency solve), after a residual R = B - A X, per right-hand side j over
+// rows < N:
 //   rel_j = max( max|r_j| / (|A| max|x_j|),
 //                max|Im r_j| / (|A| max(max|Im x_j|, 0.01 max|x_j|)) ).
 // The second term asks the imaginary part for the same relative accuracy: at
@@ -2849,79 +2816,126 @@ __global__ void k_refine_reset(RefineCtrl* ctrl) {
 // can leave it inaccurate. The 0.01 floor keeps its roundoff floor (about
 // 1e2 u |A| |x|) under the tolerance. Converged when max_j rel_j <= tol;
 // stops (not converged) on stagnation (rel > 0.5 prev) or after max_iter
-// corrections; any NaN in r or x makes rel NaN, read as not converged. One
-// work-group of 1024 with two rows in flight per thread; per-wave maxima by
-// shuffles.
-constexpr int RC_THREADS = 1024;
+// corrections; any NaN in r or x makes rel NaN, read as not converged.
 
 // max that keeps NaN (fmax drops it): a NaN residual or solution must read as
 // not converged, never as a zero residual
 __device__ inline double nmax(double a, double b) { return (b > a || b != b) ? b : a; }
 
-__global__ __launch_bounds__(RC_THREADS) void k_refine_check(int N, int Np, int nr, const double* Rr,
-                                                             const double* Ri, const double* Xr, const double* Xi,
-                                                             const double* anorm, RefineCtrl* ctrl, double tol,
-                                                             int max_iter) {
-    if (ctrl->done) return;
-    constexpr int NW = RC_THREADS / 64;
-    __shared__ double sm[NW][4][MV_MAXR];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = 0; j < nr; j++) {
-        double mr = 0, mx = 0, mri = 0, mxi = 0;   // max |r|, |x|, |Im r|, |Im x|
-        double mr2 = 0, mx2 = 0, mri2 = 0, mxi2 = 0;
-        int r = threadIdx.x;
-        for (; r + RC_THREADS < N; r += 2 * RC_THREADS) {
-            size_t o = (size_t)j * Np + r, o2 = o + RC_THREADS;
-            double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
-            double rr2 = Rr[o2], ri2 = Ri[o2], xr2 = Xr[o2], xi2 = Xi[o2];
-            mr = nmax(mr, sqrt(rr * rr + ri * ri));
-            mx = nmax(mx, sqrt(xr * xr + xi * xi));
-            mri = nmax(mri, fabs(ri));
-            mxi = nmax(mxi, fabs(xi));
-            mr2 = nmax(mr2, sqrt(rr2 * rr2 + ri2 * ri2));
-            mx2 = nmax(mx2, sqrt(xr2 * xr2 + xi2 * xi2));
-            mri2 = nmax(mri2, fabs(ri2));
-            mxi2 = nmax(mxi2, fabs(xi2));
+// rel = max(rel, rel_j) for the column maxima a = max|r|, b = max|x|,
+// c = max|Im r|, d = max|Im x|, and |A| = an (any NaN: 1e300).
+__device__ inline double refine_column(double rel, double a, double b, double c, double d, double an) {
+    double den = an * b, deni = an * fmax(d, 0.01 * b);
+    double v = den > 0 ? a / den : (a > 0 ? 1e300 : 0.0);
+    double vi = deni > 0 ? c / deni : (c > 0 ? 1e300 : 0.0);
+    const bool finite = a == a && b == b && c == c && d == d && an == an;
+    return finite ? nmax(rel, nmax(v, vi)) : 1e300;
+}
+__device__ inline void refine_decide(RefineCtrl* ctrl, double rel, double tol, int max_iter) {
+    if (!(rel == rel)) rel = 1e300;
+    ctrl->prev = ctrl->rel;
+    ctrl->rel = rel;
+    if (rel <= tol) { ctrl->done = 1; ctrl->converged = 1; }
+    else if (ctrl->iters > 0 && rel > 0.5 * ctrl->prev) ctrl->done = 1;
+    else if (ctrl->iters >= max_iter) ctrl->done = 1;
+    else ctrl->iters += 1;
+}
+
+// The refinement's residual and convergence check in one launch, after the
+// partial products of k_matvec_mfma (slab sums, as for k_matvec_reduce):
+//   r = b - sum_ks P into R, and its FP32 copy into R32 (the right-hand side
+//   of the next correction solve);
+//   the column maxima of the criterion above: each wave (64 rows of one
+//   column: Np is a multiple of 128) reduces them and posts them to the
+//   refinement state with atomicMax on their bit patterns (non-negative
+//   doubles order as their bits, and a NaN's bits lie above those of +inf,
+//   so a NaN survives as with nmax);
+//   with Pabs (the first iteration) also |A|_inf: every column-0 thread sums
+//   its row over the slabs (four running sums, so that several loads are in
+//   flight), and a NaN row sum is dropped;
+//   the last work-group to finish (a counter taken after a device-scope
+//   fence) takes the decision and clears the slots for the next iteration.
+__global__ __launch_bounds__(256) void k_refine_reduce(int N, int Np, int nr, int nks, const double* __restrict__ Pr,
+                                                       const double* __restrict__ Pi, const double* Br,
+                                                       const double* Bi, double* Rr, double* Ri, float* R32r,
+                                                       float* R32i, const double* Xr, const double* Xi,
+                                                       const double* Pabs, double* anorm, RefineCtrl* ctrl,
+                                                       double tol, int max_iter) {
+    if (ctrl->done) return;   // uniform over the grid
+    __shared__ int last;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const size_t total = (size_t)nr * Np;
+    const size_t t = (size_t)blockIdx.x * 256 + tid;
+    double mr = 0.0, mx = 0.0, mri = 0.0, mxi = 0.0, an = 0.0;
+    if (t < total) {
+        const int j = (int)(t / Np), r = (int)(t % Np);
+        double sr = 0.0, si = 0.0;
+        for (int ks = 0; ks < nks; ks++) {
+            size_t o = ((size_t)ks * MV_MAXR + j) * Np + r;
+            sr += Pr[o]; si += Pi[o];
         }
+        const size_t y = (size_t)j * Np + r;
+        sr = Br[y] - sr; si = Bi[y] - si;
+        Rr[y] = sr; Ri[y] = si;
+        R32r[y] = (float)sr; R32i[y] = (float)si;
         if (r < N) {
-            size_t o = (size_t)j * Np + r;
-            double rr = Rr[o], ri = Ri[o], xr = Xr[o], xi = Xi[o];
-            mr = nmax(mr, sqrt(rr * rr + ri * ri));
-            mx = nmax(mx, sqrt(xr * xr + xi * xi));
-            mri = nmax(mri, fabs(ri));
-            mxi = nmax(mxi, fabs(xi));
+            const double xr = Xr[y], xi = Xi[y];
+            mr = sqrt(sr * sr + si * si);
+            mx = sqrt(xr * xr + xi * xi);
+            mri = fabs(si);
+            mxi = fabs(xi);
         }
-        mr = nmax(mr, mr2); mx = nmax(mx, mx2); mri = nmax(mri, mri2); mxi = nmax(mxi, mxi2);
-        for (int off = 32; off > 0; off >>= 1) {
-            mr = nmax(mr, __shfl_xor(mr, off));
-            mx = nmax(mx, __shfl_xor(mx, off));
-            mri = nmax(mri, __shfl_xor(mri, off));
-            mxi = nmax(mxi, __shfl_xor(mxi, off));
-        }
-        if (lane == 0) { sm[wave][0][j] = mr; sm[wave][1][j] = mx; sm[wave][2][j] = mri; sm[wave][3][j] = mxi; }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double an = anorm[0], rel = 0.0;
-        for (int j = 0; j < nr; j++) {
-            double a = 0, b = 0, c = 0, d = 0;
-            for (int w = 0; w < NW; w++) {
-                a = nmax(a, sm[w][0][j]); b = nmax(b, sm[w][1][j]);
-                c = nmax(c, sm[w][2][j]); d = nmax(d, sm[w][3][j]);
+        if (Pabs && j == 0) {
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+            int ks = 0;
+            for (; ks + 4 <= nks; ks += 4) {
+                v0 += Pabs[(size_t)ks * Np + r];
+                v1 += Pabs[(size_t)(ks + 1) * Np + r];
+                v2 += Pabs[(size_t)(ks + 2) * Np + r];
+                v3 += Pabs[(size_t)(ks + 3) * Np + r];
             }
-            double den = an * b, deni = an * fmax(d, 0.01 * b);
-            double v = den > 0 ? a / den : (a > 0 ? 1e300 : 0.0);
-            double vi = deni > 0 ? c / deni : (c > 0 ? 1e300 : 0.0);
-            const bool finite = a == a && b == b && c == c && d == d && an == an;
-            rel = finite ? nmax(rel, nmax(v, vi)) : 1e300;
+            for (; ks < nks; ks++) v0 += Pabs[(size_t)ks * Np + r];
+            an = fmax(0.0, (v0 + v1) + (v2 + v3));
         }
-        if (!(rel == rel)) rel = 1e300;
-        ctrl->prev = ctrl->rel;
-        ctrl->rel = rel;
-        if (rel <= tol) { ctrl->done = 1; ctrl->converged = 1; }
-        else if (ctrl->iters > 0 && rel > 0.5 * ctrl->prev) ctrl->done = 1;
-        else if (ctrl->iters >= max_iter) ctrl->done = 1;
-        else ctrl->iters += 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = nmax(mr, __shfl_xor(mr, off));
+        mx = nmax(mx, __shfl_xor(mx, off));
+        mri = nmax(mri, __shfl_xor(mri, off));
+        mxi = nmax(mxi, __shfl_xor(mxi, off));
+        an = fmax(an, __shfl_xor(an, off));
+    }
+    const size_t tw = t - lane;                    // the wave's first element
+    if (lane == 0 && tw < total) {
+        const int jw = (int)(tw / Np);
+        atomicMax(&ctrl->mx[0][jw], (unsigned long long)__double_as_longlong(mr));
+        atomicMax(&ctrl->mx[1][jw], (unsigned long long)__double_as_longlong(mx));
+        atomicMax(&ctrl->mx[2][jw], (unsigned long long)__double_as_longlong(mri));
+        atomicMax(&ctrl->mx[3][jw], (unsigned long long)__double_as_longlong(mxi));
+        if (Pabs) atomicMax(&ctrl->an, (unsigned long long)__double_as_longlong(an));
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&ctrl->count, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (tid == 0) {
+        auto rd = [](unsigned long long* p) {
+            return __longlong_as_double((long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        };
+        double a_n;
+        if (Pabs) { a_n = rd(&ctrl->an); anorm[0] = a_n; }
+        else a_n = anorm[0];
+        double rel = 0.0;
+        for (int j = 0; j < nr; j++)
+            rel = refine_column(rel, rd(&ctrl->mx[0][j]), rd(&ctrl->mx[1][j]), rd(&ctrl->mx[2][j]), rd(&ctrl->mx[3][j]),
+                                a_n);
+        refine_decide(ctrl, rel, tol, max_iter);
+        for (int q = 0; q < 4; q++)
+            for (int j = 0; j < kMaxRhs; j++) ctrl->mx[q][j] = 0ULL;
+        ctrl->an = 0ULL;
+        ctrl->count = 0u;
     }
 }
 
@@ -3170,13 +3184,18 @@ int measure_peaks(hipStream_t s, double* out) {
 
 int matvec_kslabs(int Np) { return Np >= 4096 ? 32 : (Np >= 1024 ? 16 : 4); }
 
-template <typename TM>
-void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
-                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
-                   double* Pabs, double* norm, const int* skip) {
+// K slabs of the tall-skinny products (nks) and their width (kslab).
+int matvec_slabs(int Np, int& kslab) {
     int nks = matvec_kslabs(Np);
-    int kslab = (Np / nks + 63) / 64 * 64;
-    nks = (Np + kslab - 1) / kslab;
+    kslab = (Np / nks + 63) / 64 * 64;
+    return (Np + kslab - 1) / kslab;
+}
+
+template <typename TM>
+void launch_matvec_partials(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr,
+                            const double* Xi, double* Pr, double* Pi, double* Pabs, const int* skip) {
+    int kslab = 0;
+    const int nks = matvec_slabs(Np, kslab);
     dim3 grid(Np / MV_ROWS, nks);
     if (Pabs)
         hipLaunchKernelGGL((k_matvec_mfma<TM, true>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr, Pi,
@@ -3184,13 +3203,22 @@ void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, co
     else
         hipLaunchKernelGGL((k_matvec_mfma<TM, false>), grid, dim3(256), 0, s, Np, nr, kslab, Mr, Mi, Xr, Xi, Pr,
                            Pi, Pabs, skip);
+}
+template void launch_matvec_partials<double>(hipStream_t, int, int, const double*, const double*, const double*,
+                                             const double*, double*, double*, double*, const int*);
+
+template <typename TM>
+void launch_matvec(hipStream_t s, int Np, int nr, const TM* Mr, const TM* Mi, const double* Xr, const double* Xi,
+                   double* Pr, double* Pi, const double* Br, const double* Bi, int mode, double* Yr, double* Yi,
+                   const int* skip) {
+    int kslab = 0;
+    const int nks = matvec_slabs(Np, kslab);
+    launch_matvec_partials<TM>(s, Np, nr, Mr, Mi, Xr, Xi, Pr, Pi, nullptr, skip);
     hipLaunchKernelGGL(k_matvec_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, Np, nr, nks, Pr, Pi, Br, Bi,
                        mode, Yr, Yi, skip);
-    if (Pabs && norm) hipLaunchKernelGGL(k_rownorm, dim3(1), dim3(RN_THREADS), 0, s, Np, nks, Pabs, norm, skip);
 }
 template void launch_matvec<double>(hipStream_t, int, int, const double*, const double*, const double*, const double*,
-                                    double*, double*, const double*, const double*, int, double*, double*, double*,
-                                    double*, const int*);
+                                    double*, double*, const double*, const double*, int, double*, double*, const int*);
 
 // ---------------------------------------------------------------------------
 // GEMM tile autotuner. The tile shapes of k_cgemm_sub are timed on the device
@@ -3754,36 +3782,35 @@ template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,
                      const int* flag_any, TX* Xr, TX* Xi, unsigned long long* counter, unsigned long long* pub,
-                     const int* skip) {
+                     const int* skip, double* X64r, double* X64i, int x64_add) {
     if (flag_any) hipLaunchKernelGGL(k_apply_ipiv<TX>, dim3(1), dim3(64), 0, s, Np, nr, ipiv, flag_any, Xr, Xi, skip);
     int nblk = Np / LU_NB;
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 1, Ar, Ai, w.Linv_r, w.Linv_i,
-                       w.PL_r, w.PL_i, Xr, Xi, pub, counter, skip);
+                       w.PL_r, w.PL_i, Xr, Xi, pub, counter, skip, nullptr, nullptr, 0);
     hipLaunchKernelGGL((k_trsm_persist<T, TX>), dim3(nblk), dim3(256), 0, s, Np, nr, 0, Ar, Ai, w.Uinv_r, w.Uinv_i,
-                       w.PU_r, w.PU_i, Xr, Xi, pub, counter, skip);
+                       w.PU_r, w.PU_i, Xr, Xi, pub, counter, skip, sizeof(TX) == 4 ? X64r : nullptr,
+                       sizeof(TX) == 4 ? X64i : nullptr, x64_add);
 }
 template void launch_lu_solve<double, double>(hipStream_t, int, int, const double*, const double*,
                                               const LuWork<double>&, const int*, const int*, double*, double*,
-                                              unsigned long long*, unsigned long long*, const int*);
+                                              unsigned long long*, unsigned long long*, const int*, double*, double*,
+                                              int);
 template void launch_lu_solve<float, float>(hipStream_t, int, int, const float*, const float*, const LuWork<float>&,
                                             const int*, const int*, float*, float*, unsigned long long*,
-                                            unsigned long long*, const int*);
+                                            unsigned long long*, const int*, double*, double*, int);
 
-void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip) {
-    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);
+void launch_to_f32(hipStream_t s, size_t n, const double* a, const double* b, float* fa, float* fb, const int* skip,
+                   RefineCtrl* reset) {
+    hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip, reset);
 }
-void launch_add_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb, const int* skip) {
-    hipLaunchKernelGGL(k_add_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb, skip);
-}
-void launch_from_f32(hipStream_t s, size_t n, double* a, double* b, const float* fa, const float* fb) {
-    hipLaunchKernelGGL(k_from_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, n, a, b, fa, fb);
-}
-void launch_refine_reset(hipStream_t s, RefineCtrl* ctrl) {
-    hipLaunchKernelGGL(k_refine_reset, dim3(1), dim3(64), 0, s, ctrl);
-}
-void launch_refine_check(hipStream_t s, int N, int Np, int nr, const double* Rr, const double* Ri, const double* Xr,
-                         const double* Xi, const double* anorm, RefineCtrl* ctrl, double tol, int max_iter) {
-    hipLaunchKernelGGL(k_refine_check, dim3(1), dim3(RC_THREADS), 0, s, N, Np, nr, Rr, Ri, Xr, Xi, anorm, ctrl, tol, max_iter);
+void launch_refine_reduce(hipStream_t s, int N, int Np, int nr, const double* Pr, const double* Pi, const double* Br,
+                          const double* Bi, double* Rr, double* Ri, float* R32r, float* R32i, const double* Xr,
+                          const double* Xi, const double* Pabs, double* anorm, RefineCtrl* ctrl, double tol,
+                          int max_iter) {
+    int kslab = 0;
+    const int nks = matvec_slabs(Np, kslab);
+    hipLaunchKernelGGL(k_refine_reduce, dim3(cdiv((size_t)nr * Np, 256)), dim3(256), 0, s, N, Np, nr, nks, Pr, Pi, Br,
+                       Bi, Rr, Ri, R32r, R32i, Xr, Xi, Pabs, anorm, ctrl, tol, max_iter);
 }
 void launch_pack_planar(hipStream_t s, int n_rows, int ncol, int Np, int ncolp, const double* src, double* re,
                         double* im, int ident) {
