@@ -3727,10 +3727,18 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                                w.Linv_r + (size_t)p * NB2, w.Linv_i + (size_t)p * NB2, umax);
     };
     const int NB = LU_NB, group = lu_group<T>(Np, split);
+    // A factorisation that shares the device with others (several lanes)
+    // below kWidePanelNp factors every panel on the main stream: the
+    // lookahead's overlap is what the other lanes' work gives anyway, and
+    // each active stream is one more hardware queue for the command
+    // processor to serve (64 seeded bodies x 16 frequencies: 512 -> 531
+    // mixed solves/s; the semi-sub sweep unchanged).
+    const bool one_stream = !latency && Np < kWidePanelNp;
+    if (one_stream) s2 = s;
     panel(s, 0, 0);
     for (int p = 0; p < npanel; p += group) {
         const int g = std::min(group, npanel - p), k0 = p * NB;
-        if (p > 0) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);   // panel p factored on s2
+        if (p > 0 && !one_stream) (void)hipStreamWaitEvent(s, ev[2 * p + 1], 0);   // panel p factored on s2
         for (int j = 0; j < g; j++) {
             const int q = p + j;
             // j > 0: the group's K = j NB update of panel q's columns was applied
@@ -3748,10 +3756,12 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         // the next group's first panel (thin: its operands split in the tiles),
         // then hand it to the panel stream
         gemm_sub<T>(s, dev, Np, restg, NB, g * NB, Ar, Ai, kg, k0, k0, kg, kg, kg, prof, 1, umax);
-        (void)hipEventRecord(ev[2 * (p + g)], s);
-        (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
+        if (!one_stream) {
+            (void)hipEventRecord(ev[2 * (p + g)], s);
+            (void)hipStreamWaitEvent(s2, ev[2 * (p + g)], 0);
+        }
         panel(s2, p + g, kg);
-        (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
+        if (!one_stream) (void)hipEventRecord(ev[2 * (p + g) + 1], s2);
         if (restg > NB) {   // trailing update of everything right of it
             // split arithmetic: its operands (rows kg.. of the group's L21,
             // columns kg+NB.. of its U12) in fp16 split form once, beside the
