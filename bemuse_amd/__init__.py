@@ -163,6 +163,8 @@ _SIGS = {
     "bem_set_refine_limit": (c_int, [c_void_p, c_int]),
     "bem_set_lu_arith": (c_int, [c_void_p, c_int]),
     "bem_set_lu_tile": (c_int, [c_void_p, c_int]),
+    "bem_set_lu_speculative": (c_int, [c_void_p, c_int]),
+    "bem_set_pivot_threshold": (c_int, [c_void_p, c_double]),
     "bem_set_wave_far32": (c_int, [c_void_p, c_double]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
@@ -460,6 +462,17 @@ class Context:
         LDS chunks), 8 the 128x128 tile with direct memory-to-LDS operand copies. Results do
         not depend on it."""
         _check(lib().bem_set_lu_tile(self._h, int(tile)))
+
+    def set_lu_speculative(self, enable):
+        """Solve lanes factor speculatively (default True: no per-panel pivot check, a
+        frequency needing interchanges re-run with it) or with the pivot check after every
+        panel (False). Results are the same bit for bit wherever no panel is rejected."""
+        _check(lib().bem_set_lu_speculative(self._h, int(bool(enable))))
+
+    def set_pivot_threshold(self, tau):
+        """Multiplier bound of the pivot check (0..1, default 1): panels with a multiplier
+        |l| > tau are factored again with partial pivoting (0: every panel)."""
+        _check(lib().bem_set_pivot_threshold(self._h, float(tau)))
 
     def set_graphs(self, enable):
         """Capture and replay per-frequency HIP graphs on the solve lanes (default off)."""

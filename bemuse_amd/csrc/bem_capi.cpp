@@ -113,6 +113,8 @@ struct bem_ctx {
     hipEvent_t na_t[3] = {};
     SurState* surrogate = nullptr;     // trained wave-term surrogate, if any
     bool graphs = false;               // replay captured frequency graphs on the lanes (bem_set_graphs)
+    int lu_spec = 1;                   // solve lanes factor speculatively (bem_set_lu_speculative)
+    double pivot_tau = 1.0;            // multiplier bound of the pivot check (bem_set_pivot_threshold)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
     std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch
@@ -243,6 +245,7 @@ struct Lane {
     // (speculative), and the correction steps enqueued with the solve
     int lu_spec = 0;
     int refine_eager = kRefineMaxIter;
+    double pivot_tau = 1.0;   // the pivot check's multiplier bound
     bool sur_used = false;
     double omega = 0, kappa = 0;
     // captured frequency graph and the configuration it was captured for
@@ -339,6 +342,7 @@ struct SolveView {
     int lu_latency = 1;    // one frequency in flight (launch_lu's latency)
     int lu_spec = 0;       // speculative factorisation (kLuSpeculative): the caller re-runs flagged systems
     int refine_eager = kRefineMaxIter;   // correction steps enqueued with the solve (the rest: refine_tail)
+    double pivot_tau = 1.0;              // the pivot check's multiplier bound
 };
 
 SolveView lane_view(Lane& L) {
@@ -348,6 +352,7 @@ SolveView lane_view(Lane& L) {
     v.lu_latency = L.lu_latency;
     v.lu_spec = L.lu_spec;
     v.refine_eager = L.refine_eager;
+    v.pivot_tau = L.pivot_tau;
     return v;
 }
 
@@ -357,10 +362,10 @@ void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr)
     const int spec = V.lu_spec ? kLuSpeculative : 0;
     if (precision == 64)
         launch_lu<double>(V.s1, V.s2, V.ev, s.Np, s.Dr, s.Di, s.w64, s.flag, s.ipiv, prof, kLuArithF32 | spec,
-                          V.lu_latency);
+                          V.lu_latency, V.pivot_tau);
     else
         launch_lu<float>(V.s1, V.s2, V.ev, s.Np, s.D32r, s.D32i, s.w32, s.flag32, s.ipiv32, prof, V.lu_arith | spec,
-                         V.lu_latency);
+                         V.lu_latency, V.pivot_tau);
 }
 void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
 
@@ -945,8 +950,9 @@ int enqueue_results(bem_body* b, Lane& L, bool timing) {
 int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, double kappa, int compose, int rad_only,
                     int nb, bool pivot = false) {
     if (b->Np > L.sys.capNp) return fail(BEM_ERR_STATE, "lane workspace smaller than the body");
-    L.lu_spec = pivot ? 0 : 1;
+    L.lu_spec = (pivot || !b->ctx->lu_spec) ? 0 : 1;
     L.refine_eager = kRefineEager;
+    L.pivot_tau = b->ctx->pivot_tau;
     LinSys& s = L.sys;
     s.N = b->N;
     s.Np = b->Np;
@@ -1199,6 +1205,18 @@ int bem_set_lu_arith(bem_ctx* c, int arith) {
         return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32, BEM_LU_SPLIT_F16 or BEM_LU_SPLIT_F16_TILE");
     const int base = arith == BEM_LU_FP32 ? kLuArithF32 : arith == BEM_LU_SPLIT_F16 ? kLuArithSplitF16 : kLuArithSplitF16Tile;
     c->lu_arith = (c->lu_arith & ~kLuArithMask) | base;
+    return BEM_OK;
+}
+
+int bem_set_pivot_threshold(bem_ctx* c, double tau) {
+    if (!c || !(tau >= 0.0 && tau <= 1.0)) return fail(BEM_ERR_ARG, "bem_set_pivot_threshold: tau in [0, 1]");
+    c->pivot_tau = tau;
+    return BEM_OK;
+}
+
+int bem_set_lu_speculative(bem_ctx* c, int enable) {
+    if (!c) return fail(BEM_ERR_ARG, "bem_set_lu_speculative: null context");
+    c->lu_spec = enable ? 1 : 0;
     return BEM_OK;
 }
 
@@ -1914,6 +1932,7 @@ int bem_dense_solve(bem_ctx* c, int precision, int n, const double* A, int nrhs,
     if (r) return r;
     L.refine_max = c->refine_max;
     L.lu_arith = c->lu_arith;
+    L.pivot_tau = c->pivot_tau;
     LinSys& s = L.sys;
     int Np = s.Np;
     {
@@ -2021,6 +2040,7 @@ int bem_dense_solve_noalloc(bem_ctx* c, int precision, int n, const double* A, i
     V.s1 = c->stream; V.s2 = c->na_s2; V.ev = c->na_ev; V.sys = &s;
     V.counter = (unsigned long long*)at(L.counter); V.pub = (unsigned long long*)at(L.pub);
     V.ctrl = (RefineCtrl*)at(L.ctrl); V.refine_max = c->refine_max; V.lu_arith = c->lu_arith;
+    V.pivot_tau = c->pivot_tau;
     hipStream_t st = c->stream;
     HIP_TRY(hipMemcpyAsync(stA, A, 16 * (size_t)n * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(stB, B, 16 * (size_t)n * nrhs, hipMemcpyHostToDevice, st));

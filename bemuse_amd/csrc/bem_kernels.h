@@ -109,9 +109,13 @@ constexpr int kSplitGroup8Np = 12288, kSplitGroupMax = 8;
 // device to itself (one frequency in flight), so the panel kernels take the
 // variant with the shorter critical path and more registers; the results are
 // the same bit for bit.
+// pivot_tau (0..1): a panel with a multiplier |l| > pivot_tau takes the
+// partial-pivoting fallback (1: exactly where partial pivoting would pick
+// another pivot than the diagonal).
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof = nullptr, int arith = kLuArithF32, int latency = 1);
+               int* ipiv, LuProfile* prof = nullptr, int arith = kLuArithF32, int latency = 1,
+               double pivot_tau = 1.0);
 // Times the trailing-update tile shapes on the GEMM shapes of a factorisation
 // of padded order Np (once per device, precision and Np) using the Np x Np
 // scratch matrix Ar/Ai; returns the number of shape classes tuned.

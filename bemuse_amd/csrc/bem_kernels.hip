@@ -505,7 +505,7 @@ template <typename T>
 __device__ inline bool pivot_ok(T den) { return den > (T)0 && den < (T)INFINITY; }
 
 template <typename T>
-__device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad, T (&ar)[DB], T (&ai)[DB]) {
+__device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad, T (&ar)[DB], T (&ai)[DB], T tau2) {
     const int rho = lane & 15;
     const bool writer = lane < DB;
 #pragma unroll
@@ -521,7 +521,7 @@ __device__ inline void factor16(DiagLds<T>& S, int o, int lane, int& mybad, T (&
         T qr = pr * inv, qi = -pi * inv;
         const bool below = rho > j;
         T mr = ar[j] * qr - ai[j] * qi, mi = ar[j] * qi + ai[j] * qr;
-        if (below && !(cabs2(mr, mi) <= (T)1)) mybad |= 1;   // also NaN
+        if (below && !(cabs2(mr, mi) <= tau2)) mybad |= 1;   // also NaN
         mr = below ? mr : (T)0;
         mi = below ? mi : (T)0;
         ar[j] = below ? mr : ar[j];
@@ -725,7 +725,7 @@ __device__ inline void inv_column(DiagLds<T>& S, T* Li_r, T* Li_i, T* Ui_r, T* U
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T* save_r, T* save_i, T* Li_r,
-                                                 T* Li_i, T* Ui_r, T* Ui_i, int* flag) {
+                                                 T* Li_i, T* Ui_r, T* Ui_i, int* flag, T tau2) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
     for (int kb = 0; kb < 4; kb++) {
         const int o = DB * kb;
         T far_[DB], fai_[DB];                        // wave 0: its rows of the factored block
-        if (wave == 0) factor16<T>(S, o, lane, mybad, far_, fai_);
+        if (wave == 0) factor16<T>(S, o, lane, mybad, far_, fai_, tau2);
         __syncthreads();
         // the two triangle inverses side by side, each in one wave's registers
         if (wave == 0) linv16_rows<T>(S, kb, lane, far_, fai_);
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     int row = MT::row(g, r);
-                    if (!(cabs2(cr[r], ci[r]) <= (T)1)) mybad |= 1;
+                    if (!(cabs2(cr[r], ci[r]) <= tau2)) mybad |= 1;
                     S.ar[o + i][DB * b + row] = cr[r]; S.ai[o + i][DB * b + row] = ci[r];
                 }
             } else {
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 // keeps the updated rows in LDS, and saves and factors those.
 template <typename T, bool PIPE>
 __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, int kp, T* Ar, T* Ai, const T* Ui_r, const T* Ui_i,
-                                                T* save_r, T* save_i, int* flag) {
+                                                T* save_r, T* save_i, int* flag, T tau2) {
     // on the factorisation's critical path: issue ahead of other lanes' GEMM waves on the CU
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, int kp, T* Ar, T
         size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + MT::row(g, r);
         T lr = cr[r], li = ci[r];
         Ar[o] = lr; Ai[o] = li;
-        if (!(cabs2(lr, li) <= (T)1)) mybad = 1;
+        if (!(cabs2(lr, li) <= tau2)) mybad = 1;
     }
     if (mybad) atomicOr(flag + k0 / LU_NB, 1);
 }
@@ -1053,7 +1053,7 @@ __device__ inline void cmma_k_wide(const T* ar_, const T* ai_, size_t lda, const
 // of all four 16-row blocks.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_l21_wide(int Np, int k0, int kp, T* Ar, T* Ai, const T* Ui_r,
-                                                     const T* Ui_i, T* save_r, T* save_i, int* flag) {
+                                                     const T* Ui_i, T* save_r, T* save_i, int* flag, T tau2) {
     __builtin_amdgcn_s_setprio(3);
     typedef MfmaT<T> MT;
     typedef typename MT::acc acc_t;
@@ -1117,7 +1117,7 @@ __global__ __launch_bounds__(256) void k_lu_l21_wide(int Np, int k0, int kp, T* 
             const size_t o = (size_t)(k0 + 16 * b + i) * Np + row0 + 16 * q + MT::row(g, r);
             const T lr = cr[q][r], li = ci[q][r];
             Ar[o] = lr; Ai[o] = li;
-            if (!(cabs2(lr, li) <= (T)1)) mybad = 1;
+            if (!(cabs2(lr, li) <= tau2)) mybad = 1;
         }
     if (mybad) atomicOr(flag + k0 / LU_NB, 1);
 }
@@ -3672,12 +3672,13 @@ template double replay_lu_gemms<double>(const LuProfile&, hipStream_t);
 // ev[2 q], ev[2 q + 1] (q the first panel of a group) order the two streams.
 template <typename T>
 void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* Ai, const LuWork<T>& w, int* flag,
-               int* ipiv, LuProfile* prof, int arith, int latency) {
+               int* ipiv, LuProfile* prof, int arith, int latency, double pivot_tau) {
     int npanel = Np / LU_NB;    // even: Np is a multiple of 128
     int dev = 0;
     (void)hipGetDevice(&dev);
     const int force_cfg = ((arith >> kLuTileShift) & kLuTileMask) - 1;   // forced tile of the pre-split updates, -1: tuned
     const bool spec = (arith & kLuSpeculative) != 0;    // no pivoting fallback (the caller re-runs flagged systems)
+    const T tau2 = (T)(pivot_tau * pivot_tau);           // multipliers with |l|^2 above it flag their panel
     arith &= kLuArithMask;
     const bool split = sizeof(T) == 4 && (arith == kLuArithSplitF16 || arith == kLuArithSplitF16Tile) && w.umax;
     const bool presplit = split && arith == kLuArithSplitF16 && w.opA && w.opB;
@@ -3690,18 +3691,18 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
         T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
         T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
         hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, w.sd_r, w.sd_i, Lir, Lii, Uir,
-                           Uii, flag);
+                           Uii, flag, tau2);
         auto l21k = latency ? k_lu_l21<T, true> : k_lu_l21<T, false>;
         if constexpr (sizeof(T) == 4) {
             if (rest > 0 && !latency && Np >= kWidePanelNp) {   // shared device: 64 rows per work-group
                 hipLaunchKernelGGL(k_lu_l21_wide<T>, dim3(rest / (16 * WIDE)), dim3(256), 0, st, Np, k0, kp, Ar, Ai,
-                                   Uir, Uii, w.s21_r, w.s21_i, flag);
+                                   Uir, Uii, w.s21_r, w.s21_i, flag, tau2);
                 return;
             }
         }
         if (rest > 0)
             hipLaunchKernelGGL(l21k, dim3(rest / 16), dim3(256), 0, st, Np, k0, kp, Ar, Ai, Uir, Uii, w.s21_r, w.s21_i,
-                               flag);
+                               flag, tau2);
     };
     auto fix = [&](int p) {      // pivot check and fallback of panel p, row swaps elsewhere
         int k0 = p * LU_NB;
@@ -3785,9 +3786,9 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
                        w.Uinv_i, w.PL_r, w.PL_i, w.PU_r, w.PU_i);
 }
 template void launch_lu<double>(hipStream_t, hipStream_t, hipEvent_t*, int, double*, double*, const LuWork<double>&,
-                                 int*, int*, LuProfile*, int, int);
+                                 int*, int*, LuProfile*, int, int, double);
 template void launch_lu<float>(hipStream_t, hipStream_t, hipEvent_t*, int, float*, float*, const LuWork<float>&, int*,
-                                int*, LuProfile*, int, int);
+                                int*, LuProfile*, int, int, double);
 
 template <typename T, typename TX>
 void launch_lu_solve(hipStream_t s, int Np, int nr, const T* Ar, const T* Ai, const LuWork<T>& w, const int* ipiv,

@@ -206,6 +206,25 @@ int bem_set_lu_arith(bem_ctx* ctx, int arith);
  * counterpart in the reference. */
 int bem_set_lu_tile(bem_ctx* ctx, int tile);
 
+/* How the solve lanes (bem_solve_frequency, bem_solve_sweep,
+ * bem_solve_batch) factor: enable = 1 (default) speculatively, every panel
+ * with its diagonal as pivot and no pivot-check launch between the panel
+ * kernels, a frequency whose factorisation met a multiplier above the pivot
+ * threshold (bem_set_pivot_threshold) being run again as with 0; enable = 0:
+ * the pivot check after every panel, with the partial-pivoting
+ * refactorisation of a rejected one in place. Both give the same results bit
+ * for bit wherever no panel is rejected. No counterpart in the reference. */
+int bem_set_lu_speculative(bem_ctx* ctx, int enable);
+
+/* Multiplier bound of the factorisations' pivot check, tau in [0, 1]: a panel
+ * with a multiplier |l| > tau (or a zero or non-finite pivot) is factored
+ * again with partial pivoting (the reference's PartialPivLU, Solver_Base.h:85,
+ * pivots by largest modulus). 1 (default): exactly the panels where partial
+ * pivoting would choose another pivot than the diagonal; below 1 more panels
+ * take the partial-pivoting path (0: every panel). Applies to every solve
+ * path (lanes, bem_dense_solve, the no-allocation solve). */
+int bem_set_pivot_threshold(bem_ctx* ctx, double tau);
+
 /* Measurement option, off by default: the wave term of receiver / source
  * pairs whose first mirrored quadrature node lies farther than near_radius
  * (metres) from the receiver is evaluated with FP32 exp, log, Bessel, Struve

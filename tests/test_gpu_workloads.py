@@ -104,6 +104,63 @@ def test_cylinder1024_lane_count_does_not_change_results(cyl1024, mode):
         np.testing.assert_array_equal(one[key], two[key], err_msg=key)
 
 
+@pytest.mark.parametrize("mode", MODES)
+def test_cylinder1024_speculative_equals_checked(cyl1024, ctx, mode):
+    """The solve lanes' speculative factorisation (no pivot-check launch)
+    against the checked one (a pivot check after every panel): the same
+    results bit for bit, alone on the device (one lane) and sharing it
+    (three)."""
+    body, ob, omegas, refs = cyl1024
+    w = omegas[1:7]
+    betas = [0.0, 0.9]
+    try:
+        for lanes in (1, 3):
+            ctx.set_lu_speculative(True)
+            spec = body.solve_sweep(w, betas=betas, mode=mode, lanes=lanes)
+            ctx.set_lu_speculative(False)
+            chk = body.solve_sweep(w, betas=betas, mode=mode, lanes=lanes)
+            for key in ("A", "B", "F", "FK", "SC"):
+                np.testing.assert_array_equal(spec[key], chk[key], err_msg=f"{key} lanes {lanes}")
+            assert all(i["pivot_fallback"] == 0 for i in spec["info"])
+    finally:
+        ctx.set_lu_speculative(True)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("tau", [0.0, 0.05])
+def test_cylinder1024_pivot_threshold_reruns(cyl1024, ctx, mode, tau):
+    """A pivot-check bound below 1 sends panels to the partial-pivoting
+    fallback: a speculative factorisation that meets such a panel is run again
+    with the check (the frequency's results then equal those of the checked
+    path bit for bit), and the results stay at the tight level against the
+    default bound (different pivots, different rounding)."""
+    body, ob, omegas, refs = cyl1024
+    w = omegas[[0, 4, 9]]
+    betas = [0.0]
+    base = body.solve_sweep(w, betas=betas, mode=mode, lanes=2)
+    try:
+        ctx.set_pivot_threshold(tau)
+        spec = body.solve_sweep(w, betas=betas, mode=mode, lanes=2)
+        ctx.set_lu_speculative(False)
+        chk = body.solve_sweep(w, betas=betas, mode=mode, lanes=2)
+    finally:
+        ctx.set_pivot_threshold(1.0)
+        ctx.set_lu_speculative(True)
+    assert any(i["pivot_fallback"] > 0 for i in spec["info"]), spec["info"]
+    if tau == 0.0:   # every panel of a 1024-panel system (16 panels) takes the fallback
+        assert all(i["pivot_fallback"] == 16 for i in spec["info"]), spec["info"]
+    for key in ("A", "B", "F"):
+        np.testing.assert_array_equal(spec[key], chk[key], err_msg=key)
+        for k in range(len(w)):
+            e = relerr(spec[key][k], base[key][k])
+            assert e < TIGHT[mode], (key, k, e)
+    if mode == bm.MODE_MIXED:
+        assert all(i["fell_back_fp64"] == 0 and i["residual"] < 1e-13 for i in spec["info"])
+    for bad in (-0.1, 1.5, float("nan")):
+        with pytest.raises(bm.BemError):
+            ctx.set_pivot_threshold(bad)
+
+
 # ---------------------------------------------------------------------------
 # 6018-panel semi-sub, 40 frequencies, 5 headings
 # ---------------------------------------------------------------------------
@@ -166,6 +223,16 @@ def test_semisub6018_mixed_vs_fp64_all_frequencies(semisub):
     assert all(i["lanes"] == 10 for i in sauto["info"]), sauto["info"][0]
     for k in ("A", "B", "F"):
         np.testing.assert_array_equal(sauto[k], smx[k], err_msg=k)
+    # the checked factorisation (pivot check after every panel) on every 8th
+    # frequency: the same results bit for bit
+    ctx = body.ctx
+    try:
+        ctx.set_lu_speculative(False)
+        schk = body.solve_sweep(omegas[::8], betas=betas, mode=bm.MODE_MIXED, lanes=5)
+    finally:
+        ctx.set_lu_speculative(True)
+    for k in ("A", "B", "F"):
+        np.testing.assert_array_equal(schk[k], smx[k][::8], err_msg=f"{k} checked")
 
 
 # ---------------------------------------------------------------------------
