@@ -1030,7 +1030,10 @@ int harvest(bem_body* b, Lane& L, double* A36, double* B36, double* F, double* F
         loc.refine_iters = L.hout->ctrl.iters;
         loc.residual = L.hout->ctrl.rel;
         if (!L.hout->ctrl.converged) {
-            int r = issue_frequency(b, L, L.pending, BEM_MODE_FP64, L.omega, L.kappa, L.compose, L.rad_only, L.nb);
+            // the FP64 re-solve takes the checked factorisation (its flags
+            // are not looked at again)
+            int r = issue_frequency(b, L, L.pending, BEM_MODE_FP64, L.omega, L.kappa, L.compose, L.rad_only, L.nb,
+                                    true);
             if (r) return r;
             HIP_TRY(hipEventSynchronize(L.done));
             fell_back = 1;
