@@ -369,10 +369,6 @@ void factor_system(const SolveView& V, int precision, LuProfile* prof = nullptr)
 }
 void factor_system(Lane& L, int precision, LuProfile* prof = nullptr) { factor_system(lane_view(L), precision, prof); }
 
-// Solve D X = B for nr columns (B in the workspace, X out). FP32 mode:
-// FP32 LU solve followed by FP64 iterative refinement against the FP64
-// matrix s.Dr/s.Di, controlled on the device (no host round trip), with at
-// most V.refine_max correction steps.
 // Refinement iterations it0 .. it1 of an FP32 solve: each is a correction
 // (it > 0: d = LU32^-1 r, x += d) followed by the residual r = b - A x in
 // FP64 (the first also yields |A|_inf) and the convergence check; all of
@@ -402,6 +398,11 @@ void refine_steps(const SolveView& V, int nr, int it0, int it1) {
 int refine_last(const SolveView& V) { return std::max(0, std::min(V.refine_max, kRefineMaxIter)); }
 int refine_first_tail(const SolveView& V) { return std::min(V.refine_eager, refine_last(V)) + 1; }
 
+// Solve D X = B for nr columns (B in the workspace, X out). FP32 mode:
+// FP32 LU solve followed by FP64 iterative refinement against the FP64
+// matrix s.Dr/s.Di, controlled on the device (no host round trip), with at
+// most V.refine_max correction steps, of which the first V.refine_eager are
+// enqueued here (refine_tail enqueues the others).
 void solve_system(const SolveView& V, int precision, int nr) {
     LinSys& s = *V.sys;
     hipStream_t st = V.s1;
