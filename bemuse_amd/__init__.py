@@ -164,6 +164,7 @@ _SIGS = {
     "bem_set_lu_arith": (c_int, [c_void_p, c_int]),
     "bem_set_lu_tile": (c_int, [c_void_p, c_int]),
     "bem_set_lu_speculative": (c_int, [c_void_p, c_int]),
+    "bem_set_refine_eager": (c_int, [c_void_p, c_int]),
     "bem_set_pivot_threshold": (c_int, [c_void_p, c_double]),
     "bem_set_wave_far32": (c_int, [c_void_p, c_double]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
@@ -444,6 +445,11 @@ class Context:
         """Correction steps allowed to the mixed modes' refinement (0..4, default 4);
         0 sends every mixed frequency to the FP64 re-solve."""
         _check(lib().bem_set_refine_limit(self._h, int(max_steps)))
+
+    def set_refine_eager(self, steps):
+        """Correction steps a solve lane enqueues with its frequency (0..4, default 2); the
+        rest follow when the refinement needs them. Results do not depend on it."""
+        _check(lib().bem_set_refine_eager(self._h, int(steps)))
 
     def set_wave_far32(self, near_radius):
         """Measurement option: the wave term of pairs farther than near_radius (m) with FP32

@@ -115,6 +115,7 @@ struct bem_ctx {
     bool graphs = false;               // replay captured frequency graphs on the lanes (bem_set_graphs)
     int lu_spec = 1;                   // solve lanes factor speculatively (bem_set_lu_speculative)
     double pivot_tau = 1.0;            // multiplier bound of the pivot check (bem_set_pivot_threshold)
+    int refine_eager = kRefineEager;   // correction steps a lane enqueues with its frequency (bem_set_refine_eager)
     hipStream_t stream = nullptr;      // setup work (uploads, Rankine assembly)
     hipEvent_t ev[2] = {};
     std::vector<struct Lane*> pool;    // lanes shared by the bodies of bem_solve_batch
@@ -952,7 +953,7 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
                     int nb, bool pivot = false) {
     if (b->Np > L.sys.capNp) return fail(BEM_ERR_STATE, "lane workspace smaller than the body");
     L.lu_spec = (pivot || !b->ctx->lu_spec) ? 0 : 1;
-    L.refine_eager = kRefineEager;
+    L.refine_eager = b->ctx->refine_eager;
     L.pivot_tau = b->ctx->pivot_tau;
     LinSys& s = L.sys;
     s.N = b->N;
@@ -1209,6 +1210,13 @@ int bem_set_lu_arith(bem_ctx* c, int arith) {
         return fail(BEM_ERR_ARG, "bem_set_lu_arith: BEM_LU_FP32, BEM_LU_SPLIT_F16 or BEM_LU_SPLIT_F16_TILE");
     const int base = arith == BEM_LU_FP32 ? kLuArithF32 : arith == BEM_LU_SPLIT_F16 ? kLuArithSplitF16 : kLuArithSplitF16Tile;
     c->lu_arith = (c->lu_arith & ~kLuArithMask) | base;
+    return BEM_OK;
+}
+
+int bem_set_refine_eager(bem_ctx* c, int steps) {
+    if (!c || steps < 0 || steps > kRefineMaxIter)
+        return fail(BEM_ERR_ARG, "bem_set_refine_eager: steps must be in [0, 4]");
+    c->refine_eager = steps;
     return BEM_OK;
 }
 

@@ -180,6 +180,15 @@ int bem_set_graphs(bem_ctx* ctx, int enable);
  * that path). No counterpart in the reference (it has no refinement). */
 int bem_set_refine_limit(bem_ctx* ctx, int max_steps);
 
+/* Correction steps of the mixed modes' refinement that a solve lane enqueues
+ * with its frequency (0..4, default 2): every enqueued step that the device
+ * skips once the refinement has converged still costs a launch on the lane's
+ * chain, so the lane enqueues these and, when its results come back with the
+ * refinement neither converged nor stopped, enqueues the rest (up to the
+ * limit of bem_set_refine_limit). Results are the same for any value. No
+ * counterpart in the reference. */
+int bem_set_refine_eager(bem_ctx* ctx, int steps);
+
 /* Arithmetic of the trailing updates of the FP32 factorisation in the mixed
  * modes (the FP64 refinement that follows makes both give the same results
  * to the refinement tolerance):

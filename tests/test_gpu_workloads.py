@@ -126,6 +126,27 @@ def test_cylinder1024_speculative_equals_checked(cyl1024, ctx, mode):
         ctx.set_lu_speculative(True)
 
 
+def test_cylinder1024_refinement_steps_enqueued_later(cyl1024, ctx):
+    """The refinement steps a lane leaves for after its results (all of them
+    with 0 enqueued up front, one with 1) give the results of the default
+    (2, enough for these systems) bit for bit, with the same step counts."""
+    body, ob, omegas, refs = cyl1024
+    w = omegas[[1, 5, 8]]
+    ref = body.solve_sweep(w, betas=[0.0, 0.4], mode=bm.MODE_MIXED, lanes=2)
+    try:
+        for eager in (0, 1):
+            ctx.set_refine_eager(eager)
+            got = body.solve_sweep(w, betas=[0.0, 0.4], mode=bm.MODE_MIXED, lanes=2)
+            for key in ("A", "B", "F", "FK", "SC"):
+                np.testing.assert_array_equal(got[key], ref[key], err_msg=f"{key} eager {eager}")
+            assert [i["refine_iters"] for i in got["info"]] == [i["refine_iters"] for i in ref["info"]]
+            assert all(i["fell_back_fp64"] == 0 for i in got["info"])
+    finally:
+        ctx.set_refine_eager(2)
+    with pytest.raises(bm.BemError):
+        ctx.set_refine_eager(5)
+
+
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("tau", [0.0, 0.05])
 def test_cylinder1024_pivot_threshold_reruns(cyl1024, ctx, mode, tau):
