@@ -19,6 +19,8 @@
 #include <fstream>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <chrono>
 #include <vector>
 
 using namespace bemuse;
@@ -1121,8 +1123,8 @@ struct Task {
 };
 
 // Runs the tasks in order on the lanes: each goes to a free lane, else to the
-// first lane whose previous task has completed, else (all busy) to the lane
-// that has been busy longest, after that lane's results are collected.
+// first lane whose previous task has completed (polled while all are busy),
+// after that lane's results are collected.
 int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, int mode, int compose, int nb,
               double* A36, double* B36, double* F, double* FK, double* SC, bem_solve_info* infos) {
     size_t fs = (size_t)6 * nb * 2;
@@ -1130,8 +1132,6 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
     // one lane: its factorisations have the device to themselves
     for (Lane* L : lanes) L->lu_latency = nl == 1;
     std::vector<int> busy(nl, -1);
-    std::vector<unsigned long long> since(nl, 0);
-    unsigned long long seq = 0;
     size_t next = 0;
     int numeric = BEM_OK;
     std::string numeric_msg;
@@ -1147,11 +1147,15 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
         if (next < tasks.size())
             for (int l = 0; l < nl && pick < 0; l++)
                 if (busy[l] < 0) pick = l;
-        for (int l = 0; l < nl && pick < 0; l++)
-            if (busy[l] >= 0 && hipEventQuery(lanes[l]->done) == hipSuccess) pick = l;
-        if (pick < 0)   // every lane busy and none finished: the one busy longest
-            for (int l = 0; l < nl; l++)
-                if (busy[l] >= 0 && (pick < 0 || since[l] < since[pick])) pick = l;
+        // every lane busy: the first to finish (polled, so that no finished
+        // lane waits while the host blocks on another)
+        bool any_busy = false;
+        for (int l = 0; l < nl; l++) any_busy = any_busy || busy[l] >= 0;
+        while (pick < 0 && any_busy) {
+            for (int l = 0; l < nl && pick < 0; l++)
+                if (busy[l] >= 0 && hipEventQuery(lanes[l]->done) == hipSuccess) pick = l;
+            if (pick < 0) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
         if (pick < 0) break;   // nothing in flight and nothing left
         int r;
         if (busy[pick] >= 0) {
@@ -1175,7 +1179,6 @@ int run_tasks(const std::vector<Lane*>& lanes, const std::vector<Task>& tasks, i
                 return drain(r);
             }
             busy[pick] = (int)next;
-            since[pick] = seq++;
             next++;
         }
     }
