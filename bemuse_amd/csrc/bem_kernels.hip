@@ -424,9 +424,10 @@ __device__ inline void cmma_k(const T* ar_, const T* ai_, size_t lda, const T* b
 //   then the trailing blocks A_ij -= L_ik U_kj.
 // Then wave j forms block column j of both inverses, keeping its blocks in
 // MFMA accumulators: X_ij = -Li_ii sum_{k=j}^{i-1} L_ik X_kj (L^-1) and
-// Y_ij = -Ui_ii sum_{k=i+1}^{j} U_ik Y_kj (U^-1). Saves the block, writes L\U,
-// L^-1 and U^-1 (NB x NB column-major), and sets flag[panel] bit 1 if a
-// multiplier exceeds 1 in modulus (1|2 on a zero pivot).
+// Y_ij = -Ui_ii sum_{k=i+1}^{j} U_ik Y_kj (U^-1). Saves the block (save_r
+// non-null), writes L\U, L^-1 and U^-1 (NB x NB column-major), and sets
+// flag[panel] bit 1 if a multiplier exceeds the bound in modulus (tau2 its
+// square; 1|2 on a zero pivot).
 // ---------------------------------------------------------------------------
 constexpr int DB = 16;
 constexpr int DLD = LU_NB + 4;       // LDS column stride of the block
@@ -751,7 +752,7 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
         for (int n = 0; n < PER; n++) {
             int t = tid + 256 * n, c = t / LU_NB, r = t % LU_NB;
             S.ar[c][r] = vr[n]; S.ai[c][r] = vi[n];
-            save_r[t] = vr[n]; save_i[t] = vi[n];
+            if (save_r) { save_r[t] = vr[n]; save_i[t] = vi[n]; }
         }
     }
     __syncthreads();
@@ -830,8 +831,9 @@ __global__ __launch_bounds__(256) void k_lu_diag(int Np, int k0, T* Ar, T* Ai, T
 // rows and wave b the 16 columns 16 b .. 16 b + 15 of them (a quarter of the
 // MFMA chain each, four times the work-groups of one wave per 16 rows); all
 // waves read all NB columns of the rows, so they meet before overwriting.
-// Saves A21 (rows relative to k0+NB, ld Np) for the pivoting fallback and
-// flags multipliers with |l| > 1.
+// Saves A21 (rows relative to k0+NB, ld Np) for the pivoting fallback
+// (save_r non-null) and flags multipliers with |l| above the bound (tau2 its
+// square).
 // kp < k0: the panel is not the first of its group (whose first column is
 // kp), and its columns still lack the group's deferred update. The
 // work-group applies it to its own rows first,
@@ -861,8 +863,10 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, int kp, T* Ar, T
             size_t o = (size_t)(k0 + col) * Np + row0 + row;
             T vr = Ar[o] - ur[r], vi = Ai[o] - ui[r];
             wr[col][row] = vr; wi[col][row] = vi;
-            save_r[(size_t)col * Np + rowl + row] = vr;
-            save_i[(size_t)col * Np + rowl + row] = vi;
+            if (save_r) {
+                save_r[(size_t)col * Np + rowl + row] = vr;
+                save_i[(size_t)col * Np + rowl + row] = vi;
+            }
         }
         __syncthreads();
         cmma64<T>(&wr[0][0], &wi[0][0], 17, Ui_r + (16 * b) * LU_NB, Ui_i + (16 * b) * LU_NB, LU_NB, cr, ci, i, g);
@@ -876,11 +880,13 @@ __global__ __launch_bounds__(256) void k_lu_l21(int Np, int k0, int kp, T* Ar, T
                 size_t o = (size_t)(k0 + c) * Np + row0 + r;
                 vr[n] = Ar[o]; vi[n] = Ai[o];
             }
+            if (save_r) {
 #pragma unroll
-            for (int n = 0; n < PER; n++) {
-                int t = tid + 256 * n, r = t % 16, c = t / 16;
-                save_r[(size_t)c * Np + rowl + r] = vr[n];
-                save_i[(size_t)c * Np + rowl + r] = vi[n];
+                for (int n = 0; n < PER; n++) {
+                    int t = tid + 256 * n, r = t % 16, c = t / 16;
+                    save_r[(size_t)c * Np + rowl + r] = vr[n];
+                    save_i[(size_t)c * Np + rowl + r] = vi[n];
+                }
             }
         }
         cmma64<T>(Ar + (size_t)k0 * Np + row0, Ai + (size_t)k0 * Np + row0, Np, Ui_r + (16 * b) * LU_NB,
@@ -1082,8 +1088,10 @@ __global__ __launch_bounds__(256) void k_lu_l21_wide(int Np, int k0, int kp, T* 
                 const size_t o = (size_t)(k0 + col) * Np + row0 + row;
                 const T vr = Ar[o] - ur[q][r], vi = Ai[o] - ui[q][r];
                 wr[col][row] = vr; wi[col][row] = vi;
-                save_r[(size_t)col * Np + rowl + row] = vr;
-                save_i[(size_t)col * Np + rowl + row] = vi;
+                if (save_r) {
+                    save_r[(size_t)col * Np + rowl + row] = vr;
+                    save_i[(size_t)col * Np + rowl + row] = vi;
+                }
             }
         __syncthreads();
         cmma_k_wide<T, true>(&wr[0][0], &wi[0][0], WLD, Ui_r + (16 * b) * LU_NB, Ui_i + (16 * b) * LU_NB, LU_NB,
@@ -1098,11 +1106,13 @@ __global__ __launch_bounds__(256) void k_lu_l21_wide(int Np, int k0, int kp, T* 
                 const size_t o = (size_t)(k0 + c) * Np + row0 + r;
                 vr[n] = Ar[o]; vi[n] = Ai[o];
             }
+            if (save_r) {
 #pragma unroll
-            for (int n = 0; n < PER; n++) {
-                const int t = tid + 256 * n, r = t % RW, c = t / RW;
-                save_r[(size_t)c * Np + rowl + r] = vr[n];
-                save_i[(size_t)c * Np + rowl + r] = vi[n];
+                for (int n = 0; n < PER; n++) {
+                    const int t = tid + 256 * n, r = t % RW, c = t / RW;
+                    save_r[(size_t)c * Np + rowl + r] = vr[n];
+                    save_i[(size_t)c * Np + rowl + r] = vi[n];
+                }
             }
         }
         cmma_k_wide<T, true>(Ar + (size_t)k0 * Np + row0, Ai + (size_t)k0 * Np + row0, Np, Ui_r + (16 * b) * LU_NB,
@@ -3686,22 +3696,28 @@ void launch_lu(hipStream_t s, hipStream_t s2, hipEvent_t* ev, int Np, T* Ar, T* 
     hipLaunchKernelGGL(k_init_lu, dim3(cdiv(Np + 1, 256)), dim3(256), 0, s, Np, npanel, flag, ipiv, umax);
     // kp: first column of the panel's group (kp < k0: the group's deferred
     // update of the panel's rows below the diagonal block is applied by L21)
+    // the copies that the pivoting fallback restores a panel from (none in a
+    // speculative factorisation, which has no fallback)
+    T* sd_r = spec ? nullptr : w.sd_r;
+    T* sd_i = spec ? nullptr : w.sd_i;
+    T* s21_r = spec ? nullptr : w.s21_r;
+    T* s21_i = spec ? nullptr : w.s21_i;
     auto panel = [&](hipStream_t st, int p, int kp) {
         int k0 = p * LU_NB, rest = Np - k0 - LU_NB;
         T *Lir = w.Linv_r + (size_t)p * NB2, *Lii = w.Linv_i + (size_t)p * NB2;
         T *Uir = w.Uinv_r + (size_t)p * NB2, *Uii = w.Uinv_i + (size_t)p * NB2;
-        hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, w.sd_r, w.sd_i, Lir, Lii, Uir,
-                           Uii, flag, tau2);
+        hipLaunchKernelGGL(k_lu_diag<T>, dim3(1), dim3(256), 0, st, Np, k0, Ar, Ai, sd_r, sd_i, Lir, Lii, Uir, Uii,
+                           flag, tau2);
         auto l21k = latency ? k_lu_l21<T, true> : k_lu_l21<T, false>;
         if constexpr (sizeof(T) == 4) {
             if (rest > 0 && !latency && Np >= kWidePanelNp) {   // shared device: 64 rows per work-group
                 hipLaunchKernelGGL(k_lu_l21_wide<T>, dim3(rest / (16 * WIDE)), dim3(256), 0, st, Np, k0, kp, Ar, Ai,
-                                   Uir, Uii, w.s21_r, w.s21_i, flag, tau2);
+                                   Uir, Uii, s21_r, s21_i, flag, tau2);
                 return;
             }
         }
         if (rest > 0)
-            hipLaunchKernelGGL(l21k, dim3(rest / 16), dim3(256), 0, st, Np, k0, kp, Ar, Ai, Uir, Uii, w.s21_r, w.s21_i,
+            hipLaunchKernelGGL(l21k, dim3(rest / 16), dim3(256), 0, st, Np, k0, kp, Ar, Ai, Uir, Uii, s21_r, s21_i,
                                flag, tau2);
     };
     auto fix = [&](int p) {      // pivot check and fallback of panel p, row swaps elsewhere
