@@ -167,6 +167,7 @@ _SIGS = {
     "bem_set_refine_eager": (c_int, [c_void_p, c_int]),
     "bem_set_pivot_threshold": (c_int, [c_void_p, c_double]),
     "bem_set_wave_far32": (c_int, [c_void_p, c_double]),
+    "bem_set_wave_wg_per_cu": (c_int, [c_void_p, c_int]),
     "bem_measure_peaks": (c_int, [c_void_p, _PD]),
     "bem_kochin": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, c_int, _PD, _PD, _PD, _PD, _PD]),
     "bem_fs_elevation": (c_int, [c_void_p, c_void_p, c_double, c_double, c_int, _PD, c_int, _PD, _PD, _PD, _PD,
@@ -455,6 +456,12 @@ class Context:
         """Measurement option: the wave term of pairs farther than near_radius (m) with FP32
         special functions; near_radius < 0 (default) turns it off."""
         _check(lib().bem_set_wave_far32(self._h, float(near_radius)))
+
+    def set_wave_wg_per_cu(self, n):
+        """Work-groups per CU of the wave assembly: -1 by order (default: 2 below padded
+        order 6144, else 0), 0 one work-group per tile, n > 0 that many per CU taking the
+        tiles in turn. Results do not depend on it."""
+        _check(lib().bem_set_wave_wg_per_cu(self._h, int(n)))
 
     def set_lu_arith(self, arith):
         """Arithmetic of the mixed modes' FP32 LU updates: LU_SPLIT_F16 (default), LU_FP32, or

@@ -97,6 +97,14 @@ constexpr double kRefineTol = 1e-13;
 // enqueued step that the device skips still costs a launch on the lane's
 // chain). The benchmark systems converge in two.
 constexpr int kRefineEager = 2;
+// Below this padded order the wave assembly runs about two work-groups per CU
+// (wg_per_cu of launch_wave_compose) instead of one per tile: the small
+// bodies' lanes spend most of their time in chains of short LU launches, and
+// an assembly that fills every CU delays those chains. Measured on the 64-body
+// batch (orders 640-4096): 620 frequency-solves/s uncapped, 634-638 capped;
+// on the 6018-panel sweep, uncapped is 1% faster.
+constexpr int kWaveCapNp = 6144;
+constexpr int kWaveCapWgPerCu = 2;
 
 // Largest order of bem_dense_solve_noalloc and its padded order.
 constexpr int kNaMaxN = BEM_NOALLOC_MAX_N;
@@ -108,6 +116,7 @@ struct bem_ctx {
     int refine_max = kRefineMaxIter;   // correction steps allowed (bem_set_refine_limit)
     int lu_arith = kLuArithSplitF16;   // arithmetic of the FP32 factorisation's updates (bem_set_lu_arith)
     double wave_far_r2 = -1.0;         // bem_set_wave_far32: squared radius beyond which the wave term runs FP32 special functions
+    int wave_wg_per_cu = -1;           // bem_set_wave_wg_per_cu: -1 by order (kWaveCapNp), 0 one work-group per tile
     // streams and events of bem_dense_solve_noalloc, made here so that the
     // entry point itself creates nothing
     hipStream_t na_s2 = nullptr;
@@ -202,11 +211,12 @@ struct LinSys {
 // surrogate use.
 struct GraphKey {
     unsigned long long body = 0;
-    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1, arith = -1;
+    int Np = 0, mode = -1, nb = -1, rad_only = -1, compose = -1, sur = -1, refine = -1, arith = -1, wave_cap = -2;
     double far_r2 = -2.0;
     bool operator==(const GraphKey& o) const {
         return body == o.body && Np == o.Np && mode == o.mode && nb == o.nb && rad_only == o.rad_only &&
-               compose == o.compose && sur == o.sur && refine == o.refine && arith == o.arith && far_r2 == o.far_r2;
+               compose == o.compose && sur == o.sur && refine == o.refine && arith == o.arith &&
+               wave_cap == o.wave_cap && far_r2 == o.far_r2;
     }
 };
 
@@ -884,6 +894,12 @@ void set_frequency(Lane& L, double omega, double kappa) {
     (void)hipMemcpyAsync(L.dout->freq, L.hout->freq, sizeof(L.hout->freq), hipMemcpyHostToDevice, L.s1);
 }
 
+// Work-groups per CU of the body's wave assembly (0: one per tile).
+int wave_cap(const bem_body* b) {
+    const int c = b->ctx->wave_wg_per_cu;
+    return c >= 0 ? c : (b->Np < kWaveCapNp ? kWaveCapWgPerCu : 0);
+}
+
 // Wave assembly + compose at the lane's device frequency; kappa (the same
 // value) selects the surrogate path on the host.
 void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
@@ -898,7 +914,7 @@ void assemble(bem_body* b, Lane& L, int mode, double kappa, int compose) {
     }
     launch_wave_compose(L.s1, b->N, b->Np, L.dout->freq, compose, b->d_quad, b->d_recv, b->Ssrc, b->Srefl, b->Dsrc, b->Drefl,
                         L.Sr, L.Si, L.sys.Dr, L.sys.Di, f32 ? L.sys.D32r : nullptr, f32 ? L.sys.D32i : nullptr,
-                        b->ctx->wave_far_r2);
+                        b->ctx->wave_far_r2, wave_cap(b));
 }
 
 int enqueue_results(bem_body* b, Lane& L, bool timing);
@@ -970,7 +986,7 @@ int issue_frequency(bem_body* b, Lane& L, int index, int mode, double omega, dou
     L.hout->freq[1] = kappa;
     const bool sur = mode == BEM_MODE_MIXED_SURROGATE && compose == BEM_COMPOSE_WAVE && kappa != 0.0;
     const GraphKey key{b->id, b->Np, mode, nb, rad_only, compose, sur ? 1 : 0, L.refine_max, L.lu_arith,
-                       b->ctx->wave_far_r2};
+                       wave_cap(b), b->ctx->wave_far_r2};
     const bool use_graph = !pivot && b->ctx->graphs && (L.gexec && L.gkey == key ? true : L.last_key == key);
     L.last_key = key;
     L.graphed = use_graph;
@@ -1205,6 +1221,12 @@ int bem_set_refine_limit(bem_ctx* c, int max_steps) {
 int bem_set_wave_far32(bem_ctx* c, double near_radius) {
     if (!c || std::isnan(near_radius)) return fail(BEM_ERR_ARG, "bem_set_wave_far32: null context or NaN radius");
     c->wave_far_r2 = near_radius < 0.0 ? -1.0 : near_radius * near_radius;
+    return BEM_OK;
+}
+
+int bem_set_wave_wg_per_cu(bem_ctx* c, int n) {
+    if (!c || n < -1 || n > 64) return fail(BEM_ERR_ARG, "bem_set_wave_wg_per_cu: null context or n not in [-1, 64]");
+    c->wave_wg_per_cu = n;
     return BEM_OK;
 }
 

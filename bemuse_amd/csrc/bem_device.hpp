@@ -97,8 +97,10 @@ __host__ __device__ inline void struve_h01(double x, double y0, double y1, doubl
 // Bessel J0(x), J1(x) for x >= 0 and, for x > 3, Y0(x), Y1(x) (0 below):
 // what HavelockGF needs (Kernels.cpp:31-34, 284-295, 327-338; Y only above
 // 3, :56-74). x <= 3: power series. x > 3: Hankel form with P0, Q0, P1, Q1
-// as degree-23 Chebyshev series in t = 6/x - 1 (coefficients and method in
-// gen_bessel.py) and one shared sin/cos of x (three-piece pi/2 reduction,
+// as their degree-23 Chebyshev interpolants in t = 6/x - 1, rewritten in
+// powers of t and evaluated by Horner's rule (coefficients, method and the
+// conditioning of the power form in gen_bessel.py) and one shared sin/cos of
+// x (three-piece pi/2 reduction,
 // Taylor series on |r| <= pi/4). Beyond x = 1.6e6 the reduction is no longer
 // exact and the device library functions are used.
 __host__ __device__ inline void bessel01_series(double x, double& J0, double& J1) {
@@ -124,18 +126,16 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
         J0 = j0(x); J1 = j1(x); Y0 = y0(x); Y1 = y1(x);
         return;
     }
-    const double u = 3.0 * frcp(x), t = 2.0 * u - 1.0, t2 = 2.0 * t;
-    // Clenshaw for the four series at once
-    double p1 = 0.0, p2 = 0.0, q1 = 0.0, q2 = 0.0, r1 = 0.0, r2 = 0.0, s1 = 0.0, s2 = 0.0;
+    const double u = 3.0 * frcp(x), t = 2.0 * u - 1.0;
+    // Horner for the four series at once
+    double P0 = kBesP0[23], Q0 = kBesQ0[23], P1 = kBesP1[23], Q1 = kBesQ1[23];
 #pragma unroll
-    for (int k = 23; k >= 1; k--) {
-        double tp = fma(t2, p1, kBesP0[k] - p2); p2 = p1; p1 = tp;
-        double tq = fma(t2, q1, kBesQ0[k] - q2); q2 = q1; q1 = tq;
-        double tr = fma(t2, r1, kBesP1[k] - r2); r2 = r1; r1 = tr;
-        double ts = fma(t2, s1, kBesQ1[k] - s2); s2 = s1; s1 = ts;
+    for (int k = 22; k >= 0; k--) {
+        P0 = fma_k(P0, t, kBesP0[k]);
+        Q0 = fma_k(Q0, t, kBesQ0[k]);
+        P1 = fma_k(P1, t, kBesP1[k]);
+        Q1 = fma_k(Q1, t, kBesQ1[k]);
     }
-    const double P0 = fma(t, p1, kBesP0[0] - p2), Q0 = fma(t, q1, kBesQ0[0] - q2);
-    const double P1 = fma(t, r1, kBesP1[0] - r2), Q1 = fma(t, s1, kBesQ1[0] - s2);
     // x = n pi/2 + r, |r| <= pi/4 (n pi/2 pieces exact for n < 2^21)
     const double n = rint(x * kTwoOverPi);
     double r = fma(-n, kPio2A, x);
@@ -167,23 +167,22 @@ __host__ __device__ inline void bessel01(double x, double& J0, double& J1, doubl
 // FP32 versions of the special functions above, for the wave term of far
 // pairs (a measured option; the exact path stays FP64). Same methods, each
 // series cut where its terms fall below FP32 resolution: the Struve power
-// series at 6 terms, the Chebyshev series at degree 10, sin/cos of the reduced
+// series at 6 terms, the Chebyshev interpolants at degree 10 (in powers of t
+// like the FP64 ones), sin/cos of the reduced
 // argument at degree 9/10; the reduction x = n pi/2 + r stays FP64 (an FP32
 // argument of K R ~ 100 would already carry a phase error of 1e-5).
 // bessel01_f32 covers x > 3 only (below, the caller takes the FP64 series).
 __device__ inline void bessel01_f32(double xd, float& J0, float& J1, float& Y0, float& Y1) {
     const float x = (float)xd;
-    const float u = 3.0f / x, t = 2.0f * u - 1.0f, t2 = 2.0f * t;
-    float p1 = 0.0f, p2 = 0.0f, q1 = 0.0f, q2 = 0.0f, r1 = 0.0f, r2 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+    const float u = 3.0f / x, t = 2.0f * u - 1.0f;
+    float P0 = (float)kBesP0F[10], Q0 = (float)kBesQ0F[10], P1 = (float)kBesP1F[10], Q1 = (float)kBesQ1F[10];
 #pragma unroll
-    for (int k = 10; k >= 1; k--) {
-        float tp = fmaf(t2, p1, (float)kBesP0[k] - p2); p2 = p1; p1 = tp;
-        float tq = fmaf(t2, q1, (float)kBesQ0[k] - q2); q2 = q1; q1 = tq;
-        float tr = fmaf(t2, r1, (float)kBesP1[k] - r2); r2 = r1; r1 = tr;
-        float ts = fmaf(t2, s1, (float)kBesQ1[k] - s2); s2 = s1; s1 = ts;
+    for (int k = 9; k >= 0; k--) {
+        P0 = fmaf(P0, t, (float)kBesP0F[k]);
+        Q0 = fmaf(Q0, t, (float)kBesQ0F[k]);
+        P1 = fmaf(P1, t, (float)kBesP1F[k]);
+        Q1 = fmaf(Q1, t, (float)kBesQ1F[k]);
     }
-    const float P0 = fmaf(t, p1, (float)kBesP0[0] - p2), Q0 = fmaf(t, q1, (float)kBesQ0[0] - q2);
-    const float P1 = fmaf(t, r1, (float)kBesP1[0] - r2), Q1 = fmaf(t, s1, (float)kBesQ1[0] - s2);
     const double n = rint(xd * kTwoOverPi);
     double rd = fma(-n, kPio2A, xd);
     rd = fma(-n, kPio2B, rd);

@@ -13,11 +13,14 @@ constexpr int kMaxPanels = 1024; // largest padded order / kLuNb (Np <= 65536)
 
 void launch_rankine(hipStream_t s, int N, int Np, const double* body, const double* refl, const double* recv,
                     double* Ssrc, double* Srefl, double* Dsrc, double* Drefl);
-// freq: device pointer to (omega, kappa) of the frequency.
+// freq: device pointer to (omega, kappa) of the frequency. wg_per_cu > 0
+// (default path, far_r2 < 0) launches wg_per_cu x CUs work-groups that take
+// the tiles in turn (about that many resident per CU); 0: one work-group per
+// tile.
 void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
                          const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                          const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
-                         float* D32i, double far_r2 = -1.0);
+                         float* D32i, double far_r2 = -1.0, int wg_per_cu = 0);
 void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out,
                           bool far32 = false);
 // FP32 and FP64 MFMA TFLOP/s and streaming-copy GB/s measured on the current device; 0 on success.

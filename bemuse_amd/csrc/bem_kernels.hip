@@ -72,15 +72,17 @@ __global__ __launch_bounds__(256) void k_rankine(int N, int Np, const double* __
 // D = identity.
 // freq: device (omega, kappa) of the frequency (read on the device, so a
 // captured graph replays with the frequency written before each launch).
-__global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const double* __restrict__ freq, int compose,
-                                                      const double* __restrict__ quad, const double* __restrict__ recv,
-                                                      const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
-                                                      const double* __restrict__ Dsrc, const double* __restrict__ Drefl,
-                                                      double* Sr, double* Si, double* Dr, double* Di,
-                                                      float* D32r, float* D32i, double far_r2) {
-    __shared__ double sq[AS_TS][32];
-    const double K = freq[1];
-    int r0 = blockIdx.x * AS_TR, s0 = blockIdx.y * AS_TS;
+// FAR32: the instantiation with the FP32 far-pair branch (far_r2 >= 0). The
+// default one leaves it out: the branch's inlined FP32 functions raise the
+// kernel from 120 to 173 VGPRs, which halves the waves per SIMD (4 to 2).
+// One tile: 64 receivers from r0 x 16 sources from s0.
+template <bool FAR32>
+__device__ __forceinline__ void wave_compose_tile(int N, int Np, double K, int compose, int r0, int s0,
+                                                  double (*sq)[32], const double* __restrict__ quad,
+                                                  const double* __restrict__ recv, const double* __restrict__ Ssrc,
+                                                  const double* __restrict__ Srefl, const double* __restrict__ Dsrc,
+                                                  const double* __restrict__ Drefl, double* Sr, double* Si, double* Dr,
+                                                  double* Di, float* D32r, float* D32i, double far_r2) {
     for (int i = threadIdx.x; i < AS_TS * 32; i += 256) {
         int s = s0 + i / 32;
         sq[i / 32][i % 32] = s < N ? quad[(size_t)s * 32 + i % 32] : 0.0;
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const doubl
                 // far_r2 >= 0: pairs whose first mirrored node lies farther than
                 // sqrt(far_r2) from the receiver take the FP32 special functions
                 const double ex = px - sq[sl][0], ey = py - sq[sl][1], ez = pz - sq[sl][2];
-                if (far_r2 >= 0.0 && ex * ex + ey * ey + ez * ez > far_r2)
+                if (FAR32 && ex * ex + ey * ey + ez * ez > far_r2)
                     wave_panel<true>(sq[sl], px, py, pz, K, wsr, wsi, wdr, wdi);
                 else
                     wave_panel<false>(sq[sl], px, py, pz, K, wsr, wsi, wdr, wdi);
@@ -120,6 +122,38 @@ __global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const doubl
         }
         Sr[idx] = sre; Si[idx] = sim; Dr[idx] = dre; Di[idx] = dim;
         if (D32r) { D32r[idx] = (float)dre; D32i[idx] = (float)dim; }
+    }
+}
+
+// One work-group per tile (grid Np/64 x Np/16).
+template <bool FAR32>
+__global__ __launch_bounds__(256) void k_wave_compose(int N, int Np, const double* __restrict__ freq, int compose,
+                                                      const double* __restrict__ quad, const double* __restrict__ recv,
+                                                      const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
+                                                      const double* __restrict__ Dsrc, const double* __restrict__ Drefl,
+                                                      double* Sr, double* Si, double* Dr, double* Di,
+                                                      float* D32r, float* D32i, double far_r2) {
+    __shared__ double sq[AS_TS][32];
+    wave_compose_tile<FAR32>(N, Np, freq[1], compose, blockIdx.x * AS_TR, blockIdx.y * AS_TS, sq, quad, recv, Ssrc,
+                             Srefl, Dsrc, Drefl, Sr, Si, Dr, Di, D32r, D32i, far_r2);
+}
+
+// The same over a 1-D grid smaller than the tile count: work-group g takes
+// tiles g, g + grid, ... (tile t at receiver block t % (Np/64)), which bounds
+// the work-groups resident at a time. A separate kernel because the loop
+// raises the register count (120 to 130 VGPRs: 3 waves per SIMD instead of 4).
+__global__ __launch_bounds__(256) void k_wave_compose_strided(
+    int N, int Np, const double* __restrict__ freq, int compose, const double* __restrict__ quad,
+    const double* __restrict__ recv, const double* __restrict__ Ssrc, const double* __restrict__ Srefl,
+    const double* __restrict__ Dsrc, const double* __restrict__ Drefl, double* Sr, double* Si, double* Dr, double* Di,
+    float* D32r, float* D32i) {
+    __shared__ double sq[AS_TS][32];
+    const double K = freq[1];
+    const int tiles_r = Np / AS_TR, ntiles = tiles_r * (Np / AS_TS);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        wave_compose_tile<false>(N, Np, K, compose, (tile % tiles_r) * AS_TR, (tile / tiles_r) * AS_TS, sq, quad,
+                                 recv, Ssrc, Srefl, Dsrc, Drefl, Sr, Si, Dr, Di, D32r, D32i, -1.0);
+        __syncthreads();    // the tile's reads of sq are done before the next tile's loads
     }
 }
 
@@ -3005,10 +3039,20 @@ void launch_rankine(hipStream_t s, int N, int Np, const double* body, const doub
 void launch_wave_compose(hipStream_t s, int N, int Np, const double* freq, int compose, const double* quad,
                          const double* recv, const double* Ssrc, const double* Srefl, const double* Dsrc,
                          const double* Drefl, double* Sr, double* Si, double* Dr, double* Di, float* D32r,
-                         float* D32i, double far_r2) {
+                         float* D32i, double far_r2, int wg_per_cu) {
+    const int ntiles = (Np / AS_TR) * (Np / AS_TS);
+    int dev = 0, ncu = 0;
+    if (far_r2 < 0.0 && wg_per_cu > 0 && hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0 &&
+        wg_per_cu * ncu < ntiles) {
+        hipLaunchKernelGGL(k_wave_compose_strided, dim3(wg_per_cu * ncu), dim3(256), 0, s, N, Np, freq, compose, quad,
+                           recv, Ssrc, Srefl, Dsrc, Drefl, Sr, Si, Dr, Di, D32r, D32i);
+        return;
+    }
     dim3 grid(Np / AS_TR, Np / AS_TS);
-    hipLaunchKernelGGL(k_wave_compose, grid, dim3(256), 0, s, N, Np, freq, compose, quad, recv, Ssrc, Srefl, Dsrc,
-                       Drefl, Sr, Si, Dr, Di, D32r, D32i, far_r2);
+    auto k = far_r2 >= 0.0 ? k_wave_compose<true> : k_wave_compose<false>;
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, s, N, Np, freq, compose, quad, recv, Ssrc, Srefl, Dsrc, Drefl, Sr, Si,
+                       Dr, Di, D32r, D32i, far_r2);
 }
 
 void launch_havelock_eval(hipStream_t s, int n, const double* X, const double* Y, const double* V, double* out,

@@ -246,6 +246,16 @@ int bem_set_pivot_threshold(bem_ctx* ctx, double tau);
  * FP64, or FP32 throughout in its single-precision build). */
 int bem_set_wave_far32(bem_ctx* ctx, double near_radius);
 
+/* Work-groups per CU of the wave assembly (the exact FP64 path): n > 0 runs
+ * n x CUs work-groups that take the 64 x 16 tiles in turn, 0 one work-group
+ * per tile (every CU filled), -1 (the default) n = 2 below padded order 6144
+ * and 0 from there: the lanes of small bodies spend most of their time in
+ * chains of short factorisation launches, which an assembly filling every
+ * CU delays (the 64-body batch runs 2-3% faster capped; the 6018-panel sweep
+ * 1% faster uncapped). Results do not depend on it. No counterpart in the
+ * reference (a scheduling option). */
+int bem_set_wave_wg_per_cu(bem_ctx* ctx, int n);
+
 /* Measured machine peaks of the context's device, for reporting beside the
  * published ones: out[0] FP32 MFMA TFLOP/s, out[1] FP64 MFMA TFLOP/s (dense
  * 16x16x4 chains on registers), out[2] streaming-copy GB/s (1 GiB read +

@@ -126,6 +126,37 @@ def test_cylinder1024_speculative_equals_checked(cyl1024, ctx, mode):
         ctx.set_lu_speculative(True)
 
 
+def test_cylinder1024_wave_assembly_work_groups(cyl1024, ctx):
+    """The wave assembly with one work-group per tile and with 1, 2 or 3
+    work-groups per CU taking the 1024 tiles (64 receivers x 16 sources) in
+    turn: the same system bit for bit, in FP64 and mixed mode (which also
+    writes the FP32 copy of D), and the same A, B, F through a sweep. Bad
+    settings are rejected."""
+    body, ob, omegas, refs = cyl1024
+    kappa = 1.3 ** 2 / 9.81
+    try:
+        for mode in MODES:
+            ctx.set_wave_wg_per_cu(0)
+            S0, D0 = body.get_system(kappa, mode=mode)
+            for n in (1, 2, 3, -1):
+                ctx.set_wave_wg_per_cu(n)
+                S, D = body.get_system(kappa, mode=mode)
+                np.testing.assert_array_equal(S, S0, err_msg=f"S n={n} mode {mode}")
+                np.testing.assert_array_equal(D, D0, err_msg=f"D n={n} mode {mode}")
+        w = omegas[2:6]
+        ctx.set_wave_wg_per_cu(0)
+        ref = body.solve_sweep(w, betas=[0.0], mode=bm.MODE_MIXED, lanes=2)
+        ctx.set_wave_wg_per_cu(2)
+        got = body.solve_sweep(w, betas=[0.0], mode=bm.MODE_MIXED, lanes=2)
+        for key in ("A", "B", "F"):
+            np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
+        for bad in (-2, 65):
+            with pytest.raises(bm.BemError):
+                ctx.set_wave_wg_per_cu(bad)
+    finally:
+        ctx.set_wave_wg_per_cu(-1)
+
+
 def test_cylinder1024_refinement_steps_enqueued_later(cyl1024, ctx):
     """The refinement steps a lane leaves for after its results (all of them
     with 0 enqueued up front, one with 1) give the results of the default
