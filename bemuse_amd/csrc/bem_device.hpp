@@ -28,6 +28,25 @@ __host__ __device__ inline double frcp(double x) {
 #endif
 }
 
+// sqrt(a) for finite a >= 0 from the hardware reciprocal square root and two
+// Newton-Raphson corrections of the root and its half-reciprocal (Goldschmidt
+// form). The library sqrt adds an exponent scaling for denormal and huge
+// arguments; the arguments here (squared distances scaled by K) are neither.
+__host__ __device__ inline double fsqrt(double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rsq(a);
+    double y = a * r, h = 0.5 * r;
+    const double e = fma(-h, y, 0.5);
+    y = fma(y, e, y);
+    h = fma(h, e, h);
+    const double d = fma(-y, y, a);
+    y = fma(d, h, y);
+    return a == 0.0 ? 0.0 : y;
+#else
+    return sqrt(a);
+#endif
+}
+
 // a b + c for a coefficient c known at compile time, as one v_fma_f64 with c
 // in an SGPR pair. Left to itself the compiler selects v_fmac_f64, whose
 // addend is also its destination, and copies every 64-bit coefficient into
@@ -238,6 +257,29 @@ __device__ inline void struve_h01_f32(double xd, float y0, float y1, float& h0, 
     h1 = c1b * c2a * rab + y1;
 }
 
+// Natural logarithm of a finite normal x > 0: x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with s = (m - 1) / (m + 1),
+// |s| <= 0.1716, summed as 2 s (1 + s^2 / 3 + s^4 / 5 + ...) to s^22 (the next
+// term is below 1e-18 of the sum), and e ln 2 with ln 2 split into a 32-bit
+// head and its remainder. The library log carries extra-precision steps that
+// cost about twice the instructions.
+__host__ __device__ inline double flog(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    constexpr double kLn2Hi = 0.693147180369123816490, kLn2Lo = 1.90821492927058770002e-10;
+    int e = __builtin_amdgcn_frexp_exp(x);
+    double m = __builtin_amdgcn_frexp_mant(x);       // [0.5, 1)
+    if (m < 0.70710678118654752440) { m = 2.0 * m; e -= 1; }
+    const double s = (m - 1.0) * frcp(m + 1.0), z = s * s;
+    double p = 2.0 / 23.0;
+#pragma unroll
+    for (int k = 10; k >= 0; k--) p = fma_k(p, z, 2.0 / (2 * k + 1));
+    const double ed = (double)e;
+    return fma(ed, kLn2Hi, fma(s, p, ed * kLn2Lo));
+#else
+    return log(x);
+#endif
+}
+
 // Polynomials A..D (local flow) and dA..dC (its horizontal derivative) of the
 // Wu/Zhang/Zhu/Li/Wan/Noblesse global approximation (Eur. J. Mech. B 65 (2017)
 // eq. 14; Liang/Wu/Noblesse, Appl. Ocean Res. 74 (2018) eq. 16), coefficients
@@ -279,8 +321,8 @@ struct GFVal { double gr, gi, hr, hi, h; };
 // would swamp it.
 template <bool S32>
 __host__ __device__ inline GFVal havelock_gf_t(double X, double Y, double vv) {
-    double hh = sqrt(X * X + Y * Y);
-    double dd = sqrt(hh * hh + vv * vv);
+    double hh = fsqrt(X * X + Y * Y);
+    double dd = fsqrt(hh * hh + vv * vv);
     double rdd = frcp(dd);
     double al = -vv * rdd;
     double be = hh * rdd;
@@ -298,7 +340,7 @@ __host__ __device__ inline GFVal havelock_gf_t(double X, double Y, double vv) {
 #endif
     {
         ev = exp(vv);
-        lg = log(0.5 * (dd - vv));
+        lg = flog(0.5 * (dd - vv));
         emd = exp(-dd);
     }
     double inv1 = frcp(1.0 + dd3);
